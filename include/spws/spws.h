@@ -1,0 +1,126 @@
+/*
+ * libspws -- MI355X (gfx950) sparse-workspace kernels behind a C ABI.
+ *
+ * Two layers of entry points:
+ *   1. TACO-shaped host entry points over taco_tensor_t (host arrays in,
+ *      malloc'd host arrays out), one per reference `compute`/`assemble`.
+ *      Each comment names the reference function it replaces.
+ *   2. Device-resident entry points over plain device pointers and sizes,
+ *      used by the batched/served path and by benchmarks.  Outputs of a
+ *      device-resident call live in buffers owned by the context and stay
+ *      valid until the next call of the same family on that context.
+ *
+ * Every function returns 0 (SPWS_OK) on success and a nonzero spws_status
+ * otherwise; spws_last_error() describes the last failure of the calling
+ * thread.  Nothing here calls exit().  Contexts are independent: different
+ * host threads may use different contexts concurrently.
+ */
+#ifndef SPWS_SPWS_H
+#define SPWS_SPWS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "spws/taco_tensor.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  SPWS_OK = 0,
+  SPWS_ERR_INVALID = 1,     /* bad argument or malformed input */
+  SPWS_ERR_OOM = 2,         /* device or host allocation failed */
+  SPWS_ERR_DEVICE = 3,      /* a HIP runtime call failed */
+  SPWS_ERR_UNSUPPORTED = 4, /* input outside what the kernels handle (e.g. int32 overflow) */
+} spws_status;
+
+typedef struct spws_ctx spws_ctx;
+
+/* Row-wise SpGEMM bin boundaries by per-row upper bound (ub = products).
+ * Rows with ub <= wave_ub run one wavefront each with an LDS table of
+ * 2*wave_ub slots, ub <= wave2_ub likewise; ub <= block_ub and <= block2_ub
+ * run one workgroup each with an LDS table; longer rows use the dense
+ * accumulator when ub * dense_ratio >= ncols, else a workgroup with an
+ * HBM-resident hash table.  force_path >= 0 sends every non-empty row to
+ * one path (0 wave, 1 wave2, 2 block, 3 block2, 4 dense, 5 HBM hash), which
+ * tests use to cover each path on small inputs. */
+typedef struct {
+  int32_t wave_ub;    /* <= 64 */
+  int32_t wave2_ub;   /* <= 256 */
+  int32_t block_ub;   /* <= 1024 */
+  int32_t block2_ub;  /* <= 4096 */
+  int32_t dense_ratio;
+  int32_t force_path;
+} spws_rowwise_tuning;
+
+typedef struct {
+  int64_t products;         /* sum of row upper bounds (= multiply-adds) */
+  int64_t nnz;              /* output nonzeros */
+  int64_t rows_per_path[8]; /* rows sent to each row-wise path (index = force_path id), [6] = empty */
+  int64_t chunks;           /* outer product: output-row chunks processed */
+  int64_t sort_bits;        /* outer product: key bits radix-sorted per chunk */
+} spws_stats;
+
+/* ---------------------------------------------------------------- context */
+
+const char* spws_last_error(void);
+/* stream: a hipStream_t to run on, or NULL for a stream the context creates. */
+int spws_ctx_create(int device, void* stream, spws_ctx** out);
+int spws_ctx_destroy(spws_ctx* ctx);
+int spws_ctx_set_stream(spws_ctx* ctx, void* stream);
+void* spws_ctx_get_stream(spws_ctx* ctx);
+int spws_ctx_get_stats(spws_ctx* ctx, spws_stats* out);
+int spws_ctx_set_rowwise_tuning(spws_ctx* ctx, const spws_rowwise_tuning* t);
+int spws_ctx_get_rowwise_tuning(spws_ctx* ctx, spws_rowwise_tuning* t);
+
+/* Device memory helpers for callers without another HIP runtime binding. */
+int spws_dev_alloc(spws_ctx* ctx, size_t bytes, void** out);
+int spws_dev_free(spws_ctx* ctx, void* p);
+int spws_memcpy_h2d(spws_ctx* ctx, void* dst, const void* src, size_t bytes);
+int spws_memcpy_d2h(spws_ctx* ctx, void* dst, const void* src, size_t bytes);
+int spws_sync(spws_ctx* ctx);
+void spws_host_free(void* p); /* frees arrays the host entry points malloc'd */
+
+/* ------------------------------------------------- row-wise SpGEMM (CSR x CSR) */
+
+/* C = A * B, A m x k, B k x n, all CSR with int32 pos/crd and FP64 values,
+ * device pointers.  Output: sorted CSR, explicit zeros kept, pattern
+ * {(i,k) : exists j, A(i,j) and B(j,k) stored}.  On return *C_nnz holds
+ * nnz(C) and *C_pos / *C_crd / *C_vals point at context-owned device arrays.
+ * Replaces the hot loop of compute() in
+ * benchmark/Parallel/CSR_CSR_hash_sequential.h:232-353. */
+int spws_spgemm_rowwise(spws_ctx* ctx, int32_t m, int32_t k, int32_t n,
+                        const int32_t* A_pos, const int32_t* A_crd, const double* A_vals,
+                        const int32_t* B_pos, const int32_t* B_crd, const double* B_vals,
+                        int64_t* C_nnz, int32_t** C_pos, int32_t** C_crd, double** C_vals);
+
+/* Symbolic phase only: exact row counts into C_pos (context-owned, m+1). */
+int spws_spgemm_rowwise_symbolic(spws_ctx* ctx, int32_t m, int32_t k, int32_t n,
+                                 const int32_t* A_pos, const int32_t* A_crd,
+                                 const int32_t* B_pos, const int32_t* B_crd,
+                                 int64_t* C_nnz, int32_t** C_pos);
+
+/* Host TACO entry points (host taco_tensor_t in, malloc'd host arrays out). */
+
+/* int compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B, int32_t w_cap)
+ * of benchmark/Parallel/CSR_CSR_hash_sequential.h:232 (and the identical
+ * benchmark/CSR_CSR/CSR_CSR_hash.h).  Sets C->indices[1][0] (pos),
+ * C->indices[1][1] (crd), C->vals and C->vals_size; the caller free()s them.
+ * w_cap is accepted for signature compatibility; the device workspace is
+ * sized per row, so it does not change the result structure. */
+int spws_csr_csr_hash_compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B, int32_t w_cap);
+
+/* int assemble(C, A, B) of seperate/CSR_CSR_1.h:69-167: fills C2_pos and
+ * allocates C2_crd / C_vals (the crd array is filled as well). */
+int spws_csr_csr_assemble(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B);
+
+/* int compute(C, A, B) of seperate/CSR_CSR_1.h:4-67: fills C_vals (and
+ * C2_crd) into the arrays assemble() allocated. */
+int spws_csr_csr_compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPWS_SPWS_H */

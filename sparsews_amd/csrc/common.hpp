@@ -1,0 +1,163 @@
+// Shared device helpers and the per-context state of libspws.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "spws/spws.h"
+
+namespace spws {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------- errors
+
+void set_error(const char* fmt, ...);
+int fail_hip(hipError_t e, const char* what, const char* file, int line);
+
+#define SPWS_HIP(call)                                                  \
+  do {                                                                  \
+    hipError_t e__ = (call);                                            \
+    if (e__ != hipSuccess) return ::spws::fail_hip(e__, #call, __FILE__, __LINE__); \
+  } while (0)
+
+#define SPWS_CHECK_LAUNCH() SPWS_HIP(hipGetLastError())
+
+#define SPWS_REQUIRE(cond, code, ...)                                   \
+  do {                                                                  \
+    if (!(cond)) {                                                      \
+      ::spws::set_error(__VA_ARGS__);                                   \
+      return (code);                                                    \
+    }                                                                   \
+  } while (0)
+
+// ---------------------------------------------------------------- buffers
+
+// A device buffer that only grows.  Growing frees the old allocation, which
+// synchronises the device; steady-state calls of the same size never allocate.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t want);
+  void release();
+  template <class T>
+  T* as() const { return static_cast<T*>(ptr); }
+};
+
+struct PinnedBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t want);
+  void release();
+  template <class T>
+  T* as() const { return static_cast<T*>(ptr); }
+};
+
+}  // namespace spws
+
+struct spws_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  // named scratch and output buffers (see the users for their layouts)
+  spws::DevBuf scan_tmp;
+  spws::DevBuf row_ub, row_off, row_nnz, bin_rows, bin_meta;
+  spws::DevBuf slot_crd, slot_val, wide_tab, dense_val, dense_flag;
+  spws::DevBuf out_pos, out_crd, out_val, out_aux;
+  spws::DevBuf keys[2], vals[2], chunk_a, chunk_b;
+  spws::DevBuf stage[12];  // device copies of host inputs (TACO entry points)
+  spws::PinnedBuf host_meta;
+  // row-wise SpGEMM bin boundaries (validated against the compiled table sizes)
+  spws_rowwise_tuning rw{64, 256, 1024, 4096, 16, -1};
+  // statistics of the last call (host-readable)
+  spws_stats stats{};
+};
+
+namespace spws {
+
+int ctx_activate(spws_ctx* ctx);
+
+// ---------------------------------------------------------------- scan
+
+// Exclusive prefix sum of n values into out[0..n]; out[n] receives the total.
+// Tin/Tout in {int32_t, int64_t}.  Uses ctx->scan_tmp.
+template <class Tin, class Tout>
+int exclusive_scan(spws_ctx* ctx, const Tin* in, Tout* out, int64_t n);
+
+// ---------------------------------------------------------------- device helpers
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+template <class T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    T u = __shfl_up(v, d, kWave);
+    if (lane >= d) v += u;
+  }
+  return v;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
+  return v;
+}
+
+// Exclusive scan over a block of BLOCK threads; returns this thread's prefix
+// and writes the block total to *total.  `sh` holds BLOCK/64 + 1 entries.
+template <class T, int BLOCK>
+__device__ __forceinline__ T block_excl_scan(T v, T* sh, T* total) {
+  constexpr int NW = BLOCK / kWave;
+  const int lane = lane_id();
+  const int w = threadIdx.x / kWave;
+  T incl = wave_incl_scan(v);
+  if (lane == kWave - 1) sh[w] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T run = 0;
+    for (int i = 0; i < NW; ++i) {
+      T t = sh[i];
+      sh[i] = run;
+      run += t;
+    }
+    sh[NW] = run;
+  }
+  __syncthreads();
+  T res = sh[w] + incl - v;
+  *total = sh[NW];
+  __syncthreads();
+  return res;
+}
+
+// Fibonacci hash of a column index into a table of 2^(32 - shift) slots
+// (the high bits of the product, so strided column patterns spread out).
+__device__ __forceinline__ uint32_t hash_col(uint32_t k, uint32_t shift) {
+  return (k * 2654435761u) >> shift;
+}
+
+__host__ __device__ __forceinline__ int log2_pow2(uint32_t v) {
+  int r = 0;
+  while ((1u << r) < v) ++r;
+  return r;
+}
+
+__host__ __device__ __forceinline__ int64_t div_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+__host__ __device__ __forceinline__ uint32_t next_pow2_u32(uint32_t v) {
+  v--;
+  v |= v >> 1;
+  v |= v >> 2;
+  v |= v >> 4;
+  v |= v >> 8;
+  v |= v >> 16;
+  return v + 1;
+}
+
+}  // namespace spws

@@ -1,0 +1,321 @@
+// Context, error reporting, device buffers and the device prefix sum.
+#include <cstdarg>
+#include <cstdlib>
+
+#include "common.hpp"
+
+namespace spws {
+
+static thread_local std::string g_last_error;
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+
+int fail_hip(hipError_t e, const char* what, const char* file, int line) {
+  set_error("%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return SPWS_ERR_OOM;
+  return SPWS_ERR_DEVICE;
+}
+
+int DevBuf::ensure(size_t want) {
+  if (want <= bytes && ptr) return SPWS_OK;
+  release();
+  size_t sz = want < 256 ? 256 : want;
+  // grow geometrically so a slowly growing workload does not reallocate each call
+  sz = sz + sz / 4;
+  hipError_t e = hipMalloc(&ptr, sz);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    ptr = nullptr;
+    bytes = 0;
+    set_error("hipMalloc(%zu bytes) failed: %s", sz, hipGetErrorString(e));
+    return SPWS_ERR_OOM;
+  }
+  bytes = sz;
+  return SPWS_OK;
+}
+
+void DevBuf::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+int PinnedBuf::ensure(size_t want) {
+  if (want <= bytes && ptr) return SPWS_OK;
+  release();
+  size_t sz = want < 4096 ? 4096 : want;
+  hipError_t e = hipHostMalloc(&ptr, sz, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    ptr = nullptr;
+    bytes = 0;
+    set_error("hipHostMalloc(%zu) failed: %s", sz, hipGetErrorString(e));
+    return SPWS_ERR_OOM;
+  }
+  bytes = sz;
+  return SPWS_OK;
+}
+
+void PinnedBuf::release() {
+  if (ptr) (void)hipHostFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+int ctx_activate(spws_ctx* ctx) {
+  SPWS_REQUIRE(ctx != nullptr, SPWS_ERR_INVALID, "null context");
+  SPWS_HIP(hipSetDevice(ctx->device));
+  return SPWS_OK;
+}
+
+// ---------------------------------------------------------------- scan
+//
+// Reduce-then-scan: kernel 1 reduces tiles of kScanTile values, kernel 2
+// scans the tile sums in one workgroup, kernel 3 rescans each tile with its
+// offset.  Inputs below one tile take a single-workgroup path.
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+template <class Tin, class Tout>
+__global__ __launch_bounds__(kScanBlock) void k_scan_tile_reduce(const Tin* __restrict__ in, int64_t n,
+                                                                 Tout* __restrict__ tile_sums) {
+  __shared__ Tout sh[kScanBlock / kWave + 1];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  Tout s = 0;
+#pragma unroll
+  for (int it = 0; it < kScanItems; ++it) {
+    int64_t idx = base + (int64_t)it * kScanBlock + threadIdx.x;
+    if (idx < n) s += (Tout)in[idx];
+  }
+  s = wave_sum(s);
+  if (lane_id() == 0) sh[threadIdx.x / kWave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Tout t = 0;
+    for (int w = 0; w < kScanBlock / kWave; ++w) t += sh[w];
+    tile_sums[blockIdx.x] = t;
+  }
+}
+
+// Single-workgroup exclusive scan of n values (any n); out[n] = total.
+template <class Tin, class Tout>
+__global__ __launch_bounds__(kScanBlock) void k_scan_single(const Tin* __restrict__ in, int64_t n,
+                                                            Tout* __restrict__ out) {
+  __shared__ Tout sh[kScanBlock / kWave + 1];
+  Tout carry = 0;
+  for (int64_t base = 0; base < n; base += kScanTile) {
+    // each thread owns kScanItems consecutive values of this tile
+    Tout v[kScanItems];
+    Tout local = 0;
+    const int64_t mine = base + (int64_t)threadIdx.x * kScanItems;
+#pragma unroll
+    for (int it = 0; it < kScanItems; ++it) {
+      int64_t idx = mine + it;
+      v[it] = idx < n ? (Tout)in[idx] : (Tout)0;
+      local += v[it];
+    }
+    Tout total;
+    Tout pre = block_excl_scan<Tout, kScanBlock>(local, sh, &total);
+    Tout run = carry + pre;
+#pragma unroll
+    for (int it = 0; it < kScanItems; ++it) {
+      int64_t idx = mine + it;
+      if (idx < n) out[idx] = run;
+      run += v[it];
+    }
+    carry += total;
+  }
+  if (threadIdx.x == 0) out[n] = carry;
+}
+
+template <class Tin, class Tout>
+__global__ __launch_bounds__(kScanBlock) void k_scan_tile_apply(const Tin* __restrict__ in, int64_t n,
+                                                                const Tout* __restrict__ tile_off,
+                                                                Tout* __restrict__ out) {
+  __shared__ Tout sh[kScanBlock / kWave + 1];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  Tout v[kScanItems];
+  Tout local = 0;
+  const int64_t mine = base + (int64_t)threadIdx.x * kScanItems;
+#pragma unroll
+  for (int it = 0; it < kScanItems; ++it) {
+    int64_t idx = mine + it;
+    v[it] = idx < n ? (Tout)in[idx] : (Tout)0;
+    local += v[it];
+  }
+  Tout total;
+  Tout pre = block_excl_scan<Tout, kScanBlock>(local, sh, &total);
+  Tout run = tile_off[blockIdx.x] + pre;
+#pragma unroll
+  for (int it = 0; it < kScanItems; ++it) {
+    int64_t idx = mine + it;
+    if (idx < n) out[idx] = run;
+    run += v[it];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = tile_off[gridDim.x];
+}
+
+template <class Tin, class Tout>
+int exclusive_scan(spws_ctx* ctx, const Tin* in, Tout* out, int64_t n) {
+  hipStream_t s = ctx->stream;
+  if (n <= 4 * (int64_t)kScanTile) {
+    hipLaunchKernelGGL((k_scan_single<Tin, Tout>), dim3(1), dim3(kScanBlock), 0, s, in, n, out);
+    SPWS_CHECK_LAUNCH();
+    return SPWS_OK;
+  }
+  const int64_t tiles = div_up(n, kScanTile);
+  int rc = ctx->scan_tmp.ensure((size_t)(tiles + 1) * 2 * sizeof(Tout));
+  if (rc) return rc;
+  Tout* sums = ctx->scan_tmp.as<Tout>();
+  Tout* offs = sums + (tiles + 1);
+  hipLaunchKernelGGL((k_scan_tile_reduce<Tin, Tout>), dim3((unsigned)tiles), dim3(kScanBlock), 0, s,
+                     in, n, sums);
+  SPWS_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_scan_single<Tout, Tout>), dim3(1), dim3(kScanBlock), 0, s, sums, tiles, offs);
+  SPWS_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_scan_tile_apply<Tin, Tout>), dim3((unsigned)tiles), dim3(kScanBlock), 0, s,
+                     in, n, offs, out);
+  SPWS_CHECK_LAUNCH();
+  return SPWS_OK;
+}
+
+template int exclusive_scan<int32_t, int32_t>(spws_ctx*, const int32_t*, int32_t*, int64_t);
+template int exclusive_scan<int32_t, int64_t>(spws_ctx*, const int32_t*, int64_t*, int64_t);
+template int exclusive_scan<int64_t, int64_t>(spws_ctx*, const int64_t*, int64_t*, int64_t);
+
+}  // namespace spws
+
+using namespace spws;
+
+extern "C" {
+
+const char* spws_last_error(void) { return g_last_error.c_str(); }
+
+int spws_ctx_create(int device, void* stream, spws_ctx** out) {
+  SPWS_REQUIRE(out != nullptr, SPWS_ERR_INVALID, "null output pointer");
+  int ndev = 0;
+  SPWS_HIP(hipGetDeviceCount(&ndev));
+  SPWS_REQUIRE(device >= 0 && device < ndev, SPWS_ERR_INVALID, "device %d out of range (%d devices)",
+               device, ndev);
+  SPWS_HIP(hipSetDevice(device));
+  spws_ctx* ctx = new spws_ctx();
+  ctx->device = device;
+  if (stream) {
+    ctx->stream = static_cast<hipStream_t>(stream);
+  } else {
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete ctx;
+      return fail_hip(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
+    }
+    ctx->own_stream = true;
+  }
+  int rc = ctx->host_meta.ensure(4096);
+  if (rc) {
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return rc;
+  }
+  *out = ctx;
+  return SPWS_OK;
+}
+
+int spws_ctx_destroy(spws_ctx* ctx) {
+  if (!ctx) return SPWS_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  DevBuf* bufs[] = {&ctx->scan_tmp, &ctx->row_ub,   &ctx->row_off,    &ctx->row_nnz,   &ctx->bin_rows,
+                    &ctx->bin_meta, &ctx->slot_crd, &ctx->slot_val,   &ctx->wide_tab,  &ctx->dense_val,
+                    &ctx->dense_flag, &ctx->out_pos, &ctx->out_crd,   &ctx->out_val,   &ctx->out_aux,
+                    &ctx->keys[0],  &ctx->keys[1],  &ctx->vals[0],    &ctx->vals[1],   &ctx->chunk_a,
+                    &ctx->chunk_b};
+  for (DevBuf* b : bufs) b->release();
+  for (DevBuf& b : ctx->stage) b.release();
+  ctx->host_meta.release();
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return SPWS_OK;
+}
+
+int spws_ctx_set_stream(spws_ctx* ctx, void* stream) {
+  SPWS_REQUIRE(ctx != nullptr, SPWS_ERR_INVALID, "null context");
+  if (ctx->own_stream) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    ctx->own_stream = false;
+  }
+  if (stream) {
+    ctx->stream = static_cast<hipStream_t>(stream);
+  } else {
+    SPWS_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    ctx->own_stream = true;
+  }
+  return SPWS_OK;
+}
+
+void* spws_ctx_get_stream(spws_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int spws_ctx_get_stats(spws_ctx* ctx, spws_stats* out) {
+  SPWS_REQUIRE(ctx && out, SPWS_ERR_INVALID, "null argument");
+  *out = ctx->stats;
+  return SPWS_OK;
+}
+
+int spws_dev_alloc(spws_ctx* ctx, size_t bytes, void** out) {
+  int rc = ctx_activate(ctx);
+  if (rc) return rc;
+  SPWS_REQUIRE(out != nullptr, SPWS_ERR_INVALID, "null output pointer");
+  hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    return SPWS_ERR_OOM;
+  }
+  return SPWS_OK;
+}
+
+int spws_dev_free(spws_ctx* ctx, void* p) {
+  int rc = ctx_activate(ctx);
+  if (rc) return rc;
+  if (p) SPWS_HIP(hipFree(p));
+  return SPWS_OK;
+}
+
+int spws_memcpy_h2d(spws_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  int rc = ctx_activate(ctx);
+  if (rc) return rc;
+  if (bytes == 0) return SPWS_OK;
+  SPWS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  SPWS_HIP(hipStreamSynchronize(ctx->stream));
+  return SPWS_OK;
+}
+
+int spws_memcpy_d2h(spws_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  int rc = ctx_activate(ctx);
+  if (rc) return rc;
+  if (bytes == 0) return SPWS_OK;
+  SPWS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  SPWS_HIP(hipStreamSynchronize(ctx->stream));
+  return SPWS_OK;
+}
+
+int spws_sync(spws_ctx* ctx) {
+  int rc = ctx_activate(ctx);
+  if (rc) return rc;
+  SPWS_HIP(hipStreamSynchronize(ctx->stream));
+  return SPWS_OK;
+}
+
+void spws_host_free(void* p) { free(p); }
+
+}  // extern "C"

@@ -1,0 +1,165 @@
+"""Device-resident host API over the C ABI: contexts, device arrays, and the
+kernels' device entry points.  All device memory here is allocated through
+libspws (hipMalloc); no other GPU runtime is required."""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+from .gen import CSR
+
+
+class DeviceArray:
+    """A device allocation holding a 1-D numpy-typed array."""
+
+    def __init__(self, ctx: "Context", count: int, dtype):
+        self.ctx = ctx
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self.nbytes = max(self.count * self.dtype.itemsize, 1)
+        self.ptr = ctx.alloc(self.nbytes)
+        self.owned = True
+
+    @classmethod
+    def view(cls, ctx, ptr: int, count: int, dtype):
+        """Wrap memory owned elsewhere (e.g. a context-owned output)."""
+        self = cls.__new__(cls)
+        self.ctx, self.dtype, self.count = ctx, np.dtype(dtype), int(count)
+        self.nbytes = self.count * self.dtype.itemsize
+        self.ptr, self.owned = int(ptr or 0), False
+        return self
+
+    @classmethod
+    def from_host(cls, ctx, arr: np.ndarray):
+        arr = np.ascontiguousarray(arr)
+        self = cls(ctx, arr.size, arr.dtype)
+        if arr.size:
+            ctx.h2d(self.ptr, arr)
+        return self
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.count, dtype=self.dtype)
+        if self.count:
+            self.ctx.d2h(out, self.ptr)
+        return out
+
+    def free(self):
+        if self.owned and self.ptr:
+            self.ctx.free(self.ptr)
+        self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One spws_ctx: a device, a stream and the context-owned workspaces."""
+
+    def __init__(self, device: int = 0, stream: int | None = None):
+        h = C.c_void_p()
+        check(lib().spws_ctx_create(int(device), C.c_void_p(stream) if stream else None, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            lib().spws_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- memory
+    def alloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        check(lib().spws_dev_alloc(self.h, int(nbytes), C.byref(p)))
+        return int(p.value)
+
+    def free(self, p: int):
+        check(lib().spws_dev_free(self.h, C.c_void_p(p)))
+
+    def h2d(self, dst: int, arr: np.ndarray):
+        check(lib().spws_memcpy_h2d(self.h, C.c_void_p(dst), C.c_void_p(arr.ctypes.data), arr.nbytes))
+
+    def d2h(self, arr: np.ndarray, src: int):
+        check(lib().spws_memcpy_d2h(self.h, C.c_void_p(arr.ctypes.data), C.c_void_p(src), arr.nbytes))
+
+    def sync(self):
+        check(lib().spws_sync(self.h))
+
+    @property
+    def stream(self) -> int:
+        return int(lib().spws_ctx_get_stream(self.h) or 0)
+
+    def stats(self) -> dict:
+        s = _lib.Stats()
+        check(lib().spws_ctx_get_stats(self.h, C.byref(s)))
+        return {"products": s.products, "nnz": s.nnz, "rows_per_path": list(s.rows_per_path),
+                "chunks": s.chunks, "sort_bits": s.sort_bits}
+
+    # ---- tuning
+    def rowwise_tuning(self) -> dict:
+        t = _lib.RowwiseTuning()
+        check(lib().spws_ctx_get_rowwise_tuning(self.h, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in t._fields_}
+
+    def set_rowwise_tuning(self, **kw):
+        cur = self.rowwise_tuning()
+        cur.update(kw)
+        t = _lib.RowwiseTuning(**cur)
+        check(lib().spws_ctx_set_rowwise_tuning(self.h, C.byref(t)))
+
+
+@dataclass
+class DeviceCSR:
+    nrows: int
+    ncols: int
+    nnz: int
+    pos: DeviceArray
+    crd: DeviceArray
+    vals: DeviceArray | None
+
+    @classmethod
+    def upload(cls, ctx: Context, a: CSR, with_vals: bool = True) -> "DeviceCSR":
+        return cls(a.nrows, a.ncols, a.nnz,
+                   DeviceArray.from_host(ctx, np.asarray(a.pos, dtype=np.int32)),
+                   DeviceArray.from_host(ctx, np.asarray(a.crd, dtype=np.int32)),
+                   DeviceArray.from_host(ctx, np.asarray(a.vals, dtype=np.float64)) if with_vals else None)
+
+    def download(self) -> CSR:
+        return CSR(self.nrows, self.ncols, self.pos.to_host(), self.crd.to_host(),
+                   self.vals.to_host() if self.vals is not None else np.zeros(0))
+
+
+def spgemm_rowwise(ctx: Context, A: DeviceCSR, B: DeviceCSR) -> DeviceCSR:
+    """C = A*B on the device; the result views context-owned buffers that stay
+    valid until the next row-wise call on ``ctx``."""
+    if A.ncols != B.nrows:
+        raise ValueError("inner dimensions differ")
+    nnz = C.c_int64()
+    pp, pc, pv = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    check(lib().spws_spgemm_rowwise(ctx.h, A.nrows, A.ncols, B.ncols, A.pos.ptr, A.crd.ptr, A.vals.ptr,
+                                    B.pos.ptr, B.crd.ptr, B.vals.ptr, C.byref(nnz), C.byref(pp),
+                                    C.byref(pc), C.byref(pv)))
+    n = nnz.value
+    return DeviceCSR(A.nrows, B.ncols, n, DeviceArray.view(ctx, pp.value, A.nrows + 1, np.int32),
+                     DeviceArray.view(ctx, pc.value, n, np.int32), DeviceArray.view(ctx, pv.value, n, np.float64))
+
+
+def spgemm_rowwise_symbolic(ctx: Context, A: DeviceCSR, B: DeviceCSR) -> np.ndarray:
+    """Exact C row pointers (the assemble phase), returned to the host."""
+    nnz = C.c_int64()
+    pp = C.c_void_p()
+    check(lib().spws_spgemm_rowwise_symbolic(ctx.h, A.nrows, A.ncols, B.ncols, A.pos.ptr, A.crd.ptr,
+                                             B.pos.ptr, B.crd.ptr, C.byref(nnz), C.byref(pp)))
+    return DeviceArray.view(ctx, pp.value, A.nrows + 1, np.int32).to_host()

@@ -1,0 +1,168 @@
+"""Row-wise hash SpGEMM on the GPU vs the CPU oracle, through the C ABI.
+
+Bar: structure (row pointers, column indices, nnz) bit-exact; FP64 values
+within |x - y| <= 1e-12 * max(|y|, sum|terms|) of both the oracle's
+reference-order result and its long-double sorted sum.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import powerlaw_rows, random_csr, uniform_rows
+from sparsews_amd import taco
+from sparsews_amd.device import DeviceCSR, spgemm_rowwise, spgemm_rowwise_symbolic
+from sparsews_amd.gen import CSR, rowwise_pair
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def run(ctx, A, B):
+    dA, dB = DeviceCSR.upload(ctx, A), DeviceCSR.upload(ctx, B)
+    return spgemm_rowwise(ctx, dA, dB).download()
+
+
+def check(C, A, B, w_cap=None):
+    G = oracle.spgemm_gustavson(A, B)
+    assert C.nnz == G.nnz
+    np.testing.assert_array_equal(C.pos, G.pos)
+    np.testing.assert_array_equal(C.crd, G.crd)
+    exact, absum = oracle.spgemm_refsum(A, B, G.pos, G.crd)
+    assert oracle.max_value_error(C.vals, G.vals, absum) <= TOL
+    assert oracle.max_value_error(C.vals, exact, absum) <= TOL
+    if w_cap is not None:
+        H = oracle.spgemm_hash(A, B, w_cap)
+        assert oracle.max_value_error(C.vals, H.vals, absum) <= TOL
+
+
+def test_rowwise_workload_10k(ctx):
+    print("case: 10k x 10k, 8 per row, seed 0", flush=True)
+    A, B = rowwise_pair()
+    C = run(ctx, A, B)
+    check(C, A, B, w_cap=1 << 16)
+    st = ctx.stats()
+    assert st["products"] == 640_000
+    assert st["rows_per_path"][0] == 10_000
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_uniform_small(ctx, seed):
+    print(f"case: uniform seed {seed}", flush=True)
+    A, B = uniform_rows(500, 400, 0, 24, seed, 0), uniform_rows(400, 700, 0, 24, seed, 1)
+    check(run(ctx, A, B), A, B)
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+def test_powerlaw_mixed_paths(ctx, seed):
+    print(f"case: power-law seed {seed}", flush=True)
+    A, B = powerlaw_rows(1500, 1500, 1.8, 1500, seed, 0), powerlaw_rows(1500, 1500, 1.8, 1500, seed, 1)
+    C = run(ctx, A, B)
+    check(C, A, B)
+    st = ctx.stats()
+    used = [i for i in range(6) if st["rows_per_path"][i] > 0]
+    assert len(used) >= 3, st
+
+
+def test_wide_rows_natural(ctx):
+    """Rows with ub > 4096 and ub*16 < ncols take the HBM hash path."""
+    print("case: wide rows", flush=True)
+    n = 400_000
+    A = random_csr(40, 3000, np.full(40, 400), 6, 0)
+    B = random_csr(3000, n, np.full(3000, 15), 6, 1)
+    C = run(ctx, A, B)
+    check(C, A, B)
+    assert ctx.stats()["rows_per_path"][5] == 40
+
+
+def test_dense_rows_natural(ctx):
+    print("case: dense rows", flush=True)
+    A = random_csr(30, 500, np.full(30, 300), 7, 0)
+    B = random_csr(500, 3000, np.full(500, 40), 7, 1)
+    C = run(ctx, A, B)
+    check(C, A, B)
+    assert ctx.stats()["rows_per_path"][4] == 30
+
+
+@pytest.mark.parametrize("path", [0, 1, 2, 3, 4, 5])
+def test_forced_paths(ctx, path):
+    print(f"case: forced path {path}", flush=True)
+    A, B = uniform_rows(300, 300, 0, 8, 20 + path, 0), uniform_rows(300, 500, 0, 8, 20 + path, 1)
+    saved = ctx.rowwise_tuning()
+    try:
+        ctx.set_rowwise_tuning(force_path=path)
+        C = run(ctx, A, B)
+    finally:
+        ctx.set_rowwise_tuning(**saved)
+    check(C, A, B)
+    st = ctx.stats()
+    assert st["rows_per_path"][path] == int((oracle.spgemm_row_ub(A, B) > 0).sum())
+
+
+def test_forced_path_too_small_is_an_error(ctx):
+    from sparsews_amd import SpwsError
+    A, B = uniform_rows(50, 300, 20, 20, 30, 0), uniform_rows(300, 300, 20, 20, 30, 1)
+    saved = ctx.rowwise_tuning()
+    try:
+        ctx.set_rowwise_tuning(force_path=0)
+        with pytest.raises(SpwsError):
+            run(ctx, A, B)
+    finally:
+        ctx.set_rowwise_tuning(**saved)
+
+
+def test_edge_cases(ctx):
+    print("case: edge cases", flush=True)
+    # empty A
+    A = CSR(7, 5, np.zeros(8, dtype=np.int32), np.zeros(0, dtype=np.int32), np.zeros(0))
+    B = uniform_rows(5, 9, 1, 4, 31, 1)
+    C = run(ctx, A, B)
+    assert C.nnz == 0 and np.all(C.pos == 0)
+    # zero rows
+    A0 = CSR(0, 5, np.zeros(1, dtype=np.int32), np.zeros(0, dtype=np.int32), np.zeros(0))
+    C0 = run(ctx, A0, B)
+    assert C0.nnz == 0 and len(C0.pos) == 1
+    # identity
+    n = 200
+    I = CSR(n, n, np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), np.ones(n))
+    Bm = uniform_rows(n, 150, 0, 30, 32, 1)
+    CI = run(ctx, I, Bm)
+    np.testing.assert_array_equal(CI.pos, Bm.pos)
+    np.testing.assert_array_equal(CI.crd, Bm.crd)
+    np.testing.assert_array_equal(CI.vals, Bm.vals)
+    # one fully dense row in A
+    A1 = random_csr(3, 64, np.array([64, 0, 5]), 33, 0)
+    B1 = uniform_rows(64, 80, 0, 6, 33, 1)
+    check(run(ctx, A1, B1), A1, B1)
+
+
+def test_buffer_reuse_large_then_small(ctx):
+    print("case: reuse", flush=True)
+    A, B = uniform_rows(800, 800, 0, 30, 40, 0), uniform_rows(800, 800, 0, 30, 40, 1)
+    check(run(ctx, A, B), A, B)
+    a, b = uniform_rows(90, 60, 0, 9, 41, 0), uniform_rows(60, 70, 0, 9, 41, 1)
+    check(run(ctx, a, b), a, b)
+
+
+def test_symbolic_phase(ctx):
+    print("case: symbolic", flush=True)
+    A, B = powerlaw_rows(700, 700, 1.8, 700, 42, 0), powerlaw_rows(700, 700, 1.8, 700, 42, 1)
+    pos = spgemm_rowwise_symbolic(ctx, DeviceCSR.upload(ctx, A, False), DeviceCSR.upload(ctx, B, False))
+    np.testing.assert_array_equal(pos, oracle.spgemm_symbolic(A, B))
+
+
+def test_taco_entry_points():
+    print("case: taco compute / assemble+compute", flush=True)
+    A, B = uniform_rows(300, 250, 0, 12, 43, 0), uniform_rows(250, 260, 0, 12, 43, 1)
+    At, Bt = taco.DC_to_taco_tensor(A), taco.DC_to_taco_tensor(B)
+    Ct = taco.init_taco_tensor_DC(A.nrows, B.ncols)
+    taco.compute(Ct, At, Bt, 1 << 10)
+    C = taco.taco_to_csr(Ct)
+    taco.free_outputs(Ct)
+    check(C, A, B)
+    C2t = taco.init_taco_tensor_DC(A.nrows, B.ncols)
+    taco.assemble(C2t, At, Bt)
+    np.testing.assert_array_equal(C2t.index(1, 0, A.nrows + 1), oracle.spgemm_symbolic(A, B))
+    taco.compute_split(C2t, At, Bt)
+    C2 = taco.taco_to_csr(C2t)
+    taco.free_outputs(C2t)
+    check(C2, A, B)
