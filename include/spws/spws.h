@@ -74,6 +74,15 @@ int spws_ctx_get_stats(spws_ctx* ctx, spws_stats* out);
 int spws_ctx_set_rowwise_tuning(spws_ctx* ctx, const spws_rowwise_tuning* t);
 int spws_ctx_get_rowwise_tuning(spws_ctx* ctx, spws_rowwise_tuning* t);
 
+/* Per-kernel timing with HIP events recorded around every launch the
+ * context makes (off by default).  spws_ctx_kernel_times returns the index-th
+ * kernel name (alphabetical), its total milliseconds and launch count since
+ * the last reset; an index past the end returns SPWS_ERR_INVALID. */
+int spws_ctx_set_profiling(spws_ctx* ctx, int on);
+int spws_ctx_reset_kernel_times(spws_ctx* ctx);
+int spws_ctx_kernel_times(spws_ctx* ctx, int index, char* name, size_t name_len, double* total_ms,
+                          int64_t* launches);
+
 /* Device memory helpers for callers without another HIP runtime binding. */
 int spws_dev_alloc(spws_ctx* ctx, size_t bytes, void** out);
 int spws_dev_free(spws_ctx* ctx, void* p);
@@ -81,6 +90,9 @@ int spws_memcpy_h2d(spws_ctx* ctx, void* dst, const void* src, size_t bytes);
 int spws_memcpy_d2h(spws_ctx* ctx, void* dst, const void* src, size_t bytes);
 int spws_sync(spws_ctx* ctx);
 void spws_host_free(void* p); /* frees arrays the host entry points malloc'd */
+/* Plain device-to-device copy kernel (coalesced 16-byte loads/stores) on the
+ * context's stream; also the byte-count reference for hardware counters. */
+int spws_device_copy(spws_ctx* ctx, void* dst, const void* src, size_t bytes);
 
 /* ------------------------------------------------- row-wise SpGEMM (CSR x CSR) */
 
@@ -101,6 +113,28 @@ int spws_spgemm_rowwise_symbolic(spws_ctx* ctx, int32_t m, int32_t k, int32_t n,
                                  const int32_t* B_pos, const int32_t* B_crd,
                                  int64_t* C_nnz, int32_t** C_pos);
 
+/* -------------------------------------- outer-product SpGEMM (DCSC x DCSR) */
+
+/* C = A * B with A (m x k) in DCSC and B (k x n) in DCSR, device pointers.
+ * A1_crd[a_ncols] lists A's stored columns (sorted; NULL = all k columns,
+ * i.e. CSC), A2_pos[a_ncols+1] / A2_crd (rows, sorted within each column) /
+ * A_vals hold them; B1_crd[b_nrows] / B2_pos / B2_crd / B_vals likewise for
+ * B's stored rows (NULL = CSR).  A's columns and B's rows are co-iterated on
+ * j.  Output: sorted CSR with int64 row pointers (nnz may exceed 2^31),
+ * context-owned.  Each entry is summed over j in increasing order.
+ * Replaces compute() of benchmark/DCSC_DCSR/DCSC_DCSR_coord_index_chase.h:194. */
+int spws_spgemm_outer(spws_ctx* ctx, int32_t m, int32_t k, int32_t n,
+                      int32_t a_ncols, const int32_t* A1_crd, const int32_t* A2_pos, const int32_t* A2_crd,
+                      const double* A_vals,
+                      int32_t b_nrows, const int32_t* B1_crd, const int32_t* B2_pos, const int32_t* B2_crd,
+                      const double* B_vals,
+                      int64_t* C_nnz, int64_t** C_pos, int32_t** C_crd, double** C_vals);
+
+/* Output-row chunking of the outer product: at most max_products products
+ * and max_rows rows per chunk (0 = defaults: 2^30 products; rows limited by
+ * the 32-bit sort key).  Small values force many chunks (used by tests). */
+int spws_ctx_set_outer_chunking(spws_ctx* ctx, int64_t max_products, int32_t max_rows);
+
 /* Host TACO entry points (host taco_tensor_t in, malloc'd host arrays out). */
 
 /* int compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B, int32_t w_cap)
@@ -111,8 +145,8 @@ int spws_spgemm_rowwise_symbolic(spws_ctx* ctx, int32_t m, int32_t k, int32_t n,
  * sized per row, so it does not change the result structure. */
 int spws_csr_csr_hash_compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B, int32_t w_cap);
 
-/* int assemble(C, A, B) of seperate/CSR_CSR_1.h:69-167: fills C2_pos and
- * allocates C2_crd / C_vals (the crd array is filled as well). */
+/* int assemble(C, A, B) of seperate/CSR_CSR_1.h:69-167: fills C2_pos (the
+ * symbolic phase) and allocates C2_crd / C_vals for compute() to fill. */
 int spws_csr_csr_assemble(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B);
 
 /* int compute(C, A, B) of seperate/CSR_CSR_1.h:4-67: fills C_vals (and

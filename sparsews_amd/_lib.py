@@ -65,15 +65,22 @@ SIGNATURES = {
     "spws_ctx_get_stats": (C.c_int, [VP, C.POINTER(Stats)]),
     "spws_ctx_set_rowwise_tuning": (C.c_int, [VP, C.POINTER(RowwiseTuning)]),
     "spws_ctx_get_rowwise_tuning": (C.c_int, [VP, C.POINTER(RowwiseTuning)]),
+    "spws_ctx_set_profiling": (C.c_int, [VP, C.c_int]),
+    "spws_ctx_reset_kernel_times": (C.c_int, [VP]),
+    "spws_ctx_kernel_times": (C.c_int, [VP, C.c_int, C.c_char_p, C.c_size_t, C.POINTER(C.c_double), PI64]),
     "spws_dev_alloc": (C.c_int, [VP, C.c_size_t, C.POINTER(VP)]),
     "spws_dev_free": (C.c_int, [VP, VP]),
     "spws_memcpy_h2d": (C.c_int, [VP, VP, VP, C.c_size_t]),
     "spws_memcpy_d2h": (C.c_int, [VP, VP, VP, C.c_size_t]),
     "spws_sync": (C.c_int, [VP]),
     "spws_host_free": (None, [VP]),
+    "spws_device_copy": (C.c_int, [VP, VP, VP, C.c_size_t]),
     "spws_spgemm_rowwise": (C.c_int, [VP, I32, I32, I32, VP, VP, VP, VP, VP, VP,
                                       PI64, PVP, PVP, PVP]),
     "spws_spgemm_rowwise_symbolic": (C.c_int, [VP, I32, I32, I32, VP, VP, VP, VP, PI64, PVP]),
+    "spws_spgemm_outer": (C.c_int, [VP, I32, I32, I32, I32, VP, VP, VP, VP, I32, VP, VP, VP, VP,
+                                    PI64, PVP, PVP, PVP]),
+    "spws_ctx_set_outer_chunking": (C.c_int, [VP, I64, I32]),
     "spws_csr_csr_hash_compute": (C.c_int, [C.POINTER(TacoTensor)] * 3 + [I32]),
     "spws_csr_csr_assemble": (C.c_int, [C.POINTER(TacoTensor)] * 3),
     "spws_csr_csr_compute": (C.c_int, [C.POINTER(TacoTensor)] * 3),

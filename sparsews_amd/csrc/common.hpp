@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -57,6 +58,18 @@ struct PinnedBuf {
   T* as() const { return static_cast<T*>(ptr); }
 };
 
+// Per-kernel HIP-event timing on the context's stream (off by default).
+struct KernelProfiler {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  struct Pending {
+    const char* name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::map<std::string, std::pair<double, int64_t>> totals;  // name -> (ms, launches)
+};
+
 }  // namespace spws
 
 struct spws_ctx {
@@ -70,16 +83,37 @@ struct spws_ctx {
   spws::DevBuf out_pos, out_crd, out_val, out_aux;
   spws::DevBuf keys[2], vals[2], chunk_a, chunk_b;
   spws::DevBuf stage[12];  // device copies of host inputs (TACO entry points)
+  spws::DevBuf radix_hist;
+  spws::DevBuf obuf[8];    // outer-product per-column metadata
   spws::PinnedBuf host_meta;
   // row-wise SpGEMM bin boundaries (validated against the compiled table sizes)
   spws_rowwise_tuning rw{64, 256, 1024, 4096, 16, -1};
+  // outer-product chunk limits (0 = defaults: 2^30 products, key-width rows)
+  int64_t outer_max_products = 0;
+  int32_t outer_max_rows = 0;
   // statistics of the last call (host-readable)
   spws_stats stats{};
+  spws::KernelProfiler prof;
 };
 
 namespace spws {
 
 int ctx_activate(spws_ctx* ctx);
+
+// Kernel timing hooks: prof_begin records a start event (or returns null when
+// profiling is off), prof_end records the stop event; prof_collect turns the
+// events recorded so far into totals (call it after the stream is idle).
+hipEvent_t prof_begin(spws_ctx* ctx);
+void prof_end(spws_ctx* ctx, const char* name, hipEvent_t start);
+void prof_collect(spws_ctx* ctx);
+
+#define SPWS_LAUNCH(ctx, name, kern, grid, block, shmem, ...)                 \
+  do {                                                                        \
+    hipEvent_t e0__ = ::spws::prof_begin(ctx);                                \
+    hipLaunchKernelGGL(kern, grid, block, shmem, (ctx)->stream, __VA_ARGS__); \
+    SPWS_CHECK_LAUNCH();                                                      \
+    ::spws::prof_end(ctx, name, e0__);                                        \
+  } while (0)
 
 // ---------------------------------------------------------------- scan
 

@@ -75,6 +75,53 @@ int ctx_activate(spws_ctx* ctx) {
   return SPWS_OK;
 }
 
+static hipEvent_t prof_event(spws_ctx* ctx) {
+  if (!ctx->prof.pool.empty()) {
+    hipEvent_t e = ctx->prof.pool.back();
+    ctx->prof.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return e;
+}
+
+hipEvent_t prof_begin(spws_ctx* ctx) {
+  if (!ctx->prof.on) return nullptr;
+  hipEvent_t e = prof_event(ctx);
+  if (e) (void)hipEventRecord(e, ctx->stream);
+  return e;
+}
+
+void prof_end(spws_ctx* ctx, const char* name, hipEvent_t start) {
+  if (!ctx->prof.on || !start) return;
+  hipEvent_t e = prof_event(ctx);
+  if (!e) return;
+  (void)hipEventRecord(e, ctx->stream);
+  ctx->prof.pending.push_back({name, start, e});
+}
+
+void prof_collect(spws_ctx* ctx) {
+  if (ctx->prof.pending.empty()) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& p : ctx->prof.pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      auto& t = ctx->prof.totals[p.name];
+      t.first += ms;
+      t.second += 1;
+    } else {
+      (void)hipGetLastError();
+    }
+    ctx->prof.pool.push_back(p.a);
+    ctx->prof.pool.push_back(p.b);
+  }
+  ctx->prof.pending.clear();
+}
+
 // ---------------------------------------------------------------- scan
 //
 // Reduce-then-scan: kernel 1 reduces tiles of kScanTile values, kernel 2
@@ -166,10 +213,8 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_tile_apply(const Tin* __res
 
 template <class Tin, class Tout>
 int exclusive_scan(spws_ctx* ctx, const Tin* in, Tout* out, int64_t n) {
-  hipStream_t s = ctx->stream;
   if (n <= 4 * (int64_t)kScanTile) {
-    hipLaunchKernelGGL((k_scan_single<Tin, Tout>), dim3(1), dim3(kScanBlock), 0, s, in, n, out);
-    SPWS_CHECK_LAUNCH();
+    SPWS_LAUNCH(ctx, "scan_single", (k_scan_single<Tin, Tout>), dim3(1), dim3(kScanBlock), 0, in, n, out);
     return SPWS_OK;
   }
   const int64_t tiles = div_up(n, kScanTile);
@@ -177,15 +222,22 @@ int exclusive_scan(spws_ctx* ctx, const Tin* in, Tout* out, int64_t n) {
   if (rc) return rc;
   Tout* sums = ctx->scan_tmp.as<Tout>();
   Tout* offs = sums + (tiles + 1);
-  hipLaunchKernelGGL((k_scan_tile_reduce<Tin, Tout>), dim3((unsigned)tiles), dim3(kScanBlock), 0, s,
-                     in, n, sums);
-  SPWS_CHECK_LAUNCH();
-  hipLaunchKernelGGL((k_scan_single<Tout, Tout>), dim3(1), dim3(kScanBlock), 0, s, sums, tiles, offs);
-  SPWS_CHECK_LAUNCH();
-  hipLaunchKernelGGL((k_scan_tile_apply<Tin, Tout>), dim3((unsigned)tiles), dim3(kScanBlock), 0, s,
-                     in, n, offs, out);
-  SPWS_CHECK_LAUNCH();
+  SPWS_LAUNCH(ctx, "scan_tile_reduce", (k_scan_tile_reduce<Tin, Tout>), dim3((unsigned)tiles), dim3(kScanBlock),
+              0, in, n, sums);
+  SPWS_LAUNCH(ctx, "scan_single", (k_scan_single<Tout, Tout>), dim3(1), dim3(kScanBlock), 0, sums, tiles, offs);
+  SPWS_LAUNCH(ctx, "scan_tile_apply", (k_scan_tile_apply<Tin, Tout>), dim3((unsigned)tiles), dim3(kScanBlock), 0,
+              in, n, offs, out);
   return SPWS_OK;
+}
+
+__global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+__global__ __launch_bounds__(256) void k_copy1(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
 }
 
 template int exclusive_scan<int32_t, int32_t>(spws_ctx*, const int32_t*, int32_t*, int64_t);
@@ -240,7 +292,11 @@ int spws_ctx_destroy(spws_ctx* ctx) {
                     &ctx->chunk_b};
   for (DevBuf* b : bufs) b->release();
   for (DevBuf& b : ctx->stage) b.release();
+  for (DevBuf& b : ctx->obuf) b.release();
+  ctx->radix_hist.release();
   ctx->host_meta.release();
+  prof_collect(ctx);
+  for (hipEvent_t e : ctx->prof.pool) (void)hipEventDestroy(e);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return SPWS_OK;
@@ -317,5 +373,56 @@ int spws_sync(spws_ctx* ctx) {
 }
 
 void spws_host_free(void* p) { free(p); }
+
+int spws_device_copy(spws_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  int rc = ctx_activate(ctx);
+  if (rc) return rc;
+  SPWS_REQUIRE((dst && src) || bytes == 0, SPWS_ERR_INVALID, "null pointer");
+  SPWS_REQUIRE(((uintptr_t)dst % 16) == 0 && ((uintptr_t)src % 16) == 0, SPWS_ERR_INVALID,
+               "device_copy needs 16-byte aligned pointers");
+  const int64_t n16 = (int64_t)(bytes / 16);
+  const int64_t tail = (int64_t)(bytes % 16);
+  if (n16 > 0) {
+    int64_t g = div_up(n16, 256);
+    if (g > 8192) g = 8192;
+    SPWS_LAUNCH(ctx, "copy16", k_copy16, dim3((unsigned)g), dim3(256), 0, (const uint4*)src, (uint4*)dst, n16);
+  }
+  if (tail > 0) {
+    SPWS_LAUNCH(ctx, "copy1", k_copy1, dim3(1), dim3(256), 0, (const uint8_t*)src + n16 * 16,
+                (uint8_t*)dst + n16 * 16, tail);
+  }
+  return SPWS_OK;
+}
+
+int spws_ctx_set_profiling(spws_ctx* ctx, int on) {
+  SPWS_REQUIRE(ctx != nullptr, SPWS_ERR_INVALID, "null context");
+  prof_collect(ctx);
+  ctx->prof.on = on != 0;
+  return SPWS_OK;
+}
+
+int spws_ctx_reset_kernel_times(spws_ctx* ctx) {
+  SPWS_REQUIRE(ctx != nullptr, SPWS_ERR_INVALID, "null context");
+  prof_collect(ctx);
+  ctx->prof.totals.clear();
+  return SPWS_OK;
+}
+
+int spws_ctx_kernel_times(spws_ctx* ctx, int index, char* name, size_t name_len, double* total_ms,
+                          int64_t* launches) {
+  SPWS_REQUIRE(ctx != nullptr, SPWS_ERR_INVALID, "null context");
+  prof_collect(ctx);
+  SPWS_REQUIRE(index >= 0 && (size_t)index < ctx->prof.totals.size(), SPWS_ERR_INVALID,
+               "kernel index %d out of range", index);
+  auto it = ctx->prof.totals.begin();
+  std::advance(it, index);
+  if (name && name_len) {
+    strncpy(name, it->first.c_str(), name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (total_ms) *total_ms = it->second.first;
+  if (launches) *launches = it->second.second;
+  return SPWS_OK;
+}
 
 }  // extern "C"

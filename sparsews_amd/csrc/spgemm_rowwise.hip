@@ -673,9 +673,8 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
   int32_t* cursor = reinterpret_cast<int32_t*>(meta + 16);
   SPWS_HIP(hipMemsetAsync(meta, 0, 32 * 8, s));
   if (m > 0) {
-    hipLaunchKernelGGL(k_rowstat, dim3((unsigned)div_up(m, 256)), dim3(256), 0, s, m, n, Ap, Ai, Bp,
+    SPWS_LAUNCH(ctx, "rowstat", k_rowstat, dim3((unsigned)div_up(m, 256)), dim3(256), 0, m, n, Ap, Ai, Bp,
                        ctx->row_ub.as<int64_t>(), meta, pp);
-    SPWS_CHECK_LAUNCH();
   }
   rc = exclusive_scan<int64_t, int64_t>(ctx, ctx->row_ub.as<int64_t>(), ctx->row_off.as<int64_t>(), m);
   if (rc) return rc;
@@ -714,10 +713,9 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
     if (p < P_EMPTY) acc += (int32_t)counts[p];
   }
   if (m > 0) {
-    hipLaunchKernelGGL(k_binscatter, dim3((unsigned)div_up(m, 256)), dim3(256), 0, s, m, n,
+    SPWS_LAUNCH(ctx, "binscatter", k_binscatter, dim3((unsigned)div_up(m, 256)), dim3(256), 0, m, n,
                        ctx->row_ub.as<int64_t>(), cursor, bo, ctx->bin_rows.as<int32_t>(),
                        ctx->row_nnz.as<int32_t>(), pp);
-    SPWS_CHECK_LAUNCH();
   }
   const int32_t* rows = ctx->bin_rows.as<int32_t>();
   const int64_t* roff = ctx->row_off.as<int64_t>();
@@ -727,27 +725,19 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
   auto grid_for = [](int64_t cnt, int64_t cap) { return (unsigned)(cnt < cap ? cnt : cap); };
 
   if (counts[P_WAVE]) {
-    hipLaunchKernelGGL((k_row_wave<128, 1, kValues>), dim3(grid_for(counts[P_WAVE], 1 << 20)), dim3(64), 0, s,
-                       rows + bo.off[P_WAVE], (int32_t)counts[P_WAVE], Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz);
-    SPWS_CHECK_LAUNCH();
+    SPWS_LAUNCH(ctx, "row_wave", (k_row_wave<128, 1, kValues>), dim3(grid_for(counts[P_WAVE], 1 << 20)), dim3(64), 0, rows + bo.off[P_WAVE], (int32_t)counts[P_WAVE], Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz);
   }
   if (counts[P_WAVE2]) {
-    hipLaunchKernelGGL((k_row_wave<512, 4, kValues>), dim3(grid_for(counts[P_WAVE2], 1 << 20)), dim3(64), 0, s,
-                       rows + bo.off[P_WAVE2], (int32_t)counts[P_WAVE2], Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval,
+    SPWS_LAUNCH(ctx, "row_wave", (k_row_wave<512, 4, kValues>), dim3(grid_for(counts[P_WAVE2], 1 << 20)), dim3(64), 0, rows + bo.off[P_WAVE2], (int32_t)counts[P_WAVE2], Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval,
                        rnnz);
-    SPWS_CHECK_LAUNCH();
   }
   if (counts[P_BLOCK]) {
-    hipLaunchKernelGGL((k_row_block<2048, 1024, 256, kValues>), dim3(grid_for(counts[P_BLOCK], 1 << 16)), dim3(256),
-                       0, s, rows + bo.off[P_BLOCK], (int32_t)counts[P_BLOCK], Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd,
+    SPWS_LAUNCH(ctx, "row_block", (k_row_block<2048, 1024, 256, kValues>), dim3(grid_for(counts[P_BLOCK], 1 << 16)), dim3(256), 0, rows + bo.off[P_BLOCK], (int32_t)counts[P_BLOCK], Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd,
                        sval, rnnz);
-    SPWS_CHECK_LAUNCH();
   }
   if (counts[P_BLOCK2]) {
-    hipLaunchKernelGGL((k_row_block<8192, 4096, 512, kValues>), dim3(grid_for(counts[P_BLOCK2], 1 << 16)),
-                       dim3(512), 0, s, rows + bo.off[P_BLOCK2], (int32_t)counts[P_BLOCK2], Ap, Ai, Ax, Bp, Bi, Bx,
+    SPWS_LAUNCH(ctx, "row_block", (k_row_block<8192, 4096, 512, kValues>), dim3(grid_for(counts[P_BLOCK2], 1 << 16)), dim3(512), 0, rows + bo.off[P_BLOCK2], (int32_t)counts[P_BLOCK2], Ap, Ai, Ax, Bp, Bi, Bx,
                        roff, scrd, sval, rnnz);
-    SPWS_CHECK_LAUNCH();
   }
   if (counts[P_DENSE]) {
     // bounded scratch: at most ~2 GiB of dense accumulators
@@ -761,10 +751,9 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
     if (kValues && (rc = ctx->dense_val.ensure((size_t)g * nn * 8))) return rc;
     SPWS_HIP(hipMemsetAsync(ctx->dense_flag.ptr, 0, (size_t)g * nn, s));
     if (kValues) SPWS_HIP(hipMemsetAsync(ctx->dense_val.ptr, 0, (size_t)g * nn * 8, s));
-    hipLaunchKernelGGL((k_row_dense<kValues>), dim3((unsigned)g), dim3(kDenseBlock), 0, s, rows + bo.off[P_DENSE],
+    SPWS_LAUNCH(ctx, "row_dense", (k_row_dense<kValues>), dim3((unsigned)g), dim3(kDenseBlock), 0, rows + bo.off[P_DENSE],
                        (int32_t)counts[P_DENSE], n, Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz,
                        ctx->dense_flag.as<uint8_t>(), kValues ? ctx->dense_val.as<double>() : nullptr);
-    SPWS_CHECK_LAUNCH();
   }
   if (counts[P_WIDE]) {
     int64_t tab = (int64_t)next_pow2_u32((uint32_t)(2 * maxub[P_WIDE]));
@@ -777,19 +766,17 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
     if ((rc = ctx->wide_tab.ensure((size_t)g * per))) return rc;
     int32_t* tk = ctx->wide_tab.as<int32_t>();
     double* tv = kValues ? reinterpret_cast<double*>(ctx->wide_tab.as<char>() + (size_t)g * tab * 4) : nullptr;
-    hipLaunchKernelGGL((k_row_wide<kValues>), dim3((unsigned)g), dim3(kWideBlock), 0, s, rows + bo.off[P_WIDE],
+    SPWS_LAUNCH(ctx, "row_wide", (k_row_wide<kValues>), dim3((unsigned)g), dim3(kWideBlock), 0, rows + bo.off[P_WIDE],
                        (int32_t)counts[P_WIDE], Ap, Ai, Ax, Bp, Bi, Bx, ctx->row_ub.as<int64_t>(), roff, scrd, sval,
                        rnnz, tk, tv, tab);
-    SPWS_CHECK_LAUNCH();
   }
   rc = exclusive_scan<int32_t, int32_t>(ctx, rnnz, ctx->out_pos.as<int32_t>(), m);
   if (rc) return rc;
   if (kValues && m > 0) {
     unsigned g = (unsigned)div_up(m, 4);
     if (g > 65536) g = 65536;
-    hipLaunchKernelGGL(k_compact, dim3(g), dim3(256), 0, s, m, roff, ctx->out_pos.as<int32_t>(), scrd, sval,
+    SPWS_LAUNCH(ctx, "compact", k_compact, dim3(g), dim3(256), 0, m, roff, ctx->out_pos.as<int32_t>(), scrd, sval,
                        ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>());
-    SPWS_CHECK_LAUNCH();
   }
   int32_t* h32 = reinterpret_cast<int32_t*>(hm + 20);
   SPWS_HIP(hipMemcpyAsync(h32, ctx->out_pos.as<int32_t>() + m, 4, hipMemcpyDeviceToHost, s));

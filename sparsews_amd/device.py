@@ -10,6 +10,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, lib
+from .formats import DCSR
 from .gen import CSR
 
 
@@ -97,6 +98,9 @@ class Context:
     def sync(self):
         check(lib().spws_sync(self.h))
 
+    def copy(self, dst: int, src: int, nbytes: int):
+        check(lib().spws_device_copy(self.h, C.c_void_p(dst), C.c_void_p(src), int(nbytes)))
+
     @property
     def stream(self) -> int:
         return int(lib().spws_ctx_get_stream(self.h) or 0)
@@ -106,6 +110,23 @@ class Context:
         check(lib().spws_ctx_get_stats(self.h, C.byref(s)))
         return {"products": s.products, "nnz": s.nnz, "rows_per_path": list(s.rows_per_path),
                 "chunks": s.chunks, "sort_bits": s.sort_bits}
+
+    # ---- kernel timing (HIP events around every launch on the context's stream)
+    def set_profiling(self, on: bool):
+        check(lib().spws_ctx_set_profiling(self.h, 1 if on else 0))
+
+    def reset_kernel_times(self):
+        check(lib().spws_ctx_reset_kernel_times(self.h))
+
+    def kernel_times(self) -> dict:
+        """{kernel name: (total ms, launches)} since the last reset."""
+        out, i = {}, 0
+        buf = C.create_string_buffer(128)
+        ms, n = C.c_double(), C.c_int64()
+        while lib().spws_ctx_kernel_times(self.h, i, buf, 128, C.byref(ms), C.byref(n)) == 0:
+            out[buf.value.decode()] = (ms.value, n.value)
+            i += 1
+        return out
 
     # ---- tuning
     def rowwise_tuning(self) -> dict:
@@ -163,3 +184,72 @@ def spgemm_rowwise_symbolic(ctx: Context, A: DeviceCSR, B: DeviceCSR) -> np.ndar
     check(lib().spws_spgemm_rowwise_symbolic(ctx.h, A.nrows, A.ncols, B.ncols, A.pos.ptr, A.crd.ptr,
                                              B.pos.ptr, B.crd.ptr, C.byref(nnz), C.byref(pp)))
     return DeviceArray.view(ctx, pp.value, A.nrows + 1, np.int32).to_host()
+
+
+@dataclass
+class DeviceDCSR:
+    """DCSR (or DCSC) on the device: stored outer ids, pos, crd, vals.
+    ``l1`` is None for a dense outer level (plain CSR/CSC)."""
+    nouter: int
+    l1: DeviceArray | None
+    pos: DeviceArray
+    crd: DeviceArray
+    vals: DeviceArray
+
+    @classmethod
+    def upload(cls, ctx: Context, d: DCSR, dense_outer: bool = False) -> "DeviceDCSR":
+        if dense_outer:
+            n = len(d.l2_pos) - 1
+            return cls(n, None, DeviceArray.from_host(ctx, d.l2_pos), DeviceArray.from_host(ctx, d.l2_crd),
+                       DeviceArray.from_host(ctx, d.vals))
+        return cls(len(d.l1_crd), DeviceArray.from_host(ctx, d.l1_crd), DeviceArray.from_host(ctx, d.l2_pos),
+                   DeviceArray.from_host(ctx, d.l2_crd), DeviceArray.from_host(ctx, d.vals))
+
+    @classmethod
+    def from_csr(cls, ctx: Context, a: CSR) -> "DeviceDCSR":
+        """A plain CSR as a dense-outer DCSR (every row index stored implicitly)."""
+        return cls(a.nrows, None, DeviceArray.from_host(ctx, np.asarray(a.pos, dtype=np.int32)),
+                   DeviceArray.from_host(ctx, a.crd), DeviceArray.from_host(ctx, a.vals))
+
+
+@dataclass
+class DeviceCSR64:
+    """Context-owned CSR output with int64 row pointers."""
+    nrows: int
+    ncols: int
+    nnz: int
+    pos: DeviceArray
+    crd: DeviceArray
+    vals: DeviceArray
+
+    def download(self) -> CSR:
+        return CSR(self.nrows, self.ncols, self.pos.to_host(), self.crd.to_host(), self.vals.to_host())
+
+    def rows(self, rows) -> tuple:
+        """(pos slice, crd, vals) of selected rows, copied to the host."""
+        pos = self.pos.to_host()
+        out = []
+        for r in rows:
+            b, e = int(pos[r]), int(pos[r + 1])
+            crd = DeviceArray.view(self.crd.ctx, self.crd.ptr + 4 * b, e - b, np.int32).to_host()
+            val = DeviceArray.view(self.vals.ctx, self.vals.ptr + 8 * b, e - b, np.float64).to_host()
+            out.append((crd, val))
+        return pos, out
+
+
+def set_outer_chunking(ctx: Context, max_products: int = 0, max_rows: int = 0):
+    check(lib().spws_ctx_set_outer_chunking(ctx.h, int(max_products), int(max_rows)))
+
+
+def spgemm_outer(ctx: Context, A: DeviceDCSR, B: DeviceDCSR, m: int, k: int, n: int) -> DeviceCSR64:
+    """C = A*B with A (m x k) in DCSC and B (k x n) in DCSR: expand, radix sort
+    on packed (i, k) keys, segmented reduce; int64 row pointers."""
+    nnz = C.c_int64()
+    pp, pc, pv = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    check(lib().spws_spgemm_outer(ctx.h, int(m), int(k), int(n), A.nouter, A.l1.ptr if A.l1 else None,
+                                  A.pos.ptr, A.crd.ptr, A.vals.ptr, B.nouter, B.l1.ptr if B.l1 else None,
+                                  B.pos.ptr, B.crd.ptr, B.vals.ptr, C.byref(nnz), C.byref(pp), C.byref(pc),
+                                  C.byref(pv)))
+    n_ = nnz.value
+    return DeviceCSR64(m, n, n_, DeviceArray.view(ctx, pp.value, m + 1, np.int64),
+                       DeviceArray.view(ctx, pc.value, n_, np.int32), DeviceArray.view(ctx, pv.value, n_, np.float64))

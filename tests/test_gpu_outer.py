@@ -1,0 +1,145 @@
+"""Outer-product SpGEMM (DCSC x DCSR, expand / radix sort / segmented reduce)
+on the GPU vs the CPU oracle, through the C ABI.
+
+Structure must be bit-exact.  Values: the device sums each entry over j in
+increasing order, the order of the oracle's Gustavson restatement, so the
+values are compared bit-for-bit as well as against the 1e-12 bar.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import powerlaw_rows, random_csr, uniform_rows
+from sparsews_amd import formats, gen
+from sparsews_amd.device import DeviceDCSR, set_outer_chunking, spgemm_outer
+from sparsews_amd.gen import CSR
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def run_outer(ctx, A, B, dense_outer=False):
+    Ac, Bd = formats.dcsc_from_csr(A), formats.dcsr_from_csr(B)
+    dA = DeviceDCSR.upload(ctx, Ac, dense_outer=False)
+    dB = DeviceDCSR.upload(ctx, Bd, dense_outer=False)
+    if dense_outer:
+        dA = DeviceDCSR.from_csr(ctx, A.transpose())
+        dB = DeviceDCSR.from_csr(ctx, B)
+    return spgemm_outer(ctx, dA, dB, A.nrows, A.ncols, B.ncols).download()
+
+
+def check(C, A, B, exact_order=True, nthreads=0):
+    G = oracle.spgemm_gustavson(A, B, nthreads)
+    assert C.nnz == G.nnz
+    np.testing.assert_array_equal(np.asarray(C.pos, np.int64), np.asarray(G.pos, np.int64))
+    np.testing.assert_array_equal(C.crd, G.crd)
+    exact, absum = oracle.spgemm_refsum(A, B, G.pos, G.crd, nthreads)
+    assert oracle.max_value_error(C.vals, exact, absum) <= TOL
+    if exact_order:
+        np.testing.assert_array_equal(C.vals, G.vals)
+
+
+@pytest.mark.parametrize("seed", [60, 61])
+def test_outer_powerlaw(ctx, seed):
+    print(f"case: outer power-law seed {seed}", flush=True)
+    A, B = powerlaw_rows(600, 500, 1.8, 300, seed, 0), powerlaw_rows(500, 700, 1.8, 300, seed, 1)
+    check(run_outer(ctx, A, B), A, B)
+
+
+def test_outer_dense_outer_levels(ctx):
+    print("case: outer CSC x CSR", flush=True)
+    A, B = uniform_rows(300, 200, 0, 15, 62, 0), uniform_rows(200, 260, 0, 15, 62, 1)
+    check(run_outer(ctx, A, B, dense_outer=True), A, B)
+
+
+def test_outer_many_chunks(ctx):
+    print("case: outer many chunks", flush=True)
+    A, B = powerlaw_rows(700, 600, 1.8, 300, 63, 0), powerlaw_rows(600, 650, 1.8, 300, 63, 1)
+    try:
+        set_outer_chunking(ctx, 3000, 7)
+        C = run_outer(ctx, A, B)
+        chunks = ctx.stats()["chunks"]
+    finally:
+        set_outer_chunking(ctx, 0, 0)
+    assert chunks >= 100
+    check(C, A, B)
+
+
+def test_outer_cc_gaps(ctx):
+    print("case: outer co-iteration gaps", flush=True)
+    A = uniform_rows(80, 120, 0, 8, 64, 0)
+    B = uniform_rows(120, 90, 0, 8, 64, 1)
+    lens = B.row_lengths().copy()
+    lens[::3] = 0
+    B2 = random_csr(120, 90, lens, 64, 5)
+    check(run_outer(ctx, A, B2), A, B2)
+
+
+def test_outer_edge_cases(ctx):
+    print("case: outer edge cases", flush=True)
+    A = CSR(6, 5, np.zeros(7, dtype=np.int32), np.zeros(0, dtype=np.int32), np.zeros(0))
+    B = uniform_rows(5, 9, 1, 4, 65, 1)
+    C = run_outer(ctx, A, B)
+    assert C.nnz == 0 and np.all(C.pos == 0) and len(C.pos) == 7
+    n = 150
+    I = CSR(n, n, np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), np.ones(n))
+    Bm = uniform_rows(n, 130, 0, 20, 66, 1)
+    CI = run_outer(ctx, I, Bm)
+    np.testing.assert_array_equal(CI.pos, Bm.pos)
+    np.testing.assert_array_equal(CI.crd, Bm.crd)
+    np.testing.assert_array_equal(CI.vals, Bm.vals)
+
+
+def test_outer_wide_columns_64bit_keys(ctx):
+    """ncols > 2^24 switches to 64-bit sort keys."""
+    print("case: outer 64-bit keys", flush=True)
+    n = (1 << 24) + 123
+    A = uniform_rows(200, 150, 0, 10, 67, 0)
+    B = random_csr(150, n, np.random.default_rng(67).integers(0, 12, 150), 67, 1)
+    C = run_outer(ctx, A, B)
+    assert ctx.stats()["sort_bits"] > 32
+    check(C, A, B, nthreads=1)
+
+
+def test_outer_workload_scaled(ctx):
+    """The outer-product generator at N = 20,000, cap 2,000, A = B^T."""
+    print("case: outer scaled workload", flush=True)
+    B = gen.zipf_csr(20_000, 1.8, 2_000, seed=1, stream=2)
+    A = B.transpose()
+    dA = DeviceDCSR.upload(ctx, formats.dcsr_from_csr(B))  # CSC(A) = B's arrays
+    dB = DeviceDCSR.upload(ctx, formats.dcsr_from_csr(B))
+    C = spgemm_outer(ctx, dA, dB, 20_000, 20_000, 20_000).download()
+    check(C, A, B)
+
+
+def test_outer_workload_full(ctx):
+    """The 200,000 x 200,000, Zipf 1.8 workload at full size: products count,
+    row-pointer invariants, and 64 sampled rows exactly against the oracle."""
+    print("case: outer full workload", flush=True)
+    B = gen.outer_operand()
+    L = B.row_lengths()
+    Bd = formats.dcsr_from_csr(B)
+    dA, dB = DeviceDCSR.upload(ctx, Bd), DeviceDCSR.upload(ctx, Bd)
+    N = B.nrows
+    C = spgemm_outer(ctx, dA, dB, N, N, N)
+    st = ctx.stats()
+    assert st["products"] == int((L * L).sum())
+    rng = np.random.default_rng(123)
+    rows = np.sort(rng.choice(N, 64, replace=False))
+    pos, got = C.rows(rows)
+    assert pos[0] == 0 and pos[-1] == C.nnz and np.all(np.diff(pos) >= 0)
+    # oracle on the sampled rows of A = B^T
+    AT = B.transpose()
+    lens = AT.row_lengths()[rows]
+    sub_pos = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    sub_crd = np.concatenate([AT.crd[AT.pos[r]:AT.pos[r + 1]] for r in rows])
+    sub_val = np.concatenate([AT.vals[AT.pos[r]:AT.pos[r + 1]] for r in rows])
+    Asub = CSR(len(rows), N, sub_pos, sub_crd, sub_val)
+    G = oracle.spgemm_gustavson(Asub, B)
+    exact, absum = oracle.spgemm_refsum(Asub, B, G.pos, G.crd)
+    for t, (crd, val) in enumerate(got):
+        sl = slice(G.pos[t], G.pos[t + 1])
+        np.testing.assert_array_equal(crd, G.crd[sl])
+        np.testing.assert_array_equal(val, G.vals[sl])
+        assert oracle.max_value_error(val, exact[sl], absum[sl]) <= TOL
+    print(f"outer full: nnz(C) = {C.nnz}, products = {st['products']}, chunks = {st['chunks']}", flush=True)
