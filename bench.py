@@ -10,7 +10,8 @@ on the context's stream, recorded inside the timed region) and the CPU
 baseline (the oracle on the host cores, on a bounded sample).
 
 Workloads (sizes in the docstring of each runner):
-  outer     outer-product CSC x CSR SpGEMM, 200,000 x 200,000, Zipf 1.8 rows
+  outer     (default) outer-product DCSC x DCSR SpGEMM, 200,000 x 200,000,
+            Zipf 1.8 row lengths, A = B^T
   rowwise   row-wise CSR x CSR hash SpGEMM, 10,000 x 10,000, 8 per row
 """
 from __future__ import annotations
@@ -128,15 +129,129 @@ def run_rowwise(args):
     return res
 
 
-WORKLOADS = {"rowwise": run_rowwise}
-DEFAULT_WORKLOAD = "rowwise"
+# --------------------------------------------------------------------------- outer product
+
+def run_outer(args):
+    """Outer-product SpGEMM C = A*B, B a 200,000 x 200,000 CSR with Zipf(1.8)
+    row lengths on [1, 20,000] and uniform distinct columns, FP64 U[-1,1],
+    seed 1; A = B^T, so DCSC(A) shares B's arrays.  Output sorted CSR with
+    int64 row pointers."""
+    from sparsews_amd import formats, gen
+    from sparsews_amd.device import Context, DeviceDCSR, spgemm_outer
+    B = gen.outer_operand()
+    N = B.nrows
+    Bd = formats.dcsr_from_csr(B)
+    ctx = Context(0)
+    dA, dB = DeviceDCSR.upload(ctx, Bd), DeviceDCSR.upload(ctx, Bd)
+    out = {}
+
+    def step():
+        out["C"] = spgemm_outer(ctx, dA, dB, N, N, N)
+
+    sec, times = timed_steps(ctx, step, args.steps, args.warmup)
+    st = ctx.stats()
+    P, nnz = st["products"], out["C"].nnz
+    key_bytes = 4
+    # algorithmic bytes of each kernel over the timed region (per step):
+    #   radix_scatter  read + write of key (4 B) and value (8 B) per product per pass
+    #   radix_hist     read of the key per product per pass
+    #   outer_expand   12 B written + 12 B of B gathered per product
+    #   outer_reduce   12 B read per product + 12 B written per distinct entry
+    passes = times["radix_scatter"][1] // (args.steps * st["chunks"]) if "radix_scatter" in times else 0
+    per_step = {
+        "radix_scatter": 2 * (key_bytes + 8) * P * passes,
+        "radix_hist": key_bytes * P * passes,
+        "outer_expand": 24 * P,
+        "outer_reduce": 12 * P + 12 * nnz,
+    }
+    dom = max(per_step, key=lambda k: times.get(k, (0, 1))[0])
+    ms, launches = times[dom]
+    achieved = per_step[dom] * args.steps / (ms / 1e3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+            "avg_launch_us": round(ms / launches * 1e3, 3),
+            "algorithmic_bytes_per_launch": int(per_step[dom] * args.steps / launches)}
+    n_in = 2 * (4 * (N + 1) + 12 * B.nnz)
+    compulsory = n_in + 8 * (N + 1) + 12 * nnz
+    res = {
+        "metric": "SpGEMM GFLOP/s",
+        "value": round(2 * P / sec / 1e9, 3),
+        "unit": "GFLOP/s",
+        "ms_per_step": round(sec * 1e3, 3),
+        "dtype": "f64",
+        "config": {"workload": "outer_product_spgemm_200k_zipf1.8", "n": N, "zipf_s": 1.8, "row_cap": 20000,
+                   "nnz_B": int(B.nnz), "products": int(P), "nnz_C": int(nnz), "chunks": st["chunks"],
+                   "sort_key_bits": st["sort_bits"], "radix_passes": int(passes), "seed": 1},
+        "roofline": roof,
+        "compulsory_bytes": int(compulsory),
+        "compulsory_frac_of_hbm_roofline": round(compulsory / sec / 1e9 / HBM_PEAK_GBS, 5),
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in sorted(times.items())},
+    }
+    ctx.close()
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"], res["cpu_baseline_coord_sequential"] = outer_cpu_baseline(B, args.cpu_seconds)
+    return res
+
+
+def outer_cpu_baseline(B, seconds):
+    """The oracle on bounded row samples of the same product (rows of A = B^T):
+    OpenMP Gustavson on all cores, and the reference's sequential coordinate
+    workspace (w_cap = 2^floor(log2 nnz(B))) on one core."""
+    import oracle
+    from sparsews_amd import formats
+    from sparsews_amd.gen import CSR
+    cores = cpu_cores()
+    AT = B.transpose()
+    rng = np.random.default_rng(7)
+    L = B.row_lengths()
+    ub = np.zeros(B.ncols, dtype=np.int64)
+    np.add.at(ub, B.crd, np.repeat(L, L))
+
+    def rows_sample(budget):
+        order = rng.permutation(B.nrows)
+        csum = np.cumsum(ub[order])
+        k = int(np.searchsorted(csum, budget)) + 1
+        rows = np.sort(order[:k])
+        lens = AT.row_lengths()[rows]
+        idx = np.concatenate([np.arange(AT.pos[r], AT.pos[r + 1]) for r in rows])
+        return CSR(len(rows), B.nrows, np.concatenate([[0], np.cumsum(lens)]).astype(np.int32),
+                   AT.crd[idx], AT.vals[idx]), int(ub[rows].sum())
+
+    A1, P1 = rows_sample(2.5e8)
+    t0 = time.perf_counter()
+    oracle.spgemm_gustavson(A1, B, cores)
+    t1 = time.perf_counter() - t0
+    scale = max(1.0, seconds / max(t1, 1e-3))
+    # at most 1e9 products: the oracle's CSR output has int32 row pointers
+    A2, P2 = rows_sample(min(2.5e8 * scale, 1e9))
+    t0 = time.perf_counter()
+    oracle.spgemm_gustavson(A2, B, cores)
+    t2 = time.perf_counter() - t0
+    omp = {"value": round(2 * P2 / t2 / 1e9, 4), "unit": "GFLOP/s", "cores": cores, "kind": "port",
+           "sample": f"{A2.nrows} random output rows ({P2} products) of the 200k workload, "
+                     f"OpenMP Gustavson restatement (CSR_CSR_taco_parallel.h), {t2:.1f} s"}
+    A3, P3 = rows_sample(2e6)
+    Ac, Bd = formats.dcsc_from_csr(A3), formats.dcsr_from_csr(B)
+    w_cap = 1 << int(np.log2(B.nnz))
+    t0 = time.perf_counter()
+    oracle.outer_coord(Ac, Bd, A3.nrows, B.ncols, w_cap)
+    t3 = time.perf_counter() - t0
+    seq = {"value": round(2 * P3 / t3 / 1e9, 5), "unit": "GFLOP/s", "cores": 1, "kind": "port",
+           "sample": f"{A3.nrows} random output rows ({P3} products), sequential coordinate workspace "
+                     f"(DCSC_DCSR_coord_index_chase.h, w_cap {w_cap}), {t3:.1f} s"}
+    return omp, seq
+
+
+WORKLOADS = {"rowwise": run_rowwise, "outer": run_outer}
+DEFAULT_WORKLOAD = "outer"
+DEFAULT_STEPS = {"rowwise": (50, 5), "outer": (5, 1)}
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -147,6 +262,9 @@ def main(argv=None):
         return 2
     if int(os.environ.get("RANK", "0")) != 0:
         return 0
+    ds, dw = DEFAULT_STEPS[args.workload]
+    args.steps = ds if args.steps is None else args.steps
+    args.warmup = dw if args.warmup is None else args.warmup
     res = WORKLOADS[args.workload](args)
     line = {"metric": res.pop("metric"), "value": res.pop("value"), "unit": res.pop("unit"), "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": res.pop("ms_per_step"),

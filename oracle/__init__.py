@@ -115,13 +115,17 @@ def spgemm_gustavson(A, B, nthreads: int = 0):
     nnz = lib().orc_spgemm_gustavson(A.nrows, B.ncols, _c32(A.pos), _c32(A.crd), _cf(A.vals),
                                      _c32(B.pos), _c32(B.crd), _cf(B.vals), pos,
                                      C.byref(ci), C.byref(cx), int(nthreads))
+    if nnz < 0:
+        raise OverflowError("product has more than 2^31 - 1 entries (int32 CSR)")
     return CSR(A.nrows, B.ncols, pos, _take(ci, nnz, np.int32), _take(cx, nnz, np.float64))
 
 
 def spgemm_symbolic(A, B, nthreads: int = 0):
     pos = np.zeros(A.nrows + 1, dtype=np.int32)
-    lib().orc_spgemm_symbolic(A.nrows, B.ncols, _c32(A.pos), _c32(A.crd), _c32(B.pos),
-                              _c32(B.crd), pos, int(nthreads))
+    nnz = lib().orc_spgemm_symbolic(A.nrows, B.ncols, _c32(A.pos), _c32(A.crd), _c32(B.pos),
+                                    _c32(B.crd), pos, int(nthreads))
+    if nnz < 0:
+        raise OverflowError("product has more than 2^31 - 1 entries (int32 CSR)")
     return pos
 
 

@@ -25,6 +25,7 @@
  *   orc_mttkrp_hash        benchmark/MTTKRP/COO_CSF_DCSR_DCSR_hash_free.h:233-366
  *   orc_ttm_hash_il        benchmark/TTM/COO_CSF_DCSR_hash_il.h:247-377
  */
+#include <limits.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -277,10 +278,18 @@ ORC_API int64_t orc_spgemm_symbolic(int32_t m, int32_t n, const int32_t* Ap, con
     free(seen);
     free(list);
   }
+  int64_t run = 0;
   Cp[0] = 0;
-  for (int32_t i = 0; i < m; i++) Cp[i + 1] = Cp[i] + cnt[i];
+  for (int32_t i = 0; i < m; i++) {
+    run += cnt[i];
+    if (run > INT32_MAX) { /* the CSR output uses int32 row pointers */
+      free(cnt);
+      return -1;
+    }
+    Cp[i + 1] = (int32_t)run;
+  }
   free(cnt);
-  return Cp[m];
+  return run;
 }
 
 /* OpenMP Gustavson with a dense workspace and a per-row qsort of the index
@@ -336,6 +345,11 @@ ORC_API int64_t orc_spgemm_gustavson(int32_t m, int32_t n,
                                      const int32_t* Bp, const int32_t* Bi, const double* Bx,
                                      int32_t* Cp, int32_t** Ci, double** Cx, int nthreads) {
   int64_t nnz = orc_spgemm_symbolic(m, n, Ap, Ai, Bp, Bi, Cp, nthreads);
+  if (nnz < 0) {
+    *Ci = NULL;
+    *Cx = NULL;
+    return -1;
+  }
   int32_t* ci = (int32_t*)malloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(int32_t));
   double* cx = (double*)malloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(double));
   orc_spgemm_gustavson_numeric(m, n, Ap, Ai, Ax, Bp, Bi, Bx, Cp, ci, cx, nthreads);
