@@ -135,6 +135,35 @@ int spws_spgemm_outer(spws_ctx* ctx, int32_t m, int32_t k, int32_t n,
  * the 32-bit sort key).  Small values force many chunks (used by tests). */
 int spws_ctx_set_outer_chunking(spws_ctx* ctx, int64_t max_products, int32_t max_rows);
 
+/* ------------------------------------------------------ sparse tensor kernels */
+
+/* A 3-level CSF is passed as nroots root coordinates l1_crd (the compressed
+ * root level), l2_pos[nroots+1] / l2_crd[nfibers] (second level),
+ * l3_pos[nfibers+1] / l3_crd[nnz] (leaves) and vals[nnz]; coordinates are
+ * sorted within each parent.  Device pointers. */
+
+/* MTTKRP, M(i,r) = sum_{j,k} X(i,j,k) Bf(j,r) Cf(k,r), for a CSF with levels
+ * (j, k, i) -- the output mode is the leaf, as in the reference's B(k,l,i).
+ * Bf (J x R) and Cf (K x R) are dense row-major, 1 <= R <= 64.  Output
+ * (context-owned): M dense I x R row-major, and the sorted list of the
+ * *nrows rows that hold a nonzero (the rows of the reference's COO output).
+ * Replaces compute() of benchmark/MTTKRP/COO_CSF_DCSR_DCSR_hash_free.h:233. */
+int spws_mttkrp(spws_ctx* ctx, int32_t I, int32_t J, int32_t K, int32_t R,
+                int32_t nroots, const int32_t* l1_crd, const int32_t* l2_pos, const int32_t* l2_crd,
+                int32_t nfibers, const int32_t* l3_pos, const int32_t* l3_crd, const double* vals, int64_t nnz,
+                const double* Bf, const double* Cf, int32_t* nrows, int32_t** rows, double** M);
+
+/* TTM, Y(i,j,r) = sum_k X(i,k,j) U(k,r), for a CSF with levels (i, k, j) --
+ * the contracted mode is the middle level, as in the reference's hash_il
+ * B(k,i,j).  U (K x R) dense row-major, 1 <= R <= 256.  Output
+ * (context-owned): *nfib fibers (fib_i, fib_j) sorted by (i, j), each with a
+ * dense length-R vector in Y (nfib x R row-major).
+ * Replaces compute() of benchmark/TTM/COO_CSF_DCSR_hash_il.h:247. */
+int spws_ttm(spws_ctx* ctx, int32_t I, int32_t J, int32_t K, int32_t R,
+             int32_t nroots, const int32_t* l1_crd, const int32_t* l2_pos, const int32_t* l2_crd,
+             int32_t nfibers, const int32_t* l3_pos, const int32_t* l3_crd, const double* vals, int64_t nnz,
+             const double* U, int64_t* nfib, int32_t** fib_i, int32_t** fib_j, double** Y);
+
 /* Host TACO entry points (host taco_tensor_t in, malloc'd host arrays out). */
 
 /* int compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B, int32_t w_cap)
@@ -152,6 +181,26 @@ int spws_csr_csr_assemble(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B);
 /* int compute(C, A, B) of seperate/CSR_CSR_1.h:4-67: fills C_vals (and
  * C2_crd) into the arrays assemble() allocated. */
 int spws_csr_csr_compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B);
+
+/* int compute(C, A, B, w_cap) of
+ * benchmark/DCSC_DCSR/DCSC_DCSR_coord_index_chase.h:194: A DCSC, B DCSR
+ * ("CC" taco tensors), C CSR with int32 pos (SPWS_ERR_UNSUPPORTED when
+ * nnz(C) >= 2^31).  Sets C->indices[1][0], C->indices[1][1], C->vals. */
+int spws_dcsc_dcsr_coord_compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B, int32_t w_cap);
+
+/* int compute(A, B, C, D, w_cap) of
+ * benchmark/MTTKRP/COO_CSF_DCSR_DCSR_hash_free.h:233: B a 3-level CSF with
+ * the output mode as leaf, C and D dense factors stored as DCSR.  Sets the
+ * COO output A->indices[1][0] (row), A->indices[2][0] (rank), A->vals and
+ * A->vals_size, sorted by (row, rank). */
+int spws_mttkrp_hash_compute(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C, taco_tensor_t* D,
+                             int32_t w_cap);
+
+/* int compute(A, B, C, w_cap) of benchmark/TTM/COO_CSF_DCSR_hash_il.h:247: B
+ * a 3-level CSF with the contracted mode in the middle, C a dense factor
+ * stored as DCSR.  Sets the COO output A->indices[1][0], [2][0], [3][0],
+ * A->vals and A->vals_size, sorted by (root, leaf, column). */
+int spws_ttm_hash_il_compute(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C, int32_t w_cap);
 
 #ifdef __cplusplus
 }

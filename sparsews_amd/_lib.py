@@ -81,9 +81,16 @@ SIGNATURES = {
     "spws_spgemm_outer": (C.c_int, [VP, I32, I32, I32, I32, VP, VP, VP, VP, I32, VP, VP, VP, VP,
                                     PI64, PVP, PVP, PVP]),
     "spws_ctx_set_outer_chunking": (C.c_int, [VP, I64, I32]),
+    "spws_mttkrp": (C.c_int, [VP, I32, I32, I32, I32, I32, VP, VP, VP, I32, VP, VP, VP, I64, VP, VP,
+                              C.POINTER(I32), PVP, PVP]),
+    "spws_ttm": (C.c_int, [VP, I32, I32, I32, I32, I32, VP, VP, VP, I32, VP, VP, VP, I64, VP,
+                           PI64, PVP, PVP, PVP]),
     "spws_csr_csr_hash_compute": (C.c_int, [C.POINTER(TacoTensor)] * 3 + [I32]),
     "spws_csr_csr_assemble": (C.c_int, [C.POINTER(TacoTensor)] * 3),
     "spws_csr_csr_compute": (C.c_int, [C.POINTER(TacoTensor)] * 3),
+    "spws_dcsc_dcsr_coord_compute": (C.c_int, [C.POINTER(TacoTensor)] * 3 + [I32]),
+    "spws_mttkrp_hash_compute": (C.c_int, [C.POINTER(TacoTensor)] * 4 + [I32]),
+    "spws_ttm_hash_il_compute": (C.c_int, [C.POINTER(TacoTensor)] * 3 + [I32]),
 }
 
 _lib = None

@@ -16,6 +16,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libspws.so")
+GEN_SRC = os.path.join(HERE, "native", "gen_coo3.c")
+GEN_LIB = os.path.join(HERE, "libspws_gen.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -56,8 +58,19 @@ def _compile(src: str, force: bool, hdr_mtime: float) -> str:
     return obj
 
 
+def build_gen() -> str:
+    """Host-only synthetic tensor generator (gcc, OpenMP)."""
+    cmd = ["gcc", "-O3", "-fPIC", "-shared", "-fopenmp", "-std=c11", "-Wall", "-o", GEN_LIB + ".tmp", GEN_SRC, "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"gcc failed for {GEN_SRC}:\n{r.stdout}{r.stderr}")
+    os.replace(GEN_LIB + ".tmp", GEN_LIB)
+    return GEN_LIB
+
+
 def build(force: bool = False, jobs: int = 8) -> str:
     os.makedirs(OBJ, exist_ok=True)
+    build_gen()
     hdr = _headers_mtime()
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, force, hdr), sources()))

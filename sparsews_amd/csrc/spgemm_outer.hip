@@ -199,41 +199,84 @@ __global__ __launch_bounds__(kRadixBlock) void k_outer_heads(const K* __restrict
   }
 }
 
+// One tile of sorted keys per workgroup.  The tile is staged in LDS with
+// coalesced loads; each thread scans kRadixItems consecutive keys for run
+// heads, a block scan numbers the distinct keys, each head sums its run in
+// order (continuing in global memory when the run leaves the tile), and the
+// tile's distinct (column, value) pairs are staged in LDS and stored with
+// coalesced writes.
 template <class K>
 __global__ __launch_bounds__(kRadixBlock) void k_outer_reduce(const K* __restrict__ keys, const double* __restrict__ vals,
                                                               int64_t n, const int32_t* __restrict__ tile_uoff, int kb,
                                                               int64_t base, int32_t r0, int32_t* __restrict__ out_crd,
                                                               double* __restrict__ out_val,
                                                               int64_t* __restrict__ out_pos) {
+  __shared__ K sk[kRadixTile];
+  __shared__ double sv[kRadixTile];
+  __shared__ int32_t scrd[kRadixTile];
   __shared__ int32_t sh[kRadixBlock / kWave + 1];
   const K kmask = (((K)1) << kb) - 1;
   const int64_t tbase = (int64_t)blockIdx.x * kRadixTile;
-  const int64_t mine = tbase + (int64_t)threadIdx.x * kRadixItems;
+  const int64_t remain = n - tbase;
+  const int tile_n = remain < kRadixTile ? (int)remain : kRadixTile;
+  for (int i = threadIdx.x; i < tile_n; i += kRadixBlock) {
+    sk[i] = keys[tbase + i];
+    sv[i] = vals[tbase + i];
+  }
+  const bool has_prev = tbase > 0;
+  const K prev_key = has_prev ? keys[tbase - 1] : (K)0;
+  __syncthreads();
+  const int first = threadIdx.x * kRadixItems;
   uint32_t heads = 0;
   int32_t cnt = 0;
 #pragma unroll
   for (int it = 0; it < kRadixItems; ++it) {
-    const int64_t p = mine + it;
-    if (p < n && (p == 0 || keys[p] != keys[p - 1])) {
-      heads |= 1u << it;
-      ++cnt;
+    const int i = first + it;
+    if (i < tile_n) {
+      const bool h = i == 0 ? !has_prev || sk[0] != prev_key : sk[i] != sk[i - 1];
+      if (h) {
+        heads |= 1u << it;
+        ++cnt;
+      }
     }
   }
   int32_t tot;
   const int32_t pre = block_excl_scan<int32_t, kRadixBlock>(cnt, sh, &tot);
-  int64_t u = base + tile_uoff[blockIdx.x] + pre;
+  // run sums (registers), then stage
+  double sums[kRadixItems];
+#pragma unroll
+  for (int it = 0; it < kRadixItems; ++it) {
+    sums[it] = 0.0;
+    if (!(heads & (1u << it))) continue;
+    const int i = first + it;
+    const K key = sk[i];
+    double v = sv[i];
+    int q = i + 1;
+    while (q < tile_n && sk[q] == key) v += sv[q++];
+    if (q == tile_n) {
+      for (int64_t g = tbase + q; g < n && keys[g] == key; ++g) v += vals[g];
+    }
+    sums[it] = v;
+  }
+  const int64_t ubase = base + tile_uoff[blockIdx.x];
+  __syncthreads();
+  int32_t u = pre;
+#pragma unroll
   for (int it = 0; it < kRadixItems; ++it) {
     if (!(heads & (1u << it))) continue;
-    const int64_t p = mine + it;
-    const K key = keys[p];
-    double v = vals[p];
-    for (int64_t q = p + 1; q < n && keys[q] == key; ++q) v += vals[q];
-    out_crd[u] = (int32_t)(uint32_t)(key & kmask);
-    out_val[u] = v;
+    const int i = first + it;
+    const K key = sk[i];
+    scrd[u] = (int32_t)(uint32_t)(key & kmask);
+    sv[u] = sums[it];
     const int32_t row = (int32_t)(key >> kb);
-    const int32_t prev = p == 0 ? -1 : (int32_t)(keys[p - 1] >> kb);
-    for (int32_t r = prev + 1; r <= row; ++r) out_pos[r0 + r] = u;
+    const int32_t prow = i == 0 ? (has_prev ? (int32_t)(prev_key >> kb) : -1) : (int32_t)(sk[i - 1] >> kb);
+    for (int32_t r = prow + 1; r <= row; ++r) out_pos[r0 + r] = ubase + u;
     ++u;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < tot; i += kRadixBlock) {
+    out_crd[ubase + i] = scrd[i];
+    out_val[ubase + i] = sv[i];
   }
 }
 

@@ -1,0 +1,469 @@
+// Sparse tensor kernels over 3-level CSF inputs: MTTKRP and TTM.
+//
+// MTTKRP  M(i,r) = sum_{j,k} X(i,j,k) * Bf(j,r) * Cf(k,r)   (output mode i)
+//   Input CSF levels (j, k, i): the output mode is the leaf, as in the
+//   reference (B(k,l,i) in benchmark/MTTKRP/COO_CSF_DCSR_DCSR_hash_free.h:233;
+//   its C and D factors are Bf and Cf here, given dense, row-major R wide).
+//   Bucket workspace over output rows:
+//     1. k_seg_owner (x2)   leaf -> fiber and fiber -> root owners.
+//     2. k_mttkrp_keys      packed key (i << (bj+bk) | j << bk | k), value X.
+//     3. radix sort         stable on the i bits only: the buckets are the
+//                           output rows, CSF (j,k) order kept inside each.
+//     4. k_mttkrp_accum     each half-wavefront (32 lanes = 32 ranks) walks a
+//                           segment of the bucketed nonzeros, loading factor
+//                           rows Bf(j,:), Cf(k,:) as coalesced 256-byte rows
+//                           and accumulating the rank-R fiber in registers;
+//                           a fiber is added into M when its row changes (only
+//                           segment-boundary rows are shared, so the atomics
+//                           are few and uncontended).
+//     5. k_rows_present + scan + k_rows_compact: the sorted list of rows that
+//                           hold a nonzero (the reference's COO output rows).
+//   The product is (X * Bf) * Cf, the reference's B_vals * C_vals * D_vals
+//   (:296), and each segment sums in CSF order.
+//
+// TTM  Y(i,j,r) = sum_k X(i,k,j) * U(k,r)   (contracted mode k)
+//   Input CSF levels (i, k, j): the contracted coordinate is the middle
+//   level, as in the reference's hash_il (B(k,i,j) in
+//   benchmark/TTM/COO_CSF_DCSR_hash_il.h:247).  Hash workspace keyed on (i,j):
+//     1. k_seg_owner (x2)   leaf -> fiber and fiber -> root owners.
+//     2. k_ttm_insert       open-addressing table in HBM (64-bit keys
+//                           (i << bj) | j, atomicCAS); slot per nonzero.
+//     3. k_ttm_occupied + scan + k_ttm_collect: the occupied keys (one per
+//                           output fiber (i,j)) with their slots.
+//     4. radix sort         fiber keys, slot ids carried (the reference's Sort
+//                           of the workspace before Merge).
+//     5. k_ttm_rank         slot -> position of its fiber in sorted order.
+//     6. k_ttm_accum        R lanes per nonzero add X * U(k,:) into the dense
+//                           length-R fiber of its (i,j) (FP64 atomics).
+//   Output: semi-sparse -- the sorted fiber list (i,j) with a dense length-R
+//   vector each.
+#include <algorithm>
+#include <climits>
+
+#include "radix.hpp"
+
+namespace spws {
+
+constexpr int kOwnBlock = 256;
+constexpr int kOwnItems = 8;
+constexpr int kOwnTile = kOwnBlock * kOwnItems;
+
+// owner[e] = s for e in [pos[s], pos[s+1]), s in [0, nseg); pos[nseg] = nitems.
+// One tile of items per workgroup; the owners of the tile's first and last
+// item by binary search, then per item a binary search in the LDS slice.
+__global__ __launch_bounds__(kOwnBlock) void k_seg_owner(const int32_t* __restrict__ pos, int32_t nseg, int64_t nitems,
+                                                         int32_t* __restrict__ owner) {
+  __shared__ int32_t spos[kOwnTile + 1];
+  __shared__ int32_t s_lo, s_span;
+  const int64_t e0 = (int64_t)blockIdx.x * kOwnTile;
+  const int64_t e_end = e0 + kOwnTile < nitems ? e0 + kOwnTile : nitems;
+  if (threadIdx.x == 0) {
+    // last s with pos[s] <= x and pos[s+1] > x: search the last s with pos[s] <= x
+    // among non-empty segments (pos is non-decreasing)
+    auto last_le = [&](int64_t x) {
+      int32_t lo = 0, hi = nseg - 1;
+      while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (pos[mid] <= x) lo = mid; else hi = mid - 1;
+      }
+      return lo;
+    };
+    const int32_t a = last_le(e0);
+    const int32_t b = last_le(e_end - 1);
+    s_lo = a;
+    s_span = b - a + 1;
+  }
+  __syncthreads();
+  const int32_t lo0 = s_lo;
+  const int32_t span = s_span;
+  const bool fits = span <= kOwnTile;
+  if (fits)
+    for (int i = threadIdx.x; i <= span; i += kOwnBlock) spos[i] = pos[lo0 + i];
+  __syncthreads();
+  for (int it = 0; it < kOwnItems; ++it) {
+    const int64_t e = e0 + (int64_t)it * kOwnBlock + threadIdx.x;
+    if (e >= e_end) break;
+    int32_t lo = 0, hi = span - 1;
+    if (fits) {
+      while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (spos[mid] <= e) lo = mid; else hi = mid - 1;
+      }
+    } else {
+      while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (pos[lo0 + mid] <= e) lo = mid; else hi = mid - 1;
+      }
+    }
+    owner[e] = lo0 + lo;
+  }
+}
+
+static int seg_owner(spws_ctx* ctx, const int32_t* pos, int32_t nseg, int64_t nitems, int32_t* owner) {
+  if (nitems <= 0) return SPWS_OK;
+  SPWS_LAUNCH(ctx, "seg_owner", k_seg_owner, dim3((unsigned)div_up(nitems, kOwnTile)), dim3(kOwnBlock), 0, pos,
+              nseg, nitems, owner);
+  return SPWS_OK;
+}
+
+static int bits_of(int64_t v) {  // bits for values in [0, v)
+  int b = 0;
+  while (b < 63 && ((int64_t)1 << b) < v) ++b;
+  return b;
+}
+
+// ------------------------------------------------------------------ MTTKRP
+
+__global__ __launch_bounds__(256) void k_mttkrp_keys(int64_t nnz, const int32_t* __restrict__ leaf_fiber,
+                                                     const int32_t* __restrict__ fiber_root,
+                                                     const int32_t* __restrict__ l1_crd,
+                                                     const int32_t* __restrict__ l2_crd,
+                                                     const int32_t* __restrict__ l3_crd, int bj, int bk,
+                                                     uint64_t* __restrict__ keys) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t f = leaf_fiber[e];
+    const uint64_t j = (uint32_t)l1_crd[fiber_root[f]];
+    const uint64_t k = (uint32_t)l2_crd[f];
+    const uint64_t i = (uint32_t)l3_crd[e];
+    keys[e] = (i << (bj + bk)) | (j << bk) | k;
+  }
+}
+
+// Each half-wavefront owns `seg` consecutive bucketed nonzeros.  Lanes cover
+// ranks r = lane32, lane32 + 32, ... (R <= 64 supported with two slices).
+constexpr int kAccBlock = 256;
+
+__global__ __launch_bounds__(kAccBlock) void k_mttkrp_accum(int64_t nnz, int64_t seg, const uint64_t* __restrict__ keys,
+                                                            const double* __restrict__ vals, int bj, int bk, int R,
+                                                            const double* __restrict__ Bf,
+                                                            const double* __restrict__ Cf, double* __restrict__ M,
+                                                            int32_t* __restrict__ present) {
+  const int half = threadIdx.x >> 5;
+  const int r0 = threadIdx.x & 31;
+  const int64_t hid = (int64_t)blockIdx.x * (kAccBlock / 32) + half;
+  const int64_t e0 = hid * seg;
+  if (e0 >= nnz) return;
+  const int64_t e1 = e0 + seg < nnz ? e0 + seg : nnz;
+  const uint64_t jm = (1ull << bj) - 1, km = (1ull << bk) - 1;
+  const bool two = R > 32;
+  const int r1 = r0 + 32;
+  int64_t cur = -1;
+  double acc0 = 0.0, acc1 = 0.0;
+  for (int64_t e = e0; e < e1; ++e) {
+    const uint64_t key = keys[e];
+    const int64_t i = (int64_t)(key >> (bj + bk));
+    if (i != cur) {
+      if (cur >= 0) {
+        if (r0 < R) atomicAdd(&M[cur * R + r0], acc0);
+        if (two && r1 < R) atomicAdd(&M[cur * R + r1], acc1);
+        if (r0 == 0) present[cur] = 1;
+      }
+      cur = i;
+      acc0 = acc1 = 0.0;
+    }
+    const int64_t j = (int64_t)((key >> bk) & jm);
+    const int64_t k = (int64_t)(key & km);
+    const double x = vals[e];
+    if (r0 < R) acc0 += (x * Bf[j * R + r0]) * Cf[k * R + r0];
+    if (two && r1 < R) acc1 += (x * Bf[j * R + r1]) * Cf[k * R + r1];
+  }
+  if (cur >= 0) {
+    if (r0 < R) atomicAdd(&M[cur * R + r0], acc0);
+    if (two && r1 < R) atomicAdd(&M[cur * R + r1], acc1);
+    if (r0 == 0) present[cur] = 1;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_rows_compact(int32_t n, const int32_t* __restrict__ present,
+                                                      const int32_t* __restrict__ off, int32_t* __restrict__ rows) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && present[i]) rows[off[i]] = i;
+}
+
+// ------------------------------------------------------------------ TTM
+
+constexpr uint64_t kEmptyKey = ~0ull;
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 31;
+  x *= 0x7fb5d329728ea185ull;
+  x ^= x >> 27;
+  x *= 0x81dadef4bc2dd44dull;
+  x ^= x >> 33;
+  return x;
+}
+
+__global__ __launch_bounds__(256) void k_ttm_insert(int64_t nnz, const int32_t* __restrict__ leaf_fiber,
+                                                    const int32_t* __restrict__ fiber_root,
+                                                    const int32_t* __restrict__ l1_crd,
+                                                    const int32_t* __restrict__ l3_crd, int bj,
+                                                    unsigned long long* __restrict__ table, uint64_t mask,
+                                                    int32_t* __restrict__ leaf_slot) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = (uint32_t)l1_crd[fiber_root[leaf_fiber[e]]];
+    const uint64_t j = (uint32_t)l3_crd[e];
+    const uint64_t key = (i << bj) | j;
+    uint64_t h = mix64(key) & mask;
+    for (;;) {
+      unsigned long long cur = __hip_atomic_load(&table[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (cur == kEmptyKey) {
+        cur = atomicCAS(&table[h], (unsigned long long)kEmptyKey, (unsigned long long)key);
+        if (cur == kEmptyKey) cur = key;
+      }
+      if (cur == key) break;
+      h = (h + 1) & mask;
+    }
+    leaf_slot[e] = (int32_t)h;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ttm_occupied(int64_t T, const unsigned long long* __restrict__ table,
+                                                      int32_t* __restrict__ flag) {
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < T; s += (int64_t)gridDim.x * blockDim.x)
+    flag[s] = table[s] != kEmptyKey ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_ttm_collect(int64_t T, const unsigned long long* __restrict__ table,
+                                                     const int32_t* __restrict__ off, uint64_t* __restrict__ keys,
+                                                     double* __restrict__ slot_payload) {
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < T; s += (int64_t)gridDim.x * blockDim.x) {
+    const unsigned long long key = table[s];
+    if (key != kEmptyKey) {
+      const int32_t q = off[s];
+      keys[q] = key;
+      slot_payload[q] = __longlong_as_double((long long)s);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ttm_rank(int64_t F, const uint64_t* __restrict__ keys,
+                                                  const double* __restrict__ slot_payload, int bj,
+                                                  int32_t* __restrict__ rank_of_slot, int32_t* __restrict__ out_i,
+                                                  int32_t* __restrict__ out_j) {
+  const uint64_t jm = (1ull << bj) - 1;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < F; q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = (int64_t)__double_as_longlong(slot_payload[q]);
+    rank_of_slot[s] = (int32_t)q;
+    out_i[q] = (int32_t)(keys[q] >> bj);
+    out_j[q] = (int32_t)(keys[q] & jm);
+  }
+}
+
+// R lanes per nonzero (R <= 64, a power of two up to 64 lanes per group).
+__global__ __launch_bounds__(256) void k_ttm_accum(int64_t nnz, int R, int lanes, const int32_t* __restrict__ leaf_fiber,
+                                                   const int32_t* __restrict__ l2_crd,
+                                                   const double* __restrict__ xvals,
+                                                   const int32_t* __restrict__ leaf_slot,
+                                                   const int32_t* __restrict__ rank_of_slot,
+                                                   const double* __restrict__ U, double* __restrict__ Y) {
+  const int groups = blockDim.x / lanes;
+  const int g = threadIdx.x / lanes;
+  const int r = threadIdx.x % lanes;
+  for (int64_t e = (int64_t)blockIdx.x * groups + g; e < nnz; e += (int64_t)gridDim.x * groups) {
+    const int64_t k = l2_crd[leaf_fiber[e]];
+    const int64_t q = rank_of_slot[leaf_slot[e]];
+    const double x = xvals[e];
+    for (int rr = r; rr < R; rr += lanes) atomicAdd(&Y[q * R + rr], x * U[k * R + rr]);
+  }
+}
+
+// ------------------------------------------------------------------ drivers
+
+struct Csf3 {
+  int32_t nroots;
+  const int32_t* l1_crd;
+  const int32_t* l2_pos;
+  const int32_t* l2_crd;
+  int32_t nfibers;
+  const int32_t* l3_pos;
+  const int32_t* l3_crd;
+  const double* vals;
+  int64_t nnz;
+};
+
+static int csf_owners(spws_ctx* ctx, const Csf3& x, int32_t** leaf_fiber, int32_t** fiber_root) {
+  int rc;
+  if ((rc = ctx->obuf[0].ensure((size_t)(x.nnz > 0 ? x.nnz : 1) * 4)) ||
+      (rc = ctx->obuf[1].ensure((size_t)(x.nfibers > 0 ? x.nfibers : 1) * 4)))
+    return rc;
+  *leaf_fiber = ctx->obuf[0].as<int32_t>();
+  *fiber_root = ctx->obuf[1].as<int32_t>();
+  if ((rc = seg_owner(ctx, x.l3_pos, x.nfibers, x.nnz, *leaf_fiber))) return rc;
+  return seg_owner(ctx, x.l2_pos, x.nroots, x.nfibers, *fiber_root);
+}
+
+static int mttkrp_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_t K, int32_t R, const double* Bf,
+                      const double* Cf, int32_t* nrows_out) {
+  int rc = ctx_activate(ctx);
+  if (rc) return rc;
+  SPWS_REQUIRE(I >= 0 && J >= 0 && K >= 0 && R >= 1 && R <= 64, SPWS_ERR_INVALID,
+               "MTTKRP needs non-negative dimensions and 1 <= R <= 64");
+  const int bi = bits_of(I > 1 ? I : 2), bj = bits_of(J > 1 ? J : 2), bk = bits_of(K > 1 ? K : 2);
+  SPWS_REQUIRE(bi + bj + bk <= 64, SPWS_ERR_UNSUPPORTED, "tensor dimensions exceed the 64-bit packed key");
+  hipStream_t s = ctx->stream;
+  memset(&ctx->stats, 0, sizeof(ctx->stats));
+  const size_t nz = (size_t)(x.nnz > 0 ? x.nnz : 1);
+  const size_t ii = (size_t)(I > 0 ? I : 1);
+  if ((rc = ctx->out_val.ensure(ii * R * 8)) || (rc = ctx->out_aux.ensure(ii * 4 * 2 + 8)) ||
+      (rc = ctx->out_crd.ensure(ii * 4)) || (rc = ctx->keys[0].ensure(nz * 8)) || (rc = ctx->keys[1].ensure(nz * 8)) ||
+      (rc = ctx->vals[0].ensure(nz * 8)) || (rc = ctx->vals[1].ensure(nz * 8)))
+    return rc;
+  double* M = ctx->out_val.as<double>();
+  int32_t* present = ctx->out_aux.as<int32_t>();
+  int32_t* poff = present + ii;
+  SPWS_HIP(hipMemsetAsync(M, 0, ii * R * 8, s));
+  SPWS_HIP(hipMemsetAsync(present, 0, ii * 4, s));
+  if (x.nnz > 0) {
+    int32_t *leaf_fiber, *fiber_root;
+    if ((rc = csf_owners(ctx, x, &leaf_fiber, &fiber_root))) return rc;
+    uint64_t* keys = ctx->keys[0].as<uint64_t>();
+    double* vals = ctx->vals[0].as<double>();
+    SPWS_HIP(hipMemcpyAsync(vals, x.vals, (size_t)x.nnz * 8, hipMemcpyDeviceToDevice, s));
+    const unsigned g = (unsigned)std::min<int64_t>(div_up(x.nnz, 256), 65536);
+    SPWS_LAUNCH(ctx, "mttkrp_keys", k_mttkrp_keys, dim3(g), dim3(256), 0, x.nnz, leaf_fiber, fiber_root, x.l1_crd,
+                x.l2_crd, x.l3_crd, bj, bk, keys);
+    bool in_alt = false;
+    if ((rc = radix_sort_pairs<uint64_t>(ctx, keys, vals, ctx->keys[1].as<uint64_t>(), ctx->vals[1].as<double>(),
+                                         x.nnz, bj + bk, bj + bk + bi, &in_alt)))
+      return rc;
+    if (in_alt) {
+      keys = ctx->keys[1].as<uint64_t>();
+      vals = ctx->vals[1].as<double>();
+    }
+    // segments of 128 nonzeros per half-wavefront
+    const int64_t seg = 128;
+    const int64_t halves = div_up(x.nnz, seg);
+    SPWS_LAUNCH(ctx, "mttkrp_accum", k_mttkrp_accum, dim3((unsigned)div_up(halves, kAccBlock / 32)), dim3(kAccBlock),
+                0, x.nnz, seg, keys, vals, bj, bk, R, Bf, Cf, M, present);
+    ctx->stats.products = x.nnz * (int64_t)R;
+  }
+  if ((rc = exclusive_scan<int32_t, int32_t>(ctx, present, poff, I))) return rc;
+  if (I > 0)
+    SPWS_LAUNCH(ctx, "rows_compact", k_rows_compact, dim3((unsigned)div_up(I, 256)), dim3(256), 0, I, present, poff,
+                ctx->out_crd.as<int32_t>());
+  int32_t* hm = ctx->host_meta.as<int32_t>();
+  SPWS_HIP(hipMemcpyAsync(hm, poff + I, 4, hipMemcpyDeviceToHost, s));
+  SPWS_HIP(hipStreamSynchronize(s));
+  *nrows_out = *hm;
+  ctx->stats.nnz = (int64_t)(*hm) * R;
+  return SPWS_OK;
+}
+
+static int ttm_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_t K, int32_t R, const double* U,
+                   int64_t* nfib_out) {
+  int rc = ctx_activate(ctx);
+  if (rc) return rc;
+  SPWS_REQUIRE(I >= 0 && J >= 0 && K >= 0 && R >= 1 && R <= 256, SPWS_ERR_INVALID,
+               "TTM needs non-negative dimensions and 1 <= R <= 256");
+  const int bi = bits_of(I > 1 ? I : 2), bj = bits_of(J > 1 ? J : 2);
+  SPWS_REQUIRE(bi + bj <= 63, SPWS_ERR_UNSUPPORTED, "tensor dimensions exceed the 64-bit fiber key");
+  SPWS_REQUIRE(x.nnz < ((int64_t)1 << 30), SPWS_ERR_UNSUPPORTED, "TTM supports fewer than 2^30 nonzeros");
+  hipStream_t s = ctx->stream;
+  memset(&ctx->stats, 0, sizeof(ctx->stats));
+  const size_t nz = (size_t)(x.nnz > 0 ? x.nnz : 1);
+  // table of 2 * nnz (power of two) slots
+  uint64_t T = 1024;
+  while (T < 2 * (uint64_t)nz) T <<= 1;
+  if ((rc = ctx->wide_tab.ensure(T * 8)) || (rc = ctx->slot_crd.ensure(nz * 4)) ||
+      (rc = ctx->row_nnz.ensure((T + 1) * 4 * 2)) || (rc = ctx->keys[0].ensure(nz * 8)) ||
+      (rc = ctx->keys[1].ensure(nz * 8)) || (rc = ctx->vals[0].ensure(nz * 8)) || (rc = ctx->vals[1].ensure(nz * 8)))
+    return rc;
+  unsigned long long* table = ctx->wide_tab.as<unsigned long long>();
+  int32_t* leaf_slot = ctx->slot_crd.as<int32_t>();
+  int32_t* flag = ctx->row_nnz.as<int32_t>();
+  int32_t* off = flag + (T + 1);
+  int32_t* hm = ctx->host_meta.as<int32_t>();
+  int64_t F = 0;
+  if (x.nnz > 0) {
+    int32_t *leaf_fiber, *fiber_root;
+    if ((rc = csf_owners(ctx, x, &leaf_fiber, &fiber_root))) return rc;
+    SPWS_HIP(hipMemsetAsync(table, 0xff, T * 8, s));
+    const unsigned g = (unsigned)std::min<int64_t>(div_up(x.nnz, 256), 65536);
+    SPWS_LAUNCH(ctx, "ttm_insert", k_ttm_insert, dim3(g), dim3(256), 0, x.nnz, leaf_fiber, fiber_root, x.l1_crd,
+                x.l3_crd, bj, table, (uint64_t)(T - 1), leaf_slot);
+    const unsigned gt = (unsigned)std::min<int64_t>(div_up((int64_t)T, 256), 65536);
+    SPWS_LAUNCH(ctx, "ttm_occupied", k_ttm_occupied, dim3(gt), dim3(256), 0, (int64_t)T, table, flag);
+    if ((rc = exclusive_scan<int32_t, int32_t>(ctx, flag, off, (int64_t)T))) return rc;
+    SPWS_HIP(hipMemcpyAsync(hm, off + T, 4, hipMemcpyDeviceToHost, s));
+    SPWS_HIP(hipStreamSynchronize(s));
+    F = *hm;
+    uint64_t* keys = ctx->keys[0].as<uint64_t>();
+    double* pay = ctx->vals[0].as<double>();
+    SPWS_LAUNCH(ctx, "ttm_collect", k_ttm_collect, dim3(gt), dim3(256), 0, (int64_t)T, table, off, keys, pay);
+    bool in_alt = false;
+    if ((rc = radix_sort_pairs<uint64_t>(ctx, keys, pay, ctx->keys[1].as<uint64_t>(), ctx->vals[1].as<double>(), F, 0,
+                                         bi + bj, &in_alt)))
+      return rc;
+    if (in_alt) {
+      keys = ctx->keys[1].as<uint64_t>();
+      pay = ctx->vals[1].as<double>();
+    }
+    // rank_of_slot reuses the flag array (T entries)
+    int32_t* rank_of_slot = flag;
+    if ((rc = ctx->out_crd.ensure((size_t)(F > 0 ? F : 1) * 4 * 2)) ||
+        (rc = ctx->out_val.ensure((size_t)(F > 0 ? F : 1) * R * 8)))
+      return rc;
+    int32_t* out_i = ctx->out_crd.as<int32_t>();
+    int32_t* out_j = out_i + F;
+    const unsigned gf = (unsigned)std::min<int64_t>(div_up(F, 256), 65536);
+    SPWS_LAUNCH(ctx, "ttm_rank", k_ttm_rank, dim3(gf), dim3(256), 0, F, keys, pay, bj, rank_of_slot, out_i, out_j);
+    double* Y = ctx->out_val.as<double>();
+    SPWS_HIP(hipMemsetAsync(Y, 0, (size_t)F * R * 8, s));
+    const int lanes = R >= 64 ? 64 : (R > 32 ? 64 : (R > 16 ? 32 : (R > 8 ? 16 : (R > 4 ? 8 : (R > 2 ? 4 : R)))));
+    const int64_t groups = 256 / lanes;
+    const unsigned ga = (unsigned)std::min<int64_t>(div_up(x.nnz, groups), 65536);
+    SPWS_LAUNCH(ctx, "ttm_accum", k_ttm_accum, dim3(ga), dim3(256), 0, x.nnz, R, lanes, leaf_fiber, x.l2_crd, x.vals,
+                leaf_slot, rank_of_slot, U, Y);
+    ctx->stats.products = x.nnz * (int64_t)R;
+  } else {
+    if ((rc = ctx->out_crd.ensure(8)) || (rc = ctx->out_val.ensure(8))) return rc;
+  }
+  SPWS_HIP(hipStreamSynchronize(s));
+  *nfib_out = F;
+  ctx->stats.nnz = F * R;
+  return SPWS_OK;
+}
+
+}  // namespace spws
+
+using namespace spws;
+
+extern "C" {
+
+int spws_mttkrp(spws_ctx* ctx, int32_t I, int32_t J, int32_t K, int32_t R, int32_t nroots, const int32_t* l1_crd,
+                const int32_t* l2_pos, const int32_t* l2_crd, int32_t nfibers, const int32_t* l3_pos,
+                const int32_t* l3_crd, const double* vals, int64_t nnz, const double* Bf, const double* Cf,
+                int32_t* nrows, int32_t** rows, double** M) {
+  SPWS_REQUIRE(nnz >= 0 && nroots >= 0 && nfibers >= 0, SPWS_ERR_INVALID, "negative size");
+  SPWS_REQUIRE(nnz == 0 || (l1_crd && l2_pos && l2_crd && l3_pos && l3_crd && vals && Bf && Cf), SPWS_ERR_INVALID,
+               "null pointer argument");
+  SPWS_REQUIRE(nnz < ((int64_t)1 << 31), SPWS_ERR_UNSUPPORTED, "MTTKRP supports fewer than 2^31 nonzeros");
+  Csf3 x{nroots, l1_crd, l2_pos, l2_crd, nfibers, l3_pos, l3_crd, vals, nnz};
+  int32_t nr = 0;
+  int rc = mttkrp_run(ctx, x, I, J, K, R, Bf, Cf, &nr);
+  if (rc) return rc;
+  if (nrows) *nrows = nr;
+  if (rows) *rows = ctx->out_crd.as<int32_t>();
+  if (M) *M = ctx->out_val.as<double>();
+  return SPWS_OK;
+}
+
+int spws_ttm(spws_ctx* ctx, int32_t I, int32_t J, int32_t K, int32_t R, int32_t nroots, const int32_t* l1_crd,
+             const int32_t* l2_pos, const int32_t* l2_crd, int32_t nfibers, const int32_t* l3_pos,
+             const int32_t* l3_crd, const double* vals, int64_t nnz, const double* U, int64_t* nfib,
+             int32_t** fib_i, int32_t** fib_j, double** Y) {
+  SPWS_REQUIRE(nnz >= 0 && nroots >= 0 && nfibers >= 0, SPWS_ERR_INVALID, "negative size");
+  SPWS_REQUIRE(nnz == 0 || (l1_crd && l2_pos && l2_crd && l3_pos && l3_crd && vals && U), SPWS_ERR_INVALID,
+               "null pointer argument");
+  Csf3 x{nroots, l1_crd, l2_pos, l2_crd, nfibers, l3_pos, l3_crd, vals, nnz};
+  int64_t F = 0;
+  int rc = ttm_run(ctx, x, I, J, K, R, U, &F);
+  if (rc) return rc;
+  if (nfib) *nfib = F;
+  if (fib_i) *fib_i = ctx->out_crd.as<int32_t>();
+  if (fib_j) *fib_j = ctx->out_crd.as<int32_t>() + F;
+  if (Y) *Y = ctx->out_val.as<double>();
+  return SPWS_OK;
+}
+
+}  // extern "C"

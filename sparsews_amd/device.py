@@ -10,7 +10,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, lib
-from .formats import DCSR
+from .formats import CSF3, DCSR
 from .gen import CSR
 
 
@@ -253,3 +253,48 @@ def spgemm_outer(ctx: Context, A: DeviceDCSR, B: DeviceDCSR, m: int, k: int, n: 
     n_ = nnz.value
     return DeviceCSR64(m, n, n_, DeviceArray.view(ctx, pp.value, m + 1, np.int64),
                        DeviceArray.view(ctx, pc.value, n_, np.int32), DeviceArray.view(ctx, pv.value, n_, np.float64))
+
+
+@dataclass
+class DeviceCSF:
+    """A 3-level CSF on the device (see formats.CSF3)."""
+    nroots: int
+    nfibers: int
+    nnz: int
+    l1: DeviceArray
+    l2_pos: DeviceArray
+    l2_crd: DeviceArray
+    l3_pos: DeviceArray
+    l3_crd: DeviceArray
+    vals: DeviceArray
+
+    @classmethod
+    def upload(cls, ctx: Context, x: CSF3) -> "DeviceCSF":
+        up = lambda a, t: DeviceArray.from_host(ctx, np.ascontiguousarray(a, dtype=t))
+        return cls(len(x.l1_crd), len(x.l2_crd), x.nnz, up(x.l1_crd, np.int32), up(x.l2_pos, np.int32),
+                   up(x.l2_crd, np.int32), up(x.l3_pos, np.int32), up(x.l3_crd, np.int32), up(x.vals, np.float64))
+
+
+def mttkrp(ctx: Context, X: DeviceCSF, I: int, J: int, K: int, R: int, Bf: DeviceArray, Cf: DeviceArray):
+    """MTTKRP over a CSF with levels (j, k, i); returns (rows with a nonzero,
+    dense M view I x R) -- both context-owned device arrays."""
+    nrows = C.c_int32()
+    pr, pm = C.c_void_p(), C.c_void_p()
+    check(lib().spws_mttkrp(ctx.h, int(I), int(J), int(K), int(R), X.nroots, X.l1.ptr, X.l2_pos.ptr, X.l2_crd.ptr,
+                            X.nfibers, X.l3_pos.ptr, X.l3_crd.ptr, X.vals.ptr, X.nnz, Bf.ptr, Cf.ptr,
+                            C.byref(nrows), C.byref(pr), C.byref(pm)))
+    return (DeviceArray.view(ctx, pr.value, nrows.value, np.int32),
+            DeviceArray.view(ctx, pm.value, int(I) * int(R), np.float64))
+
+
+def ttm(ctx: Context, X: DeviceCSF, I: int, J: int, K: int, R: int, U: DeviceArray):
+    """TTM over a CSF with levels (i, k, j); returns (fib_i, fib_j, Y) views,
+    fibers sorted by (i, j), Y nfib x R."""
+    nf = C.c_int64()
+    pi, pj, py = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    check(lib().spws_ttm(ctx.h, int(I), int(J), int(K), int(R), X.nroots, X.l1.ptr, X.l2_pos.ptr, X.l2_crd.ptr,
+                         X.nfibers, X.l3_pos.ptr, X.l3_crd.ptr, X.vals.ptr, X.nnz, U.ptr, C.byref(nf),
+                         C.byref(pi), C.byref(pj), C.byref(py)))
+    F = nf.value
+    return (DeviceArray.view(ctx, pi.value, F, np.int32), DeviceArray.view(ctx, pj.value, F, np.int32),
+            DeviceArray.view(ctx, py.value, F * int(R), np.float64))

@@ -150,40 +150,43 @@ class COO3:
         return len(self.vals)
 
 
-def zipf_coo3(dims, nnz: int, exps, seed: int, stream: int, lo: float, hi: float) -> COO3:
-    """3-order COO with exactly ``nnz`` distinct coordinates.
+_GEN = None
+
+
+def _genlib():
+    global _GEN
+    if _GEN is None:
+        import ctypes as C
+        import os
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libspws_gen.so")
+        if not os.path.exists(path):
+            from .build import build_gen
+            build_gen()
+        L = C.CDLL(path)
+        L.spwsgen_coo3.restype = C.c_int64
+        L.spwsgen_coo3.argtypes = [C.c_int64] * 3 + [C.c_double] * 3 + [C.c_int64, C.c_uint64, C.c_uint64,
+                                   C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_int]
+        _GEN = L
+    return _GEN
+
+
+def zipf_coo3(dims, nnz: int, exps, seed: int, stream: int, lo: float, hi: float, threads: int = 0) -> COO3:
+    """3-order COO with exactly ``nnz`` distinct coordinates (native generator,
+    sparsews_amd/native/gen_coo3.c).
 
     Mode m draws Zipf(exps[m]) ranks mapped by a seeded permutation of
-    [0, dims[m]); ``exps[m] = None`` draws uniformly.  Duplicates are removed and
-    the deficit redrawn until exactly ``nnz`` distinct triples remain (the
-    first ``nnz`` distinct triples in draw order).  Values are U[lo, hi).
+    [0, dims[m]); ``exps[m] = None`` draws uniformly.  The coordinates are the
+    first ``nnz`` distinct triples in draw order, sorted lexicographically;
+    values are U[lo, hi).  The result does not depend on the thread count.
     """
-    rng = rng_for(seed, stream)
     d0, d1, d2 = (int(x) for x in dims)
-    perms = [rng.permutation(d).astype(np.int64) for d in (d0, d1, d2)]
-
-    def draw(count):
-        cols = []
-        for m, d in enumerate((d0, d1, d2)):
-            if exps[m] is None:
-                cols.append(rng.integers(0, d, count, dtype=np.int64))
-            else:
-                cols.append(zipf_index(rng, count, d, exps[m], perms[m]))
-        return (cols[0] * d1 + cols[1]) * d2 + cols[2]
-
-    keys = np.unique(draw(nnz))
-    while len(keys) < nnz:
-        deficit = nnz - len(keys)
-        cand = np.unique(draw(max(deficit * 2, 1024)))
-        new = np.setdiff1d(cand, keys, assume_unique=True)
-        if len(new) > deficit:
-            new = rng.choice(new, deficit, replace=False)
-        keys = np.union1d(keys, new)
-    k2 = keys % d2
-    k1 = (keys // d2) % d1
-    k0 = keys // (d1 * d2)
-    idx = np.stack([k0, k1, k2], axis=1).astype(np.int32)
-    vals = rng.uniform(lo, hi, nnz)
+    idx = np.empty((int(nnz), 3), dtype=np.int32)
+    vals = np.empty(int(nnz), dtype=np.float64)
+    s = [float(e) if e is not None else 0.0 for e in exps]
+    got = _genlib().spwsgen_coo3(d0, d1, d2, s[0], s[1], s[2], int(nnz), int(seed), int(stream),
+                                 float(lo), float(hi), idx.ctypes.data, vals.ctypes.data, int(threads))
+    if got != nnz:
+        raise ValueError("the index space cannot hold that many distinct coordinates")
     return COO3((d0, d1, d2), idx, vals)
 
 

@@ -138,3 +138,72 @@ def CSR_CSR_hash(At: Taco, Bt: Taco, Ct: Taco, w_cap: int, warmup: int, repeat: 
     if print_:
         print(taco_to_csr(Ct))
     return dt
+
+
+# --------------------------------------------------------------------------- CC, CSF and COO tensors
+
+def CC_to_taco_tensor(d, nrows: int, ncols: int) -> Taco:
+    """DCSR / DCSC taco tensor (CC_to_taco_tensor, utils/dataloader.h:504-569)."""
+    t = Taco(2, [nrows, ncols])
+    t.set_index(0, 0, np.array([0, len(d.l1_crd)], dtype=np.int32))
+    t.set_index(0, 1, d.l1_crd)
+    t.set_index(1, 0, d.l2_pos)
+    t.set_index(1, 1, d.l2_crd)
+    t.set_vals(d.vals)
+    return t
+
+
+def csf_to_taco_tensor(x) -> Taco:
+    """3-level CSF taco tensor: level l holds (pos, crd); the root pos is {0, n}."""
+    t = Taco(3, [x.dims[o] for o in x.order])
+    t.set_index(0, 0, np.array([0, len(x.l1_crd)], dtype=np.int32))
+    t.set_index(0, 1, x.l1_crd)
+    t.set_index(1, 0, x.l2_pos)
+    t.set_index(1, 1, x.l2_crd)
+    t.set_index(2, 0, x.l3_pos)
+    t.set_index(2, 1, x.l3_crd)
+    t.set_vals(x.vals)
+    return t
+
+
+def init_taco_tensor_COO(dims) -> Taco:
+    """COO output tensor (init_taco_tensor_COO, utils/dataloader.h:964-982)."""
+    t = Taco(len(dims), dims)
+    t.set_index(0, 0, np.array([dims[0]], dtype=np.int32))
+    return t
+
+
+def taco_coo(t: Taco) -> tuple:
+    """(coordinate arrays..., vals) of a COO output; frees the callee's arrays."""
+    n = int(t.t.vals_size)
+    L = lib()
+    out = []
+    for lvl in range(1, int(t.t.order) + 1):
+        out.append(t.index(lvl, 0, n))
+        p = t._level_arrays[lvl][0]
+        if p:
+            L.spws_host_free(C.cast(p, C.c_void_p))
+            t._level_arrays[lvl][0] = _P32()
+    out.append(t.vals(n))
+    if t.t.vals:
+        L.spws_host_free(C.cast(t.t.vals, C.c_void_p))
+        t.t.vals = C.POINTER(C.c_double)()
+    return tuple(out)
+
+
+def dcsc_dcsr_compute(Ct: Taco, At: Taco, Bt: Taco, w_cap: int) -> int:
+    """Outer-product coordinate workspace (DCSC_DCSR_coord_index_chase.h:194)."""
+    check(lib().spws_dcsc_dcsr_coord_compute(Ct.ref, At.ref, Bt.ref, int(w_cap)))
+    return 0
+
+
+def mttkrp_compute(At: Taco, Bt: Taco, Ct: Taco, Dt: Taco, w_cap: int) -> int:
+    """MTTKRP hash workspace (COO_CSF_DCSR_DCSR_hash_free.h:233)."""
+    check(lib().spws_mttkrp_hash_compute(At.ref, Bt.ref, Ct.ref, Dt.ref, int(w_cap)))
+    return 0
+
+
+def ttm_compute(At: Taco, Bt: Taco, Ct: Taco, w_cap: int) -> int:
+    """TTM hash_il workspace (COO_CSF_DCSR_hash_il.h:247)."""
+    check(lib().spws_ttm_hash_il_compute(At.ref, Bt.ref, Ct.ref, int(w_cap)))
+    return 0

@@ -1,0 +1,165 @@
+"""MTTKRP and TTM on the GPU vs the CPU oracle, through the C ABI.
+
+Structure (output rows / fibers) must be bit-exact; values within
+|x - y| <= 1e-12 * max(|y|, sum|terms|).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from sparsews_amd import formats, gen, taco
+from sparsews_amd.device import Context, DeviceArray, DeviceCSF, mttkrp, ttm
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def dense(X):
+    D = np.zeros(X.dims)
+    D[X.idx[:, 0], X.idx[:, 1], X.idx[:, 2]] = X.vals
+    return D
+
+
+def run_mttkrp(ctx, X, Bf, Cf):
+    R = Bf.shape[1]
+    csf = formats.csf3_from_coo(X, (1, 2, 0))
+    rows, M = mttkrp(ctx, DeviceCSF.upload(ctx, csf), X.dims[0], X.dims[1], X.dims[2], R,
+                     DeviceArray.from_host(ctx, np.ascontiguousarray(Bf.ravel())),
+                     DeviceArray.from_host(ctx, np.ascontiguousarray(Cf.ravel())))
+    return rows.to_host(), M.to_host().reshape(X.dims[0], R)
+
+
+def run_ttm(ctx, X, U):
+    R = U.shape[1]
+    csf = formats.csf3_from_coo(X, (0, 2, 1))
+    fi, fj, Y = ttm(ctx, DeviceCSF.upload(ctx, csf), X.dims[0], X.dims[1], X.dims[2], R,
+                    DeviceArray.from_host(ctx, np.ascontiguousarray(U.ravel())))
+    return fi.to_host(), fj.to_host(), Y.to_host().reshape(-1, R)
+
+
+@pytest.mark.parametrize("R", [32, 8, 48])
+def test_mttkrp_small(ctx, R):
+    print(f"case: mttkrp small R={R}", flush=True)
+    X = gen.zipf_coo3((400, 300, 200), 20000, (1.5, 1.5, 1.5), 11, 5, 0.0, 1.0)
+    rng = np.random.default_rng(R)
+    Bf, Cf = rng.random((300, R)), rng.random((200, R))
+    rows, M = run_mttkrp(ctx, X, Bf, Cf)
+    csf = formats.csf3_from_coo(X, (1, 2, 0))
+    oi, orr, ov = oracle.mttkrp_hash(csf, formats.dense_as_dcsr(Bf), formats.dense_as_dcsr(Cf), 400, R, 1 << 16)
+    np.testing.assert_array_equal(np.repeat(rows, R), oi)
+    absM = np.einsum("ijk,jr,kr->ir", np.abs(dense(X)), np.abs(Bf), np.abs(Cf))
+    assert oracle.max_value_error(M[rows].ravel(), ov, absM[rows].ravel()) <= TOL
+    # rows without a nonzero stay exactly zero
+    empty = np.setdiff1d(np.arange(400), rows)
+    assert np.all(M[empty] == 0)
+
+
+def test_mttkrp_taco_entry():
+    print("case: mttkrp taco entry", flush=True)
+    X = gen.zipf_coo3((120, 90, 70), 3000, (1.5, 1.5, 1.5), 12, 5, 0.0, 1.0)
+    R = 16
+    rng = np.random.default_rng(12)
+    Bf, Cf = rng.random((90, R)), rng.random((70, R))
+    csf = formats.csf3_from_coo(X, (1, 2, 0))
+    At = taco.init_taco_tensor_COO([120, R])
+    Bt = taco.csf_to_taco_tensor(csf)
+    Ct = taco.CC_to_taco_tensor(formats.dense_as_dcsr(Bf), 90, R)
+    Dt = taco.CC_to_taco_tensor(formats.dense_as_dcsr(Cf), 70, R)
+    taco.mttkrp_compute(At, Bt, Ct, Dt, 1 << 10)
+    gi, gr, gv = taco.taco_coo(At)
+    oi, orr, ov = oracle.mttkrp_hash(csf, formats.dense_as_dcsr(Bf), formats.dense_as_dcsr(Cf), 120, R, 1 << 10)
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gr, orr)
+    absM = np.einsum("ijk,jr,kr->ir", np.abs(dense(X)), np.abs(Bf), np.abs(Cf))
+    assert oracle.max_value_error(gv, ov, absM[oi, orr]) <= TOL
+
+
+@pytest.mark.parametrize("R", [16, 5])
+def test_ttm_small(ctx, R):
+    print(f"case: ttm small R={R}", flush=True)
+    X = gen.zipf_coo3((500, 400, 60), 20000, (1.5, 1.5, None), 13, 5, -1.0, 1.0)
+    U = np.random.default_rng(R).uniform(-1, 1, (60, R))
+    fi, fj, Y = run_ttm(ctx, X, U)
+    csf = formats.csf3_from_coo(X, (0, 2, 1))
+    ok, oj, ol, ov = oracle.ttm_hash_il(csf, formats.dense_as_dcsr(U), 500, 400, R)
+    np.testing.assert_array_equal(np.repeat(fi, R), ok)
+    np.testing.assert_array_equal(np.repeat(fj, R), oj)
+    absY = np.einsum("ijk,kr->ijr", np.abs(dense(X)), np.abs(U))
+    assert oracle.max_value_error(Y.ravel(), ov, absY[fi, fj].ravel()) <= TOL
+
+
+def test_ttm_taco_entry():
+    print("case: ttm taco entry", flush=True)
+    X = gen.zipf_coo3((80, 70, 30), 2500, (1.5, 1.5, None), 14, 5, -1.0, 1.0)
+    R = 16
+    U = np.random.default_rng(14).uniform(-1, 1, (30, R))
+    csf = formats.csf3_from_coo(X, (0, 2, 1))
+    At = taco.init_taco_tensor_COO([80, 70, R])
+    taco.ttm_compute(At, taco.csf_to_taco_tensor(csf), taco.CC_to_taco_tensor(formats.dense_as_dcsr(U), 30, R), 1)
+    gk, gj, gl, gv = taco.taco_coo(At)
+    ok, oj, ol, ov = oracle.ttm_hash_il(csf, formats.dense_as_dcsr(U), 80, 70, R)
+    np.testing.assert_array_equal(gk, ok)
+    np.testing.assert_array_equal(gj, oj)
+    np.testing.assert_array_equal(gl, ol)
+    absY = np.einsum("ijk,kr->ijr", np.abs(dense(X)), np.abs(U))
+    assert oracle.max_value_error(gv, ov, absY[ok, oj, ol]) <= TOL
+
+
+def test_tensor_edge_cases(ctx):
+    print("case: tensor edge cases", flush=True)
+    X = gen.zipf_coo3((10, 9, 8), 1, (None, None, None), 15, 5, 0.5, 1.0)
+    Bf, Cf = np.ones((9, 4)), np.ones((8, 4))
+    rows, M = run_mttkrp(ctx, X, Bf, Cf)
+    np.testing.assert_array_equal(rows, [X.idx[0, 0]])
+    np.testing.assert_array_equal(M[X.idx[0, 0]], np.full(4, X.vals[0]))
+    fi, fj, Y = run_ttm(ctx, X, np.ones((8, 3)))
+    np.testing.assert_array_equal(fi, [X.idx[0, 0]])
+    np.testing.assert_array_equal(fj, [X.idx[0, 1]])
+
+
+def sample_rows_reference(X, Bf, Cf, rows):
+    """M(rows, :) from the nonzeros of the sampled rows, and sum|terms|."""
+    sel = np.isin(X.idx[:, 0], rows)
+    xi, xj, xk, xv = X.idx[sel, 0], X.idx[sel, 1], X.idx[sel, 2], X.vals[sel]
+    where = np.searchsorted(rows, xi)
+    R = Bf.shape[1]
+    ref = np.zeros((len(rows), R))
+    ab = np.zeros((len(rows), R))
+    t = (xv[:, None] * Bf[xj]) * Cf[xk]
+    np.add.at(ref, where, t)
+    np.add.at(ab, where, np.abs(t))
+    return ref, ab
+
+
+def test_mttkrp_full_workload(ctx):
+    """50,000 x 40,000 x 30,000, 20M nonzeros, R = 32: present rows exact and
+    256 sampled rows against a float reference."""
+    print("case: mttkrp full workload (generating)", flush=True)
+    X, Bf, Cf = gen.mttkrp_tensor()
+    print("case: mttkrp full workload (running)", flush=True)
+    rows, M = run_mttkrp(ctx, X, Bf, Cf)
+    np.testing.assert_array_equal(rows, np.unique(X.idx[:, 0]))
+    samp = np.sort(np.random.default_rng(5).choice(rows, 256, replace=False))
+    ref, ab = sample_rows_reference(X, Bf, Cf, samp)
+    assert oracle.max_value_error(M[samp].ravel(), ref.ravel(), ab.ravel()) <= TOL
+
+
+def test_ttm_full_workload(ctx):
+    """100,000 x 80,000 x 2,000, 10M nonzeros, R = 16: fiber list exact and
+    sampled fibers against a float reference."""
+    print("case: ttm full workload (generating)", flush=True)
+    X, U = gen.ttm_tensor()
+    print("case: ttm full workload (running)", flush=True)
+    fi, fj, Y = run_ttm(ctx, X, U)
+    key = X.idx[:, 0].astype(np.int64) * X.dims[1] + X.idx[:, 1]
+    fib = np.unique(key)
+    np.testing.assert_array_equal(fi.astype(np.int64) * X.dims[1] + fj, fib)
+    samp = np.sort(np.random.default_rng(6).choice(len(fib), 512, replace=False))
+    sel = np.isin(key, fib[samp])
+    where = np.searchsorted(fib[samp], key[sel])
+    t = X.vals[sel, None] * U[X.idx[sel, 2]]
+    ref = np.zeros((len(samp), U.shape[1]))
+    ab = np.zeros_like(ref)
+    np.add.at(ref, where, t)
+    np.add.at(ab, where, np.abs(t))
+    assert oracle.max_value_error(Y[samp].ravel(), ref.ravel(), ab.ravel()) <= TOL
