@@ -13,6 +13,9 @@ Workloads (sizes in the docstring of each runner):
   outer     (default) outer-product DCSC x DCSR SpGEMM, 200,000 x 200,000,
             Zipf 1.8 row lengths, A = B^T
   rowwise   row-wise CSR x CSR hash SpGEMM, 10,000 x 10,000, 8 per row
+  mttkrp    mode-1 MTTKRP, 50k x 40k x 30k, 20M nonzeros, rank 32
+  ttm       mode-3 TTM, 100k x 80k x 2k, 10M nonzeros, U 2,000 x 16
+  serve     4,096 SpGEMM requests, plain vs continuously batched
 """
 from __future__ import annotations
 
@@ -242,9 +245,215 @@ def outer_cpu_baseline(B, seconds):
     return omp, seq
 
 
-WORKLOADS = {"rowwise": run_rowwise, "outer": run_outer}
+# --------------------------------------------------------------------------- tensors
+
+def _dominant(times, per_step, steps):
+    dom = max(per_step, key=lambda k: times.get(k, (0, 1))[0])
+    ms, launches = times[dom]
+    achieved = per_step[dom] * steps / (ms / 1e3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+            "avg_launch_us": round(ms / launches * 1e3, 3),
+            "algorithmic_bytes_per_launch": int(per_step[dom] * steps / launches)}
+
+
+def run_mttkrp(args):
+    """Mode-1 MTTKRP, X 50,000 x 40,000 x 30,000 with 20M distinct nonzeros
+    (per-mode Zipf 1.5 indices), FP64 U[0,1); dense factors of rank 32.  The
+    CSF (levels j, k, i: output mode as leaf, as the reference lays it out)
+    is resident in HBM; a step is the whole MTTKRP from that layout."""
+    from sparsews_amd import formats, gen
+    from sparsews_amd.device import Context, DeviceArray, DeviceCSF, mttkrp
+    X, Bf, Cf = gen.mttkrp_tensor()
+    I, J, K = X.dims
+    R = Bf.shape[1]
+    csf = formats.csf3_from_coo(X, (1, 2, 0))
+    ctx = Context(0)
+    dX = DeviceCSF.upload(ctx, csf)
+    dB = DeviceArray.from_host(ctx, np.ascontiguousarray(Bf.ravel()))
+    dC = DeviceArray.from_host(ctx, np.ascontiguousarray(Cf.ravel()))
+    out = {}
+
+    def step():
+        out["r"] = mttkrp(ctx, dX, I, J, K, R, dB, dC)
+
+    sec, times = timed_steps(ctx, step, args.steps, args.warmup)
+    nnz = X.nnz
+    nrows = out["r"][0].count
+    passes = times["radix_scatter"][1] // args.steps if "radix_scatter" in times else 0
+    per_step = {
+        # sorted (key, value) records + factor matrices once + the output rows
+        "mttkrp_accum": 16 * nnz + 8 * R * (J + K) + 8 * R * nrows,
+        "radix_scatter": 2 * 16 * nnz * passes,
+        "mttkrp_keys": 4 * nnz + 4 * nnz + 8 * nnz,
+    }
+    compulsory = 20 * nnz + 8 * R * (J + K) + 8 * R * I
+    res = {
+        "metric": "MTTKRP GFLOP/s", "value": round(3 * R * nnz / sec / 1e9, 3), "unit": "GFLOP/s",
+        "ms_per_step": round(sec * 1e3, 4), "dtype": "f64",
+        "config": {"workload": "mttkrp_mode1_50k_40k_30k_20M_R32", "dims": [I, J, K], "nnz": nnz, "rank": R,
+                   "zipf_s": 1.5, "rows_out": int(nrows), "seed": 0},
+        "roofline": _dominant(times, per_step, args.steps),
+        "compulsory_bytes": int(compulsory),
+        "compulsory_frac_of_hbm_roofline": round(compulsory / sec / 1e9 / HBM_PEAK_GBS, 5),
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(times.items())},
+    }
+    ctx.close()
+    if not args.no_cpu_baseline:
+        import oracle
+        from sparsews_amd.gen import COO3
+        rng = np.random.default_rng(9)
+        sel = np.sort(rng.choice(nnz, 1_000_000, replace=False))
+        Xs = COO3(X.dims, X.idx[sel], X.vals[sel])
+        cs = formats.csf3_from_coo(Xs, (1, 2, 0))
+        w_cap = 1 << int(np.log2(I * R * 0.1))
+        t0 = time.perf_counter()
+        oracle.mttkrp_hash(cs, formats.dense_as_dcsr(Bf), formats.dense_as_dcsr(Cf), I, R, w_cap)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(3 * R * Xs.nnz / dt / 1e9, 4), "unit": "GFLOP/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"1,000,000 random nonzeros of the tensor, sequential hash MTTKRP "
+                                         f"(COO_CSF_DCSR_DCSR_hash_free.h, w_cap {w_cap}), {dt:.1f} s"}
+    return res
+
+
+def run_ttm(args):
+    """TTM along mode 3, X 100,000 x 80,000 x 2,000 with 10M distinct nonzeros
+    (modes 1-2 Zipf 1.5, mode 3 uniform), FP64 U[-1,1], times U 2,000 x 16.
+    The CSF (levels i, k, j: contracted mode in the middle) is resident; the
+    output is the sorted (i, j) fiber list with dense length-16 vectors."""
+    from sparsews_amd import formats, gen
+    from sparsews_amd.device import Context, DeviceArray, DeviceCSF, ttm
+    X, U = gen.ttm_tensor()
+    I, J, K = X.dims
+    R = U.shape[1]
+    csf = formats.csf3_from_coo(X, (0, 2, 1))
+    ctx = Context(0)
+    dX = DeviceCSF.upload(ctx, csf)
+    dU = DeviceArray.from_host(ctx, np.ascontiguousarray(U.ravel()))
+    out = {}
+
+    def step():
+        out["r"] = ttm(ctx, dX, I, J, K, R, dU)
+
+    sec, times = timed_steps(ctx, step, args.steps, args.warmup)
+    nnz = X.nnz
+    F = out["r"][0].count
+    per_step = {
+        # per nonzero: leaf crd + value + slot + fiber id read; per fiber: R values written
+        "ttm_accum": nnz * (4 + 8 + 4 + 4 + 4) + 8 * R * F + 8 * R * K,
+        "ttm_insert": nnz * (4 + 4 + 4 + 4),
+        "radix_scatter": 0,
+    }
+    compulsory = 20 * nnz + 8 * R * K + F * (8 + 8 * R)
+    res = {
+        "metric": "TTM GFLOP/s", "value": round(2 * R * nnz / sec / 1e9, 3), "unit": "GFLOP/s",
+        "ms_per_step": round(sec * 1e3, 4), "dtype": "f64",
+        "config": {"workload": "ttm_mode3_100k_80k_2k_10M_R16", "dims": [I, J, K], "nnz": nnz, "rank": R,
+                   "fibers_out": int(F), "seed": 0},
+        "roofline": _dominant(times, per_step, args.steps),
+        "compulsory_bytes": int(compulsory),
+        "compulsory_frac_of_hbm_roofline": round(compulsory / sec / 1e9 / HBM_PEAK_GBS, 5),
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(times.items())},
+    }
+    ctx.close()
+    if not args.no_cpu_baseline:
+        import oracle
+        from sparsews_amd.gen import COO3
+        rng = np.random.default_rng(10)
+        sel = np.sort(rng.choice(nnz, 1_000_000, replace=False))
+        Xs = COO3(X.dims, X.idx[sel], X.vals[sel])
+        cs = formats.csf3_from_coo(Xs, (0, 2, 1))
+        t0 = time.perf_counter()
+        oracle.ttm_hash_il(cs, formats.dense_as_dcsr(U), I, J, R)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(2 * R * Xs.nnz / dt / 1e9, 4), "unit": "GFLOP/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"1,000,000 random nonzeros of the tensor, sequential hash_il TTM "
+                                         f"(COO_CSF_DCSR_hash_il.h), {dt:.1f} s"}
+    return res
+
+
+# --------------------------------------------------------------------------- batched / served
+
+def run_serve(args):
+    """Batched / served SpGEMM: a stream of requests A_b * B_b, square n_b
+    log-uniform in [256, 65,536], half with uniform 2..64 nonzeros per row
+    and half Zipf(1.8) capped at 4,096, FP64 U[-1,1], seed 0.  Operands are
+    resident in HBM before the run.  Measured: saturation throughput (all
+    requests queued at t = 0) of the plain path (one row-wise call per
+    request) and of the continuously batched path (2 engines, one hipGraph
+    launch per batch); then Poisson arrivals at 80% of the plain path's
+    capacity for both, and at 3x that rate for the batched path, with
+    p50/p99 latency (finish - arrival)."""
+    from sparsews_amd import gen
+    from sparsews_amd.device import Context
+    from sparsews_amd.serve import DeviceRequest, Engine, latency_summary, serve
+    stream = gen.request_stream(count=args.requests, seed=0)
+    ctx = Context(0)
+    t0 = time.perf_counter()
+    reqs, products = [], 0
+    for r in stream:
+        A, B = gen.request_operands(r)
+        d = DeviceRequest.upload(ctx, A, B)
+        reqs.append(d)
+        products += d.products
+        del A, B
+        if len(reqs) % 512 == 0:
+            log(f"serve: {len(reqs)} requests generated and uploaded ({time.perf_counter() - t0:.0f} s)")
+    N = len(reqs)
+    engines = [Engine(ctx) for _ in range(2)]
+    zero = np.zeros(N)
+    # warm-up: one pass of each mode on a prefix
+    serve(ctx, [], reqs[:32], zero[:32], "plain")
+    serve(ctx, engines, reqs[:32], zero[:32], "batched")
+    f_p, _, st_p = serve(ctx, [], reqs, zero, "plain")
+    f_b, bid, st_b = serve(ctx, engines, reqs, zero, "batched")
+    sat_plain = N / st_p["wall_seconds"]
+    sat_batch = N / st_b["wall_seconds"]
+    log(f"serve: saturation plain {sat_plain:.1f} req/s, batched {sat_batch:.1f} req/s")
+    rng = np.random.default_rng(12345)
+    lam = 0.8 * sat_plain
+    gaps = rng.exponential(1.0 / lam, N)
+    arr = np.cumsum(gaps) - gaps[0]
+    f_pl, _, st_pl = serve(ctx, [], reqs, arr, "plain")
+    f_bl, _, st_bl = serve(ctx, engines, reqs, arr, "batched")
+    arr3 = arr / 3.0
+    f_b3, _, st_b3 = serve(ctx, engines, reqs, arr3, "batched")
+    for e in engines:
+        e.close()
+    ctx.close()
+    sizes = np.array([r.products for r in reqs])
+    big = sizes >= np.percentile(sizes, 99)
+    lat_b3 = (f_b3 - arr3) * 1e3
+    res = {
+        "metric": "batched SpGEMM req/s", "value": round(sat_batch, 2), "unit": "req/s",
+        "ms_per_step": round(st_b["wall_seconds"] * 1e3, 3), "dtype": "f64",
+        "config": {"workload": "served_spgemm_4096_requests", "requests": N, "n_range": [256, 65536],
+                   "products_total": int(products), "products_median": int(np.median(sizes)),
+                   "products_q99": int(np.percentile(sizes, 99)), "engines": 2, "seed": 0},
+        "saturation": {"plain_req_per_s": round(sat_plain, 2), "batched_req_per_s": round(sat_batch, 2),
+                       "speedup": round(sat_batch / sat_plain, 3), "batches": st_b["batches"],
+                       "fallback_requests": st_b["fallback_requests"],
+                       "plain_products_per_s": round(products / st_p["wall_seconds"] / 1e9, 3),
+                       "batched_products_per_s": round(products / st_b["wall_seconds"] / 1e9, 3),
+                       "unit_products": "G products/s"},
+        "poisson_at_80pct_plain": {"rate_req_per_s": round(lam, 2),
+                                   "plain": {**latency_summary(arr, f_pl),
+                                             "req_per_s": round(N / st_pl["wall_seconds"], 2)},
+                                   "batched": {**latency_summary(arr, f_bl),
+                                               "req_per_s": round(N / st_bl["wall_seconds"], 2),
+                                               "batches": st_bl["batches"]}},
+        "poisson_batched_at_3x": {"rate_req_per_s": round(3 * lam, 2), **latency_summary(arr3, f_b3),
+                                  "req_per_s": round(N / st_b3["wall_seconds"], 2), "batches": st_b3["batches"],
+                                  "p99_ms_excluding_top1pct_requests": round(float(np.percentile(lat_b3[~big], 99)), 4)},
+    }
+    return res
+
+
+WORKLOADS = {"rowwise": run_rowwise, "outer": run_outer, "mttkrp": run_mttkrp, "ttm": run_ttm, "serve": run_serve}
 DEFAULT_WORKLOAD = "outer"
-DEFAULT_STEPS = {"rowwise": (50, 5), "outer": (5, 1)}
+DEFAULT_STEPS = {"rowwise": (50, 5), "outer": (5, 1), "mttkrp": (20, 3), "ttm": (20, 3), "serve": (1, 1)}
 
 
 def main(argv=None):
@@ -255,6 +464,7 @@ def main(argv=None):
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--requests", type=int, default=4096, help="serve workload: number of requests")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "

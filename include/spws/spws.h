@@ -164,6 +164,89 @@ int spws_ttm(spws_ctx* ctx, int32_t I, int32_t J, int32_t K, int32_t R,
              int32_t nfibers, const int32_t* l3_pos, const int32_t* l3_crd, const double* vals, int64_t nnz,
              const double* U, int64_t* nfib, int32_t** fib_i, int32_t** fib_j, double** Y);
 
+/* -------------------------------------------- batched and served SpGEMM */
+
+/* A batch engine runs many row-wise SpGEMMs C_b = A_b * B_b (device operands,
+ * int32 CSR, FP64) in one hipGraph launch: the requests are stacked into one
+ * block-diagonal product, whose rows are binned together, so the per-entry
+ * arithmetic is that of spws_spgemm_rowwise.  The engine's pools are sized
+ * once by its capacities; a request is admitted while every pool holds it
+ * (products = its upper bound of output entries = sum over A's entries of
+ * the matching B row lengths, max_row_products = the largest row's).  Each
+ * engine owns a stream; engines run concurrently.  No counterpart in the
+ * reference, which has no batching. */
+typedef struct spws_batch spws_batch;
+
+typedef struct {
+  int32_t max_requests;      /* requests per batch (<= 65535) */
+  int32_t max_rows;          /* sum of A row counts */
+  int32_t max_brows;         /* sum of B row counts */
+  int32_t reserved;
+  int64_t max_annz;          /* sum of nnz(A_b) */
+  int64_t max_bnnz;          /* sum of nnz(B_b) */
+  int64_t max_products;      /* sum of upper bounds of output entries (<= 2^31 - 1) */
+  int64_t max_row_products;  /* largest row upper bound of any request (<= 2^28) */
+} spws_batch_caps;
+
+int spws_batch_create(spws_ctx* ctx, const spws_batch_caps* caps, spws_batch** out);
+int spws_batch_destroy(spws_batch* b);
+/* Empty the batch (only while no launch is in flight). */
+int spws_batch_reset(spws_batch* b);
+/* Add one request; SPWS_ERR_OOM when the batch cannot hold it (full),
+ * SPWS_ERR_UNSUPPORTED when a row exceeds max_row_products.  *slot names the
+ * request in spws_batch_result.  The operand arrays must stay valid until
+ * the batch completes. */
+int spws_batch_add(spws_batch* b, int32_t m, int32_t k, int32_t n, const int32_t* A_pos, const int32_t* A_crd,
+                   const double* A_vals, int64_t annz, const int32_t* B_pos, const int32_t* B_crd,
+                   const double* B_vals, int64_t bnnz, int64_t products, int64_t max_row_products, int32_t* slot);
+/* Launch the batch's graph on the engine's stream (asynchronous). */
+int spws_batch_launch(spws_batch* b);
+/* *done = 1 once the launched batch has completed. */
+int spws_batch_poll(spws_batch* b, int* done);
+int spws_batch_wait(spws_batch* b);
+/* Total output nonzeros of the completed batch. */
+int spws_batch_nnz(spws_batch* b, int64_t* nnz);
+/* Result of one request of a completed batch: its row pointers rebased to 0
+ * (host array of m+1, may be NULL), nnz, and device pointers to its columns
+ * and values (engine-owned, valid until the engine's next launch). */
+int spws_batch_result(spws_batch* b, int32_t slot, int64_t* nnz, int32_t* pos_host, int32_t** crd, double** vals);
+void* spws_batch_stream(spws_batch* b);
+
+/* One request of the serving loop: device operands, the upper bounds used by
+ * admission, and the arrival time in seconds from the start of the run. */
+typedef struct {
+  int32_t m, k, n, reserved;
+  const int32_t* A_pos;
+  const int32_t* A_crd;
+  const double* A_vals;
+  int64_t annz;
+  const int32_t* B_pos;
+  const int32_t* B_crd;
+  const double* B_vals;
+  int64_t bnnz;
+  int64_t products;
+  int64_t max_row_products;
+  double arrival;
+} spws_request;
+
+typedef struct {
+  int32_t mode;                /* 0: plain (one row-wise call per request); 1: continuously batched */
+  int32_t max_batch_requests;  /* batched: requests per batch (0 = engine capacity) */
+} spws_serve_options;
+
+typedef struct {
+  double wall_seconds;
+  int64_t batches;
+  int64_t fallback_requests;   /* batched mode: requests run through the plain path */
+} spws_serve_stats;
+
+/* Run the request loop in real time: requests (sorted by arrival) arrive at
+ * their arrival times; finish[r] receives the time (s) the loop saw request
+ * r complete; batch_of[r] (optional) the batch that carried it (-1 = plain).
+ * Batched mode uses the nengines engines.  No counterpart in the reference. */
+int spws_serve(spws_ctx* ctx, spws_batch** engines, int32_t nengines, int32_t nreq, const spws_request* reqs,
+               const spws_serve_options* opt, double* finish, int32_t* batch_of, spws_serve_stats* stats);
+
 /* Host TACO entry points (host taco_tensor_t in, malloc'd host arrays out). */
 
 /* int compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B, int32_t w_cap)

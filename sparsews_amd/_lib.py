@@ -44,6 +44,27 @@ class RowwiseTuning(C.Structure):
                 ("block2_ub", C.c_int32), ("dense_ratio", C.c_int32), ("force_path", C.c_int32)]
 
 
+class BatchCaps(C.Structure):
+    _fields_ = [("max_requests", C.c_int32), ("max_rows", C.c_int32), ("max_brows", C.c_int32),
+                ("reserved", C.c_int32), ("max_annz", C.c_int64), ("max_bnnz", C.c_int64),
+                ("max_products", C.c_int64), ("max_row_products", C.c_int64)]
+
+
+class Request(C.Structure):
+    _fields_ = [("m", C.c_int32), ("k", C.c_int32), ("n", C.c_int32), ("reserved", C.c_int32),
+                ("A_pos", C.c_void_p), ("A_crd", C.c_void_p), ("A_vals", C.c_void_p), ("annz", C.c_int64),
+                ("B_pos", C.c_void_p), ("B_crd", C.c_void_p), ("B_vals", C.c_void_p), ("bnnz", C.c_int64),
+                ("products", C.c_int64), ("max_row_products", C.c_int64), ("arrival", C.c_double)]
+
+
+class ServeOptions(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("max_batch_requests", C.c_int32)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("wall_seconds", C.c_double), ("batches", C.c_int64), ("fallback_requests", C.c_int64)]
+
+
 class Stats(C.Structure):
     _fields_ = [("products", C.c_int64), ("nnz", C.c_int64), ("rows_per_path", C.c_int64 * 8),
                 ("chunks", C.c_int64), ("sort_bits", C.c_int64)]
@@ -85,6 +106,19 @@ SIGNATURES = {
                               C.POINTER(I32), PVP, PVP]),
     "spws_ttm": (C.c_int, [VP, I32, I32, I32, I32, I32, VP, VP, VP, I32, VP, VP, VP, I64, VP,
                            PI64, PVP, PVP, PVP]),
+    "spws_batch_create": (C.c_int, [VP, C.POINTER(BatchCaps), C.POINTER(VP)]),
+    "spws_batch_destroy": (C.c_int, [VP]),
+    "spws_batch_reset": (C.c_int, [VP]),
+    "spws_batch_add": (C.c_int, [VP, I32, I32, I32, VP, VP, VP, I64, VP, VP, VP, I64, I64, I64,
+                                 C.POINTER(I32)]),
+    "spws_batch_launch": (C.c_int, [VP]),
+    "spws_batch_poll": (C.c_int, [VP, C.POINTER(C.c_int)]),
+    "spws_batch_wait": (C.c_int, [VP]),
+    "spws_batch_nnz": (C.c_int, [VP, PI64]),
+    "spws_batch_result": (C.c_int, [VP, I32, PI64, VP, PVP, PVP]),
+    "spws_batch_stream": (VP, [VP]),
+    "spws_serve": (C.c_int, [VP, VP, I32, I32, C.POINTER(Request), C.POINTER(ServeOptions), VP, VP,
+                             C.POINTER(ServeStats)]),
     "spws_csr_csr_hash_compute": (C.c_int, [C.POINTER(TacoTensor)] * 3 + [I32]),
     "spws_csr_csr_assemble": (C.c_int, [C.POINTER(TacoTensor)] * 3),
     "spws_csr_csr_compute": (C.c_int, [C.POINTER(TacoTensor)] * 3),

@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libspws.so")
-GEN_SRC = os.path.join(HERE, "native", "gen_coo3.c")
+GEN_SRC = os.path.join(HERE, "native", "gen_native.c")
 GEN_LIB = os.path.join(HERE, "libspws_gen.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -1,0 +1,457 @@
+// Batched row-wise SpGEMM: many products of uneven sizes in one hipGraph
+// launch, and the request loop (plain and continuously batched) on top.
+//
+// A batch stacks its requests into one block-diagonal product: request b's
+// A rows occupy rows [row0_b, row0_b + m_b) of the batch, its B rows occupy
+// [brow0_b, brow0_b + k_b), and its output columns are shifted by col0_b.
+// Row-wise SpGEMM of blockdiag(A_b) * blockdiag(B_b) is blockdiag(A_b * B_b),
+// so the row binning of the single-product path load-balances across all
+// requests at once, and the arithmetic of every entry is unchanged.
+//
+// The pipeline is captured once per engine into a hipGraph whose kernels read
+// every size from a device-side descriptor and request table; a launch copies
+// the (pinned) table and descriptor in, so the same graph executes any batch
+// that fits the engine's capacities:
+//   H2D request table + descriptor
+//   k_batch_gather     operands -> engine pools, indices rebased
+//   k_meta_clear, k_rowstat, device-sized scan, k_bin_offsets, k_binscatter
+//   numeric kernels of the wave / wave2 / block / block2 / wide paths
+//   (persistent grids; each reads its row count from the bin metadata)
+//   device-sized scan -> C_pos, k_compact (columns shifted back per row)
+//   k_batch_total, D2H of the batch nnz
+// No memset is recorded into the graph: clears are kernels.  The dense path
+// is not used in batches: the stacked column space is the sum of the
+// requests' widths, so long rows take the HBM hash path instead.
+#include <algorithm>
+#include <chrono>
+#include <thread>
+#include <vector>
+
+#include "rowwise_kernels.hpp"
+
+struct spws_batch_req {  // device-visible per-request descriptor
+  const int32_t* A_pos;
+  const int32_t* A_crd;
+  const double* A_val;
+  const int32_t* B_pos;
+  const int32_t* B_crd;
+  const double* B_val;
+  int32_t m, k, n;
+  int32_t row0, brow0, col0;
+  int64_t annz0, bnnz0, annz, bnnz;
+};
+
+struct spws_batch_desc {
+  int32_t nreq, rows, brows, cols;
+  int64_t annz, bnnz;
+};
+
+enum BatchPool : int {
+  BP_REQ = 0, BP_DESC, BP_APOS, BP_ACRD, BP_AVAL, BP_BPOS, BP_BCRD, BP_BVAL, BP_COLOFF, BP_ROWUB, BP_ROWOFF,
+  BP_ROWNNZ, BP_BINROWS, BP_META, BP_SCRD, BP_SVAL, BP_CPOS, BP_CCRD, BP_CVAL, BP_WKEY, BP_WVAL, BP_SCAN, BP_TOTAL,
+  BP_COUNT
+};
+
+struct spws_batch {
+  spws_ctx* ctx = nullptr;
+  hipStream_t stream = nullptr;
+  spws_batch_caps caps{};
+  spws::DevBuf pool[BP_COUNT];
+  spws_batch_req* h_req = nullptr;
+  spws_batch_desc* h_desc = nullptr;
+  int64_t* h_total = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t done = nullptr;
+  bool in_flight = false;
+  spws_batch_desc cur{};
+  int64_t cur_ub = 0;
+  int64_t wide_slots = 0;
+  int32_t wide_groups = 0;
+  spws::PathParams pp{};
+};
+
+namespace spws {
+namespace {
+
+constexpr int kDScanBlocks = 256;
+constexpr int kDScanThreads = 256;
+constexpr int kDScanItems = 16;
+constexpr int kDScanTile = kDScanThreads * kDScanItems;
+
+__device__ __forceinline__ void dscan_range(int64_t n, int64_t* beg, int64_t* end) {
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  *beg = (int64_t)blockIdx.x * chunk;
+  *end = *beg + chunk < n ? *beg + chunk : n;
+  if (*beg > n) *beg = n;
+}
+
+// Exclusive scan whose length lives in device memory: each of a fixed number
+// of blocks owns a contiguous range (reduce, scan of the partials, rescan).
+template <class Tin, class Tout>
+__global__ __launch_bounds__(kDScanThreads) void k_dscan_reduce(const Tin* __restrict__ in,
+                                                                const int32_t* __restrict__ n_dev,
+                                                                Tout* __restrict__ part) {
+  __shared__ Tout sh[kDScanThreads / kWave + 1];
+  int64_t beg, end;
+  dscan_range(*n_dev, &beg, &end);
+  Tout s = 0;
+  for (int64_t i = beg + threadIdx.x; i < end; i += kDScanThreads) s += (Tout)in[i];
+  s = wave_sum(s);
+  if (lane_id() == 0) sh[threadIdx.x / kWave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Tout t = 0;
+    for (int w = 0; w < kDScanThreads / kWave; ++w) t += sh[w];
+    part[blockIdx.x] = t;
+  }
+}
+
+template <class Tout>
+__global__ __launch_bounds__(kDScanThreads) void k_dscan_top(Tout* __restrict__ part) {
+  __shared__ Tout sh[kDScanThreads / kWave + 1];
+  static_assert(kDScanBlocks == kDScanThreads, "one partial per thread");
+  const Tout v = part[threadIdx.x];
+  Tout total;
+  const Tout pre = block_excl_scan<Tout, kDScanThreads>(v, sh, &total);
+  part[threadIdx.x] = pre;
+  if (threadIdx.x == 0) part[kDScanBlocks] = total;
+}
+
+template <class Tin, class Tout>
+__global__ __launch_bounds__(kDScanThreads) void k_dscan_apply(const Tin* __restrict__ in,
+                                                               const int32_t* __restrict__ n_dev,
+                                                               const Tout* __restrict__ part, Tout* __restrict__ out) {
+  __shared__ Tout sh[kDScanThreads / kWave + 1];
+  const int64_t n = *n_dev;
+  int64_t beg, end;
+  dscan_range(n, &beg, &end);
+  Tout carry = part[blockIdx.x];
+  for (int64_t base = beg; base < end; base += kDScanTile) {
+    Tout v[kDScanItems];
+    Tout local = 0;
+    const int64_t mine = base + (int64_t)threadIdx.x * kDScanItems;
+#pragma unroll
+    for (int it = 0; it < kDScanItems; ++it) {
+      const int64_t idx = mine + it;
+      v[it] = idx < end ? (Tout)in[idx] : (Tout)0;
+      local += v[it];
+    }
+    Tout total;
+    const Tout pre = block_excl_scan<Tout, kDScanThreads>(local, sh, &total);
+    Tout run = carry + pre;
+#pragma unroll
+    for (int it = 0; it < kDScanItems; ++it) {
+      const int64_t idx = mine + it;
+      if (idx < end) out[idx] = run;
+      run += v[it];
+    }
+    carry += total;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = part[kDScanBlocks];
+}
+
+// operands of every request into the pools, indices rebased
+__global__ __launch_bounds__(256) void k_batch_gather(const spws_batch_req* __restrict__ reqs,
+                                                      const spws_batch_desc* __restrict__ desc,
+                                                      int32_t* __restrict__ A_pos, int32_t* __restrict__ A_crd,
+                                                      double* __restrict__ A_val, int32_t* __restrict__ B_pos,
+                                                      int32_t* __restrict__ B_crd, double* __restrict__ B_val,
+                                                      int32_t* __restrict__ row_coloff) {
+  const int32_t r = blockIdx.y;
+  if (r >= desc->nreq) return;
+  const spws_batch_req q = reqs[r];
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = t0; i <= q.m; i += st) {
+    A_pos[q.row0 + i] = (int32_t)(q.annz0 + q.A_pos[i]);
+    if (i < q.m) row_coloff[q.row0 + i] = q.col0;
+  }
+  for (int64_t i = t0; i <= q.k; i += st) B_pos[q.brow0 + i] = (int32_t)(q.bnnz0 + q.B_pos[i]);
+  for (int64_t p = t0; p < q.annz; p += st) {
+    A_crd[q.annz0 + p] = q.A_crd[p] + q.brow0;
+    A_val[q.annz0 + p] = q.A_val[p];
+  }
+  for (int64_t p = t0; p < q.bnnz; p += st) {
+    B_crd[q.bnnz0 + p] = q.B_crd[p] + q.col0;
+    B_val[q.bnnz0 + p] = q.B_val[p];
+  }
+}
+
+__global__ void k_batch_total(const int32_t* __restrict__ C_pos, const spws_batch_desc* __restrict__ desc,
+                              int64_t* __restrict__ total) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *total = C_pos[desc->rows];
+}
+
+}  // namespace
+
+template <class Tin, class Tout>
+static void dscan(hipStream_t s, const Tin* in, const int32_t* n_dev, Tout* part, Tout* out) {
+  hipLaunchKernelGGL((k_dscan_reduce<Tin, Tout>), dim3(kDScanBlocks), dim3(kDScanThreads), 0, s, in, n_dev, part);
+  hipLaunchKernelGGL((k_dscan_top<Tout>), dim3(1), dim3(kDScanThreads), 0, s, part);
+  hipLaunchKernelGGL((k_dscan_apply<Tin, Tout>), dim3(kDScanBlocks), dim3(kDScanThreads), 0, s, in, n_dev, part, out);
+}
+
+// Enqueue the whole batch pipeline on b->stream (captured into the graph).
+static void enqueue_batch(spws_batch* b) {
+  hipStream_t s = b->stream;
+  const spws_batch_caps& c = b->caps;
+  auto P = [&](int i) { return b->pool[i].ptr; };
+  spws_batch_req* d_req = (spws_batch_req*)P(BP_REQ);
+  spws_batch_desc* d_desc = (spws_batch_desc*)P(BP_DESC);
+  (void)hipMemcpyAsync(d_req, b->h_req, sizeof(spws_batch_req) * (size_t)c.max_requests, hipMemcpyHostToDevice, s);
+  (void)hipMemcpyAsync(d_desc, b->h_desc, sizeof(spws_batch_desc), hipMemcpyHostToDevice, s);
+  int32_t* Ap = (int32_t*)P(BP_APOS);
+  int32_t* Ai = (int32_t*)P(BP_ACRD);
+  double* Ax = (double*)P(BP_AVAL);
+  int32_t* Bp = (int32_t*)P(BP_BPOS);
+  int32_t* Bi = (int32_t*)P(BP_BCRD);
+  double* Bx = (double*)P(BP_BVAL);
+  int32_t* coloff = (int32_t*)P(BP_COLOFF);
+  hipLaunchKernelGGL(k_batch_gather, dim3(64, c.max_requests), dim3(256), 0, s, d_req, d_desc, Ap, Ai, Ax, Bp, Bi, Bx,
+                     coloff);
+  const int32_t* m_dev = &d_desc->rows;
+  int64_t* row_ub = (int64_t*)P(BP_ROWUB);
+  int64_t* row_off = (int64_t*)P(BP_ROWOFF);
+  int32_t* row_nnz = (int32_t*)P(BP_ROWNNZ);
+  int32_t* bin_rows = (int32_t*)P(BP_BINROWS);
+  int32_t* bmeta = (int32_t*)P(BP_META);
+  unsigned long long* maxub = (unsigned long long*)(bmeta + 32);
+  int32_t* scrd = (int32_t*)P(BP_SCRD);
+  double* sval = (double*)P(BP_SVAL);
+  int32_t* Cp = (int32_t*)P(BP_CPOS);
+  int32_t* Ci = (int32_t*)P(BP_CCRD);
+  double* Cx = (double*)P(BP_CVAL);
+  const unsigned grows = (unsigned)std::min<int64_t>(div_up(c.max_rows, 256), 4096);
+  hipLaunchKernelGGL(k_meta_clear, dim3(1), dim3(64), 0, s, bmeta, maxub);
+  hipLaunchKernelGGL(k_rowstat, dim3(grows), dim3(256), 0, s, 0, m_dev, 1, Ap, Ai, Bp, row_ub, bmeta, maxub, b->pp);
+  dscan<int64_t, int64_t>(s, row_ub, m_dev, (int64_t*)P(BP_SCAN), row_off);
+  hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(64), 0, s, bmeta);
+  hipLaunchKernelGGL(k_binscatter, dim3(grows), dim3(256), 0, s, 0, m_dev, 1, row_ub, bmeta, bin_rows, row_nnz, b->pp);
+  const unsigned gwave = (unsigned)std::min<int64_t>(c.max_rows, 8192);
+  const unsigned gblk = (unsigned)std::min<int64_t>(c.max_rows, 1024);
+  hipLaunchKernelGGL((k_row_wave<128, 1, true>), dim3(gwave), dim3(64), 0, s, bin_rows, bmeta, (int)P_WAVE, Ap, Ai, Ax,
+                     Bp, Bi, Bx, row_off, scrd, sval, row_nnz);
+  hipLaunchKernelGGL((k_row_wave<512, 4, true>), dim3(gwave), dim3(64), 0, s, bin_rows, bmeta, (int)P_WAVE2, Ap, Ai,
+                     Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz);
+  hipLaunchKernelGGL((k_row_block<2048, 1024, 256, true>), dim3(gblk), dim3(256), 0, s, bin_rows, bmeta, (int)P_BLOCK,
+                     Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz);
+  hipLaunchKernelGGL((k_row_block<8192, 4096, 512, true>), dim3(gblk), dim3(512), 0, s, bin_rows, bmeta,
+                     (int)P_BLOCK2, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz);
+  hipLaunchKernelGGL((k_row_wide<true>), dim3(b->wide_groups), dim3(kWideBlock), 0, s, bin_rows, bmeta, (int)P_WIDE,
+                     Ap, Ai, Ax, Bp, Bi, Bx, row_ub, row_off, scrd, sval, row_nnz, (int32_t*)P(BP_WKEY),
+                     (double*)P(BP_WVAL), b->wide_slots);
+  dscan<int32_t, int32_t>(s, row_nnz, m_dev, (int32_t*)P(BP_SCAN), Cp);
+  hipLaunchKernelGGL(k_compact, dim3((unsigned)std::min<int64_t>(div_up(c.max_rows, 4), 16384)), dim3(256), 0, s, 0,
+                     m_dev, row_off, Cp, scrd, sval, coloff, Ci, Cx);
+  hipLaunchKernelGGL(k_batch_total, dim3(1), dim3(64), 0, s, Cp, d_desc, (int64_t*)P(BP_TOTAL));
+  (void)hipMemcpyAsync(b->h_total, P(BP_TOTAL), 8, hipMemcpyDeviceToHost, s);
+}
+
+static void batch_release(spws_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->done) (void)hipEventDestroy(b->done);
+  for (DevBuf& p : b->pool) p.release();
+  if (b->h_req) (void)hipHostFree(b->h_req);
+  if (b->h_desc) (void)hipHostFree(b->h_desc);
+  if (b->h_total) (void)hipHostFree(b->h_total);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+static int batch_build(spws_batch* b) {
+  const spws_batch_caps& c = b->caps;
+  const size_t rows = (size_t)c.max_rows, brows = (size_t)c.max_brows;
+  const size_t annz = (size_t)c.max_annz, bnnz = (size_t)c.max_bnnz, ub = (size_t)c.max_products;
+  b->wide_slots = std::max<int64_t>(64, next_pow2_u32((uint32_t)(2 * c.max_row_products)));
+  b->wide_groups = 64;
+  const size_t sizes[BP_COUNT] = {
+      sizeof(spws_batch_req) * (size_t)c.max_requests, sizeof(spws_batch_desc), (rows + 1) * 4, annz * 4, annz * 8,
+      (brows + 1) * 4, bnnz * 4, bnnz * 8, rows * 4, rows * 8, (rows + 1) * 8, rows * 4, rows * 4, 64 * 8, ub * 4,
+      ub * 8, (rows + 1) * 4, ub * 4, ub * 8, (size_t)b->wide_groups * b->wide_slots * 4,
+      (size_t)b->wide_groups * b->wide_slots * 8, (kDScanBlocks + 1) * 8, 8};
+  int rc;
+  for (int i = 0; i < BP_COUNT; ++i)
+    if ((rc = b->pool[i].ensure(sizes[i] > 0 ? sizes[i] : 8))) return rc;
+  SPWS_HIP(hipHostMalloc((void**)&b->h_req, sizeof(spws_batch_req) * (size_t)c.max_requests, hipHostMallocDefault));
+  SPWS_HIP(hipHostMalloc((void**)&b->h_desc, sizeof(spws_batch_desc), hipHostMallocDefault));
+  SPWS_HIP(hipHostMalloc((void**)&b->h_total, 8, hipHostMallocDefault));
+  memset(b->h_req, 0, sizeof(spws_batch_req) * (size_t)c.max_requests);
+  memset(b->h_desc, 0, sizeof(spws_batch_desc));
+  SPWS_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  SPWS_HIP(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+  SPWS_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  enqueue_batch(b);
+  hipError_t e = hipStreamEndCapture(b->stream, &b->graph);
+  if (e != hipSuccess) return fail_hip(e, "hipStreamEndCapture (batch pipeline)", __FILE__, __LINE__);
+  SPWS_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return SPWS_OK;
+}
+
+}  // namespace spws
+
+using namespace spws;
+
+extern "C" {
+
+int spws_batch_create(spws_ctx* ctx, const spws_batch_caps* caps, spws_batch** out) {
+  int rc = ctx_activate(ctx);
+  if (rc) return rc;
+  SPWS_REQUIRE(caps && out, SPWS_ERR_INVALID, "null argument");
+  SPWS_REQUIRE(caps->max_requests >= 1 && caps->max_requests <= 65535 && caps->max_rows >= 1 &&
+                   caps->max_brows >= 1 && caps->max_annz >= 1 && caps->max_bnnz >= 1 && caps->max_products >= 1 &&
+                   caps->max_row_products >= 1,
+               SPWS_ERR_INVALID, "batch capacities must be positive (max_requests <= 65535)");
+  SPWS_REQUIRE(caps->max_products <= INT32_MAX && caps->max_annz <= INT32_MAX && caps->max_bnnz <= INT32_MAX &&
+                   caps->max_row_products <= (1 << 28),
+               SPWS_ERR_UNSUPPORTED, "batch capacities exceed the int32 index range");
+  spws_batch* b = new spws_batch();
+  b->ctx = ctx;
+  b->caps = *caps;
+  const spws_rowwise_tuning& t = ctx->rw;
+  // dense_ratio 0: the dense path is never chosen for the stacked product
+  b->pp = PathParams{t.wave_ub, t.wave2_ub, t.block_ub, t.block2_ub, 0, -1};
+  rc = batch_build(b);
+  if (rc) {
+    batch_release(b);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return SPWS_OK;
+}
+
+int spws_batch_destroy(spws_batch* b) {
+  if (!b) return SPWS_OK;
+  (void)hipSetDevice(b->ctx->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_release(b);
+  delete b;
+  return SPWS_OK;
+}
+
+int spws_batch_reset(spws_batch* b) {
+  SPWS_REQUIRE(b != nullptr, SPWS_ERR_INVALID, "null batch");
+  SPWS_REQUIRE(!b->in_flight, SPWS_ERR_INVALID, "batch is still running");
+  b->cur = spws_batch_desc{};
+  b->cur_ub = 0;
+  return SPWS_OK;
+}
+
+int spws_batch_add(spws_batch* b, int32_t m, int32_t k, int32_t n, const int32_t* A_pos, const int32_t* A_crd,
+                   const double* A_vals, int64_t annz, const int32_t* B_pos, const int32_t* B_crd,
+                   const double* B_vals, int64_t bnnz, int64_t products, int64_t max_row_products, int32_t* slot) {
+  SPWS_REQUIRE(b != nullptr, SPWS_ERR_INVALID, "null batch");
+  SPWS_REQUIRE(!b->in_flight, SPWS_ERR_INVALID, "batch is still running");
+  SPWS_REQUIRE(m >= 0 && k >= 0 && n >= 0 && annz >= 0 && bnnz >= 0 && products >= 0, SPWS_ERR_INVALID,
+               "negative size");
+  const spws_batch_caps& c = b->caps;
+  spws_batch_desc& d = b->cur;
+  // memory-based admission: every pool must hold the request (products is the
+  // upper bound of its output entries)
+  if (max_row_products > c.max_row_products) {
+    set_error("a row of %lld products exceeds the engine's row capacity", (long long)max_row_products);
+    return SPWS_ERR_UNSUPPORTED;
+  }
+  if (d.nreq + 1 > c.max_requests || (int64_t)d.rows + m > c.max_rows || (int64_t)d.brows + k > c.max_brows ||
+      d.annz + annz > c.max_annz || d.bnnz + bnnz > c.max_bnnz || b->cur_ub + products > c.max_products ||
+      (int64_t)d.cols + n > INT32_MAX) {
+    set_error("the batch is full");
+    return SPWS_ERR_OOM;
+  }
+  spws_batch_req& q = b->h_req[d.nreq];
+  q.A_pos = A_pos;
+  q.A_crd = A_crd;
+  q.A_val = A_vals;
+  q.B_pos = B_pos;
+  q.B_crd = B_crd;
+  q.B_val = B_vals;
+  q.m = m;
+  q.k = k;
+  q.n = n;
+  q.row0 = d.rows;
+  q.brow0 = d.brows;
+  q.col0 = d.cols;
+  q.annz0 = d.annz;
+  q.bnnz0 = d.bnnz;
+  q.annz = annz;
+  q.bnnz = bnnz;
+  if (slot) *slot = d.nreq;
+  d.nreq += 1;
+  d.rows += m;
+  d.brows += k;
+  d.cols += n;
+  d.annz += annz;
+  d.bnnz += bnnz;
+  b->cur_ub += products;
+  return SPWS_OK;
+}
+
+int spws_batch_launch(spws_batch* b) {
+  SPWS_REQUIRE(b != nullptr, SPWS_ERR_INVALID, "null batch");
+  SPWS_REQUIRE(!b->in_flight, SPWS_ERR_INVALID, "batch is already running");
+  int rc = ctx_activate(b->ctx);
+  if (rc) return rc;
+  *b->h_desc = b->cur;
+  *b->h_total = -1;
+  SPWS_HIP(hipGraphLaunch(b->exec, b->stream));
+  SPWS_HIP(hipEventRecord(b->done, b->stream));
+  b->in_flight = true;
+  return SPWS_OK;
+}
+
+int spws_batch_poll(spws_batch* b, int* done) {
+  SPWS_REQUIRE(b && done, SPWS_ERR_INVALID, "null argument");
+  if (!b->in_flight) {
+    *done = 1;
+    return SPWS_OK;
+  }
+  hipError_t e = hipEventQuery(b->done);
+  if (e == hipErrorNotReady) {
+    *done = 0;
+    return SPWS_OK;
+  }
+  if (e != hipSuccess) return fail_hip(e, "hipEventQuery (batch)", __FILE__, __LINE__);
+  b->in_flight = false;
+  *done = 1;
+  return SPWS_OK;
+}
+
+int spws_batch_wait(spws_batch* b) {
+  SPWS_REQUIRE(b != nullptr, SPWS_ERR_INVALID, "null batch");
+  if (!b->in_flight) return SPWS_OK;
+  SPWS_HIP(hipEventSynchronize(b->done));
+  b->in_flight = false;
+  return SPWS_OK;
+}
+
+int spws_batch_nnz(spws_batch* b, int64_t* nnz) {
+  SPWS_REQUIRE(b && nnz, SPWS_ERR_INVALID, "null argument");
+  SPWS_REQUIRE(!b->in_flight, SPWS_ERR_INVALID, "batch is still running");
+  *nnz = *b->h_total;
+  return SPWS_OK;
+}
+
+int spws_batch_result(spws_batch* b, int32_t slot, int64_t* nnz, int32_t* pos_host, int32_t** crd, double** vals) {
+  SPWS_REQUIRE(b != nullptr, SPWS_ERR_INVALID, "null batch");
+  SPWS_REQUIRE(!b->in_flight, SPWS_ERR_INVALID, "batch is still running");
+  SPWS_REQUIRE(slot >= 0 && slot < b->cur.nreq, SPWS_ERR_INVALID, "slot %d out of range", slot);
+  int rc = ctx_activate(b->ctx);
+  if (rc) return rc;
+  const spws_batch_req& q = b->h_req[slot];
+  const int32_t* Cp = b->pool[BP_CPOS].as<int32_t>();
+  std::vector<int32_t> tmp((size_t)q.m + 1);
+  SPWS_HIP(hipMemcpyAsync(tmp.data(), Cp + q.row0, ((size_t)q.m + 1) * 4, hipMemcpyDeviceToHost, b->stream));
+  SPWS_HIP(hipStreamSynchronize(b->stream));
+  const int32_t base = tmp[0];
+  if (pos_host)
+    for (int32_t i = 0; i <= q.m; ++i) pos_host[i] = tmp[i] - base;
+  if (nnz) *nnz = tmp[q.m] - base;
+  if (crd) *crd = b->pool[BP_CCRD].as<int32_t>() + base;
+  if (vals) *vals = b->pool[BP_CVAL].as<double>() + base;
+  return SPWS_OK;
+}
+
+void* spws_batch_stream(spws_batch* b) { return b ? (void*)b->stream : nullptr; }
+
+}  // extern "C"

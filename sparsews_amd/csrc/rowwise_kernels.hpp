@@ -1,0 +1,656 @@
+// Row-wise SpGEMM device code shared by the single-product driver
+// (spgemm_rowwise.hip) and the batched engine (batch.hip).  See
+// spgemm_rowwise.hip for the pipeline and the reference it follows.
+#pragma once
+#include <climits>
+
+#include "common.hpp"
+
+namespace spws {
+namespace {
+
+enum RowPath : int { P_WAVE = 0, P_WAVE2 = 1, P_BLOCK = 2, P_BLOCK2 = 3, P_DENSE = 4, P_WIDE = 5, P_EMPTY = 6 };
+constexpr int kPaths = 7;
+
+// compiled workspace capacities (max ub per path; tables hold 2x slots)
+constexpr int kWaveUbMax = 64;
+constexpr int kWave2UbMax = 256;
+constexpr int kBlockUbMax = 1024;
+constexpr int kBlock2UbMax = 4096;
+constexpr int kWideLdsSort = 4096;  // wide rows with nnz <= this sort in LDS
+
+struct PathParams {
+  int32_t wave_ub, wave2_ub, block_ub, block2_ub, dense_ratio, force;
+};
+
+__host__ __device__ __forceinline__ int classify_row(int64_t ub, int32_t ncols, const PathParams& p) {
+  if (ub == 0) return P_EMPTY;
+  if (p.force >= 0) return p.force;
+  if (ub <= p.wave_ub) return P_WAVE;
+  if (ub <= p.wave2_ub) return P_WAVE2;
+  if (ub <= p.block_ub) return P_BLOCK;
+  if (ub <= p.block2_ub) return P_BLOCK2;
+  if (ub * (int64_t)p.dense_ratio >= (int64_t)ncols) return P_DENSE;
+  return P_WIDE;
+}
+
+// Bin metadata on the device (int32 bmeta): [0, 8) rows per path, [8, 16)
+// offset of each path's rows in bin_rows, [16, 24) scatter cursors; maxub
+// (uint64): [0, 8) largest row upper bound per path.
+constexpr int kMetaCount = 0, kMetaOff = 8, kMetaCursor = 16, kMetaInts = 24;
+
+__device__ __forceinline__ int32_t dev_or(const int32_t* p, int32_t v) { return p ? *p : v; }
+
+__global__ __launch_bounds__(256) void k_rowstat(int32_t m_host, const int32_t* __restrict__ m_dev, int32_t ncols,
+                                                 const int32_t* __restrict__ Ap, const int32_t* __restrict__ Ai,
+                                                 const int32_t* __restrict__ Bp, int64_t* __restrict__ row_ub,
+                                                 int32_t* __restrict__ bmeta, unsigned long long* __restrict__ maxub,
+                                                 PathParams pp) {
+  __shared__ int cnt[8];
+  __shared__ unsigned long long mx[8];
+  if (threadIdx.x < 8) {
+    cnt[threadIdx.x] = 0;
+    mx[threadIdx.x] = 0;
+  }
+  __syncthreads();
+  const int32_t m = dev_or(m_dev, m_host);
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+    int64_t ub = 0;
+    const int32_t e = Ap[i + 1];
+    for (int32_t p = Ap[i]; p < e; ++p) {
+      const int32_t j = Ai[p];
+      ub += Bp[j + 1] - Bp[j];
+    }
+    row_ub[i] = ub;
+    const int path = classify_row(ub, ncols, pp);
+    atomicAdd(&cnt[path], 1);
+    atomicMax(&mx[path], (unsigned long long)ub);
+  }
+  __syncthreads();
+  if (threadIdx.x < kPaths && cnt[threadIdx.x]) {
+    atomicAdd(&bmeta[kMetaCount + threadIdx.x], cnt[threadIdx.x]);
+    atomicMax(&maxub[threadIdx.x], mx[threadIdx.x]);
+  }
+}
+
+// offsets of the paths' row lists (exclusive prefix of the counts)
+__global__ void k_bin_offsets(int32_t* __restrict__ bmeta) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    int32_t run = 0;
+    for (int p = 0; p < 8; ++p) {
+      bmeta[kMetaOff + p] = run;
+      if (p < P_EMPTY) run += bmeta[kMetaCount + p];
+    }
+  }
+}
+
+// zero the bin metadata (a kernel, so the sequence can be graph-captured)
+__global__ void k_meta_clear(int32_t* __restrict__ bmeta, unsigned long long* __restrict__ maxub) {
+  const int t = threadIdx.x;
+  if (t < kMetaInts) bmeta[t] = 0;
+  if (t < 8) maxub[t] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_binscatter(int32_t m_host, const int32_t* __restrict__ m_dev, int32_t ncols,
+                                                    const int64_t* __restrict__ row_ub, int32_t* __restrict__ bmeta,
+                                                    int32_t* __restrict__ bin_rows, int32_t* __restrict__ row_nnz,
+                                                    PathParams pp) {
+  const int32_t m = dev_or(m_dev, m_host);
+  const int lane = lane_id();
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  // wave-uniform trip count: every lane of a wave runs the ballots together
+  const int32_t stride = gridDim.x * blockDim.x;
+  for (int32_t base = blockIdx.x * blockDim.x; base < m; base += stride) {
+    const int32_t i = base + threadIdx.x;
+    const int path = i < m ? classify_row(row_ub[i], ncols, pp) : -1;
+    if (i < m && path == P_EMPTY) row_nnz[i] = 0;
+#pragma unroll
+    for (int p = 0; p < P_EMPTY; ++p) {
+      const uint64_t mask = __ballot(path == p);
+      if (mask == 0) continue;
+      const int leader = __ffsll((long long)mask) - 1;
+      int pos = 0;
+      if (lane == leader) pos = atomicAdd(&bmeta[kMetaCursor + p], __popcll(mask));
+      pos = __shfl(pos, leader, kWave);
+      if (path == p) bin_rows[bmeta[kMetaOff + p] + pos + __popcll(mask & lt)] = i;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ product expansion
+//
+// The products of row i are enumerated by a wavefront 64 A-entries at a time:
+// each lane loads one A entry (j, a) and len(B_j); an inclusive wave scan
+// gives the product offsets, and each lane then finds, for its product
+// index q, the owning A entry by a binary search over the lanes' offsets
+// (six ds_bpermute rounds).  Loop bounds are wave-uniform, so the shuffles
+// see every lane.
+
+template <bool kValues, class F>
+__device__ __forceinline__ void expand_products(int32_t a_beg, int32_t a_end, int32_t first, int32_t stride,
+                                                const int32_t* __restrict__ Ai, const double* __restrict__ Ax,
+                                                const int32_t* __restrict__ Bp, const int32_t* __restrict__ Bi,
+                                                const double* __restrict__ Bx, F&& insert) {
+  const int lane = lane_id();
+  for (int32_t c = a_beg + first; c < a_end; c += stride) {
+    const int32_t p = c + lane;
+    int32_t bstart = 0, blen = 0;
+    double a = 0.0;
+    if (p < a_end) {
+      const int32_t j = Ai[p];
+      bstart = Bp[j];
+      blen = Bp[j + 1] - bstart;
+      if (kValues) a = Ax[p];
+    }
+    const int32_t incl = wave_incl_scan(blen);
+    const int32_t excl = incl - blen;
+    const int32_t total = __shfl(incl, kWave - 1, kWave);
+    for (int32_t q0 = 0; q0 < total; q0 += kWave) {
+      const int32_t q = q0 + lane;
+      // owner: the last lane t with excl_t <= q
+      int lo = 0, hi = kWave - 1;
+#pragma unroll
+      for (int it = 0; it < 6; ++it) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int32_t em = __shfl(excl, mid, kWave);
+        if (em <= q) lo = mid; else hi = mid - 1;
+      }
+      const int32_t o_start = __shfl(bstart, lo, kWave);
+      const int32_t o_excl = __shfl(excl, lo, kWave);
+      double o_a = 0.0;
+      if (kValues) o_a = __shfl(a, lo, kWave);
+      if (q < total) {
+        const int32_t kb = o_start + (q - o_excl);
+        const int32_t col = Bi[kb];
+        double v = 0.0;
+        if (kValues) v = o_a * Bx[kb];
+        insert(col, v);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ LDS hash
+
+template <bool kValues>
+__device__ __forceinline__ void lds_insert(int32_t* keys, double* vals, uint32_t shift, uint32_t mask,
+                                           int32_t col, double v) {
+  uint32_t h = hash_col((uint32_t)col, shift);
+  for (;;) {
+    int32_t cur = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (cur == -1) {
+      cur = atomicCAS(&keys[h], -1, col);
+      if (cur == -1) cur = col;
+    }
+    if (cur == col) {
+      if (kValues) atomicAdd(&vals[h], v);
+      return;
+    }
+    h = (h + 1) & mask;
+  }
+}
+
+__device__ __forceinline__ double lds_lookup(const int32_t* keys, const double* vals, uint32_t shift, uint32_t mask,
+                                             int32_t col) {
+  uint32_t h = hash_col((uint32_t)col, shift);
+  while (keys[h] != col) h = (h + 1) & mask;
+  return vals[h];
+}
+
+// ------------------------------------------------------------------ bitonic sorts
+//
+// "Flip" bitonic network: every comparator is ascending; stage k first
+// compares e with e ^ (k - 1), then e with e ^ j for j = k/4 .. 1.  Padding
+// with INT_MAX (or skipping partners beyond n) therefore sorts any n.
+
+// R keys per lane, element e = r * 64 + lane.
+template <int R>
+__device__ __forceinline__ void wave_bitonic_step(int32_t (&v)[R], int d) {
+  const int lane = lane_id();
+  const int dl = d & (kWave - 1);
+  const int dr = d >> 6;
+  int32_t pv[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int32_t src = v[r ^ dr];
+    pv[r] = dl ? __shfl_xor(src, dl, kWave) : src;
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int e = r * kWave + lane;
+    const int p = e ^ d;
+    v[r] = (e < p) ? min(v[r], pv[r]) : max(v[r], pv[r]);
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void wave_bitonic(int32_t (&v)[R], int n_pow2) {
+  for (int k = 2; k <= n_pow2; k <<= 1) {
+    wave_bitonic_step<R>(v, k - 1);
+    for (int j = k >> 2; j >= 1; j >>= 1) wave_bitonic_step<R>(v, j);
+  }
+}
+
+// LDS bitonic of P (power of two) keys, all threads of the block participate.
+template <int BLOCK>
+__device__ __forceinline__ void block_bitonic_lds(int32_t* list, int P) {
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int t = threadIdx.x; t < P / 2; t += BLOCK) {
+      const int half = k >> 1;
+      const int b = t / half, o = t % half;
+      const int lo = b * k + o, hi = b * k + k - 1 - o;
+      const int32_t x = list[lo], y = list[hi];
+      if (x > y) {
+        list[lo] = y;
+        list[hi] = x;
+      }
+    }
+    __syncthreads();
+    for (int j = k >> 2; j >= 1; j >>= 1) {
+      for (int t = threadIdx.x; t < P / 2; t += BLOCK) {
+        const int b = t / j, o = t % j;
+        const int lo = b * 2 * j + o, hi = lo + j;
+        const int32_t x = list[lo], y = list[hi];
+        if (x > y) {
+          list[lo] = y;
+          list[hi] = x;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ------------------------------------------------------------------ wave-per-row kernel
+//
+// One 64-thread workgroup (one wavefront) per row, TS = 2 * max ub slots.
+
+template <int TS, int R, bool kValues>
+__global__ __launch_bounds__(64) void k_row_wave(const int32_t* __restrict__ bin_rows, const int32_t* __restrict__ bmeta,
+                                                 int path,
+                                                 const int32_t* __restrict__ Ap, const int32_t* __restrict__ Ai,
+                                                 const double* __restrict__ Ax, const int32_t* __restrict__ Bp,
+                                                 const int32_t* __restrict__ Bi, const double* __restrict__ Bx,
+                                                 const int64_t* __restrict__ row_off, int32_t* __restrict__ slot_crd,
+                                                 double* __restrict__ slot_val, int32_t* __restrict__ row_nnz) {
+  static_assert(TS >= 2 * kWave * R / 2, "table too small");
+  __shared__ int32_t keys[TS];
+  __shared__ double vals[kValues ? TS : 1];
+  __shared__ int32_t list[kWave * R];
+  constexpr uint32_t kMask = TS - 1;
+  const uint32_t shift = 32 - log2_pow2(TS);
+  const int lane = lane_id();
+  const int32_t count = bmeta[kMetaCount + path];
+  const int32_t* rows = bin_rows + bmeta[kMetaOff + path];
+  for (int32_t r = blockIdx.x; r < count; r += gridDim.x) {
+    const int32_t i = rows[r];
+#pragma unroll
+    for (int s = lane; s < TS; s += kWave) {
+      keys[s] = -1;
+      if (kValues) vals[s] = 0.0;
+    }
+    __syncthreads();
+    expand_products<kValues>(Ap[i], Ap[i + 1], 0, kWave, Ai, Ax, Bp, Bi, Bx, [&](int32_t col, double v) {
+      lds_insert<kValues>(keys, vals, shift, kMask, col, v);
+    });
+    __syncthreads();
+    // compaction of the occupied slots (order irrelevant: sorted next)
+    int32_t nnz = 0;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int s0 = 0; s0 < TS; s0 += kWave) {
+      const int32_t key = keys[s0 + lane];
+      const uint64_t occ = __ballot(key != -1);
+      if (kValues && key != -1) list[nnz + __popcll(occ & lt)] = key;
+      nnz += __popcll(occ);
+    }
+    if (!kValues) {
+      if (lane == 0) row_nnz[i] = nnz;
+      __syncthreads();
+      continue;
+    }
+    __syncthreads();
+    int32_t v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int e = q * kWave + lane;
+      v[q] = e < nnz ? list[e] : INT_MAX;
+    }
+    int n_pow2 = 2;
+    while (n_pow2 < nnz) n_pow2 <<= 1;
+    wave_bitonic<R>(v, n_pow2);
+    const int64_t off = row_off[i];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int e = q * kWave + lane;
+      if (e < nnz) {
+        slot_crd[off + e] = v[q];
+        slot_val[off + e] = lds_lookup(keys, vals, shift, kMask, v[q]);
+      }
+    }
+    if (lane == 0) row_nnz[i] = nnz;
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ workgroup-per-row kernel
+
+template <int TS, int MAXN, int BLOCK, bool kValues>
+__global__ __launch_bounds__(BLOCK) void k_row_block(const int32_t* __restrict__ bin_rows,
+                                                     const int32_t* __restrict__ bmeta, int path,
+                                                     const int32_t* __restrict__ Ap, const int32_t* __restrict__ Ai,
+                                                     const double* __restrict__ Ax, const int32_t* __restrict__ Bp,
+                                                     const int32_t* __restrict__ Bi, const double* __restrict__ Bx,
+                                                     const int64_t* __restrict__ row_off,
+                                                     int32_t* __restrict__ slot_crd, double* __restrict__ slot_val,
+                                                     int32_t* __restrict__ row_nnz) {
+  __shared__ int32_t keys[TS];
+  __shared__ double vals[kValues ? TS : 1];
+  __shared__ int32_t list[kValues ? MAXN : 1];
+  __shared__ int32_t counter;
+  constexpr uint32_t kMask = TS - 1;
+  const uint32_t shift = 32 - log2_pow2(TS);
+  const int wave = threadIdx.x / kWave;
+  constexpr int kWaves = BLOCK / kWave;
+  const int32_t count = bmeta[kMetaCount + path];
+  const int32_t* rows = bin_rows + bmeta[kMetaOff + path];
+  for (int32_t r = blockIdx.x; r < count; r += gridDim.x) {
+    const int32_t i = rows[r];
+    for (int s = threadIdx.x; s < TS; s += BLOCK) {
+      keys[s] = -1;
+      if (kValues) vals[s] = 0.0;
+    }
+    if (threadIdx.x == 0) counter = 0;
+    __syncthreads();
+    expand_products<kValues>(Ap[i], Ap[i + 1], wave * kWave, BLOCK, Ai, Ax, Bp, Bi, Bx,
+                             [&](int32_t col, double v) { lds_insert<kValues>(keys, vals, shift, kMask, col, v); });
+    __syncthreads();
+    for (int s = threadIdx.x; s < TS; s += BLOCK) {
+      const int32_t key = keys[s];
+      if (key != -1) {
+        const int pos = atomicAdd(&counter, 1);
+        if (kValues) list[pos] = key;
+      }
+    }
+    __syncthreads();
+    const int32_t nnz = counter;
+    if (!kValues) {
+      if (threadIdx.x == 0) row_nnz[i] = nnz;
+      __syncthreads();
+      continue;
+    }
+    int P = 2;
+    while (P < nnz) P <<= 1;
+    for (int e = nnz + threadIdx.x; e < P; e += BLOCK) list[e] = INT_MAX;
+    __syncthreads();
+    block_bitonic_lds<BLOCK>(list, P);
+    const int64_t off = row_off[i];
+    for (int e = threadIdx.x; e < nnz; e += BLOCK) {
+      const int32_t col = list[e];
+      slot_crd[off + e] = col;
+      slot_val[off + e] = lds_lookup(keys, vals, shift, kMask, col);
+    }
+    if (threadIdx.x == 0) row_nnz[i] = nnz;
+    __syncthreads();
+    (void)kWaves;
+  }
+}
+
+// ------------------------------------------------------------------ dense accumulator kernel
+//
+// Each workgroup owns dflag[ncols] (uint8) and dval[ncols] (FP64) in HBM,
+// both zero between rows.  Columns are emitted in order, so no sort is
+// needed; the emit pass also clears what it read.
+
+constexpr int kDenseBlock = 256;
+constexpr int kDenseItems = 16;
+
+template <bool kValues>
+__global__ __launch_bounds__(kDenseBlock) void k_row_dense(const int32_t* __restrict__ bin_rows,
+                                                           const int32_t* __restrict__ bmeta, int path,
+                                                           int32_t ncols, const int32_t* __restrict__ Ap,
+                                                           const int32_t* __restrict__ Ai, const double* __restrict__ Ax,
+                                                           const int32_t* __restrict__ Bp, const int32_t* __restrict__ Bi,
+                                                           const double* __restrict__ Bx,
+                                                           const int64_t* __restrict__ row_off,
+                                                           int32_t* __restrict__ slot_crd, double* __restrict__ slot_val,
+                                                           int32_t* __restrict__ row_nnz, uint8_t* __restrict__ dflag_all,
+                                                           double* __restrict__ dval_all) {
+  __shared__ int32_t sh[kDenseBlock / kWave + 1];
+  uint8_t* dflag = dflag_all + (size_t)blockIdx.x * (size_t)ncols;
+  double* dval = kValues ? dval_all + (size_t)blockIdx.x * (size_t)ncols : nullptr;
+  const int wave = threadIdx.x / kWave;
+  const int32_t count = bmeta[kMetaCount + path];
+  const int32_t* rows = bin_rows + bmeta[kMetaOff + path];
+  for (int32_t r = blockIdx.x; r < count; r += gridDim.x) {
+    const int32_t i = rows[r];
+    expand_products<kValues>(Ap[i], Ap[i + 1], wave * kWave, kDenseBlock, Ai, Ax, Bp, Bi, Bx,
+                             [&](int32_t col, double v) {
+                               dflag[col] = 1;
+                               if (kValues) atomicAdd(&dval[col], v);
+                             });
+    __threadfence();
+    __syncthreads();
+    const int64_t off = kValues ? row_off[i] : 0;
+    int32_t run = 0;
+    for (int32_t base = 0; base < ncols; base += kDenseBlock * kDenseItems) {
+      const int32_t mine = base + threadIdx.x * kDenseItems;
+      uint32_t bits = 0;
+      int32_t cnt = 0;
+#pragma unroll
+      for (int it = 0; it < kDenseItems; ++it) {
+        const int32_t c = mine + it;
+        if (c < ncols && dflag[c]) {
+          bits |= 1u << it;
+          ++cnt;
+        }
+      }
+      int32_t tot;
+      int32_t pre = block_excl_scan<int32_t, kDenseBlock>(cnt, sh, &tot);
+      if (bits) {
+        int32_t pos = run + pre;
+#pragma unroll
+        for (int it = 0; it < kDenseItems; ++it) {
+          if (bits & (1u << it)) {
+            const int32_t c = mine + it;
+            if (kValues) {
+              slot_crd[off + pos] = c;
+              slot_val[off + pos] = dval[c];
+              dval[c] = 0.0;
+            }
+            dflag[c] = 0;
+            ++pos;
+          }
+        }
+      }
+      run += tot;
+    }
+    if (threadIdx.x == 0) row_nnz[i] = run;
+    __threadfence();
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ HBM hash kernel (long rows)
+//
+// Each workgroup owns a table of tab_slots slots (int32 key, FP64 value);
+// a row uses the first next_pow2(2*ub) of them.  After accumulation the
+// occupied slots are compacted into the row's slot region and sorted there:
+// in LDS when nnz <= kWideLdsSort, else with a global-memory bitonic network.
+
+constexpr int kWideBlock = 256;
+
+__device__ __forceinline__ void wide_insert(int32_t* keys, double* vals, uint32_t shift, uint32_t mask, int32_t col,
+                                            double v, bool with_values) {
+  uint32_t h = hash_col((uint32_t)col, shift);
+  for (;;) {
+    int32_t cur = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cur == -1) {
+      cur = atomicCAS(&keys[h], -1, col);
+      if (cur == -1) cur = col;
+    }
+    if (cur == col) {
+      if (with_values) atomicAdd(&vals[h], v);
+      return;
+    }
+    h = (h + 1) & mask;
+  }
+}
+
+template <bool kValues>
+__global__ __launch_bounds__(kWideBlock) void k_row_wide(const int32_t* __restrict__ bin_rows,
+                                                         const int32_t* __restrict__ bmeta, int path,
+                                                         const int32_t* __restrict__ Ap, const int32_t* __restrict__ Ai,
+                                                         const double* __restrict__ Ax, const int32_t* __restrict__ Bp,
+                                                         const int32_t* __restrict__ Bi, const double* __restrict__ Bx,
+                                                         const int64_t* __restrict__ row_ub,
+                                                         const int64_t* __restrict__ row_off,
+                                                         int32_t* __restrict__ slot_crd, double* __restrict__ slot_val,
+                                                         int32_t* __restrict__ row_nnz, int32_t* __restrict__ tab_keys_all,
+                                                         double* __restrict__ tab_vals_all, int64_t tab_slots) {
+  __shared__ int32_t skeys[kValues ? kWideLdsSort : 1];
+  __shared__ double svals[kValues ? kWideLdsSort : 1];
+  __shared__ int32_t counter;
+  int32_t* tkeys = tab_keys_all + (size_t)blockIdx.x * (size_t)tab_slots;
+  double* tvals = kValues ? tab_vals_all + (size_t)blockIdx.x * (size_t)tab_slots : nullptr;
+  const int wave = threadIdx.x / kWave;
+  const int32_t count = bmeta[kMetaCount + path];
+  const int32_t* rows = bin_rows + bmeta[kMetaOff + path];
+  for (int32_t r = blockIdx.x; r < count; r += gridDim.x) {
+    const int32_t i = rows[r];
+    const int64_t ub = row_ub[i];
+    uint32_t ts = next_pow2_u32((uint32_t)(2 * ub));
+    if (ts < 64) ts = 64;
+    const uint32_t mask = ts - 1;
+    const uint32_t shift = 32 - log2_pow2(ts);
+    for (uint32_t s = threadIdx.x; s < ts; s += kWideBlock) {
+      tkeys[s] = -1;
+      if (kValues) tvals[s] = 0.0;
+    }
+    if (threadIdx.x == 0) counter = 0;
+    __threadfence();
+    __syncthreads();
+    expand_products<kValues>(Ap[i], Ap[i + 1], wave * kWave, kWideBlock, Ai, Ax, Bp, Bi, Bx,
+                             [&](int32_t col, double v) { wide_insert(tkeys, tvals, shift, mask, col, v, kValues); });
+    __threadfence();
+    __syncthreads();
+    const int64_t off = kValues ? row_off[i] : 0;
+    for (uint32_t s = threadIdx.x; s < ts; s += kWideBlock) {
+      const int32_t key = __hip_atomic_load(&tkeys[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (key != -1) {
+        const int pos = atomicAdd(&counter, 1);
+        if (kValues) {
+          slot_crd[off + pos] = key;
+          slot_val[off + pos] = __hip_atomic_load(&tvals[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+    }
+    __threadfence();
+    __syncthreads();
+    const int32_t nnz = counter;
+    if (kValues && nnz > 1) {
+      int32_t* rc = slot_crd + off;
+      double* rv = slot_val + off;
+      if (nnz <= kWideLdsSort) {
+        int P = 2;
+        while (P < nnz) P <<= 1;
+        for (int e = threadIdx.x; e < P; e += kWideBlock) {
+          skeys[e] = e < nnz ? rc[e] : INT_MAX;
+          svals[e] = e < nnz ? rv[e] : 0.0;
+        }
+        __syncthreads();
+        for (int k = 2; k <= P; k <<= 1) {
+          for (int j = k >> 1; j >= 1; j >>= 1) {
+            for (int t = threadIdx.x; t < P / 2; t += kWideBlock) {
+              int lo, hi;
+              if (j == (k >> 1)) {
+                const int b = t / j, o = t % j;
+                lo = b * k + o;
+                hi = b * k + k - 1 - o;
+              } else {
+                const int b = t / j, o = t % j;
+                lo = b * 2 * j + o;
+                hi = lo + j;
+              }
+              if (skeys[lo] > skeys[hi]) {
+                const int32_t tk = skeys[lo];
+                skeys[lo] = skeys[hi];
+                skeys[hi] = tk;
+                const double tv = svals[lo];
+                svals[lo] = svals[hi];
+                svals[hi] = tv;
+              }
+            }
+            __syncthreads();
+          }
+        }
+        for (int e = threadIdx.x; e < nnz; e += kWideBlock) {
+          rc[e] = skeys[e];
+          rv[e] = svals[e];
+        }
+      } else {
+        int P = 2;
+        while (P < nnz) P <<= 1;
+        for (int k = 2; k <= P; k <<= 1) {
+          for (int j = k >> 1; j >= 1; j >>= 1) {
+            for (int t = threadIdx.x; t < P / 2; t += kWideBlock) {
+              int lo, hi;
+              if (j == (k >> 1)) {
+                const int b = t / j, o = t % j;
+                lo = b * k + o;
+                hi = b * k + k - 1 - o;
+              } else {
+                const int b = t / j, o = t % j;
+                lo = b * 2 * j + o;
+                hi = lo + j;
+              }
+              if (hi < nnz) {
+                const int32_t x = rc[lo], y = rc[hi];
+                if (x > y) {
+                  rc[lo] = y;
+                  rc[hi] = x;
+                  const double tv = rv[lo];
+                  rv[lo] = rv[hi];
+                  rv[hi] = tv;
+                }
+              }
+            }
+            __threadfence();
+            __syncthreads();
+          }
+        }
+      }
+    }
+    if (threadIdx.x == 0) row_nnz[i] = nnz;
+    __threadfence();
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ compaction
+
+__global__ __launch_bounds__(256) void k_compact(int32_t m_host, const int32_t* __restrict__ m_dev,
+                                                 const int64_t* __restrict__ row_off,
+                                                 const int32_t* __restrict__ C_pos,
+                                                 const int32_t* __restrict__ slot_crd,
+                                                 const double* __restrict__ slot_val,
+                                                 const int32_t* __restrict__ row_coloff, int32_t* __restrict__ C_crd,
+                                                 double* __restrict__ C_val) {
+  const int32_t m = dev_or(m_dev, m_host);
+  const int lane = lane_id();
+  const int32_t waves = gridDim.x * (blockDim.x / kWave);
+  for (int32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < m; i += waves) {
+    const int32_t dst = C_pos[i];
+    const int32_t n = C_pos[i + 1] - dst;
+    const int64_t src = row_off[i];
+    const int32_t shift = row_coloff ? row_coloff[i] : 0;
+    for (int32_t e = lane; e < n; e += kWave) {
+      C_crd[dst + e] = slot_crd[src + e] - shift;
+      C_val[dst + e] = slot_val[src + e];
+    }
+  }
+}
+
+
+}  // namespace
+}  // namespace spws

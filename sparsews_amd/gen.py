@@ -166,13 +166,19 @@ def _genlib():
         L.spwsgen_coo3.restype = C.c_int64
         L.spwsgen_coo3.argtypes = [C.c_int64] * 3 + [C.c_double] * 3 + [C.c_int64, C.c_uint64, C.c_uint64,
                                    C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_int]
+        L.spwsgen_row_products.restype = None
+        L.spwsgen_row_products.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_int]
+        L.spwsgen_csr.restype = C.c_int
+        L.spwsgen_csr.argtypes = [C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64,
+                                  C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_int]
         _GEN = L
     return _GEN
 
 
 def zipf_coo3(dims, nnz: int, exps, seed: int, stream: int, lo: float, hi: float, threads: int = 0) -> COO3:
     """3-order COO with exactly ``nnz`` distinct coordinates (native generator,
-    sparsews_amd/native/gen_coo3.c).
+    sparsews_amd/native/gen_native.c).
 
     Mode m draws Zipf(exps[m]) ranks mapped by a seeded permutation of
     [0, dims[m]); ``exps[m] = None`` draws uniformly.  The coordinates are the
@@ -240,9 +246,25 @@ def request_stream(count=4096, lo=256, hi=65_536, rate=2000.0, seed=0):
             for i in range(count)]
 
 
+def native_csr(n_rows: int, n_cols: int, lengths, seed: int, stream: int, threads: int = 0) -> CSR:
+    """CSR with the given row lengths, distinct uniform sorted columns and
+    FP64 U[-1,1] values (native generator, sparsews_amd/native/gen_native.c)."""
+    lengths = np.ascontiguousarray(lengths, dtype=np.int32)
+    pos = np.zeros(n_rows + 1, dtype=np.int64)
+    np.cumsum(lengths, out=pos[1:])
+    crd = np.empty(int(pos[-1]), dtype=np.int32)
+    vals = np.empty(int(pos[-1]), dtype=np.float64)
+    rc = _genlib().spwsgen_csr(n_rows, n_cols, lengths.ctypes.data, pos.ctypes.data, int(seed), int(stream),
+                               -1.0, 1.0, crd.ctypes.data, vals.ctypes.data, int(threads))
+    if rc != 0:
+        raise ValueError("a row length exceeds the number of columns")
+    return CSR(n_rows, n_cols, pos.astype(np.int32) if pos[-1] < 2**31 - 1 else pos, crd, vals)
+
+
 def request_operands(req: Request):
-    """A_b, B_b for one request: independent square matrices of size n."""
-    rng = rng_for(req.seed, 31)
+    """A_b, B_b for one request: independent square n x n CSR matrices.  Row
+    lengths are uniform in 2..64 ("uniform" requests) or Zipf(1.8) on
+    [1, min(4096, n)] ("powerlaw" requests), capped at n."""
     mats = []
     for stream in (0, 1):
         r2 = rng_for(req.seed, 40 + stream)
@@ -250,8 +272,17 @@ def request_operands(req: Request):
             lengths = r2.integers(2, 65, req.n).clip(max=req.n)
         else:
             lengths = zipf_lengths(r2, req.n, 1.8, 1, min(4096, req.n))
-        cols = distinct_sorted_columns(r2, lengths, req.n)
-        vals = uniform_values(r2, len(cols))
-        mats.append(CSR(req.n, req.n, _pos_from_lengths(lengths), cols, vals))
-    del rng
+        mats.append(native_csr(req.n, req.n, lengths, req.seed, 50 + stream))
     return mats[0], mats[1]
+
+
+def row_products(A: CSR, B: CSR, threads: int = 0):
+    """(sum of row upper bounds, largest row upper bound) of A*B."""
+    import ctypes as C
+    t, m = C.c_int64(), C.c_int64()
+    ap = np.ascontiguousarray(A.pos, dtype=np.int32)
+    ac = np.ascontiguousarray(A.crd, dtype=np.int32)
+    bp = np.ascontiguousarray(B.pos, dtype=np.int32)
+    _genlib().spwsgen_row_products(A.nrows, ap.ctypes.data, ac.ctypes.data, bp.ctypes.data, C.addressof(t),
+                                   C.addressof(m), int(threads))
+    return int(t.value), int(m.value)

@@ -1,6 +1,10 @@
 /*
- * gen_coo3.c -- seeded synthetic 3-order COO tensors with an exact count of
- * distinct nonzeros and Zipf (or uniform) per-mode indices.
+ * gen_native.c -- seeded synthetic inputs that are too large for numpy:
+ * spwsgen_coo3 (3-order COO tensors with an exact count of distinct
+ * nonzeros and Zipf or uniform per-mode indices) and spwsgen_csr (CSR rows
+ * of given lengths with distinct uniform columns).
+ *
+ * spwsgen_coo3:
  *
  * Definition of the tensor (independent of the thread count):
  *   - mode m draws rank r in [1, d_m] with P(r) proportional to r^-s_m
@@ -211,4 +215,96 @@ GEN_API int64_t spwsgen_coo3(int64_t d0, int64_t d1, int64_t d2, double s0, doub
     free(md[m].perm);
   }
   return result;
+}
+
+/* Row r gets lengths[r] distinct columns uniform in [0, n), sorted, and
+ * values U[lo, hi) without exact zeros.  Draw t of row r uses
+ * U(seed, stream, r, t); long rows (4 * len > n) are sampled by a partial
+ * Fisher-Yates shuffle of [0, n) driven by the same uniforms.  pos[nrows+1]
+ * must be the exclusive prefix of lengths (computed by the caller). */
+static int cmp_i32(const void* a, const void* b) {
+  int32_t x = *(const int32_t*)a, y = *(const int32_t*)b;
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+GEN_API int spwsgen_csr(int64_t nrows, int64_t n, const int32_t* lengths, const int64_t* pos, uint64_t seed,
+                        uint64_t stream, double lo, double hi, int32_t* crd, double* vals, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+  for (int64_t r = 0; r < nrows; r++)
+    if (lengths[r] < 0 || lengths[r] > n) return -1;
+  const uint64_t base = splitmix64(seed * 0x100000001b3ULL ^ splitmix64(stream + 0x6a09e667ULL));
+  int bad = 0;
+#pragma omp parallel reduction(| : bad)
+  {
+    int32_t* scratch = NULL;
+    int64_t scap = 0;
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t r = 0; r < nrows; r++) {
+      const int64_t L = lengths[r];
+      int32_t* out = crd + pos[r];
+      const uint64_t rb = splitmix64(base ^ splitmix64((uint64_t)r + 0x3c6ef372ULL));
+      uint64_t t = 0;
+      if (L * 4 > n) {
+        if (scap < n) {
+          free(scratch);
+          scratch = (int32_t*)malloc((size_t)n * sizeof(int32_t));
+          scap = n;
+        }
+        for (int64_t i = 0; i < n; i++) scratch[i] = (int32_t)i;
+        for (int64_t i = 0; i < L; i++) {
+          int64_t j = i + (int64_t)(u01(rb, t++) * (double)(n - i));
+          if (j >= n) j = n - 1;
+          int32_t tmp = scratch[i];
+          scratch[i] = scratch[j];
+          scratch[j] = tmp;
+        }
+        memcpy(out, scratch, (size_t)L * sizeof(int32_t));
+        qsort(out, (size_t)L, sizeof(int32_t), cmp_i32);
+      } else {
+        int64_t have = 0;
+        while (have < L) {
+          for (int64_t i = have; i < L; i++) {
+            int64_t c = (int64_t)(u01(rb, t++) * (double)n);
+            out[i] = (int32_t)(c < n ? c : n - 1);
+          }
+          qsort(out, (size_t)L, sizeof(int32_t), cmp_i32);
+          int64_t w = 0;
+          for (int64_t i = 0; i < L; i++)
+            if (w == 0 || out[i] != out[w - 1]) out[w++] = out[i];
+          have = w;
+        }
+      }
+      const uint64_t vb = splitmix64(rb ^ 0x510e527fade682d1ULL);
+      for (int64_t i = 0; i < L; i++) {
+        double v = 0.0;
+        uint64_t k = 0;
+        while (v == 0.0) v = lo + (hi - lo) * u01(vb, (uint64_t)i * 1024 + k++);
+        vals[pos[r] + i] = v;
+      }
+    }
+    free(scratch);
+  }
+  return bad ? -1 : 0;
+}
+
+/* Upper bound of the output entries of A*B (sum over A's entries of the
+ * matching B row lengths) and of its largest row: the numbers a batch's
+ * memory-based admission is decided on. */
+GEN_API void spwsgen_row_products(int64_t nrows, const int32_t* A_pos, const int32_t* A_crd, const int32_t* B_pos,
+                                  int64_t* total, int64_t* max_row, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+  int64_t t = 0, mx = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : t) reduction(max : mx)
+  for (int64_t r = 0; r < nrows; r++) {
+    int64_t s = 0;
+    for (int32_t p = A_pos[r]; p < A_pos[r + 1]; p++) s += B_pos[A_crd[p] + 1] - B_pos[A_crd[p]];
+    t += s;
+    if (s > mx) mx = s;
+  }
+  *total = t;
+  *max_row = mx;
 }

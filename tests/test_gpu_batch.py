@@ -1,0 +1,162 @@
+"""Batched SpGEMM (one hipGraph launch for many requests) and the serving
+loop, on the GPU, through the C ABI.
+
+Every request's result must match the oracle (structure bit-exact, values
+within 1e-12), match the plain one-call path, and be the same alone and in a
+mix.  The graph is replayed several times with eager work between replays,
+and every replay is checked.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import powerlaw_rows, random_csr, uniform_rows
+from sparsews_amd import SpwsError, gen
+from sparsews_amd.device import DeviceCSR, spgemm_rowwise
+from sparsews_amd.serve import DeviceRequest, Engine, default_caps, latency_summary, serve
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+SMALL_CAPS = dict(max_requests=64, max_rows=1 << 16, max_nnz=1 << 22, max_products=1 << 25,
+                  max_row_products=1 << 20)
+
+
+def make_requests(seed, count):
+    rng = np.random.default_rng(seed)
+    out = []
+    for t in range(count):
+        n = int(rng.integers(40, 1500))
+        if t % 2 == 0:
+            A, B = uniform_rows(n, n, 0, 30, seed * 100 + t, 0), uniform_rows(n, n, 0, 30, seed * 100 + t, 1)
+        else:
+            A = powerlaw_rows(n, n, 1.8, min(n, 400), seed * 100 + t, 0)
+            B = powerlaw_rows(n, n, 1.8, min(n, 400), seed * 100 + t, 1)
+        out.append((A, B))
+    return out
+
+
+def check_result(C, A, B):
+    G = oracle.spgemm_gustavson(A, B)
+    np.testing.assert_array_equal(C.pos, G.pos)
+    np.testing.assert_array_equal(C.crd, G.crd)
+    exact, absum = oracle.spgemm_refsum(A, B, G.pos, G.crd)
+    assert oracle.max_value_error(C.vals, exact, absum) <= TOL
+    return G, absum
+
+
+@pytest.fixture(scope="module")
+def engine(ctx):
+    e = Engine(ctx, default_caps(**SMALL_CAPS))
+    yield e
+    e.close()
+
+
+def run_batch(engine, ctx, pairs):
+    engine.reset()
+    reqs = [DeviceRequest.upload(ctx, A, B) for A, B in pairs]
+    slots = [engine.add(r) for r in reqs]
+    engine.launch()
+    engine.wait()
+    return [engine.result(s, A.nrows, B.ncols) for s, (A, B) in zip(slots, pairs)], reqs
+
+
+def test_batch_matches_oracle_and_plain(ctx, engine):
+    print("case: batch of 24 mixed requests", flush=True)
+    pairs = make_requests(1, 24)
+    results, reqs = run_batch(engine, ctx, pairs)
+    total = 0
+    for (A, B), C, r in zip(pairs, results, reqs):
+        G, absum = check_result(C, A, B)
+        P = spgemm_rowwise(ctx, r.A, r.B).download()
+        np.testing.assert_array_equal(P.crd, C.crd)
+        assert oracle.max_value_error(C.vals, P.vals, absum) <= TOL
+        total += C.nnz
+    assert engine.nnz() == total
+
+
+def test_graph_replays_with_eager_work_between(ctx, engine):
+    for rep, seed in enumerate([2, 3, 4]):
+        print(f"case: graph replay {rep}", flush=True)
+        pairs = make_requests(seed, 10 + 4 * rep)
+        results, _ = run_batch(engine, ctx, pairs)
+        # eager work on the context's stream between replays
+        a, b = uniform_rows(300, 300, 1, 9, 90 + rep, 0), uniform_rows(300, 300, 1, 9, 90 + rep, 1)
+        check_result(spgemm_rowwise(ctx, DeviceCSR.upload(ctx, a), DeviceCSR.upload(ctx, b)).download(), a, b)
+        for (A, B), C in zip(pairs, results):
+            check_result(C, A, B)
+
+
+def test_same_alone_and_in_the_mix(ctx, engine):
+    print("case: alone vs mix", flush=True)
+    pairs = make_requests(5, 16)
+    A0, B0 = pairs[7]
+    alone, _ = run_batch(engine, ctx, [(A0, B0)])
+    mix, _ = run_batch(engine, ctx, pairs)
+    C1, C2 = alone[0], mix[7]
+    np.testing.assert_array_equal(C1.pos, C2.pos)
+    np.testing.assert_array_equal(C1.crd, C2.crd)
+    _, absum = check_result(C1, A0, B0)
+    assert oracle.max_value_error(C1.vals, C2.vals, absum) <= TOL
+
+
+def test_long_rows_take_the_hbm_hash_path_in_a_batch(ctx, engine):
+    print("case: long rows in a batch", flush=True)
+    A = random_csr(20, 600, np.full(20, 300), 6, 0)
+    B = random_csr(600, 20000, np.full(600, 20), 6, 1)
+    small = make_requests(6, 4)
+    results, _ = run_batch(engine, ctx, [(A, B)] + small)
+    check_result(results[0], A, B)
+    for (a, b), C in zip(small, results[1:]):
+        check_result(C, a, b)
+
+
+def test_admission_limits(ctx):
+    print("case: admission limits", flush=True)
+    caps = default_caps(max_requests=4, max_rows=1 << 12, max_nnz=1 << 16, max_products=1 << 15,
+                        max_row_products=256)
+    e = Engine(ctx, caps)
+    try:
+        tiny = [(uniform_rows(50, 50, 0, 4, 700 + t, 0), uniform_rows(50, 50, 0, 4, 700 + t, 1)) for t in range(6)]
+        reqs = [DeviceRequest.upload(ctx, A, B) for A, B in tiny]
+        assert all(r.max_row_products <= 16 for r in reqs)
+        for r in reqs[:4]:
+            e.add(r)
+        with pytest.raises(SpwsError) as ex:
+            e.add(reqs[4])
+        assert ex.value.code == 2  # the batch is full
+        e.reset()
+        A = random_csr(5, 300, np.full(5, 100), 8, 0)
+        B = random_csr(300, 300, np.full(300, 10), 8, 1)
+        with pytest.raises(SpwsError) as ex2:
+            e.add(DeviceRequest.upload(ctx, A, B))
+        assert ex2.value.code == 4  # a row beyond max_row_products
+        # an emptied engine still runs a batch correctly
+        e.reset()
+        slots = [e.add(r) for r in reqs[:3]]
+        e.launch()
+        e.wait()
+        for (a, b), sl in zip(tiny[:3], slots):
+            check_result(e.result(sl, 50, 50), a, b)
+    finally:
+        e.close()
+
+
+def test_serve_plain_and_batched(ctx):
+    print("case: serve loop", flush=True)
+    pairs = make_requests(8, 48)
+    reqs = [DeviceRequest.upload(ctx, A, B) for A, B in pairs]
+    arrivals = np.cumsum(np.random.default_rng(8).exponential(2e-4, len(reqs)))
+    arrivals -= arrivals[0]
+    engines = [Engine(ctx, default_caps(**SMALL_CAPS)) for _ in range(2)]
+    try:
+        f_plain, b_plain, st_plain = serve(ctx, [], reqs, arrivals, "plain")
+        f_bat, b_bat, st_bat = serve(ctx, engines, reqs, arrivals, "batched")
+    finally:
+        for e in engines:
+            e.close()
+    assert np.all(f_plain >= arrivals) and np.all(f_bat >= arrivals)
+    assert np.all(b_plain == -1)
+    assert st_bat["batches"] >= 1 and st_bat["batches"] <= len(reqs)
+    assert np.all(b_bat >= 0)
+    print("plain", latency_summary(arrivals, f_plain), "batched", latency_summary(arrivals, f_bat), st_bat)
