@@ -1,0 +1,38 @@
+/* Drop-in for benchmark/TTM/COO_CSF_DCSR_hash_il.h: TTM compute(A, B, C,
+ * w_cap) (:247) and the wrapper COO_CSF_DCSR_hash() (:379), on the GPU.  B
+ * is a 3-level CSF with the contracted mode in the middle; C is a dense
+ * factor stored as DCSR; A is the COO output (indices[1..3][0]). */
+#ifndef SPWS_COMPAT_TTM_HASH_IL_H
+#define SPWS_COMPAT_TTM_HASH_IL_H
+#include "spws/compat/timing.h"
+
+inline int compute(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C, int32_t w_accumulator_capacity) {
+  return spws_compat::report(spws_ttm_hash_il_compute(A, B, C, w_accumulator_capacity), "compute");
+}
+
+inline double COO_CSF_DCSR_hash(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C, int32_t w_cap, int32_t warmup,
+                                int32_t repeat, bool bench = false, bool print = false) {
+  (void)print;
+  for (int i = 0; i < warmup; i++) {
+    compute(A, B, C, w_cap);
+    if (bench) {
+      free(A->vals);
+      free(A->indices[1][0]);
+      free(A->indices[2][0]);
+      free(A->indices[3][0]);
+    }
+  }
+  const double start = spws_compat::now_seconds();
+  for (int i = 0; i < repeat; i++) {
+    compute(A, B, C, w_cap);
+    if (bench && i != repeat - 1) {
+      free(A->vals);
+      free(A->indices[1][0]);
+      free(A->indices[2][0]);
+      free(A->indices[3][0]);
+    }
+  }
+  return (spws_compat::now_seconds() - start) / (repeat > 0 ? repeat : 1);
+}
+
+#endif

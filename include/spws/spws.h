@@ -89,6 +89,9 @@ int spws_dev_free(spws_ctx* ctx, void* p);
 int spws_memcpy_h2d(spws_ctx* ctx, void* dst, const void* src, size_t bytes);
 int spws_memcpy_d2h(spws_ctx* ctx, void* dst, const void* src, size_t bytes);
 int spws_sync(spws_ctx* ctx);
+/* The calling thread's default context (device SPWS_DEVICE, default 0), the
+ * one the taco entry points use; created on first use, owned by the thread. */
+int spws_default_ctx(spws_ctx** out);
 void spws_host_free(void* p); /* frees arrays the host entry points malloc'd */
 /* Plain device-to-device copy kernel (coalesced 16-byte loads/stores) on the
  * context's stream; also the byte-count reference for hardware counters. */
@@ -163,6 +166,28 @@ int spws_ttm(spws_ctx* ctx, int32_t I, int32_t J, int32_t K, int32_t R,
              int32_t nroots, const int32_t* l1_crd, const int32_t* l2_pos, const int32_t* l2_crd,
              int32_t nfibers, const int32_t* l3_pos, const int32_t* l3_crd, const double* vals, int64_t nnz,
              const double* U, int64_t* nfib, int32_t** fib_i, int32_t** fib_j, double** Y);
+
+/* ------------------------------------------------ workspace API (wspace.h) */
+
+/* A workspace record is {int32_t crd[2]; double val;} (16 bytes), the
+ * reference's wspace struct (utils/wspace.h:10-13) with an FP64 value; an
+ * empty slot has crd[0] = -1 (init_wspace, :15-23).  Host arrays. */
+
+/* Sort(array, size, rev) of utils/wspace.h:206-217: sorts n records by
+ * (crd[0], crd[1]) in signed lexicographic order, ascending (rev = 0) or
+ * descending (rev != 0).  *nvalid receives n for ascending order and, for
+ * descending order, the number of records before the trailing empty slots. */
+int spws_wspace_sort(spws_ctx* ctx, void* records, int64_t n, int rev, int64_t* nvalid);
+
+/* Merge(COO_crd, COO_vals, COO_size, accumulator, accumulator_size) of
+ * utils/wspace.h:62-204: merges n_acc workspace records (any order) into the
+ * sorted COO (coo_crd0, coo_crd1, coo_vals; coo_size entries; capacity at
+ * least coo_size + n_acc), adding the values of equal coordinates, and writes
+ * the merged, sorted COO back; *out_size receives its length.  Equal
+ * coordinates inside the workspace are added as well (the reference assumes
+ * none). */
+int spws_wspace_merge(spws_ctx* ctx, int32_t* coo_crd0, int32_t* coo_crd1, double* coo_vals, int64_t coo_size,
+                      const void* records, int64_t n_acc, int64_t* out_size);
 
 /* -------------------------------------------- batched and served SpGEMM */
 

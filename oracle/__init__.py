@@ -69,6 +69,10 @@ def lib():
                                       C.c_int32, i32p, i32p, i32p, i32p, i32p, f64p,
                                       C.c_int32, i32p, i32p, i32p, f64p,
                                       C.POINTER(PI32), C.POINTER(PI32), C.POINTER(PI32), C.POINTER(PF64)]
+        L.orc_wspace_sort.restype = C.c_int64
+        L.orc_wspace_sort.argtypes = [C.c_void_p, C.c_int64, C.c_int]
+        L.orc_wspace_merge.restype = C.c_int64
+        L.orc_wspace_merge.argtypes = [i32p, i32p, f64p, C.c_int64, C.c_void_p, C.c_int64, C.c_int]
         L.orc_free.restype = None
         L.orc_free.argtypes = [C.c_void_p]
         _lib = L
@@ -199,6 +203,31 @@ def ttm_hash_il(Bcsf, Cd, K: int, J: int, L: int):
 # --------------------------------------------------------------------------- tolerance
 
 REL_TOL = 1e-12
+
+
+# --------------------------------------------------------------------------- workspace API
+
+WSPACE_DTYPE = np.dtype([("crd", np.int32, (2,)), ("val", np.float64)])
+
+
+def wspace_sort(records, rev: bool):
+    """Sort of utils/wspace.h:206-217 on a copy; returns (sorted, count)."""
+    w = np.array(records, dtype=WSPACE_DTYPE, copy=True)
+    n = lib().orc_wspace_sort(w.ctypes.data_as(C.c_void_p), w.size, int(bool(rev)))
+    return w, int(n)
+
+
+def wspace_merge(coo_crd0, coo_crd1, coo_vals, records, rev: bool = False):
+    """Merge of utils/wspace.h:62-204; returns the merged (crd0, crd1, vals)."""
+    acc = np.ascontiguousarray(records, dtype=WSPACE_DTYPE)
+    n0 = len(coo_crd0)
+    cap = max(n0 + acc.size, 1)
+    c0 = np.zeros(cap, np.int32)
+    c1 = np.zeros(cap, np.int32)
+    v = np.zeros(cap, np.float64)
+    c0[:n0], c1[:n0], v[:n0] = coo_crd0, coo_crd1, coo_vals
+    n = lib().orc_wspace_merge(c0, c1, v, n0, acc.ctypes.data_as(C.c_void_p), acc.size, int(bool(rev)))
+    return c0[:n].copy(), c1[:n].copy(), v[:n].copy()
 
 
 def value_error(x, y, abssum):

@@ -24,6 +24,8 @@
  *                          (TryInsert_coord_t :66-79, Sort_t :36-47, Merge_coord_t :81-192)
  *   orc_mttkrp_hash        benchmark/MTTKRP/COO_CSF_DCSR_DCSR_hash_free.h:233-366
  *   orc_ttm_hash_il        benchmark/TTM/COO_CSF_DCSR_hash_il.h:247-377
+ *   orc_wspace_sort        utils/wspace.h:206-217 (Sort; esc_cmp :24-31, esc_cmp_rev :32-39)
+ *   orc_wspace_merge       utils/wspace.h:62-204 (Merge, both directions)
  */
 #include <limits.h>
 #include <stdint.h>
@@ -697,6 +699,94 @@ ORC_API int64_t orc_ttm_hash_il(int32_t K, int32_t J, int32_t L,
   *Ol = ol;
   *Ov = ov;
   return nnz;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Coordinate workspace records: Sort and Merge of utils/wspace.h.            */
+
+typedef struct {
+  int32_t crd[2];
+  double val;
+} orc_wsrec;
+
+/* esc_cmp (utils/wspace.h:24-31) compares with int subtraction; coordinates
+ * are >= -1 there, so a three-way compare gives the same sign. */
+static int ws_cmp_up(const void* pa, const void* pb) {
+  const orc_wsrec* a = (const orc_wsrec*)pa;
+  const orc_wsrec* b = (const orc_wsrec*)pb;
+  for (int i = 0; i < 2; i++)
+    if (a->crd[i] != b->crd[i]) return a->crd[i] < b->crd[i] ? -1 : 1;
+  return 0;
+}
+
+static int ws_cmp_down(const void* pa, const void* pb) { return ws_cmp_up(pb, pa); }
+
+/* Sort (utils/wspace.h:206-217): ascending returns size; descending returns
+ * one past the last record whose crd[0] is not -1 (0 if none). */
+ORC_API int64_t orc_wspace_sort(orc_wsrec* w, int64_t size, int rev) {
+  if (rev) {
+    qsort(w, (size_t)size, sizeof(orc_wsrec), ws_cmp_down);
+    for (int64_t i = size - 1; i >= 0; i--)
+      if (w[i].crd[0] != -1) return i + 1;
+    /* the reference falls through to the ascending sort when every slot is
+     * empty; an all-empty workspace stays all-empty either way */
+  }
+  qsort(w, (size_t)size, sizeof(orc_wsrec), ws_cmp_up);
+  return size;
+}
+
+/* Merge (utils/wspace.h:62-204): two-pointer merge of a sorted COO
+ * (c0, c1, v; coo_size entries, capacity >= coo_size + acc_size) with the
+ * accumulator (ascending, or descending when rev), adding the value of an
+ * equal coordinate as acc.val + coo.val.  Returns the merged size. */
+ORC_API int64_t orc_wspace_merge(int32_t* c0, int32_t* c1, double* v, int64_t coo_size, const orc_wsrec* acc,
+                                 int64_t acc_size, int rev) {
+  const int64_t n = coo_size + acc_size;
+  int32_t* t0 = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  int32_t* t1 = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  double* tv = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  int64_t a = 0, c = 0, t = 0;
+  while (a < acc_size && c < coo_size) {
+    const orc_wsrec* r = &acc[rev ? acc_size - 1 - a : a];
+    orc_wsrec q;
+    q.crd[0] = c0[c];
+    q.crd[1] = c1[c];
+    const int cmp = ws_cmp_up(r, &q);
+    if (cmp == 0) {
+      t0[t] = r->crd[0];
+      t1[t] = r->crd[1];
+      tv[t] = r->val + v[c];
+      a++, c++, t++;
+    } else if (cmp < 0) {
+      t0[t] = r->crd[0];
+      t1[t] = r->crd[1];
+      tv[t] = r->val;
+      a++, t++;
+    } else {
+      t0[t] = c0[c];
+      t1[t] = c1[c];
+      tv[t] = v[c];
+      c++, t++;
+    }
+  }
+  for (; a < acc_size; a++, t++) {
+    const orc_wsrec* r = &acc[rev ? acc_size - 1 - a : a];
+    t0[t] = r->crd[0];
+    t1[t] = r->crd[1];
+    tv[t] = r->val;
+  }
+  for (; c < coo_size; c++, t++) {
+    t0[t] = c0[c];
+    t1[t] = c1[c];
+    tv[t] = v[c];
+  }
+  memcpy(c0, t0, sizeof(int32_t) * (size_t)t);
+  memcpy(c1, t1, sizeof(int32_t) * (size_t)t);
+  memcpy(v, tv, sizeof(double) * (size_t)t);
+  free(t0);
+  free(t1);
+  free(tv);
+  return t;
 }
 
 ORC_API void orc_free(void* p) { free(p); }

@@ -94,6 +94,7 @@ SIGNATURES = {
     "spws_memcpy_h2d": (C.c_int, [VP, VP, VP, C.c_size_t]),
     "spws_memcpy_d2h": (C.c_int, [VP, VP, VP, C.c_size_t]),
     "spws_sync": (C.c_int, [VP]),
+    "spws_default_ctx": (C.c_int, [C.POINTER(VP)]),
     "spws_host_free": (None, [VP]),
     "spws_device_copy": (C.c_int, [VP, VP, VP, C.c_size_t]),
     "spws_spgemm_rowwise": (C.c_int, [VP, I32, I32, I32, VP, VP, VP, VP, VP, VP,
@@ -106,6 +107,8 @@ SIGNATURES = {
                               C.POINTER(I32), PVP, PVP]),
     "spws_ttm": (C.c_int, [VP, I32, I32, I32, I32, I32, VP, VP, VP, I32, VP, VP, VP, I64, VP,
                            PI64, PVP, PVP, PVP]),
+    "spws_wspace_sort": (C.c_int, [VP, VP, I64, C.c_int, PI64]),
+    "spws_wspace_merge": (C.c_int, [VP, VP, VP, VP, I64, VP, I64, PI64]),
     "spws_batch_create": (C.c_int, [VP, C.POINTER(BatchCaps), C.POINTER(VP)]),
     "spws_batch_destroy": (C.c_int, [VP]),
     "spws_batch_reset": (C.c_int, [VP]),

@@ -172,6 +172,11 @@ using namespace spws;
 
 extern "C" {
 
+int spws_default_ctx(spws_ctx** out) {
+  SPWS_REQUIRE(out != nullptr, SPWS_ERR_INVALID, "null output pointer");
+  return default_ctx(out);
+}
+
 int spws_csr_csr_hash_compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B, int32_t w_cap) {
   (void)w_cap;
   int rc;
