@@ -140,11 +140,12 @@ def run_outer(args):
     seed 1; A = B^T, so DCSC(A) shares B's arrays.  Output sorted CSR with
     int64 row pointers."""
     from sparsews_amd import formats, gen
-    from sparsews_amd.device import Context, DeviceDCSR, spgemm_outer
+    from sparsews_amd.device import Context, DeviceDCSR, set_outer_method, spgemm_outer
     B = gen.outer_operand()
     N = B.nrows
     Bd = formats.dcsr_from_csr(B)
     ctx = Context(0)
+    set_outer_method(ctx, args.outer_method, args.outer_window)
     dA, dB = DeviceDCSR.upload(ctx, Bd), DeviceDCSR.upload(ctx, Bd)
     out = {}
 
@@ -156,17 +157,25 @@ def run_outer(args):
     P, nnz = st["products"], out["C"].nnz
     key_bytes = 4
     # algorithmic bytes of each kernel over the timed region (per step):
+    # LSD method
     #   radix_scatter  read + write of key (4 B) and value (8 B) per product per pass
     #   radix_hist     read of the key per product per pass
     #   outer_expand   12 B written + 12 B of B gathered per product
     #   outer_reduce   12 B read per product + 12 B written per distinct entry
+    # bucket method
+    #   bk_expand      10 B written (2 B window column + 8 B value) + 12 B of B gathered per product
+    #   bk_count       2 B read per product
+    #   bk_reduce      10 B read per product + 12 B written per distinct entry
     passes = times["radix_scatter"][1] // (args.steps * st["chunks"]) if "radix_scatter" in times else 0
-    per_step = {
-        "radix_scatter": 2 * (key_bytes + 8) * P * passes,
-        "radix_hist": key_bytes * P * passes,
-        "outer_expand": 24 * P,
-        "outer_reduce": 12 * P + 12 * nnz,
-    }
+    if st["outer_method"] == 2:
+        per_step = {"bk_expand": 22 * P, "bk_count": 2 * P, "bk_reduce": 10 * P + 12 * nnz}
+    else:
+        per_step = {
+            "radix_scatter": 2 * (key_bytes + 8) * P * passes,
+            "radix_hist": key_bytes * P * passes,
+            "outer_expand": 24 * P,
+            "outer_reduce": 12 * P + 12 * nnz,
+        }
     dom = max(per_step, key=lambda k: times.get(k, (0, 1))[0])
     ms, launches = times[dom]
     achieved = per_step[dom] * args.steps / (ms / 1e3) / 1e9
@@ -184,7 +193,9 @@ def run_outer(args):
         "dtype": "f64",
         "config": {"workload": "outer_product_spgemm_200k_zipf1.8", "n": N, "zipf_s": 1.8, "row_cap": 20000,
                    "nnz_B": int(B.nnz), "products": int(P), "nnz_C": int(nnz), "chunks": st["chunks"],
-                   "sort_key_bits": st["sort_bits"], "radix_passes": int(passes), "seed": 1},
+                   "sort_method": {1: "lsd_radix", 2: "row_window_buckets"}[st["outer_method"]],
+                   "window_bits": st["window_bits"], "sort_key_bits": st["sort_bits"],
+                   "radix_passes": int(passes), "seed": 1},
         "roofline": roof,
         "compulsory_bytes": int(compulsory),
         "compulsory_frac_of_hbm_roofline": round(compulsory / sec / 1e9 / HBM_PEAK_GBS, 5),
@@ -465,6 +476,10 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--requests", type=int, default=4096, help="serve workload: number of requests")
+    ap.add_argument("--outer-method", type=int, default=0, choices=[0, 1, 2],
+                    help="outer workload: 0 automatic, 1 LSD radix sort, 2 row/window buckets")
+    ap.add_argument("--outer-window", type=int, default=0,
+                    help="outer workload, bucket method: log2 of the column window (0 = automatic)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "

@@ -8,7 +8,8 @@
  *   2. Device-resident entry points over plain device pointers and sizes,
  *      used by the batched/served path and by benchmarks.  Outputs of a
  *      device-resident call live in buffers owned by the context and stay
- *      valid until the next call of the same family on that context.
+ *      valid until the next device-resident compute call (row-wise, outer,
+ *      MTTKRP, TTM) on that context.
  *
  * Every function returns 0 (SPWS_OK) on success and a nonzero spws_status
  * otherwise; spws_last_error() describes the last failure of the calling
@@ -59,7 +60,9 @@ typedef struct {
   int64_t nnz;              /* output nonzeros */
   int64_t rows_per_path[8]; /* rows sent to each row-wise path (index = force_path id), [6] = empty */
   int64_t chunks;           /* outer product: output-row chunks processed */
-  int64_t sort_bits;        /* outer product: key bits radix-sorted per chunk */
+  int64_t sort_bits;        /* outer product: key bits radix-sorted per chunk (LSD method) */
+  int64_t outer_method;     /* outer product: 1 LSD radix sort, 2 row/window buckets */
+  int64_t window_bits;      /* outer product, bucket method: log2 of the column window */
 } spws_stats;
 
 /* ---------------------------------------------------------------- context */
@@ -137,6 +140,19 @@ int spws_spgemm_outer(spws_ctx* ctx, int32_t m, int32_t k, int32_t n,
  * and max_rows rows per chunk (0 = defaults: 2^30 products; rows limited by
  * the 32-bit sort key).  Small values force many chunks (used by tests). */
 int spws_ctx_set_outer_chunking(spws_ctx* ctx, int64_t max_products, int32_t max_rows);
+
+/* Sort method of the outer product's coordinate workspace:
+ *   1  LSD radix sort of packed (i - r0) << kb | k keys over the whole chunk,
+ *      then a segmented reduce of the runs;
+ *   2  MSD by buckets: the expand scatters each product straight into its
+ *      (row, column window) bucket at a position computed from the operands'
+ *      structure (exact digit counts, no histogram pass over the products),
+ *      then each bucket is ordered by k within the window and reduced in LDS;
+ *   0  automatic (2 unless the windows would be too sparse or too many).
+ * window_log2 sets the bucket method's column window (6..11; 0 = chosen from
+ * the average products per row).  Both methods sum each entry in increasing
+ * j and give identical results. */
+int spws_ctx_set_outer_method(spws_ctx* ctx, int32_t method, int32_t window_log2);
 
 /* ------------------------------------------------------ sparse tensor kernels */
 

@@ -67,7 +67,8 @@ class ServeStats(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("products", C.c_int64), ("nnz", C.c_int64), ("rows_per_path", C.c_int64 * 8),
-                ("chunks", C.c_int64), ("sort_bits", C.c_int64)]
+                ("chunks", C.c_int64), ("sort_bits", C.c_int64), ("outer_method", C.c_int64),
+                ("window_bits", C.c_int64)]
 
 
 VP = C.c_void_p
@@ -103,6 +104,7 @@ SIGNATURES = {
     "spws_spgemm_outer": (C.c_int, [VP, I32, I32, I32, I32, VP, VP, VP, VP, I32, VP, VP, VP, VP,
                                     PI64, PVP, PVP, PVP]),
     "spws_ctx_set_outer_chunking": (C.c_int, [VP, I64, I32]),
+    "spws_ctx_set_outer_method": (C.c_int, [VP, I32, I32]),
     "spws_mttkrp": (C.c_int, [VP, I32, I32, I32, I32, I32, VP, VP, VP, I32, VP, VP, VP, I64, VP, VP,
                               C.POINTER(I32), PVP, PVP]),
     "spws_ttm": (C.c_int, [VP, I32, I32, I32, I32, I32, VP, VP, VP, I32, VP, VP, VP, I64, VP,

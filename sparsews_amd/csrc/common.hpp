@@ -85,12 +85,17 @@ struct spws_ctx {
   spws::DevBuf stage[12];  // device copies of host inputs (TACO entry points)
   spws::DevBuf radix_hist;
   spws::DevBuf obuf[8];    // outer-product per-column metadata
+  spws::DevBuf bk[10];     // outer-product bucket path: pairs, window bounds, bucket tables
   spws::PinnedBuf host_meta;
   // row-wise SpGEMM bin boundaries (validated against the compiled table sizes)
   spws_rowwise_tuning rw{64, 256, 1024, 4096, 16, -1};
   // outer-product chunk limits (0 = defaults: 2^30 products, key-width rows)
   int64_t outer_max_products = 0;
   int32_t outer_max_rows = 0;
+  // outer-product method (0 auto, 1 LSD radix sort, 2 row/window buckets) and
+  // bucket window width log2 (0 = chosen from the average row product count)
+  int32_t outer_method = 0;
+  int32_t outer_window_log2 = 0;
   // statistics of the last call (host-readable)
   spws_stats stats{};
   spws::KernelProfiler prof;

@@ -76,7 +76,6 @@ int spws_serve(spws_ctx* ctx, spws_batch** engines, int32_t nengines, int32_t nr
   int64_t batch_id = 0;
   while (done < nreq) {
     bool progress = false;
-    const double now = since(t0);
     for (int32_t e = 0; e < nengines; ++e) {
       if (!busy[(size_t)e]) continue;
       int fin = 0;

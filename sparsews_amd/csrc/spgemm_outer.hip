@@ -27,6 +27,24 @@
 //   k_outer_tail      row pointers of trailing empty rows.
 // Values of each entry are summed in increasing j, exactly Gustavson's
 // left-to-right order.
+//
+// Second sort method (the default): an MSD radix sort whose first digit is
+// (row i, column window k >> s).  The digit counts need no pass over the
+// products: the products of A(i,j) that land in window w are the entries of
+// B(j,:) in that window, a range found by binary search in the sorted row.
+//   k_bk_bseg         per stored B row j and window w: first entry >= w*2^s.
+//   k_bk_entries      the chunk's A entries, keyed by row, radix-sorted so
+//                     each row lists its entries in increasing j.
+//   k_bk_rowsize      bucket sizes, one wavefront per row, lanes over windows.
+//   scan              bucket starts (the digit's prefix sum).
+//   k_bk_etab         per entry and window: where its products start.
+//   k_bk_expand       each product goes straight to its bucket slot; inside a
+//                     bucket the products are in (j, k) order.
+//   k_bk_count        distinct columns per bucket (LDS bitmap).
+//   scan              output offsets of the buckets.
+//   k_bk_reduce       one wavefront per bucket: products accumulate in a dense
+//                     LDS window, entries with equal k in position (= j)
+//                     order; the window's columns are emitted in order.
 #include <algorithm>
 #include <climits>
 
@@ -68,11 +86,15 @@ __global__ __launch_bounds__(256) void k_outer_match(int32_t a_ncols, const int3
 __global__ __launch_bounds__(256) void k_outer_row_ub(int32_t a_ncols, const int32_t* __restrict__ A2_pos,
                                                       const int32_t* __restrict__ A2_crd,
                                                       const int32_t* __restrict__ blen,
-                                                      unsigned long long* __restrict__ row_ub) {
+                                                      unsigned long long* __restrict__ row_ub,
+                                                      int32_t* __restrict__ row_pairs) {
   for (int32_t ja = blockIdx.x * blockDim.x + threadIdx.x; ja < a_ncols; ja += gridDim.x * blockDim.x) {
     const int32_t L = blen[ja];
     if (L == 0) continue;
-    for (int32_t p = A2_pos[ja]; p < A2_pos[ja + 1]; ++p) atomicAdd(&row_ub[A2_crd[p]], (unsigned long long)L);
+    for (int32_t p = A2_pos[ja]; p < A2_pos[ja + 1]; ++p) {
+      atomicAdd(&row_ub[A2_crd[p]], (unsigned long long)L);
+      atomicAdd(&row_pairs[A2_crd[p]], 1);
+    }
   }
 }
 
@@ -104,7 +126,8 @@ __global__ __launch_bounds__(256) void k_outer_compact(int32_t a_ncols, const in
                                                        const int32_t* __restrict__ B2_pos,
                                                        int32_t* __restrict__ act_astart, int32_t* __restrict__ act_L,
                                                        int32_t* __restrict__ act_bstart,
-                                                       int64_t* __restrict__ act_cnt) {
+                                                       int64_t* __restrict__ act_cnt, int32_t* __restrict__ act_ja,
+                                                       int32_t* __restrict__ act_m) {
   const int32_t ja = blockIdx.x * blockDim.x + threadIdx.x;
   if (ja >= a_ncols) return;
   const int32_t m = col_m[ja];
@@ -115,6 +138,10 @@ __global__ __launch_bounds__(256) void k_outer_compact(int32_t a_ncols, const in
   act_L[a] = L;
   act_bstart[a] = B2_pos[match[ja]];
   act_cnt[a] = (int64_t)m * L;
+  if (act_ja) {
+    act_ja[a] = ja;
+    act_m[a] = m;
+  }
 }
 
 template <class K>
@@ -291,10 +318,318 @@ __global__ __launch_bounds__(256) void k_outer_tail(const K* __restrict__ keys, 
 
 __global__ void k_set_i64(int64_t* p, int64_t v) { *p = v; }
 
+// ------------------------------------------------------------ bucket method
+
+constexpr int kBkWaves = 2;   // wavefronts per workgroup in the per-bucket kernels
+constexpr int kBkUnroll = 4;  // strides of products loaded before they are used
+constexpr int kBkMinLog2 = 6;
+constexpr int kBkMaxLog2 = 11;
+
+// bseg[ja*(W+1) + w]: first entry of the B row matched with A column ja
+// (relative to the row start) whose column is >= w << s, for w in [0, W].
+__global__ __launch_bounds__(256) void k_bk_bseg(int32_t a_ncols, int32_t W, int s, const int32_t* __restrict__ match,
+                                                 const int32_t* __restrict__ blen, const int32_t* __restrict__ B2_pos,
+                                                 const int32_t* __restrict__ B2_crd, int32_t* __restrict__ bseg) {
+  const int64_t total = (int64_t)a_ncols * (W + 1);
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t ja = (int32_t)(t / (W + 1));
+    const int32_t w = (int32_t)(t - (int64_t)ja * (W + 1));
+    const int32_t L = blen[ja];
+    int32_t r = 0;
+    if (L > 0) {
+      const int32_t b = B2_pos[match[ja]];
+      const int64_t x = (int64_t)w << s;
+      r = x > INT32_MAX ? L : lower_bound_i32(B2_crd, b, b + L, (int32_t)x) - b;
+    }
+    bseg[t] = r;
+  }
+}
+
+// The chunk's matched A entries, numbered e by active column then row:
+// key = i - r0, value = e (bit pattern in a double), and e's A column.
+__global__ __launch_bounds__(256) void k_bk_entries(int32_t En, const int32_t* __restrict__ n_act_ptr,
+                                                    const int32_t* __restrict__ act_eoff,
+                                                    const int32_t* __restrict__ act_astart,
+                                                    const int32_t* __restrict__ act_ja,
+                                                    const int32_t* __restrict__ A2_crd, int32_t r0,
+                                                    uint32_t* __restrict__ pkey, double* __restrict__ pval,
+                                                    int32_t* __restrict__ ent_ja) {
+  const int32_t n_act = *n_act_ptr;
+  for (int32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < En; e += gridDim.x * blockDim.x) {
+    int32_t lo = 0, hi = n_act - 1;  // last active column c with act_eoff[c] <= e
+    while (lo < hi) {
+      const int32_t mid = (lo + hi + 1) >> 1;
+      if (act_eoff[mid] <= e) lo = mid; else hi = mid - 1;
+    }
+    const int32_t ai = act_astart[lo] + (e - act_eoff[lo]);
+    pkey[e] = (uint32_t)(A2_crd[ai] - r0);
+    pval[e] = __longlong_as_double((long long)e);
+    ent_ja[e] = act_ja[lo];
+  }
+}
+
+// One wavefront per chunk row, lanes over windows, entries in increasing j.
+// kPass 0: bsize[r*W + w] = products of the row in window w.
+// kPass 1: etab[e*W + w] = bucket start + products of the row's earlier
+//          entries in w - bseg[w], so that product (e, B entry q) of window w
+//          goes to etab[e*W + w] + q.
+template <int kPass>
+__global__ __launch_bounds__(256) void k_bk_rows(int32_t R, int32_t W, const int32_t* __restrict__ rp_off,
+                                                 const double* __restrict__ pval, const int32_t* __restrict__ ent_ja,
+                                                 const int32_t* __restrict__ bseg,
+                                                 const int32_t* __restrict__ bstart, int32_t* __restrict__ out) {
+  const int32_t nwaves = (int32_t)(gridDim.x * blockDim.x / kWave);
+  const int lane = lane_id();
+  for (int32_t r = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kWave); r < R; r += nwaves) {
+    const int32_t q0 = rp_off[r], q1 = rp_off[r + 1];
+    for (int32_t w0 = 0; w0 < W; w0 += kWave) {
+      const int32_t w = w0 + lane;
+      const bool on = w < W;
+      int32_t run = 0;
+      const int32_t start = (kPass == 1 && on) ? bstart[(int64_t)r * W + w] : 0;
+      for (int32_t q = q0; q < q1; ++q) {
+        const int32_t e = (int32_t)__double_as_longlong(pval[q]);
+        const int32_t* bs = bseg + (int64_t)ent_ja[e] * (W + 1);
+        if (on) {
+          const int32_t lo = bs[w];
+          if (kPass == 1) out[(int64_t)e * W + w] = start + run - lo;
+          run += bs[w + 1] - lo;
+        }
+      }
+      if (kPass == 0 && on) out[(int64_t)r * W + w] = run;
+    }
+  }
+}
+
+// Expand: the product of A entry e (active column a, offset ii) and B entry
+// kk of the matched row goes to slot etab[e*W + (k >> s)] + kk of its
+// (row, window) bucket; the slot holds k's offset in the window and the
+// product.  Tiles and owner search as in k_outer_expand.
+__global__ __launch_bounds__(kExpBlock) void k_bk_expand(int64_t P, const int32_t* __restrict__ n_act_ptr,
+                                                         const int64_t* __restrict__ act_off,
+                                                         const int32_t* __restrict__ act_astart,
+                                                         const int32_t* __restrict__ act_L,
+                                                         const int32_t* __restrict__ act_bstart,
+                                                         const int32_t* __restrict__ act_eoff,
+                                                         const double* __restrict__ Ax,
+                                                         const int32_t* __restrict__ B2_crd,
+                                                         const double* __restrict__ Bx, int32_t W, int s,
+                                                         const int32_t* __restrict__ etab,
+                                                         uint16_t* __restrict__ klo, double* __restrict__ pv) {
+  __shared__ int64_t soff[kExpTile + 1];
+  __shared__ int32_t s_a0, s_span;
+  const int64_t p0 = (int64_t)blockIdx.x * kExpTile;
+  const int64_t p_end = p0 + kExpTile < P ? p0 + kExpTile : P;
+  if (threadIdx.x == 0) {
+    const int32_t n_act = *n_act_ptr;
+    auto owner = [&](int64_t x) {
+      int32_t lo = 0, hi = n_act - 1;
+      while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (act_off[mid] <= x) lo = mid; else hi = mid - 1;
+      }
+      return lo;
+    };
+    const int32_t a0 = owner(p0);
+    s_a0 = a0;
+    s_span = owner(p_end - 1) - a0 + 1;
+  }
+  __syncthreads();
+  const int32_t a0 = s_a0, span = s_span;
+  for (int i = threadIdx.x; i <= span; i += kExpBlock) soff[i] = act_off[a0 + i];
+  __syncthreads();
+  const uint32_t wmask = (1u << s) - 1;
+  int32_t lo_hint = 0;
+#pragma unroll 4
+  for (int it = 0; it < kExpItems; ++it) {
+    const int64_t p = p0 + (int64_t)it * kExpBlock + threadIdx.x;
+    if (p >= p_end) break;
+    int32_t lo = lo_hint, hi = span - 1;
+    while (lo < hi) {
+      const int32_t mid = (lo + hi + 1) >> 1;
+      if (soff[mid] <= p) lo = mid; else hi = mid - 1;
+    }
+    lo_hint = lo;
+    const int32_t a = a0 + lo;
+    const uint32_t local = (uint32_t)(p - soff[lo]);
+    const uint32_t L = (uint32_t)act_L[a];
+    const uint32_t ii = local / L;
+    const uint32_t kk = local - ii * L;
+    const int32_t ai = act_astart[a] + (int32_t)ii;
+    const int32_t bi = act_bstart[a] + (int32_t)kk;
+    const uint32_t k = (uint32_t)B2_crd[bi];
+    const int32_t e = act_eoff[a] + (int32_t)ii;
+    const int32_t dest = etab[(int64_t)e * W + (k >> s)] + (int32_t)kk;
+    klo[dest] = (uint16_t)(k & wmask);
+    pv[dest] = Ax[ai] * Bx[bi];
+  }
+}
+
+// Distinct columns per bucket: one wavefront per bucket, LDS bitmap of the
+// window.
+__global__ __launch_bounds__(kBkWaves * kWave) void k_bk_count(int32_t nb, int s, const int32_t* __restrict__ bstart,
+                                                               const uint16_t* __restrict__ klo,
+                                                               int32_t* __restrict__ bnnz) {
+  extern __shared__ uint32_t bk_bits[];
+  const int words = (1 << s) >> 5;
+  uint32_t* bm = bk_bits + (threadIdx.x / kWave) * words;
+  const int lane = lane_id();
+  for (int32_t b = (int32_t)(blockIdx.x * kBkWaves + threadIdx.x / kWave); b < nb;
+       b += (int32_t)(gridDim.x * kBkWaves)) {
+    const int32_t p0 = bstart[b], p1 = bstart[b + 1];
+    if (p0 == p1) {
+      if (lane == 0) bnnz[b] = 0;
+      continue;
+    }
+    for (int q = lane; q < words; q += kWave) bm[q] = 0u;
+    __builtin_amdgcn_wave_barrier();
+    for (int32_t p = p0 + lane; p < p1; p += kWave) {
+      const uint32_t k = klo[p];
+      atomicOr(&bm[k >> 5], 1u << (k & 31));
+    }
+    __builtin_amdgcn_wave_barrier();
+    int c = 0;
+    for (int q = lane; q < words; q += kWave) c += __popc(bm[q]);
+    c = wave_sum(c);
+    if (lane == 0) bnnz[b] = c;
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Row pointers: row r0 + r starts at its first bucket's output offset.
+__global__ __launch_bounds__(256) void k_bk_rowpos(int32_t R, int32_t W, int32_t r0, int64_t base,
+                                                   const int32_t* __restrict__ boff, int64_t* __restrict__ out_pos) {
+  for (int32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < R; r += gridDim.x * blockDim.x)
+    out_pos[r0 + r] = base + boff[(int64_t)r * W];
+}
+
+// Reduce: one wavefront per bucket.  The window is a dense LDS accumulator;
+// `seen` marks the columns written so far.  Products are taken in position
+// order, 64 at a time; lanes holding the same column find each other with
+// one ballot per column bit, and the lowest of them adds the group's values
+// in lane order to the accumulator, so each entry is summed in position (=
+// increasing j) order, starting from its first product.  The set columns are
+// then written in order at the bucket's output offset.
+__global__ __launch_bounds__(kBkWaves * kWave) void k_bk_reduce(int32_t nb, int32_t W, int s,
+                                                                const int32_t* __restrict__ bstart,
+                                                                const uint16_t* __restrict__ klo,
+                                                                const double* __restrict__ pv,
+                                                                const int32_t* __restrict__ boff, int64_t base,
+                                                                int32_t* __restrict__ out_crd,
+                                                                double* __restrict__ out_val) {
+  extern __shared__ double bk_lds[];
+  const int S = 1 << s, words = S >> 5;
+  const int lane = lane_id();
+  double* acc = bk_lds + (size_t)(threadIdx.x / kWave) * (S + kWave + words / 2);
+  double* stage = acc + S;
+  uint32_t* seen = reinterpret_cast<uint32_t*>(stage + kWave);
+  for (int32_t b = (int32_t)(blockIdx.x * kBkWaves + threadIdx.x / kWave); b < nb;
+       b += (int32_t)(gridDim.x * kBkWaves)) {
+    const int32_t p0 = bstart[b], p1 = bstart[b + 1];
+    if (p0 == p1) continue;
+    for (int q = lane; q < words; q += kWave) seen[q] = 0u;
+    __builtin_amdgcn_wave_barrier();
+    for (int32_t p = p0; p < p1; p += kWave * kBkUnroll) {
+      uint32_t kk[kBkUnroll];
+      double vv[kBkUnroll];
+#pragma unroll
+      for (int u = 0; u < kBkUnroll; ++u) {
+        const int32_t q = p + u * kWave + lane;
+        kk[u] = q < p1 ? (uint32_t)klo[q] : 0u;
+        vv[u] = q < p1 ? pv[q] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kBkUnroll; ++u) {
+        if (p + u * kWave >= p1) break;
+        const bool ok = p + u * kWave + lane < p1;
+        const uint32_t k = kk[u];
+        uint64_t peers = __ballot(ok);
+        for (int bit = 0; bit < s; ++bit) {
+          const bool set = ((k >> bit) & 1u) != 0;
+          const uint64_t mset = __ballot(set);
+          peers &= set ? mset : ~mset;
+        }
+        stage[lane] = vv[u];
+        __builtin_amdgcn_wave_barrier();
+        if (ok && __builtin_ctzll(peers) == (unsigned)lane) {
+          const uint32_t bit = 1u << (k & 31);
+          const bool was = (seen[k >> 5] & bit) != 0;
+          double x = was ? acc[k] + vv[u] : vv[u];
+          for (uint64_t rest = peers & (peers - 1); rest; rest &= rest - 1) x += stage[__builtin_ctzll(rest)];
+          acc[k] = x;
+          if (!was) atomicOr(&seen[k >> 5], bit);
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+    const int64_t ob = base + boff[b];
+    const int32_t wb = (b % W) << s;
+    int32_t carry = 0;
+    for (int q0 = 0; q0 < words; q0 += kWave) {
+      const int q = q0 + lane;
+      uint32_t bits = q < words ? seen[q] : 0u;
+      const int c = __popc(bits);
+      const int incl = wave_incl_scan(c);
+      int64_t r = ob + carry + incl - c;
+      while (bits) {
+        const int t = __builtin_ctz(bits);
+        out_crd[r] = wb + q * 32 + t;
+        out_val[r] = acc[q * 32 + t];
+        ++r;
+        bits &= bits - 1;
+      }
+      carry += __shfl(incl, kWave - 1, kWave);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
 static int bits_for(int64_t v) {  // bits to represent values in [0, v)
   int b = 0;
   while (b < 63 && ((int64_t)1 << b) < v) ++b;
   return b;
+}
+
+// Output columns/values: allocated at min(total products, free memory less
+// 2 GiB), grown (with a copy of what is written) when a chunk needs more.
+static int outer_output_init(spws_ctx* ctx, int32_t m, int64_t total_products) {
+  int rc;
+  if ((rc = ctx->out_pos.ensure(((size_t)m + 1) * 8))) return rc;
+  size_t free_b = 0, tot_b = 0;
+  SPWS_HIP(hipMemGetInfo(&free_b, &tot_b));
+  const size_t cur = ctx->out_crd.bytes / 4 < ctx->out_val.bytes / 8 ? ctx->out_crd.bytes / 4 : ctx->out_val.bytes / 8;
+  const size_t avail = (free_b > ((size_t)2 << 30) ? free_b - ((size_t)2 << 30) : 0) / 12 + cur;
+  size_t want = (size_t)(total_products > 0 ? total_products : 1);
+  if (want > avail) want = avail > 1 ? avail : 1;
+  if (cur < want) {
+    ctx->out_crd.release();
+    ctx->out_val.release();
+    if ((rc = ctx->out_crd.ensure(want * 4)) || (rc = ctx->out_val.ensure(want * 8))) return rc;
+  }
+  return SPWS_OK;
+}
+
+static int outer_output_grow(spws_ctx* ctx, int64_t base, int64_t U) {
+  hipStream_t s = ctx->stream;
+  int rc;
+  const size_t have = ctx->out_crd.bytes / 4 < ctx->out_val.bytes / 8 ? ctx->out_crd.bytes / 4 : ctx->out_val.bytes / 8;
+  if ((size_t)(base + U) <= have) return SPWS_OK;
+  const size_t want = (size_t)(base + U) + (size_t)(base + U) / 2;
+  DevBuf nc, nv;
+  if ((rc = nc.ensure(want * 4)) || (rc = nv.ensure(want * 8))) {
+    nc.release();
+    nv.release();
+    return rc;
+  }
+  SPWS_HIP(hipMemcpyAsync(nc.ptr, ctx->out_crd.ptr, (size_t)base * 4, hipMemcpyDeviceToDevice, s));
+  SPWS_HIP(hipMemcpyAsync(nv.ptr, ctx->out_val.ptr, (size_t)base * 8, hipMemcpyDeviceToDevice, s));
+  SPWS_HIP(hipStreamSynchronize(s));
+  ctx->out_crd.release();
+  ctx->out_val.release();
+  ctx->out_crd = nc;
+  ctx->out_val = nv;
+  nc.ptr = nv.ptr = nullptr;
+  return SPWS_OK;
 }
 
 struct OuterArgs {
@@ -346,22 +681,7 @@ static int outer_chunks(spws_ctx* ctx, const OuterArgs& a, const std::vector<int
   if ((rc = ctx->keys[0].ensure(cap * sizeof(K))) || (rc = ctx->keys[1].ensure(cap * sizeof(K))) ||
       (rc = ctx->vals[0].ensure(cap * 8)) || (rc = ctx->vals[1].ensure(cap * 8)))
     return rc;
-  // output: start at min(total products, what fits), grow when a chunk needs more
-  const int64_t total_products = row_prefix[m];
-  if ((rc = ctx->out_pos.ensure(((size_t)m + 1) * 8))) return rc;
-  {
-    size_t free_b = 0, tot_b = 0;
-    SPWS_HIP(hipMemGetInfo(&free_b, &tot_b));
-    const size_t cur = ctx->out_crd.bytes / 4 < ctx->out_val.bytes / 8 ? ctx->out_crd.bytes / 4 : ctx->out_val.bytes / 8;
-    const size_t avail = (free_b > ((size_t)2 << 30) ? free_b - ((size_t)2 << 30) : 0) / 12 + cur;
-    size_t want = (size_t)(total_products > 0 ? total_products : 1);
-    if (want > avail) want = avail > 1 ? avail : 1;
-    if (cur < want) {
-      ctx->out_crd.release();
-      ctx->out_val.release();
-      if ((rc = ctx->out_crd.ensure(want * 4)) || (rc = ctx->out_val.ensure(want * 8))) return rc;
-    }
-  }
+  if ((rc = outer_output_init(ctx, m, row_prefix[m]))) return rc;
   const int64_t ntiles_max = div_up((int64_t)cap, kRadixTile);
   DevBuf& heads = ctx->chunk_a;
   if ((rc = heads.ensure((size_t)(2 * ntiles_max + 2) * 4))) return rc;
@@ -390,7 +710,8 @@ static int outer_chunks(spws_ctx* ctx, const OuterArgs& a, const std::vector<int
       // entries leaves act_off[a] = P for every a >= the active count
       SPWS_HIP(hipMemsetAsync(act_cnt, 0, nac * 8, s));
       SPWS_LAUNCH(ctx, "outer_compact", k_outer_compact, dim3(gcol), dim3(256), 0, na, flag_off, col_lo, col_m,
-                  match, blen, a.B2_pos, act_astart, act_L, act_bstart, act_cnt);
+                  match, blen, a.B2_pos, act_astart, act_L, act_bstart, act_cnt, (int32_t*)nullptr,
+                  (int32_t*)nullptr);
       if ((rc = exclusive_scan<int64_t, int64_t>(ctx, act_cnt, act_off, na))) return rc;
       const unsigned gexp = (unsigned)div_up(P, kExpTile);
       SPWS_LAUNCH(ctx, "outer_expand", (k_outer_expand<K>), dim3(gexp), dim3(kExpBlock), 0, P, flag_off + na, act_off,
@@ -410,26 +731,7 @@ static int outer_chunks(spws_ctx* ctx, const OuterArgs& a, const std::vector<int
       SPWS_HIP(hipMemcpyAsync(hm, tile_uoff + ntiles, 4, hipMemcpyDeviceToHost, s));
       SPWS_HIP(hipStreamSynchronize(s));
       U = *reinterpret_cast<int32_t*>(hm);
-      // grow the output if this chunk does not fit
-      const size_t have = ctx->out_crd.bytes / 4 < ctx->out_val.bytes / 8 ? ctx->out_crd.bytes / 4
-                                                                          : ctx->out_val.bytes / 8;
-      if ((size_t)(base + U) > have) {
-        const size_t want = (size_t)(base + U) + (size_t)(base + U) / 2;
-        DevBuf nc, nv;
-        if ((rc = nc.ensure(want * 4)) || (rc = nv.ensure(want * 8))) {
-          nc.release();
-          nv.release();
-          return rc;
-        }
-        SPWS_HIP(hipMemcpyAsync(nc.ptr, ctx->out_crd.ptr, (size_t)base * 4, hipMemcpyDeviceToDevice, s));
-        SPWS_HIP(hipMemcpyAsync(nv.ptr, ctx->out_val.ptr, (size_t)base * 8, hipMemcpyDeviceToDevice, s));
-        SPWS_HIP(hipStreamSynchronize(s));
-        ctx->out_crd.release();
-        ctx->out_val.release();
-        ctx->out_crd = nc;
-        ctx->out_val = nv;
-        nc.ptr = nv.ptr = nullptr;
-      }
+      if ((rc = outer_output_grow(ctx, base, U))) return rc;
       SPWS_LAUNCH(ctx, "outer_reduce", (k_outer_reduce<K>), dim3((unsigned)ntiles), dim3(kRadixBlock), 0, kbuf, vbuf,
                   P, tile_uoff, kb, base, r0, ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>(), out_pos);
     }
@@ -439,6 +741,149 @@ static int outer_chunks(spws_ctx* ctx, const OuterArgs& a, const std::vector<int
   }
   SPWS_LAUNCH(ctx, "set_i64", k_set_i64, dim3(1), dim3(1), 0, out_pos + m, base);
   SPWS_HIP(hipStreamSynchronize(s));
+  *nnz_out = base;
+  return SPWS_OK;
+}
+
+// Bucket method over output-row chunks (see the file comment).  Chunks are
+// bounded by products (int32 slots), by the entry table (entries x windows)
+// and by the bucket count (rows x windows).
+static int outer_chunks_bucket(spws_ctx* ctx, const OuterArgs& a, const std::vector<int64_t>& row_prefix,
+                               const std::vector<int64_t>& pair_prefix, const int32_t* row_pairs, int s,
+                               int64_t max_products, int32_t max_rows, int64_t* nnz_out) {
+  hipStream_t st = ctx->stream;
+  int rc;
+  const int32_t m = a.m;
+  const int32_t na = a.a_ncols;
+  const size_t nac = (size_t)(na > 0 ? na : 1);
+  const int32_t W = (int32_t)std::max<int64_t>(1, div_up((int64_t)a.n, (int64_t)1 << s));
+  int32_t* match = ctx->obuf[0].as<int32_t>();
+  int32_t* blen = ctx->obuf[1].as<int32_t>();
+  if ((rc = ctx->obuf[2].ensure(nac * 4 * 3)) || (rc = ctx->obuf[3].ensure((nac + 1) * 4)) ||
+      (rc = ctx->obuf[4].ensure(nac * 4 * 5)) || (rc = ctx->obuf[5].ensure((nac + 1) * 8 * 2)) ||
+      (rc = ctx->obuf[6].ensure((nac + 1) * 4)) || (rc = ctx->bk[0].ensure(nac * (size_t)(W + 1) * 4)))
+    return rc;
+  int32_t* col_lo = ctx->obuf[2].as<int32_t>();
+  int32_t* col_m = col_lo + nac;
+  int32_t* flag = col_m + nac;
+  int32_t* flag_off = ctx->obuf[3].as<int32_t>();
+  int32_t* act_astart = ctx->obuf[4].as<int32_t>();
+  int32_t* act_L = act_astart + nac;
+  int32_t* act_bstart = act_L + nac;
+  int32_t* act_ja = act_bstart + nac;
+  int32_t* act_m = act_ja + nac;
+  int64_t* act_cnt = ctx->obuf[5].as<int64_t>();
+  int64_t* act_off = act_cnt + (nac + 1);
+  int32_t* act_eoff = ctx->obuf[6].as<int32_t>();
+  int32_t* bseg = ctx->bk[0].as<int32_t>();
+
+  const unsigned gcol = (unsigned)div_up(na > 0 ? na : 1, 256);
+  if (na > 0) {
+    const unsigned g = (unsigned)std::min<int64_t>(div_up((int64_t)na * (W + 1), 256), 16384);
+    SPWS_LAUNCH(ctx, "bk_bseg", k_bk_bseg, dim3(g), dim3(256), 0, na, W, s, match, blen, a.B2_pos, a.B2_crd, bseg);
+  }
+
+  constexpr int64_t kMaxEtab = (int64_t)1 << 28;
+  constexpr int64_t kMaxBuckets = (int64_t)1 << 27;
+  std::vector<std::pair<int32_t, int32_t>> chunks;
+  int64_t max_chunk = 0, max_en = 0, max_r = 0;
+  for (int32_t r0 = 0; r0 < m;) {
+    int32_t r1 = r0 + 1;
+    while (r1 < m && r1 - r0 < max_rows && row_prefix[r1 + 1] - row_prefix[r0] <= max_products &&
+           (pair_prefix[r1 + 1] - pair_prefix[r0]) * W <= kMaxEtab && (int64_t)(r1 + 1 - r0) * W <= kMaxBuckets)
+      ++r1;
+    chunks.push_back({r0, r1});
+    max_chunk = std::max(max_chunk, row_prefix[r1] - row_prefix[r0]);
+    max_en = std::max(max_en, pair_prefix[r1] - pair_prefix[r0]);
+    max_r = std::max<int64_t>(max_r, r1 - r0);
+    r0 = r1;
+  }
+  SPWS_REQUIRE(max_chunk < ((int64_t)1 << 31), SPWS_ERR_UNSUPPORTED,
+               "an output row range with %lld products exceeds the int32 workspace index", (long long)max_chunk);
+  const size_t cap = (size_t)(max_chunk > 0 ? max_chunk : 1);
+  const size_t ecap = (size_t)(max_en > 0 ? max_en : 1);
+  const size_t nbcap = (size_t)max_r * (size_t)W + 1;
+  if ((rc = ctx->keys[0].ensure(cap * 2)) || (rc = ctx->vals[0].ensure(cap * 8)) ||
+      (rc = ctx->bk[1].ensure(ecap * 4)) || (rc = ctx->bk[2].ensure(ecap * 4)) ||
+      (rc = ctx->bk[3].ensure(ecap * 8)) || (rc = ctx->bk[4].ensure(ecap * 8)) ||
+      (rc = ctx->bk[5].ensure(ecap * 4)) || (rc = ctx->bk[6].ensure(ecap * (size_t)W * 4)) ||
+      (rc = ctx->bk[7].ensure(((size_t)max_r + 1) * 4)) || (rc = ctx->bk[8].ensure(nbcap * 4 * 4)))
+    return rc;
+  if ((rc = outer_output_init(ctx, m, row_prefix[m]))) return rc;
+  uint16_t* klo = ctx->keys[0].as<uint16_t>();
+  double* pv = ctx->vals[0].as<double>();
+  int32_t* ent_ja = ctx->bk[5].as<int32_t>();
+  int32_t* etab = ctx->bk[6].as<int32_t>();
+  int32_t* rp_off = ctx->bk[7].as<int32_t>();
+  int32_t* bsize = ctx->bk[8].as<int32_t>();
+  int32_t* bstart = bsize + nbcap;
+  int32_t* bnnz = bstart + nbcap;
+  int32_t* boff = bnnz + nbcap;
+  int64_t* out_pos = ctx->out_pos.as<int64_t>();
+  int32_t* hm = ctx->host_meta.as<int32_t>();
+  const size_t lds_count = (size_t)kBkWaves * (((size_t)1 << s) >> 5) * 4;
+  const size_t lds_reduce = (size_t)kBkWaves * (((size_t)1 << s) + kWave + (((size_t)1 << s) >> 6)) * 8;
+  ctx->stats.outer_method = 2;
+  ctx->stats.window_bits = s;
+  int64_t base = 0;
+
+  for (auto& ch : chunks) {
+    const int32_t r0 = ch.first, r1 = ch.second, R = r1 - r0;
+    const int64_t P = row_prefix[r1] - row_prefix[r0];
+    const int32_t En = (int32_t)(pair_prefix[r1] - pair_prefix[r0]);
+    const int32_t NB = R * W;
+    ctx->stats.chunks++;
+    int64_t U = 0;
+    if (P > 0) {
+      SPWS_LAUNCH(ctx, "outer_colrange", k_outer_colrange, dim3(gcol), dim3(256), 0, na, r0, r1, a.A2_pos, a.A2_crd,
+                  blen, col_lo, col_m, flag);
+      if ((rc = exclusive_scan<int32_t, int32_t>(ctx, flag, flag_off, na))) return rc;
+      SPWS_HIP(hipMemsetAsync(act_cnt, 0, nac * 8, st));
+      SPWS_HIP(hipMemsetAsync(act_m, 0, nac * 4, st));
+      SPWS_LAUNCH(ctx, "outer_compact", k_outer_compact, dim3(gcol), dim3(256), 0, na, flag_off, col_lo, col_m,
+                  match, blen, a.B2_pos, act_astart, act_L, act_bstart, act_cnt, act_ja, act_m);
+      if ((rc = exclusive_scan<int64_t, int64_t>(ctx, act_cnt, act_off, na))) return rc;
+      if ((rc = exclusive_scan<int32_t, int32_t>(ctx, act_m, act_eoff, na))) return rc;
+      uint32_t* pkey = ctx->bk[1].as<uint32_t>();
+      double* pval = ctx->bk[3].as<double>();
+      const unsigned ge = (unsigned)std::min<int64_t>(div_up(En, 256), 16384);
+      SPWS_LAUNCH(ctx, "bk_entries", k_bk_entries, dim3(ge), dim3(256), 0, En, flag_off + na, act_eoff, act_astart,
+                  act_ja, a.A2_crd, r0, pkey, pval, ent_ja);
+      bool alt = false;
+      if ((rc = radix_sort_pairs<uint32_t>(ctx, pkey, pval, ctx->bk[2].as<uint32_t>(), ctx->bk[4].as<double>(), En, 0,
+                                           std::max(1, bits_for(R)), &alt)))
+        return rc;
+      if (alt) pval = ctx->bk[4].as<double>();
+      if ((rc = exclusive_scan<int32_t, int32_t>(ctx, row_pairs + r0, rp_off, R))) return rc;
+      const unsigned gr = (unsigned)std::min<int64_t>(div_up((int64_t)R * kWave, 256), 16384);
+      SPWS_LAUNCH(ctx, "bk_rowsize", k_bk_rows<0>, dim3(gr), dim3(256), 0, R, W, rp_off, pval, ent_ja, bseg,
+                  (const int32_t*)nullptr, bsize);
+      if ((rc = exclusive_scan<int32_t, int32_t>(ctx, bsize, bstart, NB))) return rc;
+      SPWS_LAUNCH(ctx, "bk_etab", k_bk_rows<1>, dim3(gr), dim3(256), 0, R, W, rp_off, pval, ent_ja, bseg, bstart,
+                  etab);
+      const unsigned gexp = (unsigned)div_up(P, kExpTile);
+      SPWS_LAUNCH(ctx, "bk_expand", k_bk_expand, dim3(gexp), dim3(kExpBlock), 0, P, flag_off + na, act_off,
+                  act_astart, act_L, act_bstart, act_eoff, a.Ax, a.B2_crd, a.Bx, W, s, etab, klo, pv);
+      const unsigned gb = (unsigned)std::min<int64_t>(div_up(NB, kBkWaves), 65536);
+      SPWS_LAUNCH(ctx, "bk_count", k_bk_count, dim3(gb), dim3(kBkWaves * kWave), lds_count, NB, s, bstart, klo,
+                  bnnz);
+      if ((rc = exclusive_scan<int32_t, int32_t>(ctx, bnnz, boff, NB))) return rc;
+      SPWS_HIP(hipMemcpyAsync(hm, boff + NB, 4, hipMemcpyDeviceToHost, st));
+      SPWS_HIP(hipStreamSynchronize(st));
+      U = *hm;
+      if ((rc = outer_output_grow(ctx, base, U))) return rc;
+      SPWS_LAUNCH(ctx, "bk_rowpos", k_bk_rowpos, dim3((unsigned)std::min<int64_t>(div_up(R, 256), 4096)), dim3(256), 0,
+                  R, W, r0, base, boff, out_pos);
+      SPWS_LAUNCH(ctx, "bk_reduce", k_bk_reduce, dim3(gb), dim3(kBkWaves * kWave), lds_reduce, NB, W, s, bstart, klo,
+                  pv, boff, base, ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>());
+    } else {
+      SPWS_LAUNCH(ctx, "outer_tail", (k_outer_tail<uint32_t>), dim3((unsigned)div_up(R, 256)), dim3(256), 0,
+                  (const uint32_t*)nullptr, (int64_t)0, 0, r0, R, base, out_pos);
+    }
+    base += U;
+  }
+  SPWS_LAUNCH(ctx, "set_i64", k_set_i64, dim3(1), dim3(1), 0, out_pos + m, base);
+  SPWS_HIP(hipStreamSynchronize(st));
   *nnz_out = base;
   return SPWS_OK;
 }
@@ -457,35 +902,70 @@ static int outer_run(spws_ctx* ctx, const OuterArgs& a, int64_t* nnz_out) {
   const size_t nac = (size_t)(a.a_ncols > 0 ? a.a_ncols : 1);
   const size_t mm = (size_t)(m > 0 ? m : 1);
   if ((rc = ctx->obuf[0].ensure(nac * 4)) || (rc = ctx->obuf[1].ensure(nac * 4)) ||
-      (rc = ctx->row_ub.ensure(mm * 8)))
+      (rc = ctx->row_ub.ensure(mm * 8)) || (rc = ctx->obuf[7].ensure(mm * 4)))
     return rc;
   int32_t* match = ctx->obuf[0].as<int32_t>();
   int32_t* blen = ctx->obuf[1].as<int32_t>();
+  int32_t* row_pairs = ctx->obuf[7].as<int32_t>();
   SPWS_HIP(hipMemsetAsync(ctx->row_ub.ptr, 0, mm * 8, s));
+  SPWS_HIP(hipMemsetAsync(row_pairs, 0, mm * 4, s));
   if (a.a_ncols > 0) {
     const unsigned g = (unsigned)div_up(a.a_ncols, 256);
     SPWS_LAUNCH(ctx, "outer_match", k_outer_match, dim3(g), dim3(256), 0, a.a_ncols, a.A1_crd, a.b_nrows, a.B1_crd,
                 a.B2_pos, match, blen);
     const unsigned g2 = g < 4096 ? g : 4096;
     SPWS_LAUNCH(ctx, "outer_row_ub", k_outer_row_ub, dim3(g2), dim3(256), 0, a.a_ncols, a.A2_pos, a.A2_crd, blen,
-                (unsigned long long*)ctx->row_ub.ptr);
+                (unsigned long long*)ctx->row_ub.ptr, row_pairs);
   }
-  std::vector<int64_t> ub(mm), prefix((size_t)m + 1, 0);
+  std::vector<int64_t> ub(mm), prefix((size_t)m + 1, 0), pair_prefix((size_t)m + 1, 0);
+  std::vector<int32_t> pairs(mm);
   if (m > 0) {
     SPWS_HIP(hipMemcpyAsync(ub.data(), ctx->row_ub.ptr, (size_t)m * 8, hipMemcpyDeviceToHost, s));
+    SPWS_HIP(hipMemcpyAsync(pairs.data(), row_pairs, (size_t)m * 4, hipMemcpyDeviceToHost, s));
     SPWS_HIP(hipStreamSynchronize(s));
   }
-  for (int32_t i = 0; i < m; ++i) prefix[i + 1] = prefix[i] + ub[i];
+  int64_t busy_rows = 0;
+  for (int32_t i = 0; i < m; ++i) {
+    prefix[i + 1] = prefix[i] + ub[i];
+    pair_prefix[i + 1] = pair_prefix[i] + pairs[i];
+    busy_rows += ub[i] > 0;
+  }
   ctx->stats.products = prefix[m];
-  const int kb = std::max(1, bits_for(a.n));
   const int64_t max_products = ctx->outer_max_products > 0 ? ctx->outer_max_products : ((int64_t)1 << 30);
-  int64_t max_rows = ctx->outer_max_rows > 0 ? ctx->outer_max_rows : INT32_MAX;
-  if (kb <= 24) {
-    max_rows = std::min<int64_t>(max_rows, (int64_t)1 << (32 - kb));
-    rc = outer_chunks<uint32_t>(ctx, a, prefix, kb, max_products, (int32_t)max_rows, nnz_out);
+  const int64_t user_rows = ctx->outer_max_rows > 0 ? ctx->outer_max_rows : INT32_MAX;
+
+  // method: buckets unless the windows would hold too few products each
+  // (very sparse, very wide outputs) -- then the LSD sort over whole chunks
+  int method = ctx->outer_method;
+  int wlog = ctx->outer_window_log2;
+  if (wlog == 0) {
+    // about 1024 products per (row, window) bucket on average
+    const double avg = busy_rows > 0 ? (double)prefix[m] / (double)busy_rows : 1.0;
+    const double want = 1024.0 * (double)std::max(a.n, 1) / std::max(avg, 1.0);
+    wlog = kBkMinLog2;
+    while (wlog < kBkMaxLog2 && (double)((int64_t)1 << wlog) * 1.41421356 < want) ++wlog;
+  }
+  const int64_t W = std::max<int64_t>(1, div_up((int64_t)a.n, (int64_t)1 << wlog));
+  if (method == 0) {
+    const double per_bucket = busy_rows > 0 ? (double)prefix[m] / ((double)busy_rows * (double)W) : 0.0;
+    int32_t max_pairs = 0;
+    for (int32_t i = 0; i < m; ++i) max_pairs = std::max(max_pairs, pairs[i]);
+    method = (per_bucket >= 64.0 && W <= 4096 && (int64_t)max_pairs * W <= ((int64_t)1 << 28)) ? 2 : 1;
+  }
+  if (method == 2) {
+    rc = outer_chunks_bucket(ctx, a, prefix, pair_prefix, row_pairs, wlog, max_products,
+                             (int32_t)std::min<int64_t>(user_rows, INT32_MAX), nnz_out);
   } else {
-    max_rows = std::min<int64_t>(max_rows, (int64_t)1 << std::min(31, 63 - kb));
-    rc = outer_chunks<uint64_t>(ctx, a, prefix, kb, max_products, (int32_t)max_rows, nnz_out);
+    ctx->stats.outer_method = 1;
+    const int kb = std::max(1, bits_for(a.n));
+    int64_t max_rows = user_rows;
+    if (kb <= 24) {
+      max_rows = std::min<int64_t>(max_rows, (int64_t)1 << (32 - kb));
+      rc = outer_chunks<uint32_t>(ctx, a, prefix, kb, max_products, (int32_t)max_rows, nnz_out);
+    } else {
+      max_rows = std::min<int64_t>(max_rows, (int64_t)1 << std::min(31, 63 - kb));
+      rc = outer_chunks<uint64_t>(ctx, a, prefix, kb, max_products, (int32_t)max_rows, nnz_out);
+    }
   }
   if (rc) return rc;
   ctx->stats.nnz = *nnz_out;
@@ -503,6 +983,16 @@ int spws_ctx_set_outer_chunking(spws_ctx* ctx, int64_t max_products, int32_t max
   SPWS_REQUIRE(max_products >= 0 && max_rows >= 0, SPWS_ERR_INVALID, "negative chunk limit");
   ctx->outer_max_products = max_products;
   ctx->outer_max_rows = max_rows;
+  return SPWS_OK;
+}
+
+int spws_ctx_set_outer_method(spws_ctx* ctx, int32_t method, int32_t window_log2) {
+  SPWS_REQUIRE(ctx != nullptr, SPWS_ERR_INVALID, "null context");
+  SPWS_REQUIRE(method >= 0 && method <= 2, SPWS_ERR_INVALID, "outer method must be 0, 1 or 2");
+  SPWS_REQUIRE(window_log2 == 0 || (window_log2 >= kBkMinLog2 && window_log2 <= kBkMaxLog2), SPWS_ERR_INVALID,
+               "window_log2 must be 0 or in [%d, %d]", kBkMinLog2, kBkMaxLog2);
+  ctx->outer_method = method;
+  ctx->outer_window_log2 = window_log2;
   return SPWS_OK;
 }
 

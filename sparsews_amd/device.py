@@ -109,7 +109,8 @@ class Context:
         s = _lib.Stats()
         check(lib().spws_ctx_get_stats(self.h, C.byref(s)))
         return {"products": s.products, "nnz": s.nnz, "rows_per_path": list(s.rows_per_path),
-                "chunks": s.chunks, "sort_bits": s.sort_bits}
+                "chunks": s.chunks, "sort_bits": s.sort_bits, "outer_method": s.outer_method,
+                "window_bits": s.window_bits}
 
     # ---- kernel timing (HIP events around every launch on the context's stream)
     def set_profiling(self, on: bool):
@@ -239,6 +240,11 @@ class DeviceCSR64:
 
 def set_outer_chunking(ctx: Context, max_products: int = 0, max_rows: int = 0):
     check(lib().spws_ctx_set_outer_chunking(ctx.h, int(max_products), int(max_rows)))
+
+
+def set_outer_method(ctx: Context, method: int = 0, window_log2: int = 0):
+    """0 automatic, 1 LSD radix sort, 2 row/window buckets (see spws.h)."""
+    check(lib().spws_ctx_set_outer_method(ctx.h, int(method), int(window_log2)))
 
 
 def spgemm_outer(ctx: Context, A: DeviceDCSR, B: DeviceDCSR, m: int, k: int, n: int) -> DeviceCSR64:

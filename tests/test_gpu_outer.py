@@ -1,21 +1,34 @@
 """Outer-product SpGEMM (DCSC x DCSR, expand / radix sort / segmented reduce)
-on the GPU vs the CPU oracle, through the C ABI.
+on the GPU vs the CPU oracle, through the C ABI, with both sort methods
+(1: LSD radix sort of packed keys; 2: row/column-window buckets).
 
 Structure must be bit-exact.  Values: the device sums each entry over j in
 increasing order, the order of the oracle's Gustavson restatement, so the
 values are compared bit-for-bit as well as against the 1e-12 bar.
 """
+import contextlib
+
 import numpy as np
 import pytest
 
 import oracle
 from helpers import powerlaw_rows, random_csr, uniform_rows
 from sparsews_amd import formats, gen
-from sparsews_amd.device import DeviceDCSR, set_outer_chunking, spgemm_outer
+from sparsews_amd.device import DeviceDCSR, set_outer_chunking, set_outer_method, spgemm_outer
 from sparsews_amd.gen import CSR
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
+METHODS = [1, 2]
+
+
+@contextlib.contextmanager
+def outer_method(ctx, method, window_log2=0):
+    set_outer_method(ctx, method, window_log2)
+    try:
+        yield
+    finally:
+        set_outer_method(ctx, 0, 0)
 
 
 def run_outer(ctx, A, B, dense_outer=False):
@@ -39,55 +52,115 @@ def check(C, A, B, exact_order=True, nthreads=0):
         np.testing.assert_array_equal(C.vals, G.vals)
 
 
+@pytest.mark.parametrize("method", METHODS)
 @pytest.mark.parametrize("seed", [60, 61])
-def test_outer_powerlaw(ctx, seed):
-    print(f"case: outer power-law seed {seed}", flush=True)
+def test_outer_powerlaw(ctx, seed, method):
+    print(f"case: outer power-law seed {seed} method {method}", flush=True)
     A, B = powerlaw_rows(600, 500, 1.8, 300, seed, 0), powerlaw_rows(500, 700, 1.8, 300, seed, 1)
-    check(run_outer(ctx, A, B), A, B)
+    with outer_method(ctx, method):
+        C = run_outer(ctx, A, B)
+        assert ctx.stats()["outer_method"] == method
+    check(C, A, B)
 
 
-def test_outer_dense_outer_levels(ctx):
-    print("case: outer CSC x CSR", flush=True)
+@pytest.mark.parametrize("method", METHODS)
+def test_outer_dense_outer_levels(ctx, method):
+    print(f"case: outer CSC x CSR method {method}", flush=True)
     A, B = uniform_rows(300, 200, 0, 15, 62, 0), uniform_rows(200, 260, 0, 15, 62, 1)
-    check(run_outer(ctx, A, B, dense_outer=True), A, B)
+    with outer_method(ctx, method):
+        C = run_outer(ctx, A, B, dense_outer=True)
+    check(C, A, B)
 
 
-def test_outer_many_chunks(ctx):
-    print("case: outer many chunks", flush=True)
+@pytest.mark.parametrize("method", METHODS)
+def test_outer_many_chunks(ctx, method):
+    print(f"case: outer many chunks method {method}", flush=True)
     A, B = powerlaw_rows(700, 600, 1.8, 300, 63, 0), powerlaw_rows(600, 650, 1.8, 300, 63, 1)
     try:
         set_outer_chunking(ctx, 3000, 7)
-        C = run_outer(ctx, A, B)
-        chunks = ctx.stats()["chunks"]
+        with outer_method(ctx, method):
+            C = run_outer(ctx, A, B)
+            chunks = ctx.stats()["chunks"]
     finally:
         set_outer_chunking(ctx, 0, 0)
     assert chunks >= 100
     check(C, A, B)
 
 
-def test_outer_cc_gaps(ctx):
-    print("case: outer co-iteration gaps", flush=True)
+@pytest.mark.parametrize("method", METHODS)
+def test_outer_cc_gaps(ctx, method):
+    print(f"case: outer co-iteration gaps method {method}", flush=True)
     A = uniform_rows(80, 120, 0, 8, 64, 0)
     B = uniform_rows(120, 90, 0, 8, 64, 1)
     lens = B.row_lengths().copy()
     lens[::3] = 0
     B2 = random_csr(120, 90, lens, 64, 5)
-    check(run_outer(ctx, A, B2), A, B2)
+    with outer_method(ctx, method):
+        C = run_outer(ctx, A, B2)
+    check(C, A, B2)
 
 
-def test_outer_edge_cases(ctx):
-    print("case: outer edge cases", flush=True)
+@pytest.mark.parametrize("method", METHODS)
+def test_outer_edge_cases(ctx, method):
+    print(f"case: outer edge cases method {method}", flush=True)
     A = CSR(6, 5, np.zeros(7, dtype=np.int32), np.zeros(0, dtype=np.int32), np.zeros(0))
     B = uniform_rows(5, 9, 1, 4, 65, 1)
-    C = run_outer(ctx, A, B)
+    with outer_method(ctx, method):
+        C = run_outer(ctx, A, B)
     assert C.nnz == 0 and np.all(C.pos == 0) and len(C.pos) == 7
     n = 150
     I = CSR(n, n, np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), np.ones(n))
     Bm = uniform_rows(n, 130, 0, 20, 66, 1)
-    CI = run_outer(ctx, I, Bm)
+    with outer_method(ctx, method):
+        CI = run_outer(ctx, I, Bm)
     np.testing.assert_array_equal(CI.pos, Bm.pos)
     np.testing.assert_array_equal(CI.crd, Bm.crd)
     np.testing.assert_array_equal(CI.vals, Bm.vals)
+
+
+@pytest.mark.parametrize("wlog", [6, 8, 11])
+def test_outer_bucket_windows(ctx, wlog):
+    """Window widths from 64 to 2048 columns; the column count is not a
+    multiple of the window, and some B rows run over many windows."""
+    print(f"case: outer bucket window 2^{wlog}", flush=True)
+    A = powerlaw_rows(500, 400, 1.8, 250, 68, 0)
+    B = powerlaw_rows(400, 3001, 1.8, 900, 68, 1)
+    with outer_method(ctx, 2, wlog):
+        C = run_outer(ctx, A, B)
+        st = ctx.stats()
+    assert st["outer_method"] == 2 and st["window_bits"] == wlog
+    check(C, A, B)
+
+
+def test_outer_bucket_long_column_runs(ctx):
+    """Outputs where one entry gathers many products (dense rows of A against
+    a B whose rows all share a few columns), so one 64-lane stride holds
+    several products of the same column and of different j."""
+    print("case: outer bucket repeated columns", flush=True)
+    rng = np.random.default_rng(69)
+    A = uniform_rows(90, 300, 150, 300, 69, 0)
+    lens = rng.integers(1, 6, 300)
+    cols = np.concatenate([np.sort(rng.choice(8, size=l, replace=False)) for l in lens]).astype(np.int32)
+    pos = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    B = CSR(300, 8, pos, cols, rng.uniform(-1, 1, len(cols)))
+    with outer_method(ctx, 2, 6):
+        C = run_outer(ctx, A, B)
+    check(C, A, B)
+
+
+def test_outer_auto_method(ctx):
+    """Automatic choice: buckets for the power-law workload shape, the LSD
+    sort for very wide, very sparse outputs."""
+    print("case: outer automatic method", flush=True)
+    B = gen.zipf_csr(4000, 1.8, 400, seed=3, stream=2)
+    dB = DeviceDCSR.upload(ctx, formats.dcsr_from_csr(B))
+    spgemm_outer(ctx, dB, dB, 4000, 4000, 4000)
+    assert ctx.stats()["outer_method"] == 2
+    A = uniform_rows(100, 80, 0, 3, 70, 0)
+    Bw = random_csr(80, 1 << 22, np.full(80, 3), 70, 1)
+    C = run_outer(ctx, A, Bw)
+    assert ctx.stats()["outer_method"] == 1
+    check(C, A, Bw)
 
 
 def test_outer_wide_columns_64bit_keys(ctx):
@@ -96,19 +169,22 @@ def test_outer_wide_columns_64bit_keys(ctx):
     n = (1 << 24) + 123
     A = uniform_rows(200, 150, 0, 10, 67, 0)
     B = random_csr(150, n, np.random.default_rng(67).integers(0, 12, 150), 67, 1)
-    C = run_outer(ctx, A, B)
-    assert ctx.stats()["sort_bits"] > 32
+    with outer_method(ctx, 1):
+        C = run_outer(ctx, A, B)
+        assert ctx.stats()["sort_bits"] > 32
     check(C, A, B, nthreads=1)
 
 
-def test_outer_workload_scaled(ctx):
+@pytest.mark.parametrize("method", METHODS)
+def test_outer_workload_scaled(ctx, method):
     """The outer-product generator at N = 20,000, cap 2,000, A = B^T."""
-    print("case: outer scaled workload", flush=True)
+    print(f"case: outer scaled workload method {method}", flush=True)
     B = gen.zipf_csr(20_000, 1.8, 2_000, seed=1, stream=2)
     A = B.transpose()
     dA = DeviceDCSR.upload(ctx, formats.dcsr_from_csr(B))  # CSC(A) = B's arrays
     dB = DeviceDCSR.upload(ctx, formats.dcsr_from_csr(B))
-    C = spgemm_outer(ctx, dA, dB, 20_000, 20_000, 20_000).download()
+    with outer_method(ctx, method):
+        C = spgemm_outer(ctx, dA, dB, 20_000, 20_000, 20_000).download()
     check(C, A, B)
 
 
