@@ -83,15 +83,17 @@ __global__ __launch_bounds__(256) void k_outer_match(int32_t a_ncols, const int3
   blen[ja] = jb >= 0 ? B2_pos[jb + 1] - B2_pos[jb] : 0;
 }
 
+// One wavefront per stored column of A (columns range from 1 to ~2e4 rows).
 __global__ __launch_bounds__(256) void k_outer_row_ub(int32_t a_ncols, const int32_t* __restrict__ A2_pos,
                                                       const int32_t* __restrict__ A2_crd,
                                                       const int32_t* __restrict__ blen,
                                                       unsigned long long* __restrict__ row_ub,
                                                       int32_t* __restrict__ row_pairs) {
-  for (int32_t ja = blockIdx.x * blockDim.x + threadIdx.x; ja < a_ncols; ja += gridDim.x * blockDim.x) {
+  const int32_t nw = (int32_t)(gridDim.x * blockDim.x / kWave);
+  for (int32_t ja = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kWave); ja < a_ncols; ja += nw) {
     const int32_t L = blen[ja];
     if (L == 0) continue;
-    for (int32_t p = A2_pos[ja]; p < A2_pos[ja + 1]; ++p) {
+    for (int32_t p = A2_pos[ja] + lane_id(); p < A2_pos[ja + 1]; p += kWave) {
       atomicAdd(&row_ub[A2_crd[p]], (unsigned long long)L);
       atomicAdd(&row_pairs[A2_crd[p]], 1);
     }
@@ -321,7 +323,7 @@ __global__ void k_set_i64(int64_t* p, int64_t v) { *p = v; }
 // ------------------------------------------------------------ bucket method
 
 constexpr int kBkWaves = 2;   // wavefronts per workgroup in the per-bucket kernels
-constexpr int kBkUnroll = 4;  // strides of products loaded before they are used
+constexpr int kBkUnroll = 8;  // strides of products loaded before they are used
 constexpr int kBkMinLog2 = 6;
 constexpr int kBkMaxLog2 = 11;
 
@@ -353,7 +355,7 @@ __global__ __launch_bounds__(256) void k_bk_entries(int32_t En, const int32_t* _
                                                     const int32_t* __restrict__ act_ja,
                                                     const int32_t* __restrict__ A2_crd, int32_t r0,
                                                     uint32_t* __restrict__ pkey, double* __restrict__ pval,
-                                                    int32_t* __restrict__ ent_ja) {
+                                                    int32_t* __restrict__ ent_ja, int32_t* __restrict__ ent_ai) {
   const int32_t n_act = *n_act_ptr;
   for (int32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < En; e += gridDim.x * blockDim.x) {
     int32_t lo = 0, hi = n_act - 1;  // last active column c with act_eoff[c] <= e
@@ -365,6 +367,7 @@ __global__ __launch_bounds__(256) void k_bk_entries(int32_t En, const int32_t* _
     pkey[e] = (uint32_t)(A2_crd[ai] - r0);
     pval[e] = __longlong_as_double((long long)e);
     ent_ja[e] = act_ja[lo];
+    ent_ai[e] = ai;
   }
 }
 
@@ -401,72 +404,83 @@ __global__ __launch_bounds__(256) void k_bk_rows(int32_t R, int32_t W, const int
   }
 }
 
-// Expand: the product of A entry e (active column a, offset ii) and B entry
-// kk of the matched row goes to slot etab[e*W + (k >> s)] + kk of its
-// (row, window) bucket; the slot holds k's offset in the window and the
-// product.  Tiles and owner search as in k_outer_expand.
-__global__ __launch_bounds__(kExpBlock) void k_bk_expand(int64_t P, const int32_t* __restrict__ n_act_ptr,
-                                                         const int64_t* __restrict__ act_off,
-                                                         const int32_t* __restrict__ act_astart,
-                                                         const int32_t* __restrict__ act_L,
-                                                         const int32_t* __restrict__ act_bstart,
-                                                         const int32_t* __restrict__ act_eoff,
-                                                         const double* __restrict__ Ax,
-                                                         const int32_t* __restrict__ B2_crd,
-                                                         const double* __restrict__ Bx, int32_t W, int s,
-                                                         const int32_t* __restrict__ etab,
-                                                         uint16_t* __restrict__ klo, double* __restrict__ pv) {
-  __shared__ int64_t soff[kExpTile + 1];
-  __shared__ int32_t s_a0, s_span;
-  const int64_t p0 = (int64_t)blockIdx.x * kExpTile;
-  const int64_t p_end = p0 + kExpTile < P ? p0 + kExpTile : P;
-  if (threadIdx.x == 0) {
-    const int32_t n_act = *n_act_ptr;
-    auto owner = [&](int64_t x) {
-      int32_t lo = 0, hi = n_act - 1;
-      while (lo < hi) {
-        const int32_t mid = (lo + hi + 1) >> 1;
-        if (act_off[mid] <= x) lo = mid; else hi = mid - 1;
-      }
-      return lo;
-    };
-    const int32_t a0 = owner(p0);
-    s_a0 = a0;
-    s_span = owner(p_end - 1) - a0 + 1;
-  }
-  __syncthreads();
-  const int32_t a0 = s_a0, span = s_span;
-  for (int i = threadIdx.x; i <= span; i += kExpBlock) soff[i] = act_off[a0 + i];
-  __syncthreads();
+// Expand: one wavefront per A entry e = A(i,j) of the chunk, lanes over the
+// matched row B(j,:).  The product with B entry kk goes to slot
+// etab[e*W + (k >> s)] + kk of its (row, window) bucket; consecutive kk of
+// one window land in consecutive slots, so the stores are runs.  The slot
+// holds k's offset in the window and the product.  Entries are taken in row
+// order (the sorted entry list), so the wavefronts in flight fill the
+// adjacent runs of the same buckets and the stores stay within a few DRAM
+// pages.
+__global__ __launch_bounds__(256) void k_bk_expand(int32_t En, const double* __restrict__ order,
+                                                   const int32_t* __restrict__ ent_ai,
+                                                   const int32_t* __restrict__ ent_ja,
+                                                   const int32_t* __restrict__ match,
+                                                   const int32_t* __restrict__ blen,
+                                                   const int32_t* __restrict__ B2_pos,
+                                                   const double* __restrict__ Ax,
+                                                   const int32_t* __restrict__ B2_crd,
+                                                   const double* __restrict__ Bx, int32_t W, int s,
+                                                   const int32_t* __restrict__ etab, uint16_t* __restrict__ klo,
+                                                   double* __restrict__ pv) {
+  constexpr int U = 8;
   const uint32_t wmask = (1u << s) - 1;
-  int32_t lo_hint = 0;
-#pragma unroll 4
-  for (int it = 0; it < kExpItems; ++it) {
-    const int64_t p = p0 + (int64_t)it * kExpBlock + threadIdx.x;
-    if (p >= p_end) break;
-    int32_t lo = lo_hint, hi = span - 1;
-    while (lo < hi) {
-      const int32_t mid = (lo + hi + 1) >> 1;
-      if (soff[mid] <= p) lo = mid; else hi = mid - 1;
+  const int lane = lane_id();
+  const int32_t nw = (int32_t)(gridDim.x * blockDim.x / kWave);
+  for (int32_t q = __builtin_amdgcn_readfirstlane((int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kWave)); q < En;
+       q += nw) {
+    // wave-uniform entry: its metadata comes through scalar loads
+    const int32_t e = __builtin_amdgcn_readfirstlane((int32_t)__double_as_longlong(order[q]));
+    const int32_t ja = __builtin_amdgcn_readfirstlane(ent_ja[e]);
+    const int32_t L = __builtin_amdgcn_readfirstlane(blen[ja]);
+    const int32_t bs = __builtin_amdgcn_readfirstlane(B2_pos[__builtin_amdgcn_readfirstlane(match[ja])]);
+    const double av = Ax[__builtin_amdgcn_readfirstlane(ent_ai[e])];
+    const int32_t* et = etab + (int64_t)e * W;
+    // software pipeline: the next block of B is loaded before this block's
+    // stores are issued, so waiting for those loads never waits for stores
+    uint32_t kc[U];
+    double bv[U];
+    auto load = [&](int32_t k0, uint32_t (&kd)[U], double (&bd)[U]) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        // clamped index: every lane loads, lanes past the row are masked at the store
+        const int32_t kk = min(k0 + u * kWave + lane, L - 1);
+        kd[u] = (uint32_t)B2_crd[bs + kk];
+        bd[u] = Bx[bs + kk];
+      }
+    };
+    // two register sets, alternating blocks: no copies between them
+    auto block = [&](int32_t k0, const uint32_t (&kd)[U], const double (&bd)[U], uint32_t (&kn)[U],
+                     double (&bn)[U]) {
+      int32_t dst[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) dst[u] = et[kd[u] >> s];
+      if (k0 + kWave * U < L) load(k0 + kWave * U, kn, bn);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t kk = k0 + u * kWave + lane;
+        if (kk < L) {
+          const int32_t d = dst[u] + kk;
+          klo[d] = (uint16_t)(kd[u] & wmask);
+          pv[d] = av * bd[u];
+        }
+      }
+    };
+    uint32_t k2[U];
+    double b2[U];
+    load(0, kc, bv);
+    for (int32_t k0 = 0; k0 < L; k0 += 2 * kWave * U) {
+      block(k0, kc, bv, k2, b2);
+      if (k0 + kWave * U >= L) break;
+      block(k0 + kWave * U, k2, b2, kc, bv);
     }
-    lo_hint = lo;
-    const int32_t a = a0 + lo;
-    const uint32_t local = (uint32_t)(p - soff[lo]);
-    const uint32_t L = (uint32_t)act_L[a];
-    const uint32_t ii = local / L;
-    const uint32_t kk = local - ii * L;
-    const int32_t ai = act_astart[a] + (int32_t)ii;
-    const int32_t bi = act_bstart[a] + (int32_t)kk;
-    const uint32_t k = (uint32_t)B2_crd[bi];
-    const int32_t e = act_eoff[a] + (int32_t)ii;
-    const int32_t dest = etab[(int64_t)e * W + (k >> s)] + (int32_t)kk;
-    klo[dest] = (uint16_t)(k & wmask);
-    pv[dest] = Ax[ai] * Bx[bi];
   }
 }
 
 // Distinct columns per bucket: one wavefront per bucket, LDS bitmap of the
-// window.
+// window.  Columns are loaded kBkUnroll strides at a time, and the first
+// strides of the wavefront's next bucket are loaded before the current
+// bucket's bitmap is counted.
 __global__ __launch_bounds__(kBkWaves * kWave) void k_bk_count(int32_t nb, int s, const int32_t* __restrict__ bstart,
                                                                const uint16_t* __restrict__ klo,
                                                                int32_t* __restrict__ bnnz) {
@@ -474,24 +488,45 @@ __global__ __launch_bounds__(kBkWaves * kWave) void k_bk_count(int32_t nb, int s
   const int words = (1 << s) >> 5;
   uint32_t* bm = bk_bits + (threadIdx.x / kWave) * words;
   const int lane = lane_id();
-  for (int32_t b = (int32_t)(blockIdx.x * kBkWaves + threadIdx.x / kWave); b < nb;
-       b += (int32_t)(gridDim.x * kBkWaves)) {
-    const int32_t p0 = bstart[b], p1 = bstart[b + 1];
-    if (p0 == p1) {
-      if (lane == 0) bnnz[b] = 0;
-      continue;
+  const int32_t step = (int32_t)(gridDim.x * kBkWaves);
+  uint32_t kk[kBkUnroll];
+  auto load = [&](int32_t p, int32_t p1) {
+#pragma unroll
+    for (int u = 0; u < kBkUnroll; ++u) {
+      const int32_t q = p + u * kWave + lane;
+      kk[u] = q < p1 ? (uint32_t)klo[q] : 0xffffffffu;
     }
+  };
+  int32_t b = (int32_t)(blockIdx.x * kBkWaves + threadIdx.x / kWave);
+  int32_t p0 = 0, p1 = 0;
+  if (b < nb) {
+    p0 = bstart[b];
+    p1 = bstart[b + 1];
+    load(p0, p1);
+  }
+  while (b < nb) {
     for (int q = lane; q < words; q += kWave) bm[q] = 0u;
     __builtin_amdgcn_wave_barrier();
-    for (int32_t p = p0 + lane; p < p1; p += kWave) {
-      const uint32_t k = klo[p];
-      atomicOr(&bm[k >> 5], 1u << (k & 31));
+    for (int32_t p = p0;;) {
+#pragma unroll
+      for (int u = 0; u < kBkUnroll; ++u)
+        if (kk[u] != 0xffffffffu) atomicOr(&bm[kk[u] >> 5], 1u << (kk[u] & 31));
+      p += kWave * kBkUnroll;
+      if (p >= p1) break;
+      load(p, p1);
+    }
+    const int32_t cur = b;
+    b += step;
+    if (b < nb) {
+      p0 = bstart[b];
+      p1 = bstart[b + 1];
+      load(p0, p1);
     }
     __builtin_amdgcn_wave_barrier();
     int c = 0;
     for (int q = lane; q < words; q += kWave) c += __popc(bm[q]);
     c = wave_sum(c);
-    if (lane == 0) bnnz[b] = c;
+    if (lane == 0) bnnz[cur] = c;
     __builtin_amdgcn_wave_barrier();
   }
 }
@@ -503,13 +538,63 @@ __global__ __launch_bounds__(256) void k_bk_rowpos(int32_t R, int32_t W, int32_t
     out_pos[r0 + r] = base + boff[(int64_t)r * W];
 }
 
+// LDS of one wavefront in k_bk_reduce, for a window of 2^s columns:
+// acc (2^s doubles), seen and stride bitmaps (2^s bits each), column list
+// (2^s uint16).
+__host__ __device__ constexpr size_t bk_reduce_lds_per_wave(int s) {
+  return ((size_t)8 << s) + 2 * (((size_t)1 << s) >> 3) + ((size_t)2 << s);
+}
+
+// A batch of a bucket-walking wavefront: kBkUnroll strides of one bucket.
+struct BkBatch {
+  int32_t b, p, pend;  // bucket, first position, end of the bucket
+  bool first, last;    // first / last batch of its bucket
+};
+
+// The wavefront's batches in order: the buckets b, b + step, ... (empty ones
+// skipped), each cut into kBkUnroll strides.  b = nb marks the end.
+__device__ __forceinline__ BkBatch bk_next(const BkBatch& c, int32_t step, int32_t nb,
+                                           const int32_t* __restrict__ bstart) {
+  constexpr int32_t span = kWave * kBkUnroll;
+  BkBatch n;
+  if (c.p + span < c.pend) {
+    n.b = c.b;
+    n.p = c.p + span;
+    n.pend = c.pend;
+    n.first = false;
+  } else {
+    n.b = c.b + step;
+    n.p = n.pend = 0;
+    while (n.b < nb) {
+      n.p = bstart[n.b];
+      n.pend = bstart[n.b + 1];
+      if (n.p < n.pend) break;
+      n.b += step;
+    }
+    n.first = true;
+  }
+  n.last = n.p + span >= n.pend;
+  return n;
+}
+
+__device__ __forceinline__ BkBatch bk_first(int32_t b0, int32_t step, int32_t nb, const int32_t* __restrict__ bstart) {
+  BkBatch c;
+  c.b = b0 - step;
+  c.p = c.pend = 0;
+  return bk_next(c, step, nb, bstart);
+}
+
 // Reduce: one wavefront per bucket.  The window is a dense LDS accumulator;
 // `seen` marks the columns written so far.  Products are taken in position
-// order, 64 at a time; lanes holding the same column find each other with
-// one ballot per column bit, and the lowest of them adds the group's values
-// in lane order to the accumulator, so each entry is summed in position (=
-// increasing j) order, starting from its first product.  The set columns are
-// then written in order at the bucket's output offset.
+// order, 64 at a time.  Each lane sets its column's bit in a per-stride
+// bitmap (returning the old word) while reading its accumulator slot; a lane
+// that finds the bit already set shares its column with another lane of the
+// stride.  Lanes whose column is unique in the stride update the accumulator
+// at once; the lanes of shared columns are then applied one at a time in
+// lane order.  Each entry is thus summed in position (= increasing j) order,
+// starting from its first product.  The set columns are then listed in order
+// in LDS and written at the bucket's output offset with coalesced stores.
+// Loads run one batch ahead (two register sets), across bucket boundaries.
 __global__ __launch_bounds__(kBkWaves * kWave) void k_bk_reduce(int32_t nb, int32_t W, int s,
                                                                 const int32_t* __restrict__ bstart,
                                                                 const uint16_t* __restrict__ klo,
@@ -520,67 +605,112 @@ __global__ __launch_bounds__(kBkWaves * kWave) void k_bk_reduce(int32_t nb, int3
   extern __shared__ double bk_lds[];
   const int S = 1 << s, words = S >> 5;
   const int lane = lane_id();
-  double* acc = bk_lds + (size_t)(threadIdx.x / kWave) * (S + kWave + words / 2);
-  double* stage = acc + S;
-  uint32_t* seen = reinterpret_cast<uint32_t*>(stage + kWave);
-  for (int32_t b = (int32_t)(blockIdx.x * kBkWaves + threadIdx.x / kWave); b < nb;
-       b += (int32_t)(gridDim.x * kBkWaves)) {
-    const int32_t p0 = bstart[b], p1 = bstart[b + 1];
-    if (p0 == p1) continue;
-    for (int q = lane; q < words; q += kWave) seen[q] = 0u;
-    __builtin_amdgcn_wave_barrier();
-    for (int32_t p = p0; p < p1; p += kWave * kBkUnroll) {
-      uint32_t kk[kBkUnroll];
-      double vv[kBkUnroll];
+  char* mine = reinterpret_cast<char*>(bk_lds) + (threadIdx.x / kWave) * bk_reduce_lds_per_wave(s);
+  double* acc = reinterpret_cast<double*>(mine);
+  uint32_t* seen = reinterpret_cast<uint32_t*>(acc + S);
+  uint32_t* sb = seen + words;
+  uint16_t* list = reinterpret_cast<uint16_t*>(sb + words);
+  for (int q = lane; q < words; q += kWave) sb[q] = 0u;
+  const int32_t step = (int32_t)(gridDim.x * kBkWaves);
+
+  auto load = [&](const BkBatch& c, uint32_t (&kk)[kBkUnroll], double (&vv)[kBkUnroll]) {
+    // clamped index: every lane loads; lanes past the bucket are masked by `ok` in apply
 #pragma unroll
-      for (int u = 0; u < kBkUnroll; ++u) {
-        const int32_t q = p + u * kWave + lane;
-        kk[u] = q < p1 ? (uint32_t)klo[q] : 0u;
-        vv[u] = q < p1 ? pv[q] : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < kBkUnroll; ++u) {
-        if (p + u * kWave >= p1) break;
-        const bool ok = p + u * kWave + lane < p1;
-        const uint32_t k = kk[u];
-        uint64_t peers = __ballot(ok);
-        for (int bit = 0; bit < s; ++bit) {
-          const bool set = ((k >> bit) & 1u) != 0;
-          const uint64_t mset = __ballot(set);
-          peers &= set ? mset : ~mset;
-        }
-        stage[lane] = vv[u];
-        __builtin_amdgcn_wave_barrier();
-        if (ok && __builtin_ctzll(peers) == (unsigned)lane) {
-          const uint32_t bit = 1u << (k & 31);
-          const bool was = (seen[k >> 5] & bit) != 0;
-          double x = was ? acc[k] + vv[u] : vv[u];
-          for (uint64_t rest = peers & (peers - 1); rest; rest &= rest - 1) x += stage[__builtin_ctzll(rest)];
-          acc[k] = x;
-          if (!was) atomicOr(&seen[k >> 5], bit);
-        }
-        __builtin_amdgcn_wave_barrier();
-      }
+    for (int u = 0; u < kBkUnroll; ++u) {
+      const int32_t q = min(c.p + u * kWave + lane, c.pend - 1);
+      kk[u] = (uint32_t)klo[q];
+      vv[u] = pv[q];
     }
-    const int64_t ob = base + boff[b];
-    const int32_t wb = (b % W) << s;
+  };
+  auto apply = [&](const BkBatch& c, const uint32_t (&kk)[kBkUnroll], const double (&vv)[kBkUnroll]) {
+    if (c.first) {
+      for (int q = lane; q < words; q += kWave) seen[q] = 0u;
+      __builtin_amdgcn_wave_barrier();
+    }
+#pragma unroll
+    for (int u = 0; u < kBkUnroll; ++u) {
+      if (c.p + u * kWave >= c.pend) break;
+      const bool ok = c.p + u * kWave + lane < c.pend;
+      const uint32_t k = kk[u];
+      const double v = vv[u];
+      const uint32_t wi = k >> 5, bit = 1u << (k & 31);
+      uint32_t old = 0, sw = 0;
+      double a = 0.0;
+      if (ok) {
+        old = atomicOr(&sb[wi], bit);
+        sw = seen[wi];
+        a = acc[k];
+      }
+      const uint64_t late = __ballot(ok && (old & bit) != 0);
+      bool shared = false;
+      if (late != 0) {
+        // lanes whose column also sits in a lane that found its bit set
+        for (uint64_t m = late; m; m &= m - 1) {
+          const int l = __builtin_ctzll(m);
+          shared = shared || (ok && k == (uint32_t)__builtin_amdgcn_readlane((int)k, l));
+        }
+      }
+      if (ok && !shared) {
+        const bool was = (sw & bit) != 0;
+        acc[k] = was ? a + v : v;
+        if (!was) atomicOr(&seen[wi], bit);
+      }
+      if (late != 0) {
+        __builtin_amdgcn_wave_barrier();
+        const uint64_t sm = __ballot(shared);
+        for (uint64_t m = sm; m; m &= m - 1) {
+          const int l = __builtin_ctzll(m);
+          if (lane == l) {
+            const bool was = (seen[wi] & bit) != 0;
+            acc[k] = was ? acc[k] + v : v;
+            if (!was) atomicOr(&seen[wi], bit);
+          }
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (ok) atomicAnd(&sb[wi], ~bit);
+      __builtin_amdgcn_wave_barrier();
+    }
+    if (!c.last) return;
+    // list the set columns in order, then write them out coalesced
     int32_t carry = 0;
     for (int q0 = 0; q0 < words; q0 += kWave) {
       const int q = q0 + lane;
       uint32_t bits = q < words ? seen[q] : 0u;
-      const int c = __popc(bits);
-      const int incl = wave_incl_scan(c);
-      int64_t r = ob + carry + incl - c;
+      const int cnt = __popc(bits);
+      const int incl = wave_incl_scan(cnt);
+      int r = carry + incl - cnt;
       while (bits) {
-        const int t = __builtin_ctz(bits);
-        out_crd[r] = wb + q * 32 + t;
-        out_val[r] = acc[q * 32 + t];
-        ++r;
+        list[r++] = (uint16_t)(q * 32 + __builtin_ctz(bits));
         bits &= bits - 1;
       }
       carry += __shfl(incl, kWave - 1, kWave);
     }
     __builtin_amdgcn_wave_barrier();
+    const int64_t ob = base + boff[c.b];
+    const int32_t wb = (c.b % W) << s;
+    for (int r = lane; r < carry; r += kWave) {
+      const uint32_t k = list[r];
+      out_crd[ob + r] = wb + (int32_t)k;
+      out_val[ob + r] = acc[k];
+    }
+    __builtin_amdgcn_wave_barrier();
+  };
+
+  uint32_t k0[kBkUnroll], k1[kBkUnroll];
+  double v0[kBkUnroll], v1[kBkUnroll];
+  BkBatch c0 =
+      bk_first(__builtin_amdgcn_readfirstlane((int32_t)(blockIdx.x * kBkWaves + threadIdx.x / kWave)), step, nb, bstart);
+  if (c0.b < nb) load(c0, k0, v0);
+  while (c0.b < nb) {
+    BkBatch c1 = bk_next(c0, step, nb, bstart);
+    if (c1.b < nb) load(c1, k1, v1);
+    apply(c0, k0, v0);
+    if (c1.b >= nb) break;
+    c0 = bk_next(c1, step, nb, bstart);
+    if (c0.b < nb) load(c0, k0, v0);
+    apply(c1, k1, v1);
   }
 }
 
@@ -807,12 +937,14 @@ static int outer_chunks_bucket(spws_ctx* ctx, const OuterArgs& a, const std::vec
       (rc = ctx->bk[1].ensure(ecap * 4)) || (rc = ctx->bk[2].ensure(ecap * 4)) ||
       (rc = ctx->bk[3].ensure(ecap * 8)) || (rc = ctx->bk[4].ensure(ecap * 8)) ||
       (rc = ctx->bk[5].ensure(ecap * 4)) || (rc = ctx->bk[6].ensure(ecap * (size_t)W * 4)) ||
-      (rc = ctx->bk[7].ensure(((size_t)max_r + 1) * 4)) || (rc = ctx->bk[8].ensure(nbcap * 4 * 4)))
+      (rc = ctx->bk[7].ensure(((size_t)max_r + 1) * 4)) || (rc = ctx->bk[8].ensure(nbcap * 4 * 4)) ||
+      (rc = ctx->bk[9].ensure(ecap * 4)))
     return rc;
   if ((rc = outer_output_init(ctx, m, row_prefix[m]))) return rc;
   uint16_t* klo = ctx->keys[0].as<uint16_t>();
   double* pv = ctx->vals[0].as<double>();
   int32_t* ent_ja = ctx->bk[5].as<int32_t>();
+  int32_t* ent_ai = ctx->bk[9].as<int32_t>();
   int32_t* etab = ctx->bk[6].as<int32_t>();
   int32_t* rp_off = ctx->bk[7].as<int32_t>();
   int32_t* bsize = ctx->bk[8].as<int32_t>();
@@ -822,7 +954,7 @@ static int outer_chunks_bucket(spws_ctx* ctx, const OuterArgs& a, const std::vec
   int64_t* out_pos = ctx->out_pos.as<int64_t>();
   int32_t* hm = ctx->host_meta.as<int32_t>();
   const size_t lds_count = (size_t)kBkWaves * (((size_t)1 << s) >> 5) * 4;
-  const size_t lds_reduce = (size_t)kBkWaves * (((size_t)1 << s) + kWave + (((size_t)1 << s) >> 6)) * 8;
+  const size_t lds_reduce = (size_t)kBkWaves * bk_reduce_lds_per_wave(s);
   ctx->stats.outer_method = 2;
   ctx->stats.window_bits = s;
   int64_t base = 0;
@@ -848,7 +980,7 @@ static int outer_chunks_bucket(spws_ctx* ctx, const OuterArgs& a, const std::vec
       double* pval = ctx->bk[3].as<double>();
       const unsigned ge = (unsigned)std::min<int64_t>(div_up(En, 256), 16384);
       SPWS_LAUNCH(ctx, "bk_entries", k_bk_entries, dim3(ge), dim3(256), 0, En, flag_off + na, act_eoff, act_astart,
-                  act_ja, a.A2_crd, r0, pkey, pval, ent_ja);
+                  act_ja, a.A2_crd, r0, pkey, pval, ent_ja, ent_ai);
       bool alt = false;
       if ((rc = radix_sort_pairs<uint32_t>(ctx, pkey, pval, ctx->bk[2].as<uint32_t>(), ctx->bk[4].as<double>(), En, 0,
                                            std::max(1, bits_for(R)), &alt)))
@@ -861,9 +993,10 @@ static int outer_chunks_bucket(spws_ctx* ctx, const OuterArgs& a, const std::vec
       if ((rc = exclusive_scan<int32_t, int32_t>(ctx, bsize, bstart, NB))) return rc;
       SPWS_LAUNCH(ctx, "bk_etab", k_bk_rows<1>, dim3(gr), dim3(256), 0, R, W, rp_off, pval, ent_ja, bseg, bstart,
                   etab);
-      const unsigned gexp = (unsigned)div_up(P, kExpTile);
-      SPWS_LAUNCH(ctx, "bk_expand", k_bk_expand, dim3(gexp), dim3(kExpBlock), 0, P, flag_off + na, act_off,
-                  act_astart, act_L, act_bstart, act_eoff, a.Ax, a.B2_crd, a.Bx, W, s, etab, klo, pv);
+      const unsigned gx = (unsigned)std::min<int64_t>(div_up((int64_t)En * kWave, 256), 32768);
+      SPWS_LAUNCH(ctx, "bk_expand", k_bk_expand, dim3(gx), dim3(256), 0, En, pval, ent_ai, ent_ja, match, blen,
+                  a.B2_pos,
+                  a.Ax, a.B2_crd, a.Bx, W, s, etab, klo, pv);
       const unsigned gb = (unsigned)std::min<int64_t>(div_up(NB, kBkWaves), 65536);
       SPWS_LAUNCH(ctx, "bk_count", k_bk_count, dim3(gb), dim3(kBkWaves * kWave), lds_count, NB, s, bstart, klo,
                   bnnz);
@@ -913,7 +1046,7 @@ static int outer_run(spws_ctx* ctx, const OuterArgs& a, int64_t* nnz_out) {
     const unsigned g = (unsigned)div_up(a.a_ncols, 256);
     SPWS_LAUNCH(ctx, "outer_match", k_outer_match, dim3(g), dim3(256), 0, a.a_ncols, a.A1_crd, a.b_nrows, a.B1_crd,
                 a.B2_pos, match, blen);
-    const unsigned g2 = g < 4096 ? g : 4096;
+    const unsigned g2 = (unsigned)std::min<int64_t>(div_up(a.a_ncols, 4), 8192);
     SPWS_LAUNCH(ctx, "outer_row_ub", k_outer_row_ub, dim3(g2), dim3(256), 0, a.a_ncols, a.A2_pos, a.A2_crd, blen,
                 (unsigned long long*)ctx->row_ub.ptr, row_pairs);
   }
