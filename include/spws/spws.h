@@ -222,7 +222,9 @@ typedef struct {
   int32_t max_requests;      /* requests per batch (<= 65535) */
   int32_t max_rows;          /* sum of A row counts */
   int32_t max_brows;         /* sum of B row counts */
-  int32_t reserved;
+  int32_t max_cols;          /* widest request (n) the engine takes, <= 2^17; 0 = any
+                                width (the block bins then hash and sort instead of
+                                using the LDS column bitmap) */
   int64_t max_annz;          /* sum of nnz(A_b) */
   int64_t max_bnnz;          /* sum of nnz(B_b) */
   int64_t max_products;      /* sum of upper bounds of output entries (<= 2^31 - 1) */
@@ -234,7 +236,8 @@ int spws_batch_destroy(spws_batch* b);
 /* Empty the batch (only while no launch is in flight). */
 int spws_batch_reset(spws_batch* b);
 /* Add one request; SPWS_ERR_OOM when the batch cannot hold it (full),
- * SPWS_ERR_UNSUPPORTED when a row exceeds max_row_products.  *slot names the
+ * SPWS_ERR_UNSUPPORTED when a row exceeds max_row_products or the request
+ * is wider than max_cols.  *slot names the
  * request in spws_batch_result.  The operand arrays must stay valid until
  * the batch completes. */
 int spws_batch_add(spws_batch* b, int32_t m, int32_t k, int32_t n, const int32_t* A_pos, const int32_t* A_crd,

@@ -46,7 +46,7 @@ class RowwiseTuning(C.Structure):
 
 class BatchCaps(C.Structure):
     _fields_ = [("max_requests", C.c_int32), ("max_rows", C.c_int32), ("max_brows", C.c_int32),
-                ("reserved", C.c_int32), ("max_annz", C.c_int64), ("max_bnnz", C.c_int64),
+                ("max_cols", C.c_int32), ("max_annz", C.c_int64), ("max_bnnz", C.c_int64),
                 ("max_products", C.c_int64), ("max_row_products", C.c_int64)]
 
 

@@ -16,7 +16,9 @@
 //   k_batch_gather     operands -> engine pools, indices rebased
 //   k_meta_clear, k_rowstat, device-sized scan, k_bin_offsets, k_binscatter
 //   numeric kernels of the wave / wave2 / block / block2 / wide paths
-//   (persistent grids; each reads its row count from the bin metadata)
+//   (persistent grids; each reads its row count from the bin metadata; the
+//   block bins use the LDS bitmap kernel over each row's own column range
+//   when the engine bounds the request width by max_cols)
 //   device-sized scan -> C_pos, k_compact (columns shifted back per row)
 //   k_batch_total, D2H of the batch nnz
 // No memset is recorded into the graph: clears are kernels.  The dense path
@@ -234,10 +236,21 @@ static void enqueue_batch(spws_batch* b) {
                      Bp, Bi, Bx, row_off, scrd, sval, row_nnz);
   hipLaunchKernelGGL((k_row_wave<512, 4, true>), dim3(gwave), dim3(64), 0, s, bin_rows, bmeta, (int)P_WAVE2, Ap, Ai,
                      Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz);
-  hipLaunchKernelGGL((k_row_block<2048, 1024, 256, true>), dim3(gblk), dim3(256), 0, s, bin_rows, bmeta, (int)P_BLOCK,
-                     Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz);
-  hipLaunchKernelGGL((k_row_block<8192, 4096, 512, true>), dim3(gblk), dim3(512), 0, s, bin_rows, bmeta,
-                     (int)P_BLOCK2, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz);
+  if (c.max_cols > 0 && c.max_cols <= kBitmapCols) {
+    // every request is at most max_cols wide: LDS bitmap over each row's own column range
+    const int64_t v1 = std::min<int64_t>(b->pp.block_ub, c.max_cols), v2 = std::min<int64_t>(b->pp.block2_ub, c.max_cols);
+    hipLaunchKernelGGL((k_row_bitmap<true>), dim3(gblk), dim3(kBitmapBlock), bitmap_lds_bytes(c.max_cols, v1, true), s,
+                       bin_rows, bmeta, (int)P_BLOCK, c.max_cols, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz,
+                       coloff);
+    hipLaunchKernelGGL((k_row_bitmap<true>), dim3(gblk), dim3(kBitmapBlock), bitmap_lds_bytes(c.max_cols, v2, true), s,
+                       bin_rows, bmeta, (int)P_BLOCK2, c.max_cols, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval,
+                       row_nnz, coloff);
+  } else {
+    hipLaunchKernelGGL((k_row_block<2048, 1024, 256, true>), dim3(gblk), dim3(256), 0, s, bin_rows, bmeta,
+                       (int)P_BLOCK, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz);
+    hipLaunchKernelGGL((k_row_block<8192, 4096, 512, true>), dim3(gblk), dim3(512), 0, s, bin_rows, bmeta,
+                       (int)P_BLOCK2, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz);
+  }
   hipLaunchKernelGGL((k_row_wide<true>), dim3(b->wide_groups), dim3(kWideBlock), 0, s, bin_rows, bmeta, (int)P_WIDE,
                      Ap, Ai, Ax, Bp, Bi, Bx, row_ub, row_off, scrd, sval, row_nnz, (int32_t*)P(BP_WKEY),
                      (double*)P(BP_WVAL), b->wide_slots);
@@ -305,6 +318,8 @@ int spws_batch_create(spws_ctx* ctx, const spws_batch_caps* caps, spws_batch** o
   SPWS_REQUIRE(caps->max_products <= INT32_MAX && caps->max_annz <= INT32_MAX && caps->max_bnnz <= INT32_MAX &&
                    caps->max_row_products <= (1 << 28),
                SPWS_ERR_UNSUPPORTED, "batch capacities exceed the int32 index range");
+  SPWS_REQUIRE(caps->max_cols >= 0 && caps->max_cols <= kBitmapCols, SPWS_ERR_INVALID,
+               "max_cols must be 0 (any width) or at most %d", kBitmapCols);
   spws_batch* b = new spws_batch();
   b->ctx = ctx;
   b->caps = *caps;
@@ -351,6 +366,10 @@ int spws_batch_add(spws_batch* b, int32_t m, int32_t k, int32_t n, const int32_t
   // upper bound of its output entries)
   if (max_row_products > c.max_row_products) {
     set_error("a row of %lld products exceeds the engine's row capacity", (long long)max_row_products);
+    return SPWS_ERR_UNSUPPORTED;
+  }
+  if (c.max_cols > 0 && n > c.max_cols) {
+    set_error("a request of %d columns exceeds the engine's column capacity %d", n, c.max_cols);
     return SPWS_ERR_UNSUPPORTED;
   }
   if (d.nreq + 1 > c.max_requests || (int64_t)d.rows + m > c.max_rows || (int64_t)d.brows + k > c.max_brows ||

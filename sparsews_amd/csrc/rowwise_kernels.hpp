@@ -396,6 +396,101 @@ __global__ __launch_bounds__(BLOCK) void k_row_block(const int32_t* __restrict__
   }
 }
 
+// ------------------------------------------------------------------ bitmap kernel
+//
+// Rows of the block bins when the output has at most kBitmapCols columns:
+// one workgroup per row.  Pass 1 over the row's products sets each column's
+// bit in an LDS bitmap of all columns; a prefix of the word popcounts gives
+// every column its rank in the sorted row; pass 2 adds each product at its
+// column's rank (ds_add_f64).  The row is then written in column order with
+// no hash probing and no sort.  In a stacked batch a row's columns lie in
+// [row_colbase[i], row_colbase[i] + ncols), and the bitmap covers that range.
+
+constexpr int kBitmapBlock = 256;
+constexpr int32_t kBitmapCols = 1 << 17;
+
+__host__ __device__ constexpr size_t bitmap_lds_bytes(int32_t ncols, int64_t vcap, bool values) {
+  return (size_t)((((ncols + 31) >> 5) + 1) & ~1) * 8 + (values ? (size_t)vcap * 8 : 0);
+}
+
+template <bool kValues>
+__global__ __launch_bounds__(kBitmapBlock) void k_row_bitmap(const int32_t* __restrict__ bin_rows,
+                                                             const int32_t* __restrict__ bmeta, int path,
+                                                             int32_t ncols, const int32_t* __restrict__ Ap,
+                                                             const int32_t* __restrict__ Ai,
+                                                             const double* __restrict__ Ax,
+                                                             const int32_t* __restrict__ Bp,
+                                                             const int32_t* __restrict__ Bi,
+                                                             const double* __restrict__ Bx,
+                                                             const int64_t* __restrict__ row_off,
+                                                             int32_t* __restrict__ slot_crd,
+                                                             double* __restrict__ slot_val,
+                                                             int32_t* __restrict__ row_nnz,
+                                                             const int32_t* __restrict__ row_colbase) {
+  // declared as double so the dynamic region (which follows the static
+  // arrays) starts 8-byte aligned: acc is updated with 64-bit LDS atomics
+  extern __shared__ double bml_d[];
+  __shared__ int32_t sh[kBitmapBlock / kWave + 1];
+  const int32_t words = (ncols + 31) >> 5;
+  const int32_t wpad = (words + 1) & ~1;
+  uint32_t* bm = reinterpret_cast<uint32_t*>(bml_d);
+  uint32_t* pre = bm + wpad;
+  double* acc = reinterpret_cast<double*>(pre + wpad);
+  const int wave = threadIdx.x / kWave;
+  const int32_t count = bmeta[kMetaCount + path];
+  const int32_t* rows = bin_rows + bmeta[kMetaOff + path];
+  // each thread owns a contiguous range of bitmap words
+  const int32_t wpt = (words + kBitmapBlock - 1) / kBitmapBlock;
+  const int32_t w0 = min((int32_t)threadIdx.x * wpt, words), w1 = min(w0 + wpt, words);
+  for (int32_t r = blockIdx.x; r < count; r += gridDim.x) {
+    const int32_t i = rows[r];
+    const int32_t cb = row_colbase ? row_colbase[i] : 0;
+    for (int32_t w = threadIdx.x; w < words; w += kBitmapBlock) bm[w] = 0u;
+    __syncthreads();
+    expand_products<false>(Ap[i], Ap[i + 1], wave * kWave, kBitmapBlock, Ai, Ax, Bp, Bi, Bx, [&](int32_t col, double) {
+      const int32_t c = col - cb;
+      atomicOr(&bm[c >> 5], 1u << (c & 31));
+    });
+    __syncthreads();
+    int32_t c = 0;
+    for (int32_t w = w0; w < w1; ++w) c += __popc(bm[w]);
+    int32_t nnz;
+    int32_t run = block_excl_scan<int32_t, kBitmapBlock>(c, sh, &nnz);
+    const int32_t first_rank = run;
+    for (int32_t w = w0; w < w1; ++w) {
+      pre[w] = run;
+      run += __popc(bm[w]);
+    }
+    if (!kValues) {
+      if (threadIdx.x == 0) row_nnz[i] = nnz;
+      __syncthreads();
+      continue;
+    }
+    for (int32_t e = threadIdx.x; e < nnz; e += kBitmapBlock) acc[e] = 0.0;
+    __syncthreads();
+    expand_products<true>(Ap[i], Ap[i + 1], wave * kWave, kBitmapBlock, Ai, Ax, Bp, Bi, Bx,
+                          [&](int32_t col, double v) {
+                            const int32_t c = col - cb;
+                            const int32_t wd = c >> 5;
+                            const int32_t rank = pre[wd] + __popc(bm[wd] & ((1u << (c & 31)) - 1u));
+                            atomicAdd(&acc[rank], v);
+                          });
+    __syncthreads();
+    const int64_t off = row_off[i];
+    int32_t rk = first_rank;
+    for (int32_t w = w0; w < w1; ++w) {
+      uint32_t bits = bm[w];
+      while (bits) {
+        slot_crd[off + rk++] = cb + w * 32 + __builtin_ctz(bits);
+        bits &= bits - 1;
+      }
+    }
+    for (int32_t e = threadIdx.x; e < nnz; e += kBitmapBlock) slot_val[off + e] = acc[e];
+    if (threadIdx.x == 0) row_nnz[i] = nnz;
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------------ dense accumulator kernel
 //
 // Each workgroup owns dflag[ncols] (uint8) and dval[ncols] (FP64) in HBM,

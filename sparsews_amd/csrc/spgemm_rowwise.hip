@@ -12,7 +12,11 @@
 //        wave   / wave2   one wavefront per row, LDS open-addressing table
 //                         (2*ub slots), LDS atomicCAS keys + ds_add_f64 values,
 //                         register bitonic sort across the wavefront;
-//        block  / block2  one workgroup per row, LDS table, LDS bitonic sort;
+//        block  / block2  one workgroup per row: with at most 2^17 columns an
+//                         LDS bitmap of the columns (rank = popcount prefix)
+//                         and ds_add_f64 at each column's rank, written in
+//                         order with no sort; wider outputs use an LDS table
+//                         and an LDS bitonic sort;
 //        dense            densest rows: a dense FP64 accumulator + flag array
 //                         per workgroup in HBM, emitted in column order;
 //        wide             long, not dense rows: a per-row HBM hash table
@@ -125,13 +129,22 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
     SPWS_LAUNCH(ctx, "row_wave", (k_row_wave<512, 4, kValues>), dim3(grid_for(counts[P_WAVE2], 1 << 20)), dim3(64),
                 0, rows, bmeta, (int)P_WAVE2, Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz);
   }
-  if (counts[P_BLOCK]) {
-    SPWS_LAUNCH(ctx, "row_block", (k_row_block<2048, 1024, 256, kValues>), dim3(grid_for(counts[P_BLOCK], 1 << 16)),
-                dim3(256), 0, rows, bmeta, (int)P_BLOCK, Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz);
-  }
-  if (counts[P_BLOCK2]) {
-    SPWS_LAUNCH(ctx, "row_block", (k_row_block<8192, 4096, 512, kValues>), dim3(grid_for(counts[P_BLOCK2], 1 << 16)),
-                dim3(512), 0, rows, bmeta, (int)P_BLOCK2, Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz);
+  // block bins: the LDS bitmap kernel while all columns fit the bitmap, else
+  // the LDS hash table and bitonic sort
+  for (int path : {(int)P_BLOCK, (int)P_BLOCK2}) {
+    if (!counts[path]) continue;
+    if (n <= kBitmapCols) {
+      const int64_t vcap = std::min<int64_t>(maxub[path], n);
+      SPWS_LAUNCH(ctx, "row_bitmap", (k_row_bitmap<kValues>), dim3(grid_for(counts[path], 1 << 16)),
+                  dim3(kBitmapBlock), bitmap_lds_bytes(n, vcap, kValues), rows, bmeta, path, n, Ap, Ai, Ax, Bp, Bi,
+                  Bx, roff, scrd, sval, rnnz, (const int32_t*)nullptr);
+    } else if (path == P_BLOCK) {
+      SPWS_LAUNCH(ctx, "row_block", (k_row_block<2048, 1024, 256, kValues>), dim3(grid_for(counts[path], 1 << 16)),
+                  dim3(256), 0, rows, bmeta, path, Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz);
+    } else {
+      SPWS_LAUNCH(ctx, "row_block", (k_row_block<8192, 4096, 512, kValues>), dim3(grid_for(counts[path], 1 << 16)),
+                  dim3(512), 0, rows, bmeta, path, Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz);
+    }
   }
   if (counts[P_DENSE]) {
     // bounded scratch: at most ~2 GiB of dense accumulators

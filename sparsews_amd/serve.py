@@ -20,8 +20,10 @@ from .gen import CSR, row_products
 
 
 def default_caps(max_requests=512, max_rows=1 << 21, max_nnz=1 << 27, max_products=(1 << 29),
-                 max_row_products=1 << 21) -> _lib.BatchCaps:
-    return _lib.BatchCaps(max_requests, max_rows, max_rows, 0, max_nnz, max_nnz, max_products, max_row_products)
+                 max_row_products=1 << 21, max_cols=1 << 16) -> _lib.BatchCaps:
+    """Engine capacities; max_cols bounds the request width (0 = any width)."""
+    return _lib.BatchCaps(max_requests, max_rows, max_rows, max_cols, max_nnz, max_nnz, max_products,
+                          max_row_products)
 
 
 class Engine:

@@ -111,6 +111,30 @@ def test_long_rows_take_the_hbm_hash_path_in_a_batch(ctx, engine):
         check_result(C, a, b)
 
 
+def test_any_width_engine_and_width_limit(ctx):
+    """max_cols = 0: the block bins hash and sort; max_cols > 0: the LDS
+    column bitmap, and a wider request is refused (it runs plain)."""
+    print("case: engine width settings", flush=True)
+    pairs = make_requests(9, 10)
+    wide = dict(SMALL_CAPS, max_cols=0)
+    e = Engine(ctx, default_caps(**wide))
+    try:
+        results, _ = run_batch(e, ctx, pairs)
+        for (A, B), C in zip(pairs, results):
+            check_result(C, A, B)
+    finally:
+        e.close()
+    narrow = dict(SMALL_CAPS, max_cols=1024)
+    e2 = Engine(ctx, default_caps(**narrow))
+    try:
+        A, B = uniform_rows(100, 100, 1, 9, 95, 0), uniform_rows(100, 2000, 1, 9, 95, 1)
+        with pytest.raises(SpwsError) as ex:
+            e2.add(DeviceRequest.upload(ctx, A, B))
+        assert ex.value.code == 4
+    finally:
+        e2.close()
+
+
 def test_admission_limits(ctx):
     print("case: admission limits", flush=True)
     caps = default_caps(max_requests=4, max_rows=1 << 12, max_nnz=1 << 16, max_products=1 << 15,

@@ -83,6 +83,23 @@ def test_dense_rows_natural(ctx):
     assert ctx.stats()["rows_per_path"][4] == 30
 
 
+@pytest.mark.parametrize("ncols", [3000, 131072, 131073, 200_000])
+@pytest.mark.parametrize("per_row", [(20, 30), (40, 60)])
+def test_block_bins_bitmap_and_hash(ctx, ncols, per_row):
+    """Rows in the block bins (256 < ub <= 4096): the LDS bitmap kernel up to
+    2^17 output columns, the LDS hash table and bitonic sort above."""
+    a_len, b_len = per_row
+    print(f"case: block bins ncols={ncols} a={a_len} b={b_len}", flush=True)
+    A = random_csr(300, 2000, np.full(300, a_len), 8 + a_len, 0)
+    B = random_csr(2000, ncols, np.full(2000, b_len), 8 + a_len, 1)
+    C = run(ctx, A, B)
+    check(C, A, B)
+    st = ctx.stats()
+    assert st["rows_per_path"][2] + st["rows_per_path"][3] == 300
+    dA, dB = DeviceCSR.upload(ctx, A), DeviceCSR.upload(ctx, B)
+    np.testing.assert_array_equal(spgemm_rowwise_symbolic(ctx, dA, dB), C.pos)
+
+
 @pytest.mark.parametrize("path", [0, 1, 2, 3, 4, 5])
 def test_forced_paths(ctx, path):
     print(f"case: forced path {path}", flush=True)
