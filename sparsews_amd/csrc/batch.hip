@@ -251,9 +251,15 @@ static void enqueue_batch(spws_batch* b) {
     hipLaunchKernelGGL((k_row_block<8192, 4096, 512, true>), dim3(gblk), dim3(512), 0, s, bin_rows, bmeta,
                        (int)P_BLOCK2, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz);
   }
-  hipLaunchKernelGGL((k_row_wide<true>), dim3(b->wide_groups), dim3(kWideBlock), 0, s, bin_rows, bmeta, (int)P_WIDE,
-                     Ap, Ai, Ax, Bp, Bi, Bx, row_ub, row_off, scrd, sval, row_nnz, (int32_t*)P(BP_WKEY),
-                     (double*)P(BP_WVAL), b->wide_slots);
+  if (c.max_cols > 0 && c.max_cols <= kBitmapCols) {
+    hipLaunchKernelGGL((k_row_bitmap<true, true>), dim3(gblk), dim3(kBitmapBlock), bitmap_lds_bytes(c.max_cols, 0, false),
+                       s, bin_rows, bmeta, (int)P_WIDE, c.max_cols, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval,
+                       row_nnz, coloff);
+  } else {
+    hipLaunchKernelGGL((k_row_wide<true>), dim3(b->wide_groups), dim3(kWideBlock), 0, s, bin_rows, bmeta,
+                       (int)P_WIDE, Ap, Ai, Ax, Bp, Bi, Bx, row_ub, row_off, scrd, sval, row_nnz,
+                       (int32_t*)P(BP_WKEY), (double*)P(BP_WVAL), b->wide_slots);
+  }
   dscan<int32_t, int32_t>(s, row_nnz, m_dev, (int32_t*)P(BP_SCAN), Cp);
   hipLaunchKernelGGL(k_compact, dim3((unsigned)std::min<int64_t>(div_up(c.max_rows, 4), 16384)), dim3(256), 0, s, 0,
                      m_dev, row_off, Cp, scrd, sval, coloff, Ci, Cx);

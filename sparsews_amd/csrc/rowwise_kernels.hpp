@@ -41,6 +41,8 @@ constexpr int kMetaCount = 0, kMetaOff = 8, kMetaCursor = 16, kMetaInts = 24;
 
 __device__ __forceinline__ int32_t dev_or(const int32_t* p, int32_t v) { return p ? *p : v; }
 
+// One wavefront per row (A rows range from a few to thousands of entries):
+// lanes sum the matched B row lengths, lane 0 classifies the row.
 __global__ __launch_bounds__(256) void k_rowstat(int32_t m_host, const int32_t* __restrict__ m_dev, int32_t ncols,
                                                  const int32_t* __restrict__ Ap, const int32_t* __restrict__ Ai,
                                                  const int32_t* __restrict__ Bp, int64_t* __restrict__ row_ub,
@@ -54,17 +56,22 @@ __global__ __launch_bounds__(256) void k_rowstat(int32_t m_host, const int32_t* 
   }
   __syncthreads();
   const int32_t m = dev_or(m_dev, m_host);
-  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+  const int lane = lane_id();
+  const int32_t nw = (int32_t)(gridDim.x * blockDim.x / kWave);
+  for (int32_t i = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kWave); i < m; i += nw) {
     int64_t ub = 0;
     const int32_t e = Ap[i + 1];
-    for (int32_t p = Ap[i]; p < e; ++p) {
+    for (int32_t p = Ap[i] + lane; p < e; p += kWave) {
       const int32_t j = Ai[p];
       ub += Bp[j + 1] - Bp[j];
     }
-    row_ub[i] = ub;
-    const int path = classify_row(ub, ncols, pp);
-    atomicAdd(&cnt[path], 1);
-    atomicMax(&mx[path], (unsigned long long)ub);
+    ub = wave_sum(ub);
+    if (lane == 0) {
+      row_ub[i] = ub;
+      const int path = classify_row(ub, ncols, pp);
+      atomicAdd(&cnt[path], 1);
+      atomicMax(&mx[path], (unsigned long long)ub);
+    }
   }
   __syncthreads();
   if (threadIdx.x < kPaths && cnt[threadIdx.x]) {
@@ -398,13 +405,16 @@ __global__ __launch_bounds__(BLOCK) void k_row_block(const int32_t* __restrict__
 
 // ------------------------------------------------------------------ bitmap kernel
 //
-// Rows of the block bins when the output has at most kBitmapCols columns:
-// one workgroup per row.  Pass 1 over the row's products sets each column's
-// bit in an LDS bitmap of all columns; a prefix of the word popcounts gives
-// every column its rank in the sorted row; pass 2 adds each product at its
-// column's rank (ds_add_f64).  The row is then written in column order with
-// no hash probing and no sort.  In a stacked batch a row's columns lie in
-// [row_colbase[i], row_colbase[i] + ncols), and the bitmap covers that range.
+// Rows whose output has at most kBitmapCols columns: one workgroup per row.
+// Pass 1 over the row's products sets each column's bit in an LDS bitmap of
+// all columns; a prefix of the word popcounts gives every column its rank in
+// the sorted row; pass 2 adds each product at its column's rank.  The row is
+// then written in column order with no hash probing and no sort.  Block-bin
+// rows accumulate in LDS (ds_add_f64, kGlobal = false); longer rows
+// accumulate straight into their slot region in HBM (global FP64 atomics,
+// kGlobal = true), which holds at least nnz entries.  In a stacked batch a
+// row's columns lie in [row_colbase[i], row_colbase[i] + ncols), and the
+// bitmap covers that range.
 
 constexpr int kBitmapBlock = 256;
 constexpr int32_t kBitmapCols = 1 << 17;
@@ -413,7 +423,7 @@ __host__ __device__ constexpr size_t bitmap_lds_bytes(int32_t ncols, int64_t vca
   return (size_t)((((ncols + 31) >> 5) + 1) & ~1) * 8 + (values ? (size_t)vcap * 8 : 0);
 }
 
-template <bool kValues>
+template <bool kValues, bool kGlobal = false>
 __global__ __launch_bounds__(kBitmapBlock) void k_row_bitmap(const int32_t* __restrict__ bin_rows,
                                                              const int32_t* __restrict__ bmeta, int path,
                                                              int32_t ncols, const int32_t* __restrict__ Ap,
@@ -466,17 +476,18 @@ __global__ __launch_bounds__(kBitmapBlock) void k_row_bitmap(const int32_t* __re
       __syncthreads();
       continue;
     }
-    for (int32_t e = threadIdx.x; e < nnz; e += kBitmapBlock) acc[e] = 0.0;
+    const int64_t off = row_off[i];
+    double* dst = kGlobal ? slot_val + off : acc;
+    for (int32_t e = threadIdx.x; e < nnz; e += kBitmapBlock) dst[e] = 0.0;
     __syncthreads();
     expand_products<true>(Ap[i], Ap[i + 1], wave * kWave, kBitmapBlock, Ai, Ax, Bp, Bi, Bx,
                           [&](int32_t col, double v) {
                             const int32_t c = col - cb;
                             const int32_t wd = c >> 5;
                             const int32_t rank = pre[wd] + __popc(bm[wd] & ((1u << (c & 31)) - 1u));
-                            atomicAdd(&acc[rank], v);
+                            atomicAdd(&dst[rank], v);
                           });
     __syncthreads();
-    const int64_t off = row_off[i];
     int32_t rk = first_rank;
     for (int32_t w = w0; w < w1; ++w) {
       uint32_t bits = bm[w];
@@ -485,7 +496,8 @@ __global__ __launch_bounds__(kBitmapBlock) void k_row_bitmap(const int32_t* __re
         bits &= bits - 1;
       }
     }
-    for (int32_t e = threadIdx.x; e < nnz; e += kBitmapBlock) slot_val[off + e] = acc[e];
+    if (!kGlobal)
+      for (int32_t e = threadIdx.x; e < nnz; e += kBitmapBlock) slot_val[off + e] = acc[e];
     if (threadIdx.x == 0) row_nnz[i] = nnz;
     __syncthreads();
   }

@@ -17,10 +17,13 @@
 //                         and ds_add_f64 at each column's rank, written in
 //                         order with no sort; wider outputs use an LDS table
 //                         and an LDS bitonic sort;
-//        dense            densest rows: a dense FP64 accumulator + flag array
-//                         per workgroup in HBM, emitted in column order;
-//        wide             long, not dense rows: a per-row HBM hash table
-//                         (2*ub slots), then an LDS or global bitonic sort.
+//        dense / wide     longer rows: with at most 2^17 columns, the LDS
+//                         column bitmap with FP64 atomics into the row's slot
+//                         region; wider outputs: the densest rows use a dense
+//                         FP64 accumulator + flag array per workgroup in HBM,
+//                         emitted in column order, the others a per-row HBM
+//                         hash table (2*ub slots), then an LDS or global
+//                         bitonic sort.
 //   5. scan(nnz)      C_pos.
 //   6. k_compact      slot regions -> contiguous C_crd / C_vals.
 //   -- sync: read nnz(C)
@@ -73,7 +76,8 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
   unsigned long long* maxub_d = reinterpret_cast<unsigned long long*>(bmeta + 32);
   SPWS_LAUNCH(ctx, "meta_clear", k_meta_clear, dim3(1), dim3(64), 0, bmeta, maxub_d);
   if (m > 0) {
-    SPWS_LAUNCH(ctx, "rowstat", k_rowstat, dim3((unsigned)div_up(m, 256)), dim3(256), 0, m, nullptr, n, Ap, Ai, Bp,
+    SPWS_LAUNCH(ctx, "rowstat", k_rowstat, dim3((unsigned)std::min<int64_t>(div_up(m, 4), 65536)), dim3(256), 0, m,
+                nullptr, n, Ap, Ai, Bp,
                 ctx->row_ub.as<int64_t>(), bmeta, maxub_d, pp);
   }
   rc = exclusive_scan<int64_t, int64_t>(ctx, ctx->row_ub.as<int64_t>(), ctx->row_off.as<int64_t>(), m);
@@ -144,6 +148,18 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
     } else {
       SPWS_LAUNCH(ctx, "row_block", (k_row_block<8192, 4096, 512, kValues>), dim3(grid_for(counts[path], 1 << 16)),
                   dim3(512), 0, rows, bmeta, path, Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz);
+    }
+  }
+  // rows longer than the block bins: while the columns fit the LDS bitmap,
+  // the bitmap kernel accumulating in the rows' slot regions replaces both
+  // the dense accumulator and the HBM hash table
+  if (n <= kBitmapCols) {
+    for (int path : {(int)P_DENSE, (int)P_WIDE}) {
+      if (!counts[path]) continue;
+      SPWS_LAUNCH(ctx, "row_bitmap_long", (k_row_bitmap<kValues, true>), dim3(grid_for(counts[path], 4096)),
+                  dim3(kBitmapBlock), bitmap_lds_bytes(n, 0, false), rows, bmeta, path, n, Ap, Ai, Ax, Bp, Bi, Bx,
+                  roff, scrd, sval, rnnz, (const int32_t*)nullptr);
+      counts[path] = 0;
     }
   }
   if (counts[P_DENSE]) {

@@ -74,13 +74,27 @@ def test_wide_rows_natural(ctx):
     assert ctx.stats()["rows_per_path"][5] == 40
 
 
-def test_dense_rows_natural(ctx):
-    print("case: dense rows", flush=True)
+@pytest.mark.parametrize("ncols", [3000, 140_000])
+def test_dense_rows_natural(ctx, ncols):
+    """ub * 16 >= ncols: the column bitmap with HBM accumulation up to 2^17
+    columns, the dense HBM accumulator above."""
+    print(f"case: dense rows ncols={ncols}", flush=True)
     A = random_csr(30, 500, np.full(30, 300), 7, 0)
-    B = random_csr(500, 3000, np.full(500, 40), 7, 1)
+    B = random_csr(500, ncols, np.full(500, 40), 7, 1)
     C = run(ctx, A, B)
     check(C, A, B)
     assert ctx.stats()["rows_per_path"][4] == 30
+
+
+def test_long_rows_bitmap_hbm_accumulation(ctx):
+    """Rows of ub > 4096 with at most 2^17 columns, some of them with few
+    distinct columns (many products per entry)."""
+    print("case: long rows, bitmap + HBM accumulation", flush=True)
+    A = random_csr(64, 3000, np.full(64, 400), 9, 0)
+    B = random_csr(3000, 100_000, np.full(3000, 15), 9, 1)
+    check(run(ctx, A, B), A, B)
+    Bn = random_csr(3000, 700, np.full(3000, 15), 9, 2)
+    check(run(ctx, A, Bn), A, Bn)
 
 
 @pytest.mark.parametrize("ncols", [3000, 131072, 131073, 200_000])
