@@ -41,8 +41,11 @@ constexpr int kMetaCount = 0, kMetaOff = 8, kMetaCursor = 16, kMetaInts = 24;
 
 __device__ __forceinline__ int32_t dev_or(const int32_t* p, int32_t v) { return p ? *p : v; }
 
-// One wavefront per row (A rows range from a few to thousands of entries):
-// lanes sum the matched B row lengths, lane 0 classifies the row.
+// One thread per row; rows longer than kRowstatShort entries are summed by
+// the whole wavefront, one at a time (A rows range from a few entries to
+// thousands).  Each row's path is counted in an LDS histogram.
+constexpr int32_t kRowstatShort = 32;
+
 __global__ __launch_bounds__(256) void k_rowstat(int32_t m_host, const int32_t* __restrict__ m_dev, int32_t ncols,
                                                  const int32_t* __restrict__ Ap, const int32_t* __restrict__ Ai,
                                                  const int32_t* __restrict__ Bp, int64_t* __restrict__ row_ub,
@@ -57,16 +60,35 @@ __global__ __launch_bounds__(256) void k_rowstat(int32_t m_host, const int32_t* 
   __syncthreads();
   const int32_t m = dev_or(m_dev, m_host);
   const int lane = lane_id();
-  const int32_t nw = (int32_t)(gridDim.x * blockDim.x / kWave);
-  for (int32_t i = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kWave); i < m; i += nw) {
-    int64_t ub = 0;
-    const int32_t e = Ap[i + 1];
-    for (int32_t p = Ap[i] + lane; p < e; p += kWave) {
-      const int32_t j = Ai[p];
-      ub += Bp[j + 1] - Bp[j];
+  const int32_t stride = gridDim.x * blockDim.x;
+  // wave-uniform trip count: the cooperative part needs every lane
+  for (int32_t base = blockIdx.x * blockDim.x; base < m; base += stride) {
+    const int32_t i = base + threadIdx.x;
+    const bool valid = i < m;
+    int32_t b = 0, e = 0;
+    if (valid) {
+      b = Ap[i];
+      e = Ap[i + 1];
     }
-    ub = wave_sum(ub);
-    if (lane == 0) {
+    const bool is_long = valid && e - b > kRowstatShort;
+    int64_t ub = 0;
+    if (valid && !is_long)
+      for (int32_t p = b; p < e; ++p) {
+        const int32_t j = Ai[p];
+        ub += Bp[j + 1] - Bp[j];
+      }
+    for (uint64_t lm = __ballot(is_long); lm; lm &= lm - 1) {
+      const int l = __builtin_ctzll(lm);
+      const int32_t lb = __shfl(b, l, kWave), le = __shfl(e, l, kWave);
+      int64_t part = 0;
+      for (int32_t p = lb + lane; p < le; p += kWave) {
+        const int32_t j = Ai[p];
+        part += Bp[j + 1] - Bp[j];
+      }
+      part = wave_sum(part);
+      if (lane == l) ub = part;
+    }
+    if (valid) {
       row_ub[i] = ub;
       const int path = classify_row(ub, ncols, pp);
       atomicAdd(&cnt[path], 1);
@@ -126,20 +148,25 @@ __global__ __launch_bounds__(256) void k_binscatter(int32_t m_host, const int32_
 
 // ------------------------------------------------------------------ product expansion
 //
-// The products of row i are enumerated by a wavefront 64 A-entries at a time:
-// each lane loads one A entry (j, a) and len(B_j); an inclusive wave scan
-// gives the product offsets, and each lane then finds, for its product
-// index q, the owning A entry by a binary search over the lanes' offsets
-// (six ds_bpermute rounds).  Loop bounds are wave-uniform, so the shuffles
-// see every lane.
+// The products of row i are enumerated 64 A-entries at a time: each lane
+// loads one A entry (j, a) and len(B_j); an inclusive wave scan gives the
+// product offsets, and each lane then finds, for its product index q, the
+// owning A entry by a binary search over the lanes' offsets (six
+// ds_bpermute rounds).  Every wavefront of the workgroup scans the same
+// chunk of A entries and takes its own strides of the chunk's products, so
+// rows with few A entries but long B rows still use all wavefronts.  Loop
+// bounds are wave-uniform, so the shuffles see every lane.
 
 template <bool kValues, class F>
-__device__ __forceinline__ void expand_products(int32_t a_beg, int32_t a_end, int32_t first, int32_t stride,
+__device__ __forceinline__ void expand_products(int32_t a_beg, int32_t a_end, int32_t widx, int32_t nwaves,
                                                 const int32_t* __restrict__ Ai, const double* __restrict__ Ax,
                                                 const int32_t* __restrict__ Bp, const int32_t* __restrict__ Bi,
                                                 const double* __restrict__ Bx, F&& insert) {
+  // kU strides of products are located and loaded before any is inserted,
+  // so their B loads are in flight together
+  constexpr int kU = 4;
   const int lane = lane_id();
-  for (int32_t c = a_beg + first; c < a_end; c += stride) {
+  for (int32_t c = a_beg; c < a_end; c += kWave) {
     const int32_t p = c + lane;
     int32_t bstart = 0, blen = 0;
     double a = 0.0;
@@ -152,26 +179,39 @@ __device__ __forceinline__ void expand_products(int32_t a_beg, int32_t a_end, in
     const int32_t incl = wave_incl_scan(blen);
     const int32_t excl = incl - blen;
     const int32_t total = __shfl(incl, kWave - 1, kWave);
-    for (int32_t q0 = 0; q0 < total; q0 += kWave) {
-      const int32_t q = q0 + lane;
-      // owner: the last lane t with excl_t <= q
-      int lo = 0, hi = kWave - 1;
+    for (int32_t q0 = widx * kWave * kU; q0 < total; q0 += nwaves * kWave * kU) {
+      // strides past the row's end are skipped (wave-uniform)
+      const int nu = min(kU, (total - q0 + kWave - 1) / kWave);
+      int32_t kb[kU];
+      double oa[kU];
 #pragma unroll
-      for (int it = 0; it < 6; ++it) {
-        const int mid = (lo + hi + 1) >> 1;
-        const int32_t em = __shfl(excl, mid, kWave);
-        if (em <= q) lo = mid; else hi = mid - 1;
+      for (int u = 0; u < kU; ++u) {
+        if (u >= nu) break;
+        // product index clamped into the row (lanes past the end are masked at insert)
+        const int32_t q = min(q0 + u * kWave + lane, total - 1);
+        // owner: the last lane t with excl_t <= q
+        int lo = 0, hi = kWave - 1;
+#pragma unroll
+        for (int it = 0; it < 6; ++it) {
+          const int mid = (lo + hi + 1) >> 1;
+          const int32_t em = __shfl(excl, mid, kWave);
+          if (em <= q) lo = mid; else hi = mid - 1;
+        }
+        kb[u] = __shfl(bstart, lo, kWave) + (q - __shfl(excl, lo, kWave));
+        oa[u] = kValues ? __shfl(a, lo, kWave) : 0.0;
       }
-      const int32_t o_start = __shfl(bstart, lo, kWave);
-      const int32_t o_excl = __shfl(excl, lo, kWave);
-      double o_a = 0.0;
-      if (kValues) o_a = __shfl(a, lo, kWave);
-      if (q < total) {
-        const int32_t kb = o_start + (q - o_excl);
-        const int32_t col = Bi[kb];
-        double v = 0.0;
-        if (kValues) v = o_a * Bx[kb];
-        insert(col, v);
+      int32_t col[kU];
+      double bv[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        if (u >= nu) break;
+        col[u] = Bi[kb[u]];
+        bv[u] = kValues ? Bx[kb[u]] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        if (u >= nu) break;
+        if (q0 + u * kWave + lane < total) insert(col[u], kValues ? oa[u] * bv[u] : 0.0);
       }
     }
   }
@@ -297,7 +337,7 @@ __global__ __launch_bounds__(64) void k_row_wave(const int32_t* __restrict__ bin
       if (kValues) vals[s] = 0.0;
     }
     __syncthreads();
-    expand_products<kValues>(Ap[i], Ap[i + 1], 0, kWave, Ai, Ax, Bp, Bi, Bx, [&](int32_t col, double v) {
+    expand_products<kValues>(Ap[i], Ap[i + 1], 0, 1, Ai, Ax, Bp, Bi, Bx, [&](int32_t col, double v) {
       lds_insert<kValues>(keys, vals, shift, kMask, col, v);
     });
     __syncthreads();
@@ -369,7 +409,7 @@ __global__ __launch_bounds__(BLOCK) void k_row_block(const int32_t* __restrict__
     }
     if (threadIdx.x == 0) counter = 0;
     __syncthreads();
-    expand_products<kValues>(Ap[i], Ap[i + 1], wave * kWave, BLOCK, Ai, Ax, Bp, Bi, Bx,
+    expand_products<kValues>(Ap[i], Ap[i + 1], wave, BLOCK / kWave, Ai, Ax, Bp, Bi, Bx,
                              [&](int32_t col, double v) { lds_insert<kValues>(keys, vals, shift, kMask, col, v); });
     __syncthreads();
     for (int s = threadIdx.x; s < TS; s += BLOCK) {
@@ -457,7 +497,7 @@ __global__ __launch_bounds__(kBitmapBlock) void k_row_bitmap(const int32_t* __re
     const int32_t cb = row_colbase ? row_colbase[i] : 0;
     for (int32_t w = threadIdx.x; w < words; w += kBitmapBlock) bm[w] = 0u;
     __syncthreads();
-    expand_products<false>(Ap[i], Ap[i + 1], wave * kWave, kBitmapBlock, Ai, Ax, Bp, Bi, Bx, [&](int32_t col, double) {
+    expand_products<false>(Ap[i], Ap[i + 1], wave, kBitmapBlock / kWave, Ai, Ax, Bp, Bi, Bx, [&](int32_t col, double) {
       const int32_t c = col - cb;
       atomicOr(&bm[c >> 5], 1u << (c & 31));
     });
@@ -480,7 +520,7 @@ __global__ __launch_bounds__(kBitmapBlock) void k_row_bitmap(const int32_t* __re
     double* dst = kGlobal ? slot_val + off : acc;
     for (int32_t e = threadIdx.x; e < nnz; e += kBitmapBlock) dst[e] = 0.0;
     __syncthreads();
-    expand_products<true>(Ap[i], Ap[i + 1], wave * kWave, kBitmapBlock, Ai, Ax, Bp, Bi, Bx,
+    expand_products<true>(Ap[i], Ap[i + 1], wave, kBitmapBlock / kWave, Ai, Ax, Bp, Bi, Bx,
                           [&](int32_t col, double v) {
                             const int32_t c = col - cb;
                             const int32_t wd = c >> 5;
@@ -531,7 +571,7 @@ __global__ __launch_bounds__(kDenseBlock) void k_row_dense(const int32_t* __rest
   const int32_t* rows = bin_rows + bmeta[kMetaOff + path];
   for (int32_t r = blockIdx.x; r < count; r += gridDim.x) {
     const int32_t i = rows[r];
-    expand_products<kValues>(Ap[i], Ap[i + 1], wave * kWave, kDenseBlock, Ai, Ax, Bp, Bi, Bx,
+    expand_products<kValues>(Ap[i], Ap[i + 1], wave, kDenseBlock / kWave, Ai, Ax, Bp, Bi, Bx,
                              [&](int32_t col, double v) {
                                dflag[col] = 1;
                                if (kValues) atomicAdd(&dval[col], v);
@@ -637,7 +677,7 @@ __global__ __launch_bounds__(kWideBlock) void k_row_wide(const int32_t* __restri
     if (threadIdx.x == 0) counter = 0;
     __threadfence();
     __syncthreads();
-    expand_products<kValues>(Ap[i], Ap[i + 1], wave * kWave, kWideBlock, Ai, Ax, Bp, Bi, Bx,
+    expand_products<kValues>(Ap[i], Ap[i + 1], wave, kWideBlock / kWave, Ai, Ax, Bp, Bi, Bx,
                              [&](int32_t col, double v) { wide_insert(tkeys, tvals, shift, mask, col, v, kValues); });
     __threadfence();
     __syncthreads();
