@@ -36,7 +36,8 @@ __host__ __device__ __forceinline__ int classify_row(int64_t ub, int32_t ncols, 
 
 // Bin metadata on the device (int32 bmeta): [0, 8) rows per path, [8, 16)
 // offset of each path's rows in bin_rows, [16, 24) scatter cursors; maxub
-// (uint64): [0, 8) largest row upper bound per path.
+// (uint64): [0, 8) largest row upper bound per path, [8] sum of all upper
+// bounds.
 constexpr int kMetaCount = 0, kMetaOff = 8, kMetaCursor = 16, kMetaInts = 24;
 
 __device__ __forceinline__ int32_t dev_or(const int32_t* p, int32_t v) { return p ? *p : v; }
@@ -61,6 +62,7 @@ __global__ __launch_bounds__(256) void k_rowstat(int32_t m_host, const int32_t* 
   const int32_t m = dev_or(m_dev, m_host);
   const int lane = lane_id();
   const int32_t stride = gridDim.x * blockDim.x;
+  unsigned long long tot = 0;
   // wave-uniform trip count: the cooperative part needs every lane
   for (int32_t base = blockIdx.x * blockDim.x; base < m; base += stride) {
     const int32_t i = base + threadIdx.x;
@@ -93,8 +95,11 @@ __global__ __launch_bounds__(256) void k_rowstat(int32_t m_host, const int32_t* 
       const int path = classify_row(ub, ncols, pp);
       atomicAdd(&cnt[path], 1);
       atomicMax(&mx[path], (unsigned long long)ub);
+      tot += (unsigned long long)ub;
     }
   }
+  tot = wave_sum(tot);
+  if (lane == 0 && tot) atomicAdd(&maxub[8], tot);
   __syncthreads();
   if (threadIdx.x < kPaths && cnt[threadIdx.x]) {
     atomicAdd(&bmeta[kMetaCount + threadIdx.x], cnt[threadIdx.x]);
@@ -117,7 +122,7 @@ __global__ void k_bin_offsets(int32_t* __restrict__ bmeta) {
 __global__ void k_meta_clear(int32_t* __restrict__ bmeta, unsigned long long* __restrict__ maxub) {
   const int t = threadIdx.x;
   if (t < kMetaInts) bmeta[t] = 0;
-  if (t < 8) maxub[t] = 0;
+  if (t < 9) maxub[t] = 0;  // [8]: sum of the upper bounds
 }
 
 __global__ __launch_bounds__(256) void k_binscatter(int32_t m_host, const int32_t* __restrict__ m_dev, int32_t ncols,
@@ -312,6 +317,8 @@ __device__ __forceinline__ void block_bitonic_lds(int32_t* list, int P) {
 //
 // One 64-thread workgroup (one wavefront) per row, TS = 2 * max ub slots.
 
+// bin_rows == nullptr: rows 0 .. direct_m - 1 in order, each with a fixed
+// region of kWave slots at r * kWave (every row has ub <= 64).
 template <int TS, int R, bool kValues>
 __global__ __launch_bounds__(64) void k_row_wave(const int32_t* __restrict__ bin_rows, const int32_t* __restrict__ bmeta,
                                                  int path,
@@ -319,7 +326,8 @@ __global__ __launch_bounds__(64) void k_row_wave(const int32_t* __restrict__ bin
                                                  const double* __restrict__ Ax, const int32_t* __restrict__ Bp,
                                                  const int32_t* __restrict__ Bi, const double* __restrict__ Bx,
                                                  const int64_t* __restrict__ row_off, int32_t* __restrict__ slot_crd,
-                                                 double* __restrict__ slot_val, int32_t* __restrict__ row_nnz) {
+                                                 double* __restrict__ slot_val, int32_t* __restrict__ row_nnz,
+                                                 int32_t direct_m = 0) {
   static_assert(TS >= 2 * kWave * R / 2, "table too small");
   __shared__ int32_t keys[TS];
   __shared__ double vals[kValues ? TS : 1];
@@ -327,10 +335,11 @@ __global__ __launch_bounds__(64) void k_row_wave(const int32_t* __restrict__ bin
   constexpr uint32_t kMask = TS - 1;
   const uint32_t shift = 32 - log2_pow2(TS);
   const int lane = lane_id();
-  const int32_t count = bmeta[kMetaCount + path];
-  const int32_t* rows = bin_rows + bmeta[kMetaOff + path];
+  const bool direct = bin_rows == nullptr;
+  const int32_t count = direct ? direct_m : bmeta[kMetaCount + path];
+  const int32_t* rows = direct ? nullptr : bin_rows + bmeta[kMetaOff + path];
   for (int32_t r = blockIdx.x; r < count; r += gridDim.x) {
-    const int32_t i = rows[r];
+    const int32_t i = direct ? r : rows[r];
 #pragma unroll
     for (int s = lane; s < TS; s += kWave) {
       keys[s] = -1;
@@ -366,7 +375,7 @@ __global__ __launch_bounds__(64) void k_row_wave(const int32_t* __restrict__ bin
     int n_pow2 = 2;
     while (n_pow2 < nnz) n_pow2 <<= 1;
     wave_bitonic<R>(v, n_pow2);
-    const int64_t off = row_off[i];
+    const int64_t off = direct ? (int64_t)i * kWave : row_off[i];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int e = q * kWave + lane;
@@ -378,6 +387,59 @@ __global__ __launch_bounds__(64) void k_row_wave(const int32_t* __restrict__ bin
     if (lane == 0) row_nnz[i] = nnz;
     __syncthreads();
   }
+}
+
+// ------------------------------------------------------------------ short-row compaction
+//
+// When every row has ub <= 64, k_row_wave runs over the rows directly (no
+// bins) into fixed 64-entry slot regions, and this kernel turns the slots
+// into C: each workgroup owns kCfRows consecutive rows, adds up the counts of
+// all rows before them (at most kCfMaxRows rows, so a few hundred loads per
+// thread), scans its own rows' counts and copies the rows, writing C_pos.
+// Two launches, no upper-bound scan, no bins and no count scan.
+
+constexpr int kCfRows = 64;
+constexpr int kCfBlock = 256;
+constexpr int32_t kCfMaxRows = 1 << 16;
+
+template <bool kValues>
+__global__ __launch_bounds__(kCfBlock) void k_compact_fixed(int32_t m, const int32_t* __restrict__ row_nnz,
+                                                            const int32_t* __restrict__ slot_crd,
+                                                            const double* __restrict__ slot_val,
+                                                            int32_t* __restrict__ C_pos, int32_t* __restrict__ C_crd,
+                                                            double* __restrict__ C_val) {
+  __shared__ int32_t sh[kCfBlock / kWave + 1];
+  __shared__ int32_t pre[kCfRows];
+  __shared__ int32_t s_base, s_total;
+  const int lane = lane_id();
+  const int w = threadIdx.x / kWave;
+  const int32_t r0 = blockIdx.x * kCfRows;
+  // counts of all earlier rows
+  int32_t before = 0;
+  for (int32_t i = threadIdx.x; i < r0; i += kCfBlock) before += row_nnz[i];
+  int32_t tot;
+  (void)block_excl_scan<int32_t, kCfBlock>(before, sh, &tot);
+  if (w == 0) {
+    const int32_t c = (lane < kCfRows && r0 + lane < m) ? row_nnz[r0 + lane] : 0;
+    const int32_t incl = wave_incl_scan(c);
+    pre[lane] = incl - c;
+    if (lane == kWave - 1) s_total = incl;
+    if (lane == 0) s_base = tot;
+  }
+  __syncthreads();
+  const int32_t base = s_base;
+  for (int q = w; q < kCfRows; q += kCfBlock / kWave) {
+    const int32_t r = r0 + q;
+    if (r >= m) break;
+    const int32_t at = base + pre[q];
+    const int32_t n = row_nnz[r];
+    if (lane == 0) C_pos[r] = at;
+    if (kValues && lane < n) {
+      C_crd[at + lane] = slot_crd[(int64_t)r * kWave + lane];
+      C_val[at + lane] = slot_val[(int64_t)r * kWave + lane];
+    }
+  }
+  if (threadIdx.x == 0 && r0 + kCfRows >= m) C_pos[m] = base + s_total;
 }
 
 // ------------------------------------------------------------------ workgroup-per-row kernel

@@ -80,22 +80,18 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
                 m, nullptr, n, Ap, Ai, Bp,
                 ctx->row_ub.as<int64_t>(), bmeta, maxub_d, pp);
   }
-  rc = exclusive_scan<int64_t, int64_t>(ctx, ctx->row_ub.as<int64_t>(), ctx->row_off.as<int64_t>(), m);
-  if (rc) return rc;
-  SPWS_LAUNCH(ctx, "bin_offsets", k_bin_offsets, dim3(1), dim3(64), 0, bmeta);
   // one small read: counts, max ub per path, total products
   int64_t* hm = ctx->host_meta.as<int64_t>();
   int32_t* hc = reinterpret_cast<int32_t*>(hm + 24);
   SPWS_HIP(hipMemcpyAsync(hc, bmeta, 8 * 4, hipMemcpyDeviceToHost, s));
-  SPWS_HIP(hipMemcpyAsync(hm, maxub_d, 8 * 8, hipMemcpyDeviceToHost, s));
-  SPWS_HIP(hipMemcpyAsync(hm + 16, ctx->row_off.as<int64_t>() + m, 8, hipMemcpyDeviceToHost, s));
+  SPWS_HIP(hipMemcpyAsync(hm, maxub_d, 9 * 8, hipMemcpyDeviceToHost, s));
   SPWS_HIP(hipStreamSynchronize(s));
   int64_t counts[8], maxub[8];
   for (int p = 0; p < 8; ++p) {
     counts[p] = hc[p];
     maxub[p] = hm[p];
   }
-  const int64_t total_ub = hm[16];
+  const int64_t total_ub = hm[8];
   ctx->stats.products = total_ub;
   for (int p = 0; p < kPaths; ++p) ctx->stats.rows_per_path[p] = counts[p];
   SPWS_REQUIRE(total_ub <= (int64_t)INT32_MAX, SPWS_ERR_UNSUPPORTED,
@@ -107,6 +103,35 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
                    "forced path %d cannot hold a row of %lld products", t.force_path,
                    (long long)maxub[t.force_path]);
   }
+
+  // every row fits one wavefront: rows straight into fixed 64-entry slots,
+  // then one compaction that also forms C_pos (no bins, no scans)
+  if (t.force_path < 0 && m > 0 && m <= kCfMaxRows && counts[P_WAVE] + counts[P_EMPTY] == m &&
+      maxub[P_WAVE] <= kWaveUbMax) {
+    const size_t cap = (size_t)(total_ub > 0 ? total_ub : 1);
+    const size_t fixed = (size_t)m * kWave;
+    if ((kValues && ((rc = ctx->slot_crd.ensure(fixed * 4)) || (rc = ctx->slot_val.ensure(fixed * 8)) ||
+                     (rc = ctx->out_crd.ensure(cap * 4)) || (rc = ctx->out_val.ensure(cap * 8)))))
+      return rc;
+    SPWS_LAUNCH(ctx, "row_wave", (k_row_wave<128, 1, kValues>), dim3((unsigned)std::min<int64_t>(m, 1 << 20)),
+                dim3(64), 0, (const int32_t*)nullptr, bmeta, (int)P_WAVE, Ap, Ai, Ax, Bp, Bi, Bx,
+                (const int64_t*)nullptr, kValues ? ctx->slot_crd.as<int32_t>() : nullptr,
+                kValues ? ctx->slot_val.as<double>() : nullptr, ctx->row_nnz.as<int32_t>(), m);
+    SPWS_LAUNCH(ctx, "compact_fixed", (k_compact_fixed<kValues>), dim3((unsigned)div_up(m, kCfRows)),
+                dim3(kCfBlock), 0, m, ctx->row_nnz.as<int32_t>(), kValues ? ctx->slot_crd.as<int32_t>() : nullptr,
+                kValues ? ctx->slot_val.as<double>() : nullptr, ctx->out_pos.as<int32_t>(),
+                kValues ? ctx->out_crd.as<int32_t>() : nullptr, kValues ? ctx->out_val.as<double>() : nullptr);
+    int32_t* h32 = reinterpret_cast<int32_t*>(hm + 20);
+    SPWS_HIP(hipMemcpyAsync(h32, ctx->out_pos.as<int32_t>() + m, 4, hipMemcpyDeviceToHost, s));
+    SPWS_HIP(hipStreamSynchronize(s));
+    *nnz_out = *h32;
+    ctx->stats.nnz = *h32;
+    return SPWS_OK;
+  }
+
+  rc = exclusive_scan<int64_t, int64_t>(ctx, ctx->row_ub.as<int64_t>(), ctx->row_off.as<int64_t>(), m);
+  if (rc) return rc;
+  SPWS_LAUNCH(ctx, "bin_offsets", k_bin_offsets, dim3(1), dim3(64), 0, bmeta);
 
   const size_t slots = (size_t)(total_ub > 0 ? total_ub : 1);
   if (kValues) {

@@ -45,6 +45,23 @@ def test_rowwise_workload_10k(ctx):
     assert st["rows_per_path"][0] == 10_000
 
 
+@pytest.mark.parametrize("m,seed", [(1, 11), (63, 12), (64, 13), (65, 14), (3000, 15), (20000, 16)])
+def test_single_pass_short_rows(ctx, m, seed):
+    """Every row has ub <= 64: the single-pass kernel (rows claimed in order,
+    decoupled look-back for the output offsets) writes C directly.  Includes
+    empty rows and row counts around the 64-row look-back window."""
+    print(f"case: single-pass m={m} seed={seed}", flush=True)
+    A = uniform_rows(m, 700, 0, 8, seed, 0)
+    B = uniform_rows(700, 900, 0, 8, seed, 1)
+    assert oracle.spgemm_row_ub(A, B).max() <= 64
+    C = run(ctx, A, B)
+    check(C, A, B)
+    st = ctx.stats()
+    assert st["rows_per_path"][0] + st["rows_per_path"][6] == m
+    dA, dB = DeviceCSR.upload(ctx, A), DeviceCSR.upload(ctx, B)
+    np.testing.assert_array_equal(spgemm_rowwise_symbolic(ctx, dA, dB), C.pos)
+
+
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_uniform_small(ctx, seed):
     print(f"case: uniform seed {seed}", flush=True)
