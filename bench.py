@@ -393,10 +393,10 @@ def run_serve(args):
     and half Zipf(1.8) capped at 4,096, FP64 U[-1,1], seed 0.  Operands are
     resident in HBM before the run.  Measured: saturation throughput (all
     requests queued at t = 0) of the plain path (one row-wise call per
-    request) and of the continuously batched path (2 engines, one hipGraph
-    launch per batch); then Poisson arrivals at 80% of the plain path's
-    capacity for both, and at 3x that rate for the batched path, with
-    p50/p99 latency (finish - arrival)."""
+    request) and of the continuously batched path (4 engines, one hipGraph
+    launch per batch, at most 2^26 products per batch); then Poisson
+    arrivals at 80% of the plain path's capacity for both, and at 3x that
+    rate for the batched path, with p50/p99 latency (finish - arrival)."""
     from sparsews_amd import gen
     from sparsews_amd.device import Context
     from sparsews_amd.serve import DeviceRequest, Engine, latency_summary, serve
@@ -413,13 +413,14 @@ def run_serve(args):
         if len(reqs) % 512 == 0:
             log(f"serve: {len(reqs)} requests generated and uploaded ({time.perf_counter() - t0:.0f} s)")
     N = len(reqs)
-    engines = [Engine(ctx) for _ in range(2)]
+    n_engines, cap = 4, 1 << 26
+    engines = [Engine(ctx) for _ in range(n_engines)]
     zero = np.zeros(N)
     # warm-up: one pass of each mode on a prefix
     serve(ctx, [], reqs[:32], zero[:32], "plain")
-    serve(ctx, engines, reqs[:32], zero[:32], "batched")
+    serve(ctx, engines, reqs[:32], zero[:32], "batched", max_batch_products=cap)
     f_p, _, st_p = serve(ctx, [], reqs, zero, "plain")
-    f_b, bid, st_b = serve(ctx, engines, reqs, zero, "batched")
+    f_b, bid, st_b = serve(ctx, engines, reqs, zero, "batched", max_batch_products=cap)
     sat_plain = N / st_p["wall_seconds"]
     sat_batch = N / st_b["wall_seconds"]
     log(f"serve: saturation plain {sat_plain:.1f} req/s, batched {sat_batch:.1f} req/s")
@@ -428,9 +429,9 @@ def run_serve(args):
     gaps = rng.exponential(1.0 / lam, N)
     arr = np.cumsum(gaps) - gaps[0]
     f_pl, _, st_pl = serve(ctx, [], reqs, arr, "plain")
-    f_bl, _, st_bl = serve(ctx, engines, reqs, arr, "batched")
+    f_bl, _, st_bl = serve(ctx, engines, reqs, arr, "batched", max_batch_products=cap)
     arr3 = arr / 3.0
-    f_b3, _, st_b3 = serve(ctx, engines, reqs, arr3, "batched")
+    f_b3, _, st_b3 = serve(ctx, engines, reqs, arr3, "batched", max_batch_products=cap)
     for e in engines:
         e.close()
     ctx.close()
@@ -442,7 +443,8 @@ def run_serve(args):
         "ms_per_step": round(st_b["wall_seconds"] * 1e3, 3), "dtype": "f64",
         "config": {"workload": "served_spgemm_4096_requests", "requests": N, "n_range": [256, 65536],
                    "products_total": int(products), "products_median": int(np.median(sizes)),
-                   "products_q99": int(np.percentile(sizes, 99)), "engines": 2, "seed": 0},
+                   "products_q99": int(np.percentile(sizes, 99)), "engines": n_engines,
+                   "max_batch_products": cap, "seed": 0},
         "saturation": {"plain_req_per_s": round(sat_plain, 2), "batched_req_per_s": round(sat_batch, 2),
                        "speedup": round(sat_batch / sat_plain, 3), "batches": st_b["batches"],
                        "fallback_requests": st_b["fallback_requests"],

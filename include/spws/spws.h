@@ -276,6 +276,9 @@ typedef struct {
 typedef struct {
   int32_t mode;                /* 0: plain (one row-wise call per request); 1: continuously batched */
   int32_t max_batch_requests;  /* batched: requests per batch (0 = engine capacity) */
+  int64_t max_batch_products;  /* batched: products (output upper bound) per batch, so a
+                                  batch stays short and latency bounded; a single request
+                                  larger than this still runs alone (0 = engine capacity) */
 } spws_serve_options;
 
 typedef struct {

@@ -58,7 +58,7 @@ class Request(C.Structure):
 
 
 class ServeOptions(C.Structure):
-    _fields_ = [("mode", C.c_int32), ("max_batch_requests", C.c_int32)]
+    _fields_ = [("mode", C.c_int32), ("max_batch_requests", C.c_int32), ("max_batch_products", C.c_int64)]
 
 
 class ServeStats(C.Structure):

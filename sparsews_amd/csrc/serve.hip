@@ -8,7 +8,8 @@
 //   2. arrivals: requests whose arrival time has passed join the queue;
 //   3. admission: each idle engine takes queued requests in arrival order
 //      while its pools hold them (rows, operand nonzeros, and the upper bound
-//      of the output entries), then launches its graph.  Requests admitted
+//      of the output entries) and the batch stays under the products cap,
+//      then launches its graph.  Requests admitted
 //      while other engines run overlap with them on the device (one stream
 //      per engine).  A request no engine can ever hold runs through the
 //      plain path.
@@ -69,6 +70,7 @@ int spws_serve(spws_ctx* ctx, spws_batch** engines, int32_t nengines, int32_t nr
   }
   SPWS_REQUIRE(engines && nengines >= 1, SPWS_ERR_INVALID, "batched mode needs at least one engine");
   const int32_t cap_req = opt->max_batch_requests > 0 ? opt->max_batch_requests : INT32_MAX;
+  const int64_t cap_prod = opt->max_batch_products > 0 ? opt->max_batch_products : INT64_MAX;
   std::vector<std::vector<int32_t>> owned((size_t)nengines);
   std::vector<bool> busy((size_t)nengines, false);
   std::deque<int32_t> queue;
@@ -94,9 +96,11 @@ int spws_serve(spws_ctx* ctx, spws_batch** engines, int32_t nengines, int32_t nr
       if (busy[(size_t)e]) continue;
       if ((rc = spws_batch_reset(engines[e]))) return rc;
       int32_t count = 0;
+      int64_t prod = 0;
       while (!queue.empty() && count < cap_req) {
         const int32_t r = queue.front();
         const spws_request& q = reqs[r];
+        if (count > 0 && prod + q.products > cap_prod) break;  // keep the batch short
         int32_t slot;
         rc = spws_batch_add(engines[e], q.m, q.k, q.n, q.A_pos, q.A_crd, q.A_vals, q.annz, q.B_pos, q.B_crd,
                             q.B_vals, q.bnnz, q.products, q.max_row_products, &slot);
@@ -105,6 +109,7 @@ int spws_serve(spws_ctx* ctx, spws_batch** engines, int32_t nengines, int32_t nr
           owned[(size_t)e].push_back(r);
           if (batch_of) batch_of[r] = (int32_t)batch_id;
           ++count;
+          prod += q.products;
           continue;
         }
         if (rc == SPWS_ERR_OOM && count > 0) break;  // this batch is full

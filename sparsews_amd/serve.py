@@ -101,7 +101,8 @@ class DeviceRequest:
                             self.max_row_products, float(arrival))
 
 
-def serve(ctx: Context, engines, reqs, arrivals, mode: str, max_batch_requests: int = 0):
+def serve(ctx: Context, engines, reqs, arrivals, mode: str, max_batch_requests: int = 0,
+          max_batch_products: int = 0):
     """Run the native request loop.  Returns (finish seconds per request,
     batch id per request (-1 = plain), stats dict)."""
     order = np.argsort(np.asarray(arrivals), kind="stable")
@@ -109,7 +110,7 @@ def serve(ctx: Context, engines, reqs, arrivals, mode: str, max_batch_requests: 
         raise ValueError("requests must be given in arrival order")
     arr = (_lib.Request * len(reqs))(*[r.c_struct(a) for r, a in zip(reqs, arrivals)])
     handles = (C.c_void_p * max(len(engines), 1))(*[e.h for e in engines])
-    opt = _lib.ServeOptions(0 if mode == "plain" else 1, int(max_batch_requests))
+    opt = _lib.ServeOptions(0 if mode == "plain" else 1, int(max_batch_requests), int(max_batch_products))
     finish = np.zeros(len(reqs), dtype=np.float64)
     batch_of = np.zeros(len(reqs), dtype=np.int32)
     st = _lib.ServeStats()
