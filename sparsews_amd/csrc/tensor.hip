@@ -131,8 +131,14 @@ __global__ __launch_bounds__(256) void k_mttkrp_keys(int64_t nnz, const int32_t*
 
 // Each half-wavefront owns `seg` consecutive bucketed nonzeros.  Lanes cover
 // ranks r = lane32, lane32 + 32, ... (R <= 64 supported with two slices).
+// The half-wavefront loads 32 keys and values at once (coalesced), then
+// walks them kAccU at a time: the factor rows of kAccU nonzeros are loaded
+// before any is used, so their latencies overlap; the sums still run in
+// bucket order.
 constexpr int kAccBlock = 256;
+constexpr int kAccU = 8;
 
+template <bool kTwo>
 __global__ __launch_bounds__(kAccBlock) void k_mttkrp_accum(int64_t nnz, int64_t seg, const uint64_t* __restrict__ keys,
                                                             const double* __restrict__ vals, int bj, int bk, int R,
                                                             const double* __restrict__ Bf,
@@ -142,30 +148,56 @@ __global__ __launch_bounds__(kAccBlock) void k_mttkrp_accum(int64_t nnz, int64_t
   const int r0 = threadIdx.x & 31;
   const int64_t hid = (int64_t)blockIdx.x * (kAccBlock / 32) + half;
   const int64_t e0 = hid * seg;
-  if (e0 >= nnz) return;
+  if (e0 >= nnz) return;  // uniform per half-wavefront
   const int64_t e1 = e0 + seg < nnz ? e0 + seg : nnz;
   const uint64_t jm = (1ull << bj) - 1, km = (1ull << bk) - 1;
-  const bool two = R > 32;
+  constexpr bool two = kTwo;  // R > 32: a second slice of ranks per lane
   const int r1 = r0 + 32;
+  const int rr0 = r0 < R ? r0 : 0, rr1 = r1 < R ? r1 : 0;  // in-range rank for the loads
   int64_t cur = -1;
   double acc0 = 0.0, acc1 = 0.0;
-  for (int64_t e = e0; e < e1; ++e) {
-    const uint64_t key = keys[e];
-    const int64_t i = (int64_t)(key >> (bj + bk));
-    if (i != cur) {
-      if (cur >= 0) {
-        if (r0 < R) atomicAdd(&M[cur * R + r0], acc0);
-        if (two && r1 < R) atomicAdd(&M[cur * R + r1], acc1);
-        if (r0 == 0) present[cur] = 1;
+  for (int64_t base = e0; base < e1; base += 32) {
+    const int cnt = (int)(e1 - base < 32 ? e1 - base : 32);
+    const int64_t me = base + (r0 < cnt ? r0 : cnt - 1);
+    const uint64_t mykey = keys[me];
+    const double myval = vals[me];
+    for (int t = 0; t < cnt; t += kAccU) {
+      uint64_t kk[kAccU];
+      double xv[kAccU], b0[kAccU], c0[kAccU], b1[kAccU], c1[kAccU];
+#pragma unroll
+      for (int u = 0; u < kAccU; ++u) {
+        const int src = t + u < cnt ? t + u : cnt - 1;
+        kk[u] = __shfl(mykey, src, 32);
+        xv[u] = __shfl(myval, src, 32);
       }
-      cur = i;
-      acc0 = acc1 = 0.0;
+#pragma unroll
+      for (int u = 0; u < kAccU; ++u) {
+        const int64_t j = (int64_t)((kk[u] >> bk) & jm);
+        const int64_t k = (int64_t)(kk[u] & km);
+        b0[u] = Bf[j * R + rr0];
+        c0[u] = Cf[k * R + rr0];
+        if (two) {
+          b1[u] = Bf[j * R + rr1];
+          c1[u] = Cf[k * R + rr1];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kAccU; ++u) {
+        if (t + u >= cnt) break;  // uniform per half-wavefront
+        const int64_t i = (int64_t)(kk[u] >> (bj + bk));
+        if (i != cur) {
+          if (cur >= 0) {
+            if (r0 < R) atomicAdd(&M[cur * R + r0], acc0);
+            if (two && r1 < R) atomicAdd(&M[cur * R + r1], acc1);
+            if (r0 == 0) present[cur] = 1;
+          }
+          cur = i;
+          acc0 = acc1 = 0.0;
+        }
+        acc0 += (xv[u] * b0[u]) * c0[u];
+        if (two) acc1 += (xv[u] * b1[u]) * c1[u];
+      }
     }
-    const int64_t j = (int64_t)((key >> bk) & jm);
-    const int64_t k = (int64_t)(key & km);
-    const double x = vals[e];
-    if (r0 < R) acc0 += (x * Bf[j * R + r0]) * Cf[k * R + r0];
-    if (two && r1 < R) acc1 += (x * Bf[j * R + r1]) * Cf[k * R + r1];
   }
   if (cur >= 0) {
     if (r0 < R) atomicAdd(&M[cur * R + r0], acc0);
@@ -333,8 +365,13 @@ static int mttkrp_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_
     // segments of 128 nonzeros per half-wavefront
     const int64_t seg = 128;
     const int64_t halves = div_up(x.nnz, seg);
-    SPWS_LAUNCH(ctx, "mttkrp_accum", k_mttkrp_accum, dim3((unsigned)div_up(halves, kAccBlock / 32)), dim3(kAccBlock),
-                0, x.nnz, seg, keys, vals, bj, bk, R, Bf, Cf, M, present);
+    if (R > 32) {
+      SPWS_LAUNCH(ctx, "mttkrp_accum", k_mttkrp_accum<true>, dim3((unsigned)div_up(halves, kAccBlock / 32)),
+                  dim3(kAccBlock), 0, x.nnz, seg, keys, vals, bj, bk, R, Bf, Cf, M, present);
+    } else {
+      SPWS_LAUNCH(ctx, "mttkrp_accum", k_mttkrp_accum<false>, dim3((unsigned)div_up(halves, kAccBlock / 32)),
+                  dim3(kAccBlock), 0, x.nnz, seg, keys, vals, bj, bk, R, Bf, Cf, M, present);
+    }
     ctx->stats.products = x.nnz * (int64_t)R;
   }
   if ((rc = exclusive_scan<int32_t, int32_t>(ctx, present, poff, I))) return rc;
