@@ -33,8 +33,13 @@
 //     4. radix sort         fiber keys, slot ids carried (the reference's Sort
 //                           of the workspace before Merge).
 //     5. k_ttm_rank         slot -> position of its fiber in sorted order.
-//     6. k_ttm_accum        R lanes per nonzero add X * U(k,:) into the dense
-//                           length-R fiber of its (i,j) (FP64 atomics).
+//     6. k_ttm_keys + radix sort + k_ttm_heads: nonzeros grouped by fiber
+//                           rank (stable, so each fiber keeps CSF order).
+//     7. k_ttm_accum        one 16-lane group per work item (at most 128
+//                           nonzeros of one fiber) sums X * U(k,:) in order;
+//                           a fiber of one item writes its vector, longer
+//                           fibers write partial sums that k_ttm_combine adds
+//                           in item order (no atomics; deterministic).
 //   Output: semi-sparse -- the sorted fiber list (i,j) with a dense length-R
 //   vector each.
 #include <algorithm>
@@ -281,21 +286,118 @@ __global__ __launch_bounds__(256) void k_ttm_rank(int64_t F, const uint64_t* __r
   }
 }
 
-// R lanes per nonzero (R <= 64, a power of two up to 64 lanes per group).
-__global__ __launch_bounds__(256) void k_ttm_accum(int64_t nnz, int R, int lanes, const int32_t* __restrict__ leaf_fiber,
-                                                   const int32_t* __restrict__ l2_crd,
-                                                   const double* __restrict__ xvals,
-                                                   const int32_t* __restrict__ leaf_slot,
-                                                   const int32_t* __restrict__ rank_of_slot,
-                                                   const double* __restrict__ U, double* __restrict__ Y) {
-  const int groups = blockDim.x / lanes;
-  const int g = threadIdx.x / lanes;
-  const int r = threadIdx.x % lanes;
-  for (int64_t e = (int64_t)blockIdx.x * groups + g; e < nnz; e += (int64_t)gridDim.x * groups) {
-    const int64_t k = l2_crd[leaf_fiber[e]];
-    const int64_t q = rank_of_slot[leaf_slot[e]];
-    const double x = xvals[e];
-    for (int rr = r; rr < R; rr += lanes) atomicAdd(&Y[q * R + rr], x * U[k * R + rr]);
+// Nonzeros keyed by their fiber's rank and their contracted coordinate:
+// key = q << bk | k, value = X.  A stable sort on the q bits groups each
+// fiber's nonzeros, still in CSF order (increasing k) inside the fiber, the
+// order in which the reference's hash workspace accumulates them.
+__global__ __launch_bounds__(256) void k_ttm_keys(int64_t nnz, const int32_t* __restrict__ leaf_fiber,
+                                                  const int32_t* __restrict__ l2_crd,
+                                                  const double* __restrict__ xvals,
+                                                  const int32_t* __restrict__ leaf_slot,
+                                                  const int32_t* __restrict__ rank_of_slot, int bk,
+                                                  uint64_t* __restrict__ keys, double* __restrict__ vals) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t q = (uint32_t)rank_of_slot[leaf_slot[e]];
+    const uint64_t k = (uint32_t)l2_crd[leaf_fiber[e]];
+    keys[e] = (q << bk) | k;
+    vals[e] = xvals[e];
+  }
+}
+
+// foff[q] = first sorted position of fiber q (every fiber has a nonzero).
+__global__ __launch_bounds__(256) void k_ttm_heads(int64_t nnz, const uint64_t* __restrict__ keys, int bk, int64_t F,
+                                                   int32_t* __restrict__ foff) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t q = keys[p] >> bk;
+    if (p == 0 || (keys[p - 1] >> bk) != q) foff[q] = (int32_t)p;
+    if (p == 0) foff[F] = (int32_t)nnz;
+  }
+}
+
+// Work items of at most kTtmChunk sorted nonzeros: a fiber of n nonzeros
+// has ceil(n / kTtmChunk) items (fibers reach 2,000 nonzeros, one per k).
+constexpr int kTtmChunk = 128;
+constexpr int kTtmLanes = 16;
+constexpr int kTtmU = 16;
+
+__global__ __launch_bounds__(256) void k_ttm_chunks(int64_t F, const int32_t* __restrict__ foff,
+                                                    int32_t* __restrict__ nchunk) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < F; q += (int64_t)gridDim.x * blockDim.x)
+    nchunk[q] = (foff[q + 1] - foff[q] + kTtmChunk - 1) / kTtmChunk;
+}
+
+// item -> fiber map from the item offsets
+__global__ __launch_bounds__(256) void k_ttm_itemmap(int64_t F, const int32_t* __restrict__ coff,
+                                                     int32_t* __restrict__ item_fiber) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < F; q += (int64_t)gridDim.x * blockDim.x)
+    for (int32_t c = coff[q]; c < coff[q + 1]; ++c) item_fiber[c] = (int32_t)q;
+}
+
+// One group of kTtmLanes lanes per work item (ranks r = lane, lane +
+// kTtmLanes, ...): the group loads kTtmLanes sorted keys/values at once and
+// the rows U(k,:) of all of them, then sums in order.  A single-item fiber
+// writes its vector; the items of a longer fiber write partial sums, which
+// k_ttm_combine adds in item order (so the result does not depend on timing).
+__global__ __launch_bounds__(256) void k_ttm_accum(int64_t F, int64_t nitems, int R,
+                                                   const uint64_t* __restrict__ keys, const double* __restrict__ vals,
+                                                   const int32_t* __restrict__ foff,
+                                                   const int32_t* __restrict__ coff,
+                                                   const int32_t* __restrict__ item_fiber, int bk,
+                                                   const double* __restrict__ U, double* __restrict__ Y,
+                                                   double* __restrict__ part) {
+  const int groups = 256 / kTtmLanes;
+  const int g = threadIdx.x / kTtmLanes;
+  const int l = threadIdx.x % kTtmLanes;
+  const uint64_t km = (1ull << bk) - 1;
+  for (int64_t c = (int64_t)blockIdx.x * groups + g; c < nitems; c += (int64_t)gridDim.x * groups) {
+    const int64_t q = item_fiber[c];
+    const int32_t sub = (int32_t)(c - coff[q]);
+    const int32_t single = coff[q + 1] - coff[q] == 1;
+    const int32_t p0 = foff[q] + sub * kTtmChunk;
+    const int32_t p1 = min(p0 + kTtmChunk, foff[q + 1]);
+    for (int r0 = 0; r0 < R; r0 += kTtmLanes) {
+      const int r = r0 + l;
+      const int rr = r < R ? r : R - 1;
+      double acc = 0.0;
+      for (int32_t base = p0; base < p1; base += kTtmLanes) {
+        const int cnt = p1 - base < kTtmLanes ? p1 - base : kTtmLanes;
+        const int32_t me = base + (l < cnt ? l : cnt - 1);
+        const uint64_t mykey = keys[me];
+        const double myval = vals[me];
+        double xv[kTtmU], uv[kTtmU];
+#pragma unroll
+        for (int u = 0; u < kTtmU; ++u) {
+          const int src = u < cnt ? u : cnt - 1;
+          const uint64_t key = __shfl(mykey, src, kTtmLanes);
+          xv[u] = __shfl(myval, src, kTtmLanes);
+          uv[u] = U[(int64_t)(key & km) * R + rr];
+        }
+#pragma unroll
+        for (int u = 0; u < kTtmU; ++u)
+          if (u < cnt) acc += xv[u] * uv[u];
+      }
+      if (r < R) {
+        if (single) Y[q * R + r] = acc;
+        else part[c * R + r] = acc;
+      }
+    }
+  }
+}
+
+// Fibers of several items: Y(q,:) = sum of their partial sums in item order.
+__global__ __launch_bounds__(256) void k_ttm_combine(int64_t F, int R, const int32_t* __restrict__ coff,
+                                                     const double* __restrict__ part, double* __restrict__ Y) {
+  const int groups = 256 / kTtmLanes;
+  const int g = threadIdx.x / kTtmLanes;
+  const int l = threadIdx.x % kTtmLanes;
+  for (int64_t q = (int64_t)blockIdx.x * groups + g; q < F; q += (int64_t)gridDim.x * groups) {
+    const int32_t c0 = coff[q], c1 = coff[q + 1];
+    if (c1 - c0 <= 1) continue;
+    for (int r = l; r < R; r += kTtmLanes) {
+      double acc = part[(int64_t)c0 * R + r];
+      for (int32_t c = c0 + 1; c < c1; ++c) acc += part[(int64_t)c * R + r];
+      Y[q * R + r] = acc;
+    }
   }
 }
 
@@ -445,12 +547,47 @@ static int ttm_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_t K
     const unsigned gf = (unsigned)std::min<int64_t>(div_up(F, 256), 65536);
     SPWS_LAUNCH(ctx, "ttm_rank", k_ttm_rank, dim3(gf), dim3(256), 0, F, keys, pay, bj, rank_of_slot, out_i, out_j);
     double* Y = ctx->out_val.as<double>();
-    SPWS_HIP(hipMemsetAsync(Y, 0, (size_t)F * R * 8, s));
-    const int lanes = R >= 64 ? 64 : (R > 32 ? 64 : (R > 16 ? 32 : (R > 8 ? 16 : (R > 4 ? 8 : (R > 2 ? 4 : R)))));
-    const int64_t groups = 256 / lanes;
-    const unsigned ga = (unsigned)std::min<int64_t>(div_up(x.nnz, groups), 65536);
-    SPWS_LAUNCH(ctx, "ttm_accum", k_ttm_accum, dim3(ga), dim3(256), 0, x.nnz, R, lanes, leaf_fiber, x.l2_crd, x.vals,
-                leaf_slot, rank_of_slot, U, Y);
+    // group each fiber's nonzeros (stable sort on the fiber rank), then one
+    // pass of in-order sums per fiber
+    const int bq = bits_of(F > 1 ? F : 2);
+    const int bk = bits_of(K > 1 ? K : 2);
+    SPWS_REQUIRE(bq + bk <= 64, SPWS_ERR_UNSUPPORTED, "fiber count and K exceed the 64-bit sort key");
+    uint64_t* nkeys = ctx->keys[0].as<uint64_t>();
+    double* nvals = ctx->vals[0].as<double>();
+    SPWS_LAUNCH(ctx, "ttm_keys", k_ttm_keys, dim3(g), dim3(256), 0, x.nnz, leaf_fiber, x.l2_crd, x.vals, leaf_slot,
+                rank_of_slot, bk, nkeys, nvals);
+    bool alt2 = false;
+    if ((rc = radix_sort_pairs<uint64_t>(ctx, nkeys, nvals, ctx->keys[1].as<uint64_t>(), ctx->vals[1].as<double>(),
+                                         x.nnz, bk, bk + bq, &alt2)))
+      return rc;
+    if (alt2) {
+      nkeys = ctx->keys[1].as<uint64_t>();
+      nvals = ctx->vals[1].as<double>();
+    }
+    // fiber offsets reuse the flag array past rank_of_slot's T entries
+    int32_t* foff = off;
+    SPWS_LAUNCH(ctx, "ttm_heads", k_ttm_heads, dim3(g), dim3(256), 0, x.nnz, nkeys, bk, F, foff);
+    // work items of at most kTtmChunk nonzeros; coff = prefix of items per fiber
+    if ((rc = ctx->obuf[2].ensure(((size_t)F + 1) * 4 * 2))) return rc;
+    int32_t* nchunk = ctx->obuf[2].as<int32_t>();
+    int32_t* coff = nchunk + (F + 1);
+    const unsigned gf2 = (unsigned)std::min<int64_t>(div_up(F, 256), 65536);
+    SPWS_LAUNCH(ctx, "ttm_chunks", k_ttm_chunks, dim3(gf2), dim3(256), 0, F, foff, nchunk);
+    if ((rc = exclusive_scan<int32_t, int32_t>(ctx, nchunk, coff, F))) return rc;
+    // items <= F + nnz / kTtmChunk
+    const int64_t max_items = F + x.nnz / kTtmChunk + 1;
+    if ((rc = ctx->obuf[3].ensure((size_t)max_items * R * 8)) || (rc = ctx->obuf[4].ensure((size_t)max_items * 4)))
+      return rc;
+    SPWS_HIP(hipMemcpyAsync(hm, coff + F, 4, hipMemcpyDeviceToHost, s));
+    SPWS_HIP(hipStreamSynchronize(s));
+    const int64_t nitems = *hm;
+    int32_t* item_fiber = ctx->obuf[4].as<int32_t>();
+    SPWS_LAUNCH(ctx, "ttm_itemmap", k_ttm_itemmap, dim3(gf2), dim3(256), 0, F, coff, item_fiber);
+    const unsigned ga = (unsigned)std::min<int64_t>(div_up(nitems, 256 / kTtmLanes), 65536);
+    SPWS_LAUNCH(ctx, "ttm_accum", k_ttm_accum, dim3(ga), dim3(256), 0, F, nitems, R, nkeys, nvals, foff, coff,
+                item_fiber, bk, U, Y, ctx->obuf[3].as<double>());
+    const unsigned gc = (unsigned)std::min<int64_t>(div_up(F, 256 / kTtmLanes), 65536);
+    SPWS_LAUNCH(ctx, "ttm_combine", k_ttm_combine, dim3(gc), dim3(256), 0, F, R, coff, ctx->obuf[3].as<double>(), Y);
     ctx->stats.products = x.nnz * (int64_t)R;
   } else {
     if ((rc = ctx->out_crd.ensure(8)) || (rc = ctx->out_val.ensure(8))) return rc;
