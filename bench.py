@@ -52,12 +52,40 @@ def timed_steps(ctx, step, steps, warmup):
     return (t1 - t0) / steps, times
 
 
-def roofline(times, name, bytes_per_launch):
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` (the name the context's timer uses)
+    for `workload`, from a committed profiles/*_traffic.md summary (two
+    rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE, calibrated by a plain
+    copy kernel); None when no such summary exists."""
+    import glob
+    import re
+    for path in sorted(glob.glob(os.path.join(HERE, "profiles", "*_traffic.md"))):
+        with open(path) as f:
+            for line in f:
+                m = re.match(r"bench-traffic: workload=(\S*) kernel=(\S+) bytes_per_launch=(\d+)", line.strip())
+                if m and m.group(1) == workload and re.sub(r"<.*", "", m.group(2)) in ("k_" + kernel, kernel):
+                    return int(m.group(3))
+    return None
+
+
+def calibration_copy(ctx, nbytes):
+    """One plain device copy of nbytes (reads and writes exactly nbytes):
+    the reference point for the FETCH_SIZE / WRITE_SIZE counters."""
+    if nbytes <= 0:
+        return
+    src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
+    ctx.copy(dst, src, nbytes)
+    ctx.sync()
+    ctx.free(src)
+    ctx.free(dst)
+
+
+def roofline(times, name, bytes_per_launch, workload):
     ms, launches = times[name]
     avg_s = ms / launches / 1e3
     achieved = bytes_per_launch / avg_s / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(workload, name),
             "kernel": name, "avg_launch_us": round(avg_s * 1e6, 3),
             "algorithmic_bytes_per_launch": int(bytes_per_launch)}
 
@@ -85,6 +113,7 @@ def run_rowwise(args):
         out["C"] = spgemm_rowwise(ctx, dA, dB)
 
     sec, times = timed_steps(ctx, step, args.steps, args.warmup)
+    calibration_copy(ctx, args.calibrate_copy)
     C = out["C"]
     P = ctx.stats()["products"]
     nnz = C.nnz
@@ -104,7 +133,7 @@ def run_rowwise(args):
         "dtype": "f64",
         "config": {"workload": "rowwise_hash_spgemm_10k_r8", "m": m, "n": B.ncols, "nnz_per_row": 8,
                    "products": int(P), "nnz_C": int(nnz), "seed": 0},
-        "roofline": roofline(times, dom, dom_bytes),
+        "roofline": roofline(times, dom, dom_bytes, "rowwise"),
         "compulsory_bytes": int(compulsory),
         "compulsory_frac_of_hbm_roofline": round(compulsory / sec / 1e9 / HBM_PEAK_GBS, 5),
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(times.items())},
@@ -153,6 +182,7 @@ def run_outer(args):
         out["C"] = spgemm_outer(ctx, dA, dB, N, N, N)
 
     sec, times = timed_steps(ctx, step, args.steps, args.warmup)
+    calibration_copy(ctx, args.calibrate_copy)
     st = ctx.stats()
     P, nnz = st["products"], out["C"].nnz
     key_bytes = 4
@@ -180,7 +210,7 @@ def run_outer(args):
     ms, launches = times[dom]
     achieved = per_step[dom] * args.steps / (ms / 1e3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("outer", dom), "kernel": dom,
             "avg_launch_us": round(ms / launches * 1e3, 3),
             "algorithmic_bytes_per_launch": int(per_step[dom] * args.steps / launches)}
     n_in = 2 * (4 * (N + 1) + 12 * B.nnz)
@@ -258,12 +288,12 @@ def outer_cpu_baseline(B, seconds):
 
 # --------------------------------------------------------------------------- tensors
 
-def _dominant(times, per_step, steps):
+def _dominant(times, per_step, steps, workload):
     dom = max(per_step, key=lambda k: times.get(k, (0, 1))[0])
     ms, launches = times[dom]
     achieved = per_step[dom] * steps / (ms / 1e3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(workload, dom), "kernel": dom,
             "avg_launch_us": round(ms / launches * 1e3, 3),
             "algorithmic_bytes_per_launch": int(per_step[dom] * steps / launches)}
 
@@ -289,6 +319,7 @@ def run_mttkrp(args):
         out["r"] = mttkrp(ctx, dX, I, J, K, R, dB, dC)
 
     sec, times = timed_steps(ctx, step, args.steps, args.warmup)
+    calibration_copy(ctx, args.calibrate_copy)
     nnz = X.nnz
     nrows = out["r"][0].count
     passes = times["radix_scatter"][1] // args.steps if "radix_scatter" in times else 0
@@ -304,7 +335,7 @@ def run_mttkrp(args):
         "ms_per_step": round(sec * 1e3, 4), "dtype": "f64",
         "config": {"workload": "mttkrp_mode1_50k_40k_30k_20M_R32", "dims": [I, J, K], "nnz": nnz, "rank": R,
                    "zipf_s": 1.5, "rows_out": int(nrows), "seed": 0},
-        "roofline": _dominant(times, per_step, args.steps),
+        "roofline": _dominant(times, per_step, args.steps, "mttkrp"),
         "compulsory_bytes": int(compulsory),
         "compulsory_frac_of_hbm_roofline": round(compulsory / sec / 1e9 / HBM_PEAK_GBS, 5),
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(times.items())},
@@ -348,6 +379,7 @@ def run_ttm(args):
         out["r"] = ttm(ctx, dX, I, J, K, R, dU)
 
     sec, times = timed_steps(ctx, step, args.steps, args.warmup)
+    calibration_copy(ctx, args.calibrate_copy)
     nnz = X.nnz
     F = out["r"][0].count
     per_step = {
@@ -362,7 +394,7 @@ def run_ttm(args):
         "ms_per_step": round(sec * 1e3, 4), "dtype": "f64",
         "config": {"workload": "ttm_mode3_100k_80k_2k_10M_R16", "dims": [I, J, K], "nnz": nnz, "rank": R,
                    "fibers_out": int(F), "seed": 0},
-        "roofline": _dominant(times, per_step, args.steps),
+        "roofline": _dominant(times, per_step, args.steps, "ttm"),
         "compulsory_bytes": int(compulsory),
         "compulsory_frac_of_hbm_roofline": round(compulsory / sec / 1e9 / HBM_PEAK_GBS, 5),
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(times.items())},
@@ -480,6 +512,9 @@ def main(argv=None):
     ap.add_argument("--requests", type=int, default=4096, help="serve workload: number of requests")
     ap.add_argument("--outer-method", type=int, default=0, choices=[0, 1, 2],
                     help="outer workload: 0 automatic, 1 LSD radix sort, 2 row/window buckets")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="after the timed steps, one plain device copy of this many bytes (the reference "
+                         "point for rocprofv3 FETCH_SIZE / WRITE_SIZE runs)")
     ap.add_argument("--outer-window", type=int, default=0,
                     help="outer workload, bucket method: log2 of the column window (0 = automatic)")
     args = ap.parse_args(argv)
