@@ -477,57 +477,137 @@ __global__ __launch_bounds__(256) void k_bk_expand(int32_t En, const double* __r
   }
 }
 
+// A batch of a bucket-walking wavefront: up to kSpan positions of one bucket.
+struct BkBatch {
+  int64_t b;         // bucket (-1: no more)
+  int32_t p, pend;   // first position, end of the bucket
+  int32_t off;       // bucket's output offset (when the walker loads them)
+  bool last;         // last batch of its bucket
+};
+
+// Walks the buckets b0, b0 + step, b0 + 2 step, ... of one wavefront, in
+// batches of kSpan positions, skipping empty buckets.  The bounds (and
+// output offsets) of 64 buckets are loaded at once, one bucket per lane, and
+// a group ahead of use, so that moving to the next bucket is register work:
+// a ballot of the non-empty lanes and readlanes.  Empty buckets of a group
+// get their count (zero) written by the lane that loaded them.
+template <int kSpan, bool kOff>
+struct BkWalk {
+  int64_t gb, step, nb;          // first bucket of the current group, stride, bucket count
+  int32_t cs, ce, co, ns, ne, no;  // per lane: current / next group's bucket bounds and offset
+  uint64_t mask;                 // current group's non-empty buckets not yet started
+  int32_t* zero_cnt;             // when set, zero written here for empty buckets
+
+  __device__ __forceinline__ void dload(int64_t g, const int32_t* __restrict__ bstart, const int32_t* __restrict__ boff,
+                                        int32_t& ds, int32_t& de, int32_t& dofs) {
+    const int64_t bi = g + (int64_t)lane_id() * step;
+    ds = de = dofs = 0;
+    if (bi < nb) {
+      ds = bstart[bi];
+      de = bstart[bi + 1];
+      if (kOff) dofs = boff[bi];
+    }
+  }
+  __device__ __forceinline__ void settle() {
+    mask = __ballot(ce > cs);
+    if (zero_cnt) {
+      const int64_t bi = gb + (int64_t)lane_id() * step;
+      if (bi < nb && ce <= cs) zero_cnt[bi] = 0;
+    }
+  }
+  __device__ __forceinline__ void init(int64_t b0, int64_t step_, int64_t nb_, const int32_t* __restrict__ bstart,
+                                       const int32_t* __restrict__ boff, int32_t* zc) {
+    gb = b0;
+    step = step_;
+    nb = nb_;
+    zero_cnt = zc;
+    dload(gb, bstart, boff, cs, ce, co);
+    dload(gb + step * kWave, bstart, boff, ns, ne, no);
+    settle();
+  }
+  __device__ __forceinline__ BkBatch next(const BkBatch& c, const int32_t* __restrict__ bstart,
+                                          const int32_t* __restrict__ boff) {
+    BkBatch n = c;
+    if (c.b >= 0 && c.p + kSpan < c.pend) {
+      n.p = c.p + kSpan;
+      n.last = n.p + kSpan >= n.pend;
+      return n;
+    }
+    while (mask == 0) {
+      gb += step * kWave;
+      if (gb >= nb) {
+        n.b = -1;
+        return n;
+      }
+      cs = ns;
+      ce = ne;
+      co = no;
+      dload(gb + step * kWave, bstart, boff, ns, ne, no);
+      settle();
+    }
+    const int i = __builtin_ctzll(mask);
+    mask &= mask - 1;
+    n.b = gb + (int64_t)i * step;
+    n.p = __builtin_amdgcn_readlane(cs, i);
+    n.pend = __builtin_amdgcn_readlane(ce, i);
+    n.off = kOff ? __builtin_amdgcn_readlane(co, i) : 0;
+    n.last = n.p + kSpan >= n.pend;
+    return n;
+  }
+};
+
 // Distinct columns per bucket: one wavefront per bucket, LDS bitmap of the
-// window.  Columns are loaded kBkUnroll strides at a time, and the first
-// strides of the wavefront's next bucket are loaded before the current
-// bucket's bitmap is counted.
+// window.  Columns are loaded kBkUnroll strides at a time, one batch ahead
+// (two register sets) across bucket boundaries; the bitmap is counted and
+// cleared at the end of each bucket.
 __global__ __launch_bounds__(kBkWaves * kWave) void k_bk_count(int32_t nb, int s, const int32_t* __restrict__ bstart,
                                                                const uint16_t* __restrict__ klo,
                                                                int32_t* __restrict__ bnnz) {
+  constexpr int kSpan = kWave * kBkUnroll;
   extern __shared__ uint32_t bk_bits[];
-  const int words = (1 << s) >> 5;
+  const int words = (1 << s) >> 5;  // <= 64: one per lane
   uint32_t* bm = bk_bits + (threadIdx.x / kWave) * words;
   const int lane = lane_id();
-  const int32_t step = (int32_t)(gridDim.x * kBkWaves);
-  uint32_t kk[kBkUnroll];
-  auto load = [&](int32_t p, int32_t p1) {
+  if (lane < words) bm[lane] = 0u;
+  __builtin_amdgcn_wave_barrier();
+  auto load = [&](const BkBatch& c, uint32_t (&kk)[kBkUnroll]) {
 #pragma unroll
     for (int u = 0; u < kBkUnroll; ++u) {
-      const int32_t q = p + u * kWave + lane;
-      kk[u] = q < p1 ? (uint32_t)klo[q] : 0xffffffffu;
+      const int32_t q = c.p + u * kWave + lane;
+      kk[u] = (uint32_t)klo[min(q, c.pend - 1)];
     }
   };
-  int32_t b = (int32_t)(blockIdx.x * kBkWaves + threadIdx.x / kWave);
-  int32_t p0 = 0, p1 = 0;
-  if (b < nb) {
-    p0 = bstart[b];
-    p1 = bstart[b + 1];
-    load(p0, p1);
-  }
-  while (b < nb) {
-    for (int q = lane; q < words; q += kWave) bm[q] = 0u;
-    __builtin_amdgcn_wave_barrier();
-    for (int32_t p = p0;;) {
+  auto apply = [&](const BkBatch& c, const uint32_t (&kk)[kBkUnroll]) {
 #pragma unroll
-      for (int u = 0; u < kBkUnroll; ++u)
-        if (kk[u] != 0xffffffffu) atomicOr(&bm[kk[u] >> 5], 1u << (kk[u] & 31));
-      p += kWave * kBkUnroll;
-      if (p >= p1) break;
-      load(p, p1);
-    }
-    const int32_t cur = b;
-    b += step;
-    if (b < nb) {
-      p0 = bstart[b];
-      p1 = bstart[b + 1];
-      load(p0, p1);
-    }
+    for (int u = 0; u < kBkUnroll; ++u)
+      if (c.p + u * kWave + lane < c.pend) atomicOr(&bm[kk[u] >> 5], 1u << (kk[u] & 31));
+    if (!c.last) return;
     __builtin_amdgcn_wave_barrier();
-    int c = 0;
-    for (int q = lane; q < words; q += kWave) c += __popc(bm[q]);
-    c = wave_sum(c);
-    if (lane == 0) bnnz[cur] = c;
+    uint32_t w = 0;
+    if (lane < words) {
+      w = bm[lane];
+      bm[lane] = 0u;
+    }
+    const int cnt = wave_sum((int)__popc(w));
+    if (lane == 0) bnnz[c.b] = cnt;
     __builtin_amdgcn_wave_barrier();
+  };
+  BkWalk<kSpan, false> wk;
+  wk.init(__builtin_amdgcn_readfirstlane((int32_t)(blockIdx.x * kBkWaves + threadIdx.x / kWave)),
+          (int64_t)gridDim.x * kBkWaves, nb, bstart, nullptr, bnnz);
+  uint32_t k0[kBkUnroll], k1[kBkUnroll];
+  BkBatch c0;
+  c0.b = -1;
+  c0 = wk.next(c0, bstart, nullptr);
+  if (c0.b >= 0) load(c0, k0);
+  while (c0.b >= 0) {
+    BkBatch c1 = wk.next(c0, bstart, nullptr);
+    if (c1.b >= 0) load(c1, k1);
+    apply(c0, k0);
+    if (c1.b < 0) break;
+    c0 = wk.next(c1, bstart, nullptr);
+    if (c0.b >= 0) load(c0, k0);
+    apply(c1, k1);
   }
 }
 
@@ -538,97 +618,73 @@ __global__ __launch_bounds__(256) void k_bk_rowpos(int32_t R, int32_t W, int32_t
     out_pos[r0 + r] = base + boff[(int64_t)r * W];
 }
 
-// LDS of one wavefront in k_bk_reduce, for a window of 2^s columns:
-// acc (2^s doubles), seen and stride bitmaps (2^s bits each), column list
-// (2^s uint16).
+// LDS of one wavefront in k_bk_reduce, for a window of 2^s columns (s <= 11):
+// acc (2^s doubles), the seen and stride bitmaps (2^s bits each), the column
+// list (2^s uint16).
 __host__ __device__ constexpr size_t bk_reduce_lds_per_wave(int s) {
   return ((size_t)8 << s) + 2 * (((size_t)1 << s) >> 3) + ((size_t)2 << s);
 }
 
-// A batch of a bucket-walking wavefront: kBkUnroll strides of one bucket.
-struct BkBatch {
-  int32_t b, p, pend;  // bucket, first position, end of the bucket
-  bool first, last;    // first / last batch of its bucket
-};
+constexpr int kBkRedWaves = 1;   // wavefronts per workgroup in k_bk_reduce
+constexpr int kBkRedUnroll = 16; // strides per batch of k_bk_reduce (two batches in flight)
 
-// The wavefront's batches in order: the buckets b, b + step, ... (empty ones
-// skipped), each cut into kBkUnroll strides.  b = nb marks the end.
-__device__ __forceinline__ BkBatch bk_next(const BkBatch& c, int32_t step, int32_t nb,
-                                           const int32_t* __restrict__ bstart) {
-  constexpr int32_t span = kWave * kBkUnroll;
-  BkBatch n;
-  if (c.p + span < c.pend) {
-    n.b = c.b;
-    n.p = c.p + span;
-    n.pend = c.pend;
-    n.first = false;
-  } else {
-    n.b = c.b + step;
-    n.p = n.pend = 0;
-    while (n.b < nb) {
-      n.p = bstart[n.b];
-      n.pend = bstart[n.b + 1];
-      if (n.p < n.pend) break;
-      n.b += step;
-    }
-    n.first = true;
-  }
-  n.last = n.p + span >= n.pend;
-  return n;
-}
-
-__device__ __forceinline__ BkBatch bk_first(int32_t b0, int32_t step, int32_t nb, const int32_t* __restrict__ bstart) {
-  BkBatch c;
-  c.b = b0 - step;
-  c.p = c.pend = 0;
-  return bk_next(c, step, nb, bstart);
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
 // Reduce: one wavefront per bucket.  The window is a dense LDS accumulator;
 // `seen` marks the columns written so far.  Products are taken in position
-// order, 64 at a time.  Each lane sets its column's bit in a per-stride
-// bitmap (returning the old word) while reading its accumulator slot; a lane
-// that finds the bit already set shares its column with another lane of the
-// stride.  Lanes whose column is unique in the stride update the accumulator
-// at once; the lanes of shared columns are then applied one at a time in
-// lane order.  Each entry is thus summed in position (= increasing j) order,
-// starting from its first product.  The set columns are then listed in order
-// in LDS and written at the bucket's output offset with coalesced stores.
-// Loads run one batch ahead (two register sets), across bucket boundaries.
-__global__ __launch_bounds__(kBkWaves * kWave) void k_bk_reduce(int32_t nb, int32_t W, int s,
-                                                                const int32_t* __restrict__ bstart,
-                                                                const uint16_t* __restrict__ klo,
-                                                                const double* __restrict__ pv,
-                                                                const int32_t* __restrict__ boff, int64_t base,
-                                                                int32_t* __restrict__ out_crd,
-                                                                double* __restrict__ out_val) {
+// order, 64 at a time (a stride).  Each lane sets its column's bit in a
+// per-stride bitmap (returning the old word) while reading its accumulator
+// slot and seen word; a lane that finds the bit already set shares its column
+// with another lane of the stride.  Every lane starts its sum from the
+// accumulator (or from its own product for a new column).  For each group of
+// lanes sharing a column, the lowest lane (the earliest position) then adds
+// the other members' products in lane order, taken from registers, and alone
+// writes the slot.  Each entry is thus summed in position (= increasing j)
+// order, starting from its first product, with one LDS round trip per stride.
+// At the end of a bucket each lane lists the set columns of its seen word in
+// LDS, at its prefix among the words, and the list is stored 64 entries at
+// a time with every lane active (four accumulator reads in flight), so the
+// stores are full and consecutive.  Loads run one batch ahead (two register
+// sets), across bucket boundaries.  One wavefront per workgroup: the LDS of
+// a 2^11-column window (about 21 KB) then allows seven per CU.
+template <int kU>
+__global__ __launch_bounds__(kBkRedWaves * kWave) void k_bk_reduce(int32_t nb, int32_t W, int s,
+                                                                   const int32_t* __restrict__ bstart,
+                                                                   const uint16_t* __restrict__ klo,
+                                                                   const double* __restrict__ pv,
+                                                                   const int32_t* __restrict__ boff, int64_t base,
+                                                                   int32_t* __restrict__ out_crd,
+                                                                   double* __restrict__ out_val) {
   extern __shared__ double bk_lds[];
-  const int S = 1 << s, words = S >> 5;
+  const int S = 1 << s, words = S >> 5;  // words <= 64: one per lane
   const int lane = lane_id();
   char* mine = reinterpret_cast<char*>(bk_lds) + (threadIdx.x / kWave) * bk_reduce_lds_per_wave(s);
   double* acc = reinterpret_cast<double*>(mine);
   uint32_t* seen = reinterpret_cast<uint32_t*>(acc + S);
   uint32_t* sb = seen + words;
   uint16_t* list = reinterpret_cast<uint16_t*>(sb + words);
-  for (int q = lane; q < words; q += kWave) sb[q] = 0u;
-  const int32_t step = (int32_t)(gridDim.x * kBkWaves);
-
-  auto load = [&](const BkBatch& c, uint32_t (&kk)[kBkUnroll], double (&vv)[kBkUnroll]) {
+  if (lane < words) {
+    sb[lane] = 0u;
+    seen[lane] = 0u;
+  }
+  __builtin_amdgcn_wave_barrier();
+  auto load = [&](const BkBatch& c, uint32_t (&kk)[kU], double (&vv)[kU]) {
     // clamped index: every lane loads; lanes past the bucket are masked by `ok` in apply
 #pragma unroll
-    for (int u = 0; u < kBkUnroll; ++u) {
+    for (int u = 0; u < kU; ++u) {
       const int32_t q = min(c.p + u * kWave + lane, c.pend - 1);
       kk[u] = (uint32_t)klo[q];
       vv[u] = pv[q];
     }
   };
-  auto apply = [&](const BkBatch& c, const uint32_t (&kk)[kBkUnroll], const double (&vv)[kBkUnroll]) {
-    if (c.first) {
-      for (int q = lane; q < words; q += kWave) seen[q] = 0u;
-      __builtin_amdgcn_wave_barrier();
-    }
+  auto apply = [&](const BkBatch& c, const uint32_t (&kk)[kU], const double (&vv)[kU]) {
 #pragma unroll
-    for (int u = 0; u < kBkUnroll; ++u) {
+    for (int u = 0; u < kU; ++u) {
       if (c.p + u * kWave >= c.pend) break;
       const bool ok = c.p + u * kWave + lane < c.pend;
       const uint32_t k = kk[u];
@@ -641,75 +697,80 @@ __global__ __launch_bounds__(kBkWaves * kWave) void k_bk_reduce(int32_t nb, int3
         sw = seen[wi];
         a = acc[k];
       }
-      const uint64_t late = __ballot(ok && (old & bit) != 0);
-      bool shared = false;
-      if (late != 0) {
-        // lanes whose column also sits in a lane that found its bit set
-        for (uint64_t m = late; m; m &= m - 1) {
-          const int l = __builtin_ctzll(m);
-          shared = shared || (ok && k == (uint32_t)__builtin_amdgcn_readlane((int)k, l));
+      const bool was = (sw & bit) != 0;
+      double sum = was ? a + v : v;
+      bool lead = ok;
+      uint64_t todo = __ballot(ok && (old & bit) != 0);
+      while (todo) {
+        const int l = __builtin_ctzll(todo);
+        const uint32_t kl = (uint32_t)__builtin_amdgcn_readlane((int)k, l);
+        const uint64_t grp = __ballot(ok && k == kl);
+        todo &= ~grp;
+        const int ld = __builtin_ctzll(grp);
+        for (uint64_t mm = grp & (grp - 1); mm; mm &= mm - 1) {
+          const double vl = readlane_f64(v, __builtin_ctzll(mm));
+          if (lane == ld) sum = sum + vl;
         }
+        if (lane != ld && ((grp >> lane) & 1)) lead = false;
       }
-      if (ok && !shared) {
-        const bool was = (sw & bit) != 0;
-        acc[k] = was ? a + v : v;
+      if (lead) {
+        acc[k] = sum;
         if (!was) atomicOr(&seen[wi], bit);
       }
-      if (late != 0) {
-        __builtin_amdgcn_wave_barrier();
-        const uint64_t sm = __ballot(shared);
-        for (uint64_t m = sm; m; m &= m - 1) {
-          const int l = __builtin_ctzll(m);
-          if (lane == l) {
-            const bool was = (seen[wi] & bit) != 0;
-            acc[k] = was ? acc[k] + v : v;
-            if (!was) atomicOr(&seen[wi], bit);
-          }
-          __builtin_amdgcn_wave_barrier();
-        }
-      }
-      __builtin_amdgcn_wave_barrier();
       if (ok) atomicAnd(&sb[wi], ~bit);
       __builtin_amdgcn_wave_barrier();
     }
     if (!c.last) return;
-    // list the set columns in order, then write them out coalesced
-    int32_t carry = 0;
-    for (int q0 = 0; q0 < words; q0 += kWave) {
-      const int q = q0 + lane;
-      uint32_t bits = q < words ? seen[q] : 0u;
-      const int cnt = __popc(bits);
-      const int incl = wave_incl_scan(cnt);
-      int r = carry + incl - cnt;
-      while (bits) {
-        list[r++] = (uint16_t)(q * 32 + __builtin_ctz(bits));
-        bits &= bits - 1;
-      }
-      carry += __shfl(incl, kWave - 1, kWave);
-    }
+    // write the set columns in order: 64-column slabs, one lane per column
+    const uint32_t myw = lane < words ? seen[lane] : 0u;
     __builtin_amdgcn_wave_barrier();
-    const int64_t ob = base + boff[c.b];
-    const int32_t wb = (c.b % W) << s;
-    for (int r = lane; r < carry; r += kWave) {
-      const uint32_t k = list[r];
-      out_crd[ob + r] = wb + (int32_t)k;
-      out_val[ob + r] = acc[k];
+    if (lane < words) seen[lane] = 0u;
+    const int64_t ob = base + c.off;
+    const int32_t wb = (int32_t)(c.b % W) << s;
+    int32_t* oc = out_crd + ob;
+    double* ov = out_val + ob;
+    // list every set column in order (each lane its word), then store 64 at a time
+    const int cnt = __popc(myw);
+    const int incl = wave_incl_scan(cnt);
+    int r = incl - cnt;
+    for (uint32_t bits = myw; bits; bits &= bits - 1) list[r++] = (uint16_t)(lane * 32 + __builtin_ctz(bits));
+    const int total = __builtin_amdgcn_readlane(incl, kWave - 1);
+    __builtin_amdgcn_wave_barrier();
+    for (int r0 = 0; r0 < total; r0 += 4 * kWave) {
+      uint32_t kq[4];
+      double aq[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) kq[i] = list[min(r0 + i * kWave + lane, total - 1)];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) aq[i] = acc[kq[i]];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int q = r0 + i * kWave + lane;
+        if (q < total) {
+          oc[q] = wb + (int32_t)kq[i];
+          ov[q] = aq[i];
+        }
+      }
     }
     __builtin_amdgcn_wave_barrier();
   };
 
-  uint32_t k0[kBkUnroll], k1[kBkUnroll];
-  double v0[kBkUnroll], v1[kBkUnroll];
-  BkBatch c0 =
-      bk_first(__builtin_amdgcn_readfirstlane((int32_t)(blockIdx.x * kBkWaves + threadIdx.x / kWave)), step, nb, bstart);
-  if (c0.b < nb) load(c0, k0, v0);
-  while (c0.b < nb) {
-    BkBatch c1 = bk_next(c0, step, nb, bstart);
-    if (c1.b < nb) load(c1, k1, v1);
+  BkWalk<kWave * kU, true> wk;
+  wk.init(__builtin_amdgcn_readfirstlane((int32_t)(blockIdx.x * kBkRedWaves + threadIdx.x / kWave)), (int64_t)gridDim.x * kBkRedWaves,
+          nb, bstart, boff, nullptr);
+  uint32_t k0[kU], k1[kU];
+  double v0[kU], v1[kU];
+  BkBatch c0;
+  c0.b = -1;
+  c0 = wk.next(c0, bstart, boff);
+  if (c0.b >= 0) load(c0, k0, v0);
+  while (c0.b >= 0) {
+    BkBatch c1 = wk.next(c0, bstart, boff);
+    if (c1.b >= 0) load(c1, k1, v1);
     apply(c0, k0, v0);
-    if (c1.b >= nb) break;
-    c0 = bk_next(c1, step, nb, bstart);
-    if (c0.b < nb) load(c0, k0, v0);
+    if (c1.b < 0) break;
+    c0 = wk.next(c1, bstart, boff);
+    if (c0.b >= 0) load(c0, k0, v0);
     apply(c1, k1, v1);
   }
 }
@@ -954,7 +1015,7 @@ static int outer_chunks_bucket(spws_ctx* ctx, const OuterArgs& a, const std::vec
   int64_t* out_pos = ctx->out_pos.as<int64_t>();
   int32_t* hm = ctx->host_meta.as<int32_t>();
   const size_t lds_count = (size_t)kBkWaves * (((size_t)1 << s) >> 5) * 4;
-  const size_t lds_reduce = (size_t)kBkWaves * bk_reduce_lds_per_wave(s);
+  const size_t lds_reduce = (size_t)kBkRedWaves * bk_reduce_lds_per_wave(s);
   ctx->stats.outer_method = 2;
   ctx->stats.window_bits = s;
   int64_t base = 0;
@@ -1007,8 +1068,9 @@ static int outer_chunks_bucket(spws_ctx* ctx, const OuterArgs& a, const std::vec
       if ((rc = outer_output_grow(ctx, base, U))) return rc;
       SPWS_LAUNCH(ctx, "bk_rowpos", k_bk_rowpos, dim3((unsigned)std::min<int64_t>(div_up(R, 256), 4096)), dim3(256), 0,
                   R, W, r0, base, boff, out_pos);
-      SPWS_LAUNCH(ctx, "bk_reduce", k_bk_reduce, dim3(gb), dim3(kBkWaves * kWave), lds_reduce, NB, W, s, bstart, klo,
-                  pv, boff, base, ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>());
+      const unsigned gred = (unsigned)std::min<int64_t>(div_up(NB, kBkRedWaves), 65536);
+      SPWS_LAUNCH(ctx, "bk_reduce", k_bk_reduce<kBkRedUnroll>, dim3(gred), dim3(kBkRedWaves * kWave), lds_reduce, NB, W,
+                  s, bstart, klo, pv, boff, base, ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>());
     } else {
       SPWS_LAUNCH(ctx, "outer_tail", (k_outer_tail<uint32_t>), dim3((unsigned)div_up(R, 256)), dim3(256), 0,
                   (const uint32_t*)nullptr, (int64_t)0, 0, r0, R, base, out_pos);
