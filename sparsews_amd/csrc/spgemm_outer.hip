@@ -322,8 +322,7 @@ __global__ void k_set_i64(int64_t* p, int64_t v) { *p = v; }
 
 // ------------------------------------------------------------ bucket method
 
-constexpr int kBkWaves = 2;   // wavefronts per workgroup in the per-bucket kernels
-constexpr int kBkUnroll = 8;  // strides of products loaded before they are used
+constexpr int kBkWaves = 2;   // wavefronts per workgroup in k_bk_count
 constexpr int kBkMinLog2 = 6;
 constexpr int kBkMaxLog2 = 11;
 
@@ -477,10 +476,95 @@ __global__ __launch_bounds__(256) void k_bk_expand(int32_t En, const double* __r
   }
 }
 
+// Wait until every vector memory operation of the wavefront has completed.
+// Loads and stores share one counter on this architecture, so once stores
+// are in flight the compiler can only wait for a load by waiting for
+// everything; a kernel that issues this just before a batch's loads makes
+// every such wait one on operations issued a batch earlier.
+__device__ __forceinline__ void vm_drain() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+constexpr int kBkExpandLdsW = 1024;  // widest window table k_bk_expand_lds keeps in LDS
+
+// Expand with the entry's window table in LDS (W <= kBkExpandLdsW): one
+// wavefront per A entry, as k_bk_expand.  The entry's W bucket slots are
+// copied to LDS, so a product's slot is an LDS read of its window.  Each
+// block of kWave * U products starts with a drain of the wavefront's memory
+// operations (the block's B loads and the previous block's stores, both
+// issued a block earlier), then issues the next block's loads, then looks up
+// the slots and stores: no wait inside a block falls on memory issued
+// within it.
+template <int U>
+__global__ __launch_bounds__(256) void k_bk_expand_lds(int32_t En, const double* __restrict__ order,
+                                                       const int32_t* __restrict__ ent_ai,
+                                                       const int32_t* __restrict__ ent_ja,
+                                                       const int32_t* __restrict__ match,
+                                                       const int32_t* __restrict__ blen,
+                                                       const int32_t* __restrict__ B2_pos,
+                                                       const double* __restrict__ Ax,
+                                                       const int32_t* __restrict__ B2_crd,
+                                                       const double* __restrict__ Bx, int32_t W, int s,
+                                                       const int32_t* __restrict__ etab, uint16_t* __restrict__ klo,
+                                                       double* __restrict__ pv) {
+  extern __shared__ int32_t ex_lds[];
+  int32_t* wt = ex_lds + (threadIdx.x / kWave) * W;
+  const uint32_t wmask = (1u << s) - 1;
+  const int lane = lane_id();
+  const int32_t nw = (int32_t)(gridDim.x * blockDim.x / kWave);
+  for (int32_t q = __builtin_amdgcn_readfirstlane((int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kWave)); q < En;
+       q += nw) {
+    const int32_t e = __builtin_amdgcn_readfirstlane((int32_t)__double_as_longlong(order[q]));
+    const int32_t ja = __builtin_amdgcn_readfirstlane(ent_ja[e]);
+    const int32_t L = __builtin_amdgcn_readfirstlane(blen[ja]);
+    const int32_t bs = __builtin_amdgcn_readfirstlane(B2_pos[__builtin_amdgcn_readfirstlane(match[ja])]);
+    const double av = Ax[__builtin_amdgcn_readfirstlane(ent_ai[e])];
+    const int32_t* et = etab + (int64_t)e * W;
+    __builtin_amdgcn_wave_barrier();
+    for (int32_t w = lane; w < W; w += kWave) wt[w] = et[w];
+    __builtin_amdgcn_wave_barrier();
+    uint32_t k0[U], k1[U];
+    double b0[U], b1[U];
+    auto load = [&](int32_t kb, uint32_t (&kd)[U], double (&bd)[U]) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        // clamped index: every lane loads, lanes past the row are masked at the store
+        const int32_t kk = min(kb + u * kWave + lane, L - 1);
+        kd[u] = (uint32_t)B2_crd[bs + kk];
+        bd[u] = Bx[bs + kk];
+      }
+    };
+    auto block = [&](int32_t kb, const uint32_t (&kd)[U], const double (&bd)[U]) {
+      int32_t dst[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) dst[u] = wt[kd[u] >> s];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t kk = kb + u * kWave + lane;
+        if (kk < L) {
+          const int32_t d = dst[u] + kk;
+          klo[d] = (uint16_t)(kd[u] & wmask);
+          pv[d] = av * bd[u];
+        }
+      }
+    };
+    constexpr int32_t span = kWave * U;
+    load(0, k0, b0);
+    for (int32_t kb = 0; kb < L; kb += 2 * span) {
+      vm_drain();
+      if (kb + span < L) load(kb + span, k1, b1);
+      block(kb, k0, b0);
+      if (kb + span >= L) break;
+      vm_drain();
+      if (kb + 2 * span < L) load(kb + 2 * span, k0, b0);
+      block(kb + span, k1, b1);
+    }
+  }
+}
+
 // A batch of a bucket-walking wavefront: up to kSpan positions of one bucket.
 struct BkBatch {
   int64_t b;         // bucket (-1: no more)
-  int32_t p, pend;   // first position, end of the bucket
+  int32_t p, pend;   // first position of the batch, end of the bucket
+  int32_t p0;        // first position of the bucket
   int32_t off;       // bucket's output offset (when the walker loads them)
   bool last;         // last batch of its bucket
 };
@@ -490,8 +574,10 @@ struct BkBatch {
 // output offsets) of 64 buckets are loaded at once, one bucket per lane, and
 // a group ahead of use, so that moving to the next bucket is register work:
 // a ballot of the non-empty lanes and readlanes.  Empty buckets of a group
-// get their count (zero) written by the lane that loaded them.
-template <int kSpan, bool kOff>
+// get their count (zero) written by the lane that loaded them.  With kAlign
+// a bucket's first batch starts at its first position rounded down to a
+// multiple of 4 (for 4-element loads; p0 keeps the true start).
+template <int kSpan, bool kOff, bool kAlign = false>
 struct BkWalk {
   int64_t gb, step, nb;          // first bucket of the current group, stride, bucket count
   int32_t cs, ce, co, ns, ne, no;  // per lane: current / next group's bucket bounds and offset
@@ -548,7 +634,8 @@ struct BkWalk {
     const int i = __builtin_ctzll(mask);
     mask &= mask - 1;
     n.b = gb + (int64_t)i * step;
-    n.p = __builtin_amdgcn_readlane(cs, i);
+    n.p0 = __builtin_amdgcn_readlane(cs, i);
+    n.p = kAlign ? (n.p0 & ~3) : n.p0;
     n.pend = __builtin_amdgcn_readlane(ce, i);
     n.off = kOff ? __builtin_amdgcn_readlane(co, i) : 0;
     n.last = n.p + kSpan >= n.pend;
@@ -557,30 +644,39 @@ struct BkWalk {
 };
 
 // Distinct columns per bucket: one wavefront per bucket, LDS bitmap of the
-// window.  Columns are loaded kBkUnroll strides at a time, one batch ahead
-// (two register sets) across bucket boundaries; the bitmap is counted and
-// cleared at the end of each bucket.
+// window.  Each lane loads 8 bytes (four column offsets) per stride, from a
+// 4-aligned position, kBkCountUnroll strides at a time and one batch ahead
+// (two register sets) across bucket boundaries; offsets outside the bucket
+// are masked.  The bitmap is counted and cleared at the end of each bucket.
+constexpr int kBkCountUnroll = 4;
+
 __global__ __launch_bounds__(kBkWaves * kWave) void k_bk_count(int32_t nb, int s, const int32_t* __restrict__ bstart,
                                                                const uint16_t* __restrict__ klo,
                                                                int32_t* __restrict__ bnnz) {
-  constexpr int kSpan = kWave * kBkUnroll;
+  constexpr int U = kBkCountUnroll;
+  constexpr int kSpan = 4 * kWave * U;
   extern __shared__ uint32_t bk_bits[];
   const int words = (1 << s) >> 5;  // <= 64: one per lane
   uint32_t* bm = bk_bits + (threadIdx.x / kWave) * words;
   const int lane = lane_id();
   if (lane < words) bm[lane] = 0u;
   __builtin_amdgcn_wave_barrier();
-  auto load = [&](const BkBatch& c, uint32_t (&kk)[kBkUnroll]) {
+  const uint2* kq = reinterpret_cast<const uint2*>(klo);
+  auto load = [&](const BkBatch& c, uint2 (&kk)[U]) {
+    // clamped to the bucket's last 4-element group: every lane loads in bounds
+    const int32_t qlast = (c.pend - 1) >> 2;
 #pragma unroll
-    for (int u = 0; u < kBkUnroll; ++u) {
-      const int32_t q = c.p + u * kWave + lane;
-      kk[u] = (uint32_t)klo[min(q, c.pend - 1)];
-    }
+    for (int u = 0; u < U; ++u) kk[u] = kq[min((c.p >> 2) + u * kWave + lane, qlast)];
   };
-  auto apply = [&](const BkBatch& c, const uint32_t (&kk)[kBkUnroll]) {
+  auto apply = [&](const BkBatch& c, const uint2 (&kk)[U]) {
 #pragma unroll
-    for (int u = 0; u < kBkUnroll; ++u)
-      if (c.p + u * kWave + lane < c.pend) atomicOr(&bm[kk[u] >> 5], 1u << (kk[u] & 31));
+    for (int u = 0; u < U; ++u) {
+      const int32_t e0 = c.p + 4 * (u * kWave + lane);
+      const uint32_t v[4] = {kk[u].x & 0xffffu, kk[u].x >> 16, kk[u].y & 0xffffu, kk[u].y >> 16};
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (e0 + i >= c.p0 && e0 + i < c.pend) atomicOr(&bm[v[i] >> 5], 1u << (v[i] & 31));
+    }
     if (!c.last) return;
     __builtin_amdgcn_wave_barrier();
     uint32_t w = 0;
@@ -592,10 +688,10 @@ __global__ __launch_bounds__(kBkWaves * kWave) void k_bk_count(int32_t nb, int s
     if (lane == 0) bnnz[c.b] = cnt;
     __builtin_amdgcn_wave_barrier();
   };
-  BkWalk<kSpan, false> wk;
+  BkWalk<kSpan, false, true> wk;
   wk.init(__builtin_amdgcn_readfirstlane((int32_t)(blockIdx.x * kBkWaves + threadIdx.x / kWave)),
           (int64_t)gridDim.x * kBkWaves, nb, bstart, nullptr, bnnz);
-  uint32_t k0[kBkUnroll], k1[kBkUnroll];
+  uint2 k0[U], k1[U];
   BkBatch c0;
   c0.b = -1;
   c0 = wk.next(c0, bstart, nullptr);
@@ -994,7 +1090,8 @@ static int outer_chunks_bucket(spws_ctx* ctx, const OuterArgs& a, const std::vec
   const size_t cap = (size_t)(max_chunk > 0 ? max_chunk : 1);
   const size_t ecap = (size_t)(max_en > 0 ? max_en : 1);
   const size_t nbcap = (size_t)max_r * (size_t)W + 1;
-  if ((rc = ctx->keys[0].ensure(cap * 2)) || (rc = ctx->vals[0].ensure(cap * 8)) ||
+  // column offsets: k_bk_count reads them in aligned groups of 4
+  if ((rc = ctx->keys[0].ensure(cap * 2 + 8)) || (rc = ctx->vals[0].ensure(cap * 8)) ||
       (rc = ctx->bk[1].ensure(ecap * 4)) || (rc = ctx->bk[2].ensure(ecap * 4)) ||
       (rc = ctx->bk[3].ensure(ecap * 8)) || (rc = ctx->bk[4].ensure(ecap * 8)) ||
       (rc = ctx->bk[5].ensure(ecap * 4)) || (rc = ctx->bk[6].ensure(ecap * (size_t)W * 4)) ||
@@ -1055,9 +1152,13 @@ static int outer_chunks_bucket(spws_ctx* ctx, const OuterArgs& a, const std::vec
       SPWS_LAUNCH(ctx, "bk_etab", k_bk_rows<1>, dim3(gr), dim3(256), 0, R, W, rp_off, pval, ent_ja, bseg, bstart,
                   etab);
       const unsigned gx = (unsigned)std::min<int64_t>(div_up((int64_t)En * kWave, 256), 32768);
-      SPWS_LAUNCH(ctx, "bk_expand", k_bk_expand, dim3(gx), dim3(256), 0, En, pval, ent_ai, ent_ja, match, blen,
-                  a.B2_pos,
-                  a.Ax, a.B2_crd, a.Bx, W, s, etab, klo, pv);
+      if (W <= kBkExpandLdsW) {
+        SPWS_LAUNCH(ctx, "bk_expand", k_bk_expand_lds<8>, dim3(gx), dim3(256), (size_t)(256 / kWave) * W * 4, En, pval,
+                    ent_ai, ent_ja, match, blen, a.B2_pos, a.Ax, a.B2_crd, a.Bx, W, s, etab, klo, pv);
+      } else {
+        SPWS_LAUNCH(ctx, "bk_expand", k_bk_expand, dim3(gx), dim3(256), 0, En, pval, ent_ai, ent_ja, match, blen,
+                    a.B2_pos, a.Ax, a.B2_crd, a.Bx, W, s, etab, klo, pv);
+      }
       const unsigned gb = (unsigned)std::min<int64_t>(div_up(NB, kBkWaves), 65536);
       SPWS_LAUNCH(ctx, "bk_count", k_bk_count, dim3(gb), dim3(kBkWaves * kWave), lds_count, NB, s, bstart, klo,
                   bnnz);

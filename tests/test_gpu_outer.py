@@ -132,6 +132,20 @@ def test_outer_bucket_windows(ctx, wlog):
     check(C, A, B)
 
 
+def test_outer_bucket_wide_window_table(ctx):
+    """More than 1024 windows per row (64-column windows over 80,000
+    columns): the expand reads each product's bucket slot from the global
+    window table instead of a per-entry copy in LDS."""
+    print("case: outer bucket 1250 windows per row", flush=True)
+    A = powerlaw_rows(300, 400, 1.8, 200, 71, 0)
+    B = powerlaw_rows(400, 80000, 1.8, 3000, 71, 1)
+    with outer_method(ctx, 2, 6):
+        C = run_outer(ctx, A, B)
+        st = ctx.stats()
+    assert st["outer_method"] == 2 and st["window_bits"] == 6
+    check(C, A, B)
+
+
 def test_outer_bucket_long_column_runs(ctx):
     """Outputs where one entry gathers many products (dense rows of A against
     a B whose rows all share a few columns), so one 64-lane stride holds
