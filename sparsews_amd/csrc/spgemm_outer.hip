@@ -697,10 +697,12 @@ __global__ __launch_bounds__(kBkWaves * kWave) void k_bk_count(int32_t nb, int s
   c0 = wk.next(c0, bstart, nullptr);
   if (c0.b >= 0) load(c0, k0);
   while (c0.b >= 0) {
+    vm_drain();
     BkBatch c1 = wk.next(c0, bstart, nullptr);
     if (c1.b >= 0) load(c1, k1);
     apply(c0, k0);
     if (c1.b < 0) break;
+    vm_drain();
     c0 = wk.next(c1, bstart, nullptr);
     if (c0.b >= 0) load(c0, k0);
     apply(c1, k1);
@@ -861,10 +863,14 @@ __global__ __launch_bounds__(kBkRedWaves * kWave) void k_bk_reduce(int32_t nb, i
   c0 = wk.next(c0, bstart, boff);
   if (c0.b >= 0) load(c0, k0, v0);
   while (c0.b >= 0) {
+    // the drain falls on this batch's loads and the last bucket's stores,
+    // both issued a batch earlier, not on the loads issued next
+    vm_drain();
     BkBatch c1 = wk.next(c0, bstart, boff);
     if (c1.b >= 0) load(c1, k1, v1);
     apply(c0, k0, v0);
     if (c1.b < 0) break;
+    vm_drain();
     c0 = wk.next(c1, bstart, boff);
     if (c0.b >= 0) load(c0, k0, v0);
     apply(c1, k1, v1);
