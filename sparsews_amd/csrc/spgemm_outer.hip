@@ -737,13 +737,15 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
 // `seen` marks the columns written so far.  Products are taken in position
 // order, 64 at a time (a stride).  Each lane sets its column's bit in a
 // per-stride bitmap (returning the old word) while reading its accumulator
-// slot and seen word; a lane that finds the bit already set shares its column
-// with another lane of the stride.  Every lane starts its sum from the
-// accumulator (or from its own product for a new column).  For each group of
-// lanes sharing a column, the lowest lane (the earliest position) then adds
-// the other members' products in lane order, taken from registers, and alone
-// writes the slot.  Each entry is thus summed in position (= increasing j)
-// order, starting from its first product, with one LDS round trip per stride.
+// slot; a lane that finds the bit already set shares its column with another
+// lane of the stride.  Every lane starts its sum from the accumulator, which
+// holds -0.0 for a column not yet seen: -0.0 + v is v, bit for bit, for
+// every v but a NaN, so a sum starts from its first product exactly.  For
+// each group of lanes sharing a column, the lowest lane (the earliest
+// position) then adds the other members' products in lane order, taken from
+// registers, and alone writes the slot.  Each entry is thus summed in
+// position (= increasing j) order, starting from its first product, with one
+// LDS round trip per stride.
 // At the end of a bucket each lane lists the set columns of its seen word in
 // LDS, at its prefix among the words, and the list is stored 64 entries at
 // a time with every lane active (four accumulator reads in flight), so the
@@ -770,6 +772,9 @@ __global__ __launch_bounds__(kBkRedWaves * kWave) void k_bk_reduce(int32_t nb, i
     sb[lane] = 0u;
     seen[lane] = 0u;
   }
+  // an untouched slot holds -0.0 (-0.0 + v is v for every v but a NaN), so
+  // the first product of a column is added like the others
+  for (int q = lane; q < S; q += kWave) acc[q] = -0.0;
   __builtin_amdgcn_wave_barrier();
   auto load = [&](const BkBatch& c, uint32_t (&kk)[kU], double (&vv)[kU]) {
     // clamped index: every lane loads; lanes past the bucket are masked by `ok` in apply
@@ -788,15 +793,13 @@ __global__ __launch_bounds__(kBkRedWaves * kWave) void k_bk_reduce(int32_t nb, i
       const uint32_t k = kk[u];
       const double v = vv[u];
       const uint32_t wi = k >> 5, bit = 1u << (k & 31);
-      uint32_t old = 0, sw = 0;
+      uint32_t old = 0;
       double a = 0.0;
       if (ok) {
         old = atomicOr(&sb[wi], bit);
-        sw = seen[wi];
         a = acc[k];
       }
-      const bool was = (sw & bit) != 0;
-      double sum = was ? a + v : v;
+      double sum = a + v;
       bool lead = ok;
       uint64_t todo = __ballot(ok && (old & bit) != 0);
       while (todo) {
@@ -813,13 +816,13 @@ __global__ __launch_bounds__(kBkRedWaves * kWave) void k_bk_reduce(int32_t nb, i
       }
       if (lead) {
         acc[k] = sum;
-        if (!was) atomicOr(&seen[wi], bit);
+        atomicOr(&seen[wi], bit);
       }
       if (ok) atomicAnd(&sb[wi], ~bit);
       __builtin_amdgcn_wave_barrier();
     }
     if (!c.last) return;
-    // write the set columns in order: 64-column slabs, one lane per column
+    // write the set columns in order
     const uint32_t myw = lane < words ? seen[lane] : 0u;
     __builtin_amdgcn_wave_barrier();
     if (lane < words) seen[lane] = 0u;
@@ -847,6 +850,7 @@ __global__ __launch_bounds__(kBkRedWaves * kWave) void k_bk_reduce(int32_t nb, i
         if (q < total) {
           oc[q] = wb + (int32_t)kq[i];
           ov[q] = aq[i];
+          acc[kq[i]] = -0.0;
         }
       }
     }
