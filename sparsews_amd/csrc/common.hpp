@@ -85,7 +85,7 @@ struct spws_ctx {
   spws::DevBuf stage[12];  // device copies of host inputs (TACO entry points)
   spws::DevBuf radix_hist;
   spws::DevBuf obuf[8];    // outer-product per-column metadata
-  spws::DevBuf bk[10];     // outer-product bucket path: pairs, window bounds, bucket tables
+  spws::DevBuf bk[11];     // outer-product bucket path: pairs, window bounds, bucket tables (two sets)
   spws::PinnedBuf host_meta;
   // row-wise SpGEMM bin boundaries (validated against the compiled table sizes)
   spws_rowwise_tuning rw{64, 256, 1024, 4096, 16, -1};
@@ -96,6 +96,11 @@ struct spws_ctx {
   // bucket window width log2 (0 = chosen from the average row product count)
   int32_t outer_method = 0;
   int32_t outer_window_log2 = 0;
+  // second stream (default priority) and its events, created on first use:
+  // the outer product's per-chunk reduce overlaps the next chunk's expand
+  hipStream_t aux_stream = nullptr;
+  hipEvent_t aux_ready = nullptr;
+  hipEvent_t aux_done[2] = {nullptr, nullptr};
   // statistics of the last call (host-readable)
   spws_stats stats{};
   spws::KernelProfiler prof;
@@ -104,6 +109,8 @@ struct spws_ctx {
 namespace spws {
 
 int ctx_activate(spws_ctx* ctx);
+// The context's auxiliary stream (created with its events on first use).
+int ctx_aux_stream(spws_ctx* ctx, hipStream_t* out);
 
 // Kernel timing hooks: prof_begin records a start event (or returns null when
 // profiling is off), prof_end records the stop event; prof_collect turns the

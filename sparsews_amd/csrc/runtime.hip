@@ -75,6 +75,16 @@ int ctx_activate(spws_ctx* ctx) {
   return SPWS_OK;
 }
 
+int ctx_aux_stream(spws_ctx* ctx, hipStream_t* out) {
+  if (!ctx->aux_stream) {
+    SPWS_HIP(hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
+    SPWS_HIP(hipEventCreateWithFlags(&ctx->aux_ready, hipEventDisableTiming));
+    for (hipEvent_t& e : ctx->aux_done) SPWS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  *out = ctx->aux_stream;
+  return SPWS_OK;
+}
+
 static hipEvent_t prof_event(spws_ctx* ctx) {
   if (!ctx->prof.pool.empty()) {
     hipEvent_t e = ctx->prof.pool.back();
@@ -290,9 +300,15 @@ int spws_ctx_destroy(spws_ctx* ctx) {
                     &ctx->dense_flag, &ctx->out_pos, &ctx->out_crd,   &ctx->out_val,   &ctx->out_aux,
                     &ctx->keys[0],  &ctx->keys[1],  &ctx->vals[0],    &ctx->vals[1],   &ctx->chunk_a,
                     &ctx->chunk_b};
+  if (ctx->aux_stream) (void)hipStreamSynchronize(ctx->aux_stream);
   for (DevBuf* b : bufs) b->release();
   for (DevBuf& b : ctx->stage) b.release();
   for (DevBuf& b : ctx->obuf) b.release();
+  for (DevBuf& b : ctx->bk) b.release();
+  if (ctx->aux_ready) (void)hipEventDestroy(ctx->aux_ready);
+  for (hipEvent_t e : ctx->aux_done)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
   ctx->radix_hist.release();
   ctx->host_meta.release();
   prof_collect(ctx);

@@ -906,6 +906,21 @@ static int outer_output_init(spws_ctx* ctx, int32_t m, int64_t total_products) {
   return SPWS_OK;
 }
 
+static size_t outer_output_capacity(const spws_ctx* ctx) {
+  return ctx->out_crd.bytes / 4 < ctx->out_val.bytes / 8 ? ctx->out_crd.bytes / 4 : ctx->out_val.bytes / 8;
+}
+
+// Launches inside the scope go to stream s (when not null) instead of the
+// context's stream, timing included.
+struct StreamScope {
+  spws_ctx* ctx;
+  hipStream_t saved;
+  StreamScope(spws_ctx* c, hipStream_t s) : ctx(c), saved(c->stream) {
+    if (s) c->stream = s;
+  }
+  ~StreamScope() { ctx->stream = saved; }
+};
+
 static int outer_output_grow(spws_ctx* ctx, int64_t base, int64_t U) {
   hipStream_t s = ctx->stream;
   int rc;
@@ -1100,25 +1115,30 @@ static int outer_chunks_bucket(spws_ctx* ctx, const OuterArgs& a, const std::vec
   const size_t cap = (size_t)(max_chunk > 0 ? max_chunk : 1);
   const size_t ecap = (size_t)(max_en > 0 ? max_en : 1);
   const size_t nbcap = (size_t)max_r * (size_t)W + 1;
-  // column offsets: k_bk_count reads them in aligned groups of 4
-  if ((rc = ctx->keys[0].ensure(cap * 2 + 8)) || (rc = ctx->vals[0].ensure(cap * 8)) ||
-      (rc = ctx->bk[1].ensure(ecap * 4)) || (rc = ctx->bk[2].ensure(ecap * 4)) ||
+  // Two sets of the per-chunk workspace (column offsets, products, bucket
+  // tables): the reduce of chunk c runs on the auxiliary stream while the
+  // main stream expands chunk c + 1 into the other set.  Column offsets:
+  // k_bk_count reads them in aligned groups of 4.
+  const int nsets = chunks.size() > 1 ? 2 : 1;
+  DevBuf* tabs[2] = {&ctx->bk[8], &ctx->bk[10]};
+  for (int t = 0; t < nsets; ++t)
+    if ((rc = ctx->keys[t].ensure(cap * 2 + 8)) || (rc = ctx->vals[t].ensure(cap * 8)) ||
+        (rc = tabs[t]->ensure(nbcap * 4 * 4)))
+      return rc;
+  if ((rc = ctx->bk[1].ensure(ecap * 4)) || (rc = ctx->bk[2].ensure(ecap * 4)) ||
       (rc = ctx->bk[3].ensure(ecap * 8)) || (rc = ctx->bk[4].ensure(ecap * 8)) ||
       (rc = ctx->bk[5].ensure(ecap * 4)) || (rc = ctx->bk[6].ensure(ecap * (size_t)W * 4)) ||
-      (rc = ctx->bk[7].ensure(((size_t)max_r + 1) * 4)) || (rc = ctx->bk[8].ensure(nbcap * 4 * 4)) ||
-      (rc = ctx->bk[9].ensure(ecap * 4)))
+      (rc = ctx->bk[7].ensure(((size_t)max_r + 1) * 4)) || (rc = ctx->bk[9].ensure(ecap * 4)))
     return rc;
   if ((rc = outer_output_init(ctx, m, row_prefix[m]))) return rc;
-  uint16_t* klo = ctx->keys[0].as<uint16_t>();
-  double* pv = ctx->vals[0].as<double>();
+  hipStream_t aux = nullptr;
+  if (nsets > 1) {
+    if ((rc = ctx_aux_stream(ctx, &aux))) return rc;
+  }
   int32_t* ent_ja = ctx->bk[5].as<int32_t>();
   int32_t* ent_ai = ctx->bk[9].as<int32_t>();
   int32_t* etab = ctx->bk[6].as<int32_t>();
   int32_t* rp_off = ctx->bk[7].as<int32_t>();
-  int32_t* bsize = ctx->bk[8].as<int32_t>();
-  int32_t* bstart = bsize + nbcap;
-  int32_t* bnnz = bstart + nbcap;
-  int32_t* boff = bnnz + nbcap;
   int64_t* out_pos = ctx->out_pos.as<int64_t>();
   int32_t* hm = ctx->host_meta.as<int32_t>();
   const size_t lds_count = (size_t)kBkWaves * (((size_t)1 << s) >> 5) * 4;
@@ -1126,8 +1146,24 @@ static int outer_chunks_bucket(spws_ctx* ctx, const OuterArgs& a, const std::vec
   ctx->stats.outer_method = 2;
   ctx->stats.window_bits = s;
   int64_t base = 0;
+  bool pending[2] = {false, false};  // set t's last reduce not yet waited for
+  // on any error return, let the auxiliary stream's reduce finish first
+  struct AuxDrain {
+    hipStream_t s;
+    ~AuxDrain() {
+      if (s) (void)hipStreamSynchronize(s);
+    }
+  } aux_drain{aux};
+  int ci = 0;
 
   for (auto& ch : chunks) {
+    const int t = ci++ % nsets;
+    uint16_t* klo = ctx->keys[t].as<uint16_t>();
+    double* pv = ctx->vals[t].as<double>();
+    int32_t* bsize = tabs[t]->as<int32_t>();
+    int32_t* bstart = bsize + nbcap;
+    int32_t* bnnz = bstart + nbcap;
+    int32_t* boff = bnnz + nbcap;
     const int32_t r0 = ch.first, r1 = ch.second, R = r1 - r0;
     const int64_t P = row_prefix[r1] - row_prefix[r0];
     const int32_t En = (int32_t)(pair_prefix[r1] - pair_prefix[r0]);
@@ -1155,6 +1191,11 @@ static int outer_chunks_bucket(spws_ctx* ctx, const OuterArgs& a, const std::vec
         return rc;
       if (alt) pval = ctx->bk[4].as<double>();
       if ((rc = exclusive_scan<int32_t, int32_t>(ctx, row_pairs + r0, rp_off, R))) return rc;
+      // set t is free once the reduce that last read it has finished
+      if (pending[t]) {
+        SPWS_HIP(hipStreamWaitEvent(st, ctx->aux_done[t], 0));
+        pending[t] = false;
+      }
       const unsigned gr = (unsigned)std::min<int64_t>(div_up((int64_t)R * kWave, 256), 16384);
       SPWS_LAUNCH(ctx, "bk_rowsize", k_bk_rows<0>, dim3(gr), dim3(256), 0, R, W, rp_off, pval, ent_ja, bseg,
                   (const int32_t*)nullptr, bsize);
@@ -1176,18 +1217,33 @@ static int outer_chunks_bucket(spws_ctx* ctx, const OuterArgs& a, const std::vec
       SPWS_HIP(hipMemcpyAsync(hm, boff + NB, 4, hipMemcpyDeviceToHost, st));
       SPWS_HIP(hipStreamSynchronize(st));
       U = *hm;
+      if (outer_output_capacity(ctx) < (size_t)(base + U) && aux) {
+        // growing moves the output: the reduce in flight must finish first
+        SPWS_HIP(hipStreamSynchronize(aux));
+      }
       if ((rc = outer_output_grow(ctx, base, U))) return rc;
+      // row pointers and the reduce: on the auxiliary stream when there is one
+      StreamScope on(ctx, aux);
+      if (aux) {
+        SPWS_HIP(hipEventRecord(ctx->aux_ready, st));
+        SPWS_HIP(hipStreamWaitEvent(aux, ctx->aux_ready, 0));
+      }
       SPWS_LAUNCH(ctx, "bk_rowpos", k_bk_rowpos, dim3((unsigned)std::min<int64_t>(div_up(R, 256), 4096)), dim3(256), 0,
                   R, W, r0, base, boff, out_pos);
       const unsigned gred = (unsigned)std::min<int64_t>(div_up(NB, kBkRedWaves), 65536);
       SPWS_LAUNCH(ctx, "bk_reduce", k_bk_reduce<kBkRedUnroll>, dim3(gred), dim3(kBkRedWaves * kWave), lds_reduce, NB, W,
                   s, bstart, klo, pv, boff, base, ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>());
+      if (aux) {
+        SPWS_HIP(hipEventRecord(ctx->aux_done[t], aux));
+        pending[t] = true;
+      }
     } else {
       SPWS_LAUNCH(ctx, "outer_tail", (k_outer_tail<uint32_t>), dim3((unsigned)div_up(R, 256)), dim3(256), 0,
                   (const uint32_t*)nullptr, (int64_t)0, 0, r0, R, base, out_pos);
     }
     base += U;
   }
+  if (aux) SPWS_HIP(hipStreamSynchronize(aux));
   SPWS_LAUNCH(ctx, "set_i64", k_set_i64, dim3(1), dim3(1), 0, out_pos + m, base);
   SPWS_HIP(hipStreamSynchronize(st));
   *nnz_out = base;
