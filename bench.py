@@ -419,6 +419,41 @@ def run_ttm(args):
 
 # --------------------------------------------------------------------------- batched / served
 
+P99_TARGET_MS = 2.0
+
+
+def latency_bound_rate(ctx, engines, reqs, mode, sat, cap, steps=7):
+    """Highest Poisson arrival rate at which `mode` keeps the p99 latency of
+    `reqs` within P99_TARGET_MS: bisection (geometric) between 2% and 100%
+    of the saturation rate `sat`, the same unit-rate arrival pattern scaled.
+    Returns (rate, p99 at that rate); rate 0 when even the lowest fails."""
+    from sparsews_amd.serve import serve
+    n = len(reqs)
+    gaps = np.random.default_rng(777).exponential(1.0, n)
+    unit = np.cumsum(gaps) - gaps[0]
+
+    def p99_at(rate):
+        arr = unit / rate
+        fin, _, _ = serve(ctx, engines, reqs, arr, mode, max_batch_products=cap)
+        return float(np.percentile((fin - arr) * 1e3, 99))
+
+    lo, hi = 0.02 * sat, sat
+    p_lo = p99_at(lo)
+    if p_lo > P99_TARGET_MS:
+        return 0.0, p_lo
+    p_hi = p99_at(hi)
+    if p_hi <= P99_TARGET_MS:
+        return hi, p_hi
+    for _ in range(steps):
+        mid = (lo * hi) ** 0.5
+        p = p99_at(mid)
+        if p <= P99_TARGET_MS:
+            lo, p_lo = mid, p
+        else:
+            hi = mid
+    return lo, p_lo
+
+
 def run_serve(args):
     """Batched / served SpGEMM: a stream of requests A_b * B_b, square n_b
     log-uniform in [256, 65,536], half with uniform 2..64 nonzeros per row
@@ -428,7 +463,10 @@ def run_serve(args):
     request) and of the continuously batched path (4 engines, one hipGraph
     launch per batch, at most 2^26 products per batch); then Poisson
     arrivals at 80% of the plain path's capacity for both, and at 3x that
-    rate for the batched path, with p50/p99 latency (finish - arrival)."""
+    rate for the batched path, with p50/p99 latency (finish - arrival).
+    Last, the latency-bound rate of each path: the highest Poisson arrival
+    rate (bisection over the rate, the first 1,024 requests) at which the
+    p99 latency stays within 2 ms."""
     from sparsews_amd import gen
     from sparsews_amd.device import Context
     from sparsews_amd.serve import DeviceRequest, Engine, latency_summary, serve
@@ -445,7 +483,7 @@ def run_serve(args):
         if len(reqs) % 512 == 0:
             log(f"serve: {len(reqs)} requests generated and uploaded ({time.perf_counter() - t0:.0f} s)")
     N = len(reqs)
-    n_engines, cap = 4, 1 << 26
+    n_engines, cap = args.serve_engines, 1 << args.serve_cap_log2
     engines = [Engine(ctx) for _ in range(n_engines)]
     zero = np.zeros(N)
     # warm-up: one pass of each mode on a prefix
@@ -464,6 +502,12 @@ def run_serve(args):
     f_bl, _, st_bl = serve(ctx, engines, reqs, arr, "batched", max_batch_products=cap)
     arr3 = arr / 3.0
     f_b3, _, st_b3 = serve(ctx, engines, reqs, arr3, "batched", max_batch_products=cap)
+    bound = {"p99_target_ms": P99_TARGET_MS, "requests": min(N, 1024)}
+    for mode, sat in (("plain", sat_plain), ("batched", sat_batch)):
+        rate, p99 = latency_bound_rate(ctx, engines if mode == "batched" else [], reqs[:1024], mode, sat, cap)
+        bound[mode] = {"req_per_s": round(rate, 2), "p99_ms": round(p99, 4)}
+        log(f"serve: {mode} sustains {rate:.1f} req/s at p99 {p99:.3f} ms")
+    bound["speedup"] = round(bound["batched"]["req_per_s"] / max(bound["plain"]["req_per_s"], 1e-9), 3)
     for e in engines:
         e.close()
     ctx.close()
@@ -489,6 +533,7 @@ def run_serve(args):
                                    "batched": {**latency_summary(arr, f_bl),
                                                "req_per_s": round(N / st_bl["wall_seconds"], 2),
                                                "batches": st_bl["batches"]}},
+        "latency_bound": bound,
         "poisson_batched_at_3x": {"rate_req_per_s": round(3 * lam, 2), **latency_summary(arr3, f_b3),
                                   "req_per_s": round(N / st_b3["wall_seconds"], 2), "batches": st_b3["batches"],
                                   "p99_ms_excluding_top1pct_requests": round(float(np.percentile(lat_b3[~big], 99)), 4)},
@@ -510,6 +555,9 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--requests", type=int, default=4096, help="serve workload: number of requests")
+    ap.add_argument("--serve-engines", type=int, default=4, help="serve workload: batch engines")
+    ap.add_argument("--serve-cap-log2", type=int, default=26,
+                    help="serve workload: log2 of the products (output upper bound) per batch")
     ap.add_argument("--outer-method", type=int, default=0, choices=[0, 1, 2],
                     help="outer workload: 0 automatic, 1 LSD radix sort, 2 row/window buckets")
     ap.add_argument("--calibrate-copy", type=int, default=0,

@@ -4,7 +4,7 @@
     python -m sparsews_amd.profsum --trace RESULTS_DB --title T --command "CMD" [--top N]
     python -m sparsews_amd.profsum --pmc RESULTS_DB [RESULTS_DB ...] --title T --command "CMD"
     python -m sparsews_amd.profsum --traffic FETCH_DB WRITE_DB --kernel K [--copy-kernel C]
-                                   [--copy-bytes N] --title T --command "CMD"
+                                   [--copy-bytes N] [--launch-name L] --title T --command "CMD"
 
 The second form reads the counter values of ``rocprofv3 --pmc`` runs from
 their ``*_results.db`` files and prints, per kernel, the dispatch count and
@@ -82,10 +82,12 @@ def summarise_pmc(paths, title: str, command: str) -> str:
 
 
 def traffic_summary(fetch_db: str, write_db: str, kernel: str, copy_kernel: str, copy_bytes: int,
-                    title: str, command: str, workload: str = "") -> str:
+                    title: str, command: str, workload: str = "", launch_name: str = "") -> str:
     """HBM bytes per launch of `kernel` from a FETCH_SIZE run and a WRITE_SIZE
     run, each scaled by what the counter reads for a plain copy of
-    `copy_bytes` bytes (which reads and writes exactly that many bytes)."""
+    `copy_bytes` bytes (which reads and writes exactly that many bytes).
+    `launch_name` (default: the kernel name) is the name the library's own
+    per-kernel timer gives the launch, written in the bench-traffic line."""
     f = pmc_per_kernel(fetch_db)
     w = pmc_per_kernel(write_db)
     fk = f[(kernel, "FETCH_SIZE")]
@@ -105,7 +107,7 @@ def traffic_summary(fetch_db: str, write_db: str, kernel: str, copy_kernel: str,
              f"| `{copy_kernel}` FETCH_SIZE ({copy_bytes} B copy) | {fc[0]} | {fc[1] / fc[0]:.1f} | | |",
              f"| `{copy_kernel}` WRITE_SIZE ({copy_bytes} B copy) | {wc[0]} | {wc[1] / wc[0]:.1f} | | |",
              "",
-             f"bench-traffic: workload={workload} kernel={kernel} bytes_per_launch={fetch_b + write_b:.0f} "
+             f"bench-traffic: workload={workload} kernel={launch_name or kernel} bytes_per_launch={fetch_b + write_b:.0f} "
              f"fetch_bytes={fetch_b:.0f} write_bytes={write_b:.0f}"]
     return "\n".join(lines) + "\n"
 
@@ -119,6 +121,7 @@ def main(argv=None):
     ap.add_argument("--copy-kernel", default="k_copy16")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--workload", default="")
+    ap.add_argument("--launch-name", default="", help="the library timer's name for the kernel's launches")
     ap.add_argument("--trace", default=None, help="rocprofv3 --kernel-trace results db")
     ap.add_argument("--title", required=True)
     ap.add_argument("--command", required=True)
@@ -129,7 +132,7 @@ def main(argv=None):
         text = trace_summary(a.trace, a.title, a.command, a.top)
     elif a.traffic:
         text = traffic_summary(a.traffic[0], a.traffic[1], a.kernel, a.copy_kernel, a.copy_bytes, a.title, a.command,
-                               a.workload)
+                               a.workload, a.launch_name)
     elif a.pmc:
         text = summarise_pmc(a.pmc, a.title, a.command)
     else:
