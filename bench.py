@@ -382,18 +382,25 @@ def run_ttm(args):
     calibration_copy(ctx, args.calibrate_copy)
     nnz = X.nnz
     F = out["r"][0].count
+    # the nonzeros are grouped by one stable radix sort on the (i, j) bits of
+    # their packed (i, j, k) keys: 8-bit digits
+    sort_bits = max(1, (I - 1).bit_length()) + max(1, (J - 1).bit_length())
+    passes = -(-sort_bits // 8)
     per_step = {
-        # per nonzero: leaf crd + value + slot + fiber id read; per fiber: R values written
-        "ttm_accum": nnz * (4 + 8 + 4 + 4 + 4) + 8 * R * F + 8 * R * K,
-        "ttm_insert": nnz * (4 + 4 + 4 + 4),
-        "radix_scatter": 0,
+        # per nonzero: key + value read; per fiber: R values written; U read once
+        "ttm_accum": nnz * (8 + 8) + 8 * R * F + 8 * R * K,
+        # per nonzero and pass: key + value read and written
+        "radix_scatter": nnz * 32 * passes,
+        # per nonzero: leaf fiber, fiber root, root / middle / leaf coordinates
+        # and value read; key + value written
+        "ttm_keys": nnz * (4 + 4 + 4 + 4 + 4 + 8 + 16),
     }
     compulsory = 20 * nnz + 8 * R * K + F * (8 + 8 * R)
     res = {
         "metric": "TTM GFLOP/s", "value": round(2 * R * nnz / sec / 1e9, 3), "unit": "GFLOP/s",
         "ms_per_step": round(sec * 1e3, 4), "dtype": "f64",
         "config": {"workload": "ttm_mode3_100k_80k_2k_10M_R16", "dims": [I, J, K], "nnz": nnz, "rank": R,
-                   "fibers_out": int(F), "seed": 0},
+                   "fibers_out": int(F), "sort_bits": sort_bits, "radix_passes": passes, "seed": 0},
         "roofline": _dominant(times, per_step, args.steps, "ttm"),
         "compulsory_bytes": int(compulsory),
         "compulsory_frac_of_hbm_roofline": round(compulsory / sec / 1e9 / HBM_PEAK_GBS, 5),

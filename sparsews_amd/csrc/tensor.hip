@@ -415,6 +415,54 @@ struct Csf3 {
   int64_t nnz;
 };
 
+// Direct TTM grouping: the nonzeros carry packed (i, j, k) keys, and one
+// stable radix sort on the (i, j) bits puts each output fiber together, in
+// (i, j) order, with its nonzeros in increasing k (the CSF order (i, k, j)
+// has them so within a root) -- the order hash_il accumulates them
+// (COO_CSF_DCSR_hash_il.h:247-377).
+__global__ __launch_bounds__(256) void k_ttm_direct_keys(int64_t nnz, const int32_t* __restrict__ leaf_fiber,
+                                                         const int32_t* __restrict__ fiber_root,
+                                                         const int32_t* __restrict__ l1_crd,
+                                                         const int32_t* __restrict__ l2_crd,
+                                                         const int32_t* __restrict__ l3_crd,
+                                                         const double* __restrict__ xvals, int bj, int bk,
+                                                         uint64_t* __restrict__ keys, double* __restrict__ vals) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t f = leaf_fiber[e];
+    const uint64_t i = (uint32_t)l1_crd[fiber_root[f]];
+    const uint64_t k = (uint32_t)l2_crd[f];
+    const uint64_t j = (uint32_t)l3_crd[e];
+    keys[e] = (i << (bj + bk)) | (j << bk) | k;
+    vals[e] = xvals[e];
+  }
+}
+
+// flag[p] = 1 where sorted nonzero p starts a fiber ((i, j) differs from p - 1)
+__global__ __launch_bounds__(256) void k_ttm_fiber_flags(int64_t nnz, const uint64_t* __restrict__ keys, int bk,
+                                                         int32_t* __restrict__ flag) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x)
+    flag[p] = (p == 0 || (keys[p - 1] >> bk) != (keys[p] >> bk)) ? 1 : 0;
+}
+
+// With rank = exclusive prefix of the flags (rank[nnz] = fiber count F):
+// fiber rank[p] starts at p; its coordinates come from the key.
+__global__ __launch_bounds__(256) void k_ttm_fiber_starts(int64_t nnz, const uint64_t* __restrict__ keys, int bj,
+                                                          int bk, const int32_t* __restrict__ flag,
+                                                          const int32_t* __restrict__ rank, int32_t* __restrict__ foff,
+                                                          int32_t* __restrict__ out_i, int32_t* __restrict__ out_j) {
+  const uint64_t jm = (1ull << bj) - 1;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
+    if (flag[p]) {
+      const int32_t q = rank[p];
+      const uint64_t key = keys[p];
+      foff[q] = (int32_t)p;
+      out_i[q] = (int32_t)(key >> (bj + bk));
+      out_j[q] = (int32_t)((key >> bk) & jm);
+    }
+    if (p == 0) foff[rank[nnz]] = (int32_t)nnz;
+  }
+}
+
 static int csf_owners(spws_ctx* ctx, const Csf3& x, int32_t** leaf_fiber, int32_t** fiber_root) {
   int rc;
   if ((rc = ctx->obuf[0].ensure((size_t)(x.nnz > 0 ? x.nnz : 1) * 4)) ||
@@ -488,6 +536,38 @@ static int mttkrp_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_
   return SPWS_OK;
 }
 
+// Per-fiber sums over nonzeros grouped by fiber (foff[F + 1], keys' low bk
+// bits = k): work items of at most kTtmChunk nonzeros, in-order sums, then
+// the partial sums of long fibers combined in item order.
+static int ttm_accumulate(spws_ctx* ctx, int64_t nnz, int64_t F, int32_t R, int bk, const uint64_t* nkeys,
+                          const double* nvals, const int32_t* foff, const double* U, double* Y) {
+  hipStream_t s = ctx->stream;
+  int32_t* hm = ctx->host_meta.as<int32_t>();
+  int rc;
+  // coff = prefix of items per fiber
+  if ((rc = ctx->obuf[2].ensure(((size_t)F + 1) * 4 * 2))) return rc;
+  int32_t* nchunk = ctx->obuf[2].as<int32_t>();
+  int32_t* coff = nchunk + (F + 1);
+  const unsigned gf2 = (unsigned)std::min<int64_t>(div_up(F, 256), 65536);
+  SPWS_LAUNCH(ctx, "ttm_chunks", k_ttm_chunks, dim3(gf2), dim3(256), 0, F, foff, nchunk);
+  if ((rc = exclusive_scan<int32_t, int32_t>(ctx, nchunk, coff, F))) return rc;
+  // items <= F + nnz / kTtmChunk
+  const int64_t max_items = F + nnz / kTtmChunk + 1;
+  if ((rc = ctx->obuf[3].ensure((size_t)max_items * R * 8)) || (rc = ctx->obuf[4].ensure((size_t)max_items * 4)))
+    return rc;
+  SPWS_HIP(hipMemcpyAsync(hm, coff + F, 4, hipMemcpyDeviceToHost, s));
+  SPWS_HIP(hipStreamSynchronize(s));
+  const int64_t nitems = *hm;
+  int32_t* item_fiber = ctx->obuf[4].as<int32_t>();
+  SPWS_LAUNCH(ctx, "ttm_itemmap", k_ttm_itemmap, dim3(gf2), dim3(256), 0, F, coff, item_fiber);
+  const unsigned ga = (unsigned)std::min<int64_t>(div_up(nitems, 256 / kTtmLanes), 65536);
+  SPWS_LAUNCH(ctx, "ttm_accum", k_ttm_accum, dim3(ga), dim3(256), 0, F, nitems, R, nkeys, nvals, foff, coff, item_fiber,
+              bk, U, Y, ctx->obuf[3].as<double>());
+  const unsigned gc = (unsigned)std::min<int64_t>(div_up(F, 256 / kTtmLanes), 65536);
+  SPWS_LAUNCH(ctx, "ttm_combine", k_ttm_combine, dim3(gc), dim3(256), 0, F, R, coff, ctx->obuf[3].as<double>(), Y);
+  return SPWS_OK;
+}
+
 static int ttm_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_t K, int32_t R, const double* U,
                    int64_t* nfib_out) {
   int rc = ctx_activate(ctx);
@@ -500,20 +580,60 @@ static int ttm_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_t K
   hipStream_t s = ctx->stream;
   memset(&ctx->stats, 0, sizeof(ctx->stats));
   const size_t nz = (size_t)(x.nnz > 0 ? x.nnz : 1);
-  // table of 2 * nnz (power of two) slots
+  const int bk_all = bits_of(K > 1 ? K : 2);
+  const bool direct = bi + bj + bk_all <= 64;
+  // hash path: a table of 2 * nnz (power of two) slots
   uint64_t T = 1024;
-  while (T < 2 * (uint64_t)nz) T <<= 1;
-  if ((rc = ctx->wide_tab.ensure(T * 8)) || (rc = ctx->slot_crd.ensure(nz * 4)) ||
-      (rc = ctx->row_nnz.ensure((T + 1) * 4 * 2)) || (rc = ctx->keys[0].ensure(nz * 8)) ||
+  while (!direct && T < 2 * (uint64_t)nz) T <<= 1;
+  const size_t nflag = direct ? nz + 1 : T + 1;
+  if ((rc = ctx->row_nnz.ensure(nflag * 4 * 2)) || (rc = ctx->keys[0].ensure(nz * 8)) ||
       (rc = ctx->keys[1].ensure(nz * 8)) || (rc = ctx->vals[0].ensure(nz * 8)) || (rc = ctx->vals[1].ensure(nz * 8)))
     return rc;
+  if (!direct && ((rc = ctx->wide_tab.ensure(T * 8)) || (rc = ctx->slot_crd.ensure(nz * 4)))) return rc;
   unsigned long long* table = ctx->wide_tab.as<unsigned long long>();
   int32_t* leaf_slot = ctx->slot_crd.as<int32_t>();
   int32_t* flag = ctx->row_nnz.as<int32_t>();
-  int32_t* off = flag + (T + 1);
+  int32_t* off = flag + nflag;
   int32_t* hm = ctx->host_meta.as<int32_t>();
   int64_t F = 0;
-  if (x.nnz > 0) {
+  if (x.nnz > 0 && direct) {
+    // one sort of the nonzeros on (i, j, k)
+    int32_t *leaf_fiber, *fiber_root;
+    if ((rc = csf_owners(ctx, x, &leaf_fiber, &fiber_root))) return rc;
+    const int bk = bk_all;
+    uint64_t* nkeys = ctx->keys[0].as<uint64_t>();
+    double* nvals = ctx->vals[0].as<double>();
+    const unsigned g = (unsigned)std::min<int64_t>(div_up(x.nnz, 256), 65536);
+    SPWS_LAUNCH(ctx, "ttm_keys", k_ttm_direct_keys, dim3(g), dim3(256), 0, x.nnz, leaf_fiber, fiber_root, x.l1_crd,
+                x.l2_crd, x.l3_crd, x.vals, bj, bk, nkeys, nvals);
+    // the sort covers (i, j) only: within an i slice the CSF order already
+    // has each (i, j)'s nonzeros in increasing k, and the sort is stable
+    bool alt = false;
+    if ((rc = radix_sort_pairs<uint64_t>(ctx, nkeys, nvals, ctx->keys[1].as<uint64_t>(), ctx->vals[1].as<double>(),
+                                         x.nnz, bk, bi + bj + bk, &alt)))
+      return rc;
+    if (alt) {
+      nkeys = ctx->keys[1].as<uint64_t>();
+      nvals = ctx->vals[1].as<double>();
+    }
+    // fiber flags and their prefix (flag / off hold nnz + 1 entries here)
+    SPWS_LAUNCH(ctx, "ttm_flags", k_ttm_fiber_flags, dim3(g), dim3(256), 0, x.nnz, nkeys, bk, flag);
+    if ((rc = exclusive_scan<int32_t, int32_t>(ctx, flag, off, x.nnz))) return rc;
+    SPWS_HIP(hipMemcpyAsync(hm, off + x.nnz, 4, hipMemcpyDeviceToHost, s));
+    SPWS_HIP(hipStreamSynchronize(s));
+    F = *hm;
+    if ((rc = ctx->out_crd.ensure((size_t)(F > 0 ? F : 1) * 4 * 2)) ||
+        (rc = ctx->out_val.ensure((size_t)(F > 0 ? F : 1) * R * 8)) ||
+        (rc = ctx->obuf[5].ensure(((size_t)F + 1) * 4)))
+      return rc;
+    int32_t* out_i = ctx->out_crd.as<int32_t>();
+    int32_t* out_j = out_i + F;
+    int32_t* foff = ctx->obuf[5].as<int32_t>();
+    SPWS_LAUNCH(ctx, "ttm_starts", k_ttm_fiber_starts, dim3(g), dim3(256), 0, x.nnz, nkeys, bj, bk, flag, off, foff,
+                out_i, out_j);
+    if ((rc = ttm_accumulate(ctx, x.nnz, F, R, bk, nkeys, nvals, foff, U, ctx->out_val.as<double>()))) return rc;
+    ctx->stats.products = x.nnz * (int64_t)R;
+  } else if (x.nnz > 0) {
     int32_t *leaf_fiber, *fiber_root;
     if ((rc = csf_owners(ctx, x, &leaf_fiber, &fiber_root))) return rc;
     SPWS_HIP(hipMemsetAsync(table, 0xff, T * 8, s));
@@ -567,27 +687,7 @@ static int ttm_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_t K
     // fiber offsets reuse the flag array past rank_of_slot's T entries
     int32_t* foff = off;
     SPWS_LAUNCH(ctx, "ttm_heads", k_ttm_heads, dim3(g), dim3(256), 0, x.nnz, nkeys, bk, F, foff);
-    // work items of at most kTtmChunk nonzeros; coff = prefix of items per fiber
-    if ((rc = ctx->obuf[2].ensure(((size_t)F + 1) * 4 * 2))) return rc;
-    int32_t* nchunk = ctx->obuf[2].as<int32_t>();
-    int32_t* coff = nchunk + (F + 1);
-    const unsigned gf2 = (unsigned)std::min<int64_t>(div_up(F, 256), 65536);
-    SPWS_LAUNCH(ctx, "ttm_chunks", k_ttm_chunks, dim3(gf2), dim3(256), 0, F, foff, nchunk);
-    if ((rc = exclusive_scan<int32_t, int32_t>(ctx, nchunk, coff, F))) return rc;
-    // items <= F + nnz / kTtmChunk
-    const int64_t max_items = F + x.nnz / kTtmChunk + 1;
-    if ((rc = ctx->obuf[3].ensure((size_t)max_items * R * 8)) || (rc = ctx->obuf[4].ensure((size_t)max_items * 4)))
-      return rc;
-    SPWS_HIP(hipMemcpyAsync(hm, coff + F, 4, hipMemcpyDeviceToHost, s));
-    SPWS_HIP(hipStreamSynchronize(s));
-    const int64_t nitems = *hm;
-    int32_t* item_fiber = ctx->obuf[4].as<int32_t>();
-    SPWS_LAUNCH(ctx, "ttm_itemmap", k_ttm_itemmap, dim3(gf2), dim3(256), 0, F, coff, item_fiber);
-    const unsigned ga = (unsigned)std::min<int64_t>(div_up(nitems, 256 / kTtmLanes), 65536);
-    SPWS_LAUNCH(ctx, "ttm_accum", k_ttm_accum, dim3(ga), dim3(256), 0, F, nitems, R, nkeys, nvals, foff, coff,
-                item_fiber, bk, U, Y, ctx->obuf[3].as<double>());
-    const unsigned gc = (unsigned)std::min<int64_t>(div_up(F, 256 / kTtmLanes), 65536);
-    SPWS_LAUNCH(ctx, "ttm_combine", k_ttm_combine, dim3(gc), dim3(256), 0, F, R, coff, ctx->obuf[3].as<double>(), Y);
+    if ((rc = ttm_accumulate(ctx, x.nnz, F, R, bk, nkeys, nvals, foff, U, Y))) return rc;
     ctx->stats.products = x.nnz * (int64_t)R;
   } else {
     if ((rc = ctx->out_crd.ensure(8)) || (rc = ctx->out_val.ensure(8))) return rc;

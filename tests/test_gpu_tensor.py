@@ -88,6 +88,33 @@ def test_ttm_small(ctx, R):
     assert oracle.max_value_error(Y.ravel(), ov, absY[fi, fj].ravel()) <= TOL
 
 
+def test_ttm_wide_coordinates(ctx):
+    """Dimensions whose packed (i, j, k) coordinates need more than 64 bits
+    (2^24 x 2^24 x 2^17): the fibers are grouped through the hash table
+    instead of one sort on (i, j, k).  Fibers and their sums are checked
+    against a numpy grouping of the nonzeros, summed in increasing k."""
+    print("case: ttm 65-bit coordinates", flush=True)
+    dims = (1 << 24, 1 << 24, 1 << 17)
+    X = gen.zipf_coo3(dims, 3000, (1.5, 1.5, None), 16, 5, -1.0, 1.0)
+    R = 4
+    U = np.random.default_rng(16).uniform(-1, 1, (dims[2], R))
+    fi, fj, Y = run_ttm(ctx, X, U)
+    i, j, k, v = X.idx[:, 0], X.idx[:, 1], X.idx[:, 2], X.vals
+    order = np.lexsort((k, j, i))
+    i, j, k, v = i[order], j[order], k[order], v[order]
+    head = np.ones(len(i), dtype=bool)
+    head[1:] = (i[1:] != i[:-1]) | (j[1:] != j[:-1])
+    np.testing.assert_array_equal(fi, i[head])
+    np.testing.assert_array_equal(fj, j[head])
+    fib = np.cumsum(head) - 1
+    ref = np.zeros((head.sum(), R))
+    ab = np.zeros((head.sum(), R))
+    for p in range(len(i)):  # in increasing k within each fiber
+        ref[fib[p]] += v[p] * U[k[p]]
+        ab[fib[p]] += np.abs(v[p] * U[k[p]])
+    assert oracle.max_value_error(Y.ravel(), ref.ravel(), ab.ravel()) <= TOL
+
+
 def test_ttm_taco_entry():
     print("case: ttm taco entry", flush=True)
     X = gen.zipf_coo3((80, 70, 30), 2500, (1.5, 1.5, None), 14, 5, -1.0, 1.0)
