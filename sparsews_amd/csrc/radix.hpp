@@ -5,7 +5,7 @@
 namespace spws {
 
 constexpr int kRadixBlock = 256;
-constexpr int kRadixItems = 16;
+constexpr int kRadixItems = 8;
 constexpr int kRadixTile = kRadixBlock * kRadixItems;  // keys per workgroup tile
 constexpr int kRadixDigit = 8;                          // bits per pass
 constexpr int kRadixBuckets = 1 << kRadixDigit;
