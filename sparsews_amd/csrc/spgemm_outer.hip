@@ -1115,16 +1115,16 @@ static int outer_chunks_bucket(spws_ctx* ctx, const OuterArgs& a, const std::vec
   const size_t cap = (size_t)(max_chunk > 0 ? max_chunk : 1);
   const size_t ecap = (size_t)(max_en > 0 ? max_en : 1);
   const size_t nbcap = (size_t)max_r * (size_t)W + 1;
-  // Two sets of the per-chunk workspace (column offsets, products, bucket
-  // tables): the reduce of chunk c runs on the auxiliary stream while the
-  // main stream expands chunk c + 1 into the other set.  Column offsets:
+  // The reduce of chunk c runs on the auxiliary stream while the main
+  // stream prepares chunk c + 1 (column ranges, entry sort, bucket tables);
+  // the expand of c + 1 waits for that reduce.  So the bucket tables come in
+  // two sets and the products and column offsets in one.  Column offsets:
   // k_bk_count reads them in aligned groups of 4.
   const int nsets = chunks.size() > 1 ? 2 : 1;
   DevBuf* tabs[2] = {&ctx->bk[8], &ctx->bk[10]};
+  if ((rc = ctx->keys[0].ensure(cap * 2 + 8)) || (rc = ctx->vals[0].ensure(cap * 8))) return rc;
   for (int t = 0; t < nsets; ++t)
-    if ((rc = ctx->keys[t].ensure(cap * 2 + 8)) || (rc = ctx->vals[t].ensure(cap * 8)) ||
-        (rc = tabs[t]->ensure(nbcap * 4 * 4)))
-      return rc;
+    if ((rc = tabs[t]->ensure(nbcap * 4 * 4))) return rc;
   if ((rc = ctx->bk[1].ensure(ecap * 4)) || (rc = ctx->bk[2].ensure(ecap * 4)) ||
       (rc = ctx->bk[3].ensure(ecap * 8)) || (rc = ctx->bk[4].ensure(ecap * 8)) ||
       (rc = ctx->bk[5].ensure(ecap * 4)) || (rc = ctx->bk[6].ensure(ecap * (size_t)W * 4)) ||
@@ -1158,8 +1158,8 @@ static int outer_chunks_bucket(spws_ctx* ctx, const OuterArgs& a, const std::vec
 
   for (auto& ch : chunks) {
     const int t = ci++ % nsets;
-    uint16_t* klo = ctx->keys[t].as<uint16_t>();
-    double* pv = ctx->vals[t].as<double>();
+    uint16_t* klo = ctx->keys[0].as<uint16_t>();
+    double* pv = ctx->vals[0].as<double>();
     int32_t* bsize = tabs[t]->as<int32_t>();
     int32_t* bstart = bsize + nbcap;
     int32_t* bnnz = bstart + nbcap;
@@ -1202,6 +1202,13 @@ static int outer_chunks_bucket(spws_ctx* ctx, const OuterArgs& a, const std::vec
       if ((rc = exclusive_scan<int32_t, int32_t>(ctx, bsize, bstart, NB))) return rc;
       SPWS_LAUNCH(ctx, "bk_etab", k_bk_rows<1>, dim3(gr), dim3(256), 0, R, W, rp_off, pval, ent_ja, bseg, bstart,
                   etab);
+      // the expand waits for the previous chunk's reduce: only this chunk's
+      // preparation (column ranges, entry sort, bucket tables) overlaps it,
+      // so the two memory-bound kernels never share the GPU
+      if (pending[1 - t]) {
+        SPWS_HIP(hipStreamWaitEvent(st, ctx->aux_done[1 - t], 0));
+        pending[1 - t] = false;
+      }
       const unsigned gx = (unsigned)std::min<int64_t>(div_up((int64_t)En * kWave, 256), 32768);
       if (W <= kBkExpandLdsW) {
         SPWS_LAUNCH(ctx, "bk_expand", k_bk_expand_lds<8>, dim3(gx), dim3(256), (size_t)(256 / kWave) * W * 4, En, pval,
