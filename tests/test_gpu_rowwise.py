@@ -62,6 +62,21 @@ def test_single_pass_short_rows(ctx, m, seed):
     np.testing.assert_array_equal(spgemm_rowwise_symbolic(ctx, dA, dB), C.pos)
 
 
+def test_matrix_market_file(ctx, tmp_path):
+    """A symmetric Matrix Market file (lower triangle stored) read with its
+    values, squared on the device."""
+    print("case: row-wise on a Matrix Market file", flush=True)
+    import scipy.io
+    import scipy.sparse as sp
+    from sparsews_amd import io as spio
+    M = sp.tril(sp.random(800, 800, density=0.01, random_state=31)).tocoo()
+    path = tmp_path / "sym.mtx"
+    scipy.io.mmwrite(str(path), M, symmetry="symmetric")
+    A = spio.read_mtx(path)
+    C = spgemm_rowwise(ctx, DeviceCSR.upload(ctx, A), DeviceCSR.upload(ctx, A)).download()
+    check(C, A, A)
+
+
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_uniform_small(ctx, seed):
     print(f"case: uniform seed {seed}", flush=True)

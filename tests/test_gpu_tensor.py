@@ -115,6 +115,25 @@ def test_ttm_wide_coordinates(ctx):
     assert oracle.max_value_error(Y.ravel(), ref.ravel(), ab.ravel()) <= TOL
 
 
+def test_ttm_frostt_file(ctx, tmp_path):
+    """A FROSTT .tns file read with its values, then TTM on the device."""
+    print("case: ttm on a FROSTT file", flush=True)
+    from sparsews_amd import io as spio
+    X0 = gen.zipf_coo3((300, 200, 40), 8000, (1.5, 1.5, None), 17, 5, -1.0, 1.0)
+    path = tmp_path / "x.tns"
+    spio.write_tns(path, X0)
+    X = spio.read_tns(path, dims=X0.dims)
+    R = 8
+    U = np.random.default_rng(17).uniform(-1, 1, (40, R))
+    fi, fj, Y = run_ttm(ctx, X, U)
+    csf = formats.csf3_from_coo(X, (0, 2, 1))
+    ok, oj, ol, ov = oracle.ttm_hash_il(csf, formats.dense_as_dcsr(U), 300, 200, R)
+    np.testing.assert_array_equal(np.repeat(fi, R), ok)
+    np.testing.assert_array_equal(np.repeat(fj, R), oj)
+    absY = np.einsum("ijk,kr->ijr", np.abs(dense(X)), np.abs(U))
+    assert oracle.max_value_error(Y.ravel(), ov, absY[fi, fj].ravel()) <= TOL
+
+
 def test_ttm_taco_entry():
     print("case: ttm taco entry", flush=True)
     X = gen.zipf_coo3((80, 70, 30), 2500, (1.5, 1.5, None), 14, 5, -1.0, 1.0)
