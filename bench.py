@@ -90,6 +90,35 @@ def roofline(times, name, bytes_per_launch, workload):
             "algorithmic_bytes_per_launch": int(bytes_per_launch)}
 
 
+def rocsparse_leg(ctx, dA, dB, C, products, steps, warmup):
+    """rocSPARSE's SpGEMM (installed with ROCm) on the same device operands,
+    for context: wall time per product over `steps` products (its nnz and
+    compute stages, buffers reused), and whether its output has our
+    structure and values (within 1e-12 relative)."""
+    from sparsews_amd import rocsparse_cmp
+    if not rocsparse_cmp.available():
+        return {"error": "librocsparse.so not found"}
+    try:
+        r = rocsparse_cmp.RocsparseSpgemm(ctx, dA, dB)
+        for _ in range(warmup):
+            r.run()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r.run()
+        ctx.sync()
+        sec = (time.perf_counter() - t0) / steps
+        pos, crd, vals = r.result()
+        r.close()
+    except RuntimeError as e:
+        return {"error": str(e)}
+    ours = C.download()
+    same = bool(len(crd) == ours.nnz and np.array_equal(pos, np.asarray(ours.pos)) and np.array_equal(crd, ours.crd))
+    rel = float(np.max(np.abs(vals - ours.vals) / np.maximum(np.abs(ours.vals), 1e-300))) if same and len(crd) else None
+    return {"ms_per_step": round(sec * 1e3, 4), "value": round(2 * products / sec / 1e9, 3), "unit": "GFLOP/s",
+            "same_structure": same, "max_rel_value_diff": rel}
+
+
 def cpu_cores():
     try:
         return len(os.sched_getaffinity(0))
@@ -138,6 +167,8 @@ def run_rowwise(args):
         "compulsory_frac_of_hbm_roofline": round(compulsory / sec / 1e9 / HBM_PEAK_GBS, 5),
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(times.items())},
     }
+    if not args.no_rocsparse:
+        res["rocsparse"] = rocsparse_leg(ctx, dA, dB, C, P, args.steps, args.warmup)
     if not args.no_cpu_baseline:
         import oracle
         cores = cpu_cores()
@@ -560,6 +591,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-rocsparse", action="store_true", help="row-wise workload: skip the rocSPARSE comparison")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--requests", type=int, default=4096, help="serve workload: number of requests")
     ap.add_argument("--serve-engines", type=int, default=4, help="serve workload: batch engines")
