@@ -248,8 +248,10 @@ def set_outer_method(ctx: Context, method: int = 0, window_log2: int = 0):
 
 
 def spgemm_outer(ctx: Context, A: DeviceDCSR, B: DeviceDCSR, m: int, k: int, n: int) -> DeviceCSR64:
-    """C = A*B with A (m x k) in DCSC and B (k x n) in DCSR: expand, radix sort
-    on packed (i, k) keys, segmented reduce; int64 row pointers."""
+    """C = A*B with A (m x k) in DCSC and B (k x n) in DCSR through the
+    coordinate workspace: expand, sort on (i, k) (row/column-window buckets
+    or an LSD radix sort of packed keys, see set_outer_method), segmented
+    reduce; int64 row pointers."""
     nnz = C.c_int64()
     pp, pc, pv = C.c_void_p(), C.c_void_p(), C.c_void_p()
     check(lib().spws_spgemm_outer(ctx.h, int(m), int(k), int(n), A.nouter, A.l1.ptr if A.l1 else None,
