@@ -12,7 +12,9 @@
 //     4. k_mttkrp_accum     each half-wavefront (32 lanes = 32 ranks) walks a
 //                           segment of the bucketed nonzeros, loading factor
 //                           rows Bf(j,:), Cf(k,:) as coalesced 256-byte rows
-//                           and accumulating the rank-R fiber in registers;
+//                           (Bf(j,:) only when j changes: within a row the
+//                           nonzeros come in (j, k) order) and accumulating
+//                           the rank-R fiber in registers;
 //                           a fiber is added into M when its row changes (only
 //                           segment-boundary rows are shared, so the atomics
 //                           are few and uncontended).
@@ -24,22 +26,25 @@
 // TTM  Y(i,j,r) = sum_k X(i,k,j) * U(k,r)   (contracted mode k)
 //   Input CSF levels (i, k, j): the contracted coordinate is the middle
 //   level, as in the reference's hash_il (B(k,i,j) in
-//   benchmark/TTM/COO_CSF_DCSR_hash_il.h:247).  Hash workspace keyed on (i,j):
+//   benchmark/TTM/COO_CSF_DCSR_hash_il.h:247), whose hash workspace is keyed
+//   on the output fiber (i,j).  Here the fibers are grouped by sorting:
 //     1. k_seg_owner (x2)   leaf -> fiber and fiber -> root owners.
-//     2. k_ttm_insert       open-addressing table in HBM (64-bit keys
-//                           (i << bj) | j, atomicCAS); slot per nonzero.
-//     3. k_ttm_occupied + scan + k_ttm_collect: the occupied keys (one per
-//                           output fiber (i,j)) with their slots.
-//     4. radix sort         fiber keys, slot ids carried (the reference's Sort
-//                           of the workspace before Merge).
-//     5. k_ttm_rank         slot -> position of its fiber in sorted order.
-//     6. k_ttm_keys + radix sort + k_ttm_heads: nonzeros grouped by fiber
-//                           rank (stable, so each fiber keeps CSF order).
-//     7. k_ttm_accum        one 16-lane group per work item (at most 128
+//     2. k_ttm_direct_keys  packed (i, j, k) key and value per nonzero.
+//     3. radix sort         stable, on the (i, j) bits only: each fiber's
+//                           nonzeros end up together and keep the CSF order,
+//                           increasing k -- the order hash_il sums them.
+//     4. k_ttm_fiber_flags + scan + k_ttm_fiber_starts: fiber heads, fiber
+//                           offsets and the output coordinates (i, j).
+//     5. k_ttm_accum        one 16-lane group per work item (at most 128
 //                           nonzeros of one fiber) sums X * U(k,:) in order;
 //                           a fiber of one item writes its vector, longer
 //                           fibers write partial sums that k_ttm_combine adds
 //                           in item order (no atomics; deterministic).
+//   When i, j and k need more than 64 bits together, the fibers are found
+//   through an open-addressing table in HBM (k_ttm_insert, 64-bit keys
+//   (i << bj) | j, atomicCAS), collected and sorted (k_ttm_occupied,
+//   k_ttm_collect, radix sort, k_ttm_rank), and the nonzeros are then
+//   sorted by fiber rank (k_ttm_keys, k_ttm_heads) before step 5.
 //   Output: semi-sparse -- the sorted fiber list (i,j) with a dense length-R
 //   vector each.
 #include <algorithm>
@@ -161,6 +166,10 @@ __global__ __launch_bounds__(kAccBlock) void k_mttkrp_accum(int64_t nnz, int64_t
   const int rr0 = r0 < R ? r0 : 0, rr1 = r1 < R ? r1 : 0;  // in-range rank for the loads
   int64_t cur = -1;
   double acc0 = 0.0, acc1 = 0.0;
+  // Bf(j,:) of the last nonzero: within a row the nonzeros come in (j, k)
+  // order, so runs of equal j reuse it instead of gathering it again
+  int64_t jprev = -1;
+  double bp0 = 0.0, bp1 = 0.0;
   for (int64_t base = e0; base < e1; base += 32) {
     const int cnt = (int)(e1 - base < 32 ? e1 - base : 32);
     const int64_t me = base + (r0 < cnt ? r0 : cnt - 1);
@@ -175,16 +184,30 @@ __global__ __launch_bounds__(kAccBlock) void k_mttkrp_accum(int64_t nnz, int64_t
         kk[u] = __shfl(mykey, src, 32);
         xv[u] = __shfl(myval, src, 32);
       }
+      bool nb[kAccU];  // Bf(j,:) differs from the previous nonzero's (uniform per half-wavefront)
 #pragma unroll
       for (int u = 0; u < kAccU; ++u) {
         const int64_t j = (int64_t)((kk[u] >> bk) & jm);
         const int64_t k = (int64_t)(kk[u] & km);
-        b0[u] = Bf[j * R + rr0];
-        c0[u] = Cf[k * R + rr0];
-        if (two) {
-          b1[u] = Bf[j * R + rr1];
-          c1[u] = Cf[k * R + rr1];
+        const int64_t jbefore = u == 0 ? jprev : (int64_t)((kk[u - 1] >> bk) & jm);
+        nb[u] = j != jbefore;
+        b0[u] = b1[u] = 0.0;
+        if (nb[u]) {
+          b0[u] = Bf[j * R + rr0];
+          if (two) b1[u] = Bf[j * R + rr1];
         }
+        c0[u] = Cf[k * R + rr0];
+        if (two) c1[u] = Cf[k * R + rr1];
+      }
+      jprev = (int64_t)((kk[kAccU - 1] >> bk) & jm);
+#pragma unroll
+      for (int u = 0; u < kAccU; ++u) {
+        if (nb[u]) {
+          bp0 = b0[u];
+          if (two) bp1 = b1[u];
+        }
+        b0[u] = bp0;
+        if (two) b1[u] = bp1;
       }
 #pragma unroll
       for (int u = 0; u < kAccU; ++u) {
