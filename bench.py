@@ -576,7 +576,32 @@ def run_serve(args):
                                   "req_per_s": round(N / st_b3["wall_seconds"], 2), "batches": st_b3["batches"],
                                   "p99_ms_excluding_top1pct_requests": round(float(np.percentile(lat_b3[~big], 99)), 4)},
     }
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = serve_cpu_baseline(stream, args.cpu_seconds)
     return res
+
+
+def serve_cpu_baseline(stream, seconds):
+    """The oracle's OpenMP Gustavson (CSR_CSR_taco_parallel.h) on all host
+    cores, one request after another from the start of the same request
+    stream (operands regenerated from their seeds), until about `seconds`
+    of work: requests per second on the CPU."""
+    import oracle
+    from sparsews_amd import gen
+    cores = cpu_cores()
+    done, products, busy = 0, 0, 0.0
+    for r in stream:
+        A, B = gen.request_operands(r)
+        t0 = time.perf_counter()
+        oracle.spgemm_gustavson(A, B, cores)
+        busy += time.perf_counter() - t0
+        products += gen.row_products(A, B)[0]
+        done += 1
+        if busy >= seconds:
+            break
+    return {"value": round(done / busy, 2), "unit": "req/s", "cores": cores, "kind": "port",
+            "sample": f"first {done} requests of the stream ({products} products), OpenMP Gustavson restatement "
+                      f"(CSR_CSR_taco_parallel.h), {busy:.1f} s"}
 
 
 WORKLOADS = {"rowwise": run_rowwise, "outer": run_outer, "mttkrp": run_mttkrp, "ttm": run_ttm, "serve": run_serve}
