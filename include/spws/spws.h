@@ -318,6 +318,15 @@ int spws_csr_csr_compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B);
  * nnz(C) >= 2^31).  Sets C->indices[1][0], C->indices[1][1], C->vals. */
 int spws_dcsc_dcsr_coord_compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B, int32_t w_cap);
 
+/* int compute(C, A, B, w_cap) of benchmark/CSC_CSR_T/CSC_CSR_T_hash.h:232:
+ * the transposed-output product C = (A B)^T with A CSC (m x k, stored by
+ * column), B CSR (k x n) and C CSR (n x m) -- the workspace keyed on (k, i),
+ * fused with the product.  Computed as the outer product B^T A^T (the
+ * operands' arrays swapped), each entry summed over j in increasing order.
+ * Sets C->indices[1][0], C->indices[1][1], C->vals (SPWS_ERR_UNSUPPORTED
+ * when nnz(C) >= 2^31). */
+int spws_csc_csr_t_hash_compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B, int32_t w_cap);
+
 /* int compute(A, B, C, D, w_cap) of
  * benchmark/MTTKRP/COO_CSF_DCSR_DCSR_hash_free.h:233: B a 3-level CSF with
  * the output mode as leaf, C and D dense factors stored as DCSR.  Sets the

@@ -128,6 +128,7 @@ SIGNATURES = {
     "spws_csr_csr_assemble": (C.c_int, [C.POINTER(TacoTensor)] * 3),
     "spws_csr_csr_compute": (C.c_int, [C.POINTER(TacoTensor)] * 3),
     "spws_dcsc_dcsr_coord_compute": (C.c_int, [C.POINTER(TacoTensor)] * 3 + [I32]),
+    "spws_csc_csr_t_hash_compute": (C.c_int, [C.POINTER(TacoTensor)] * 3 + [I32]),
     "spws_mttkrp_hash_compute": (C.c_int, [C.POINTER(TacoTensor)] * 4 + [I32]),
     "spws_ttm_hash_il_compute": (C.c_int, [C.POINTER(TacoTensor)] * 3 + [I32]),
 }

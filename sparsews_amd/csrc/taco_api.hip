@@ -334,6 +334,64 @@ int spws_dcsc_dcsr_coord_compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor
   return SPWS_OK;
 }
 
+int spws_csc_csr_t_hash_compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B, int32_t w_cap) {
+  (void)w_cap;
+  SPWS_REQUIRE(C && C->dimensions && C->indices && C->indices[1], SPWS_ERR_INVALID,
+               "C must be initialised as a 2-level tensor (init_taco_tensor_DC)");
+  SPWS_REQUIRE(A && B && A->dimensions && B->dimensions && A->indices && B->indices && A->indices[1] &&
+                   B->indices[1] && A->indices[1][0] && B->indices[1][0],
+               SPWS_ERR_INVALID, "A must be CSC and B CSR taco tensors");
+  // C (n x m) = (A B)^T = B^T A^T: B's rows j are the stored columns of the
+  // left operand B^T, A's columns j the stored rows of the right operand A^T;
+  // both outer levels are dense (every j)
+  const int32_t n = C->dimensions[0], m = C->dimensions[1], k = B->dimensions[0];
+  SPWS_REQUIRE(B->dimensions[1] == n && A->dimensions[0] == m && A->dimensions[1] == k, SPWS_ERR_INVALID,
+               "dimensions: A %d x %d (CSC), B %d x %d, C must be %d x %d", A->dimensions[0], A->dimensions[1],
+               B->dimensions[0], B->dimensions[1], B->dimensions[1], A->dimensions[0]);
+  int rc;
+  spws_ctx* ctx;
+  if ((rc = default_ctx(&ctx)) || (rc = ctx_activate(ctx))) return rc;
+  const int64_t annz = A->indices[1][0][k], bnnz = B->indices[1][0][k];
+  const int32_t *apos, *acrd, *bpos, *bcrd;
+  const double *aval, *bval;
+  if ((rc = stage_in(ctx, 0, A->indices[1][0], (int64_t)k + 1, &apos)) ||
+      (rc = stage_in(ctx, 1, A->indices[1][1], annz, &acrd)) || (rc = stage_in(ctx, 2, A->vals, annz, &aval)) ||
+      (rc = stage_in(ctx, 3, B->indices[1][0], (int64_t)k + 1, &bpos)) ||
+      (rc = stage_in(ctx, 4, B->indices[1][1], bnnz, &bcrd)) || (rc = stage_in(ctx, 5, B->vals, bnnz, &bval)))
+    return rc;
+  int64_t nnz = 0;
+  int64_t* dpos;
+  int32_t* dcrd;
+  double* dval;
+  rc = spws_spgemm_outer(ctx, n, k, m, k, nullptr, bpos, bcrd, bval, k, nullptr, apos, acrd, aval, &nnz, &dpos, &dcrd,
+                         &dval);
+  if (rc) return rc;
+  SPWS_REQUIRE(nnz <= INT32_MAX, SPWS_ERR_UNSUPPORTED, "nnz(C) = %lld does not fit the int32 CSR of taco_tensor_t",
+               (long long)nnz);
+  int64_t* p64 = nullptr;
+  int32_t* hc = nullptr;
+  double* hv = nullptr;
+  if ((rc = d2h_alloc(ctx, dpos, (int64_t)n + 1, &p64))) return rc;
+  int32_t* hp = host_alloc<int32_t>((int64_t)n + 1);
+  if (!hp) {
+    free(p64);
+    set_error("host allocation failed");
+    return SPWS_ERR_OOM;
+  }
+  for (int32_t i = 0; i <= n; ++i) hp[i] = (int32_t)p64[i];
+  free(p64);
+  if ((rc = d2h_alloc(ctx, dcrd, nnz, &hc)) || (rc = d2h_alloc(ctx, dval, nnz, &hv))) {
+    free(hp);
+    free(hc);
+    return rc;
+  }
+  C->indices[1][0] = hp;
+  C->indices[1][1] = hc;
+  C->vals = hv;
+  C->vals_size = (int32_t)nnz;
+  return SPWS_OK;
+}
+
 int spws_mttkrp_hash_compute(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C, taco_tensor_t* D, int32_t w_cap) {
   (void)w_cap;
   SPWS_REQUIRE(A && A->dimensions && A->indices && A->indices[1] && A->indices[2], SPWS_ERR_INVALID,

@@ -78,6 +78,19 @@ def DC_to_taco_tensor(a: CSR) -> Taco:
     return t
 
 
+def DC_to_taco_tensor_csc(a: CSR) -> Taco:
+    """CSC taco tensor of the matrix whose transpose is ``a`` (``a`` holds the
+    columns as rows): dimensions {a.ncols, a.nrows}, mode_ordering {1, 0},
+    indices[1] = (column pointers, row indices)."""
+    t = Taco(2, [a.ncols, a.nrows])
+    t._ord[:] = [1, 0]
+    t.set_index(0, 0, np.array([a.nrows], dtype=np.int32))
+    t.set_index(1, 0, a.pos)
+    t.set_index(1, 1, a.crd)
+    t.set_vals(a.vals)
+    return t
+
+
 def init_taco_tensor_DC(nrow: int, ncol: int) -> Taco:
     t = Taco(2, [nrow, ncol])
     t.set_index(0, 0, np.array([nrow], dtype=np.int32))
@@ -194,6 +207,13 @@ def taco_coo(t: Taco) -> tuple:
 def dcsc_dcsr_compute(Ct: Taco, At: Taco, Bt: Taco, w_cap: int) -> int:
     """Outer-product coordinate workspace (DCSC_DCSR_coord_index_chase.h:194)."""
     check(lib().spws_dcsc_dcsr_coord_compute(Ct.ref, At.ref, Bt.ref, int(w_cap)))
+    return 0
+
+
+def csc_csr_t_compute(Ct: Taco, At: Taco, Bt: Taco, w_cap: int) -> int:
+    """Transposed-output product C = (A B)^T, A CSC, B CSR
+    (CSC_CSR_T_hash.h:232)."""
+    check(lib().spws_csc_csr_t_hash_compute(Ct.ref, At.ref, Bt.ref, int(w_cap)))
     return 0
 
 

@@ -233,3 +233,33 @@ def test_outer_workload_full(ctx):
         np.testing.assert_array_equal(val, G.vals[sl])
         assert oracle.max_value_error(val, exact[sl], absum[sl]) <= TOL
     print(f"outer full: nnz(C) = {C.nnz}, products = {st['products']}, chunks = {st['chunks']}", flush=True)
+
+
+@pytest.mark.parametrize("w_cap", [1 << 10, 1 << 16])
+def test_csc_csr_transposed_output(w_cap):
+    """C = (A B)^T through the reference's CSC x CSR transposed-output entry
+    (CSC_CSR_T_hash.h:232): the entries of row k of C are summed over j in
+    increasing order, so C equals the oracle's Gustavson product B^T A^T bit
+    for bit; it also equals scipy's (A B)^T within the tolerance."""
+    print(f"case: CSC x CSR transposed output w_cap={w_cap}", flush=True)
+    import scipy.sparse as sp
+    from sparsews_amd import taco
+    A = powerlaw_rows(300, 250, 1.8, 120, 72, 0)   # m x k
+    B = powerlaw_rows(250, 400, 1.8, 150, 72, 1)   # k x n
+    At = taco.DC_to_taco_tensor_csc(A.transpose())
+    Bt = taco.DC_to_taco_tensor(B)
+    Ct = taco.init_taco_tensor_DC(B.ncols, A.nrows)
+    taco.csc_csr_t_compute(Ct, At, Bt, w_cap)
+    CT = taco.taco_to_csr(Ct)
+    taco.free_outputs(Ct)
+    G = oracle.spgemm_gustavson(B.transpose(), A.transpose())
+    np.testing.assert_array_equal(np.asarray(CT.pos, np.int64), np.asarray(G.pos, np.int64))
+    np.testing.assert_array_equal(CT.crd, G.crd)
+    np.testing.assert_array_equal(CT.vals, G.vals)
+    sa = sp.csr_matrix((A.vals, A.crd, A.pos), shape=(A.nrows, A.ncols))
+    sb = sp.csr_matrix((B.vals, B.crd, B.pos), shape=(B.nrows, B.ncols))
+    ref = (sa @ sb).T.toarray()
+    rows = np.repeat(np.arange(CT.nrows), np.diff(np.asarray(CT.pos, np.int64)))
+    dense = np.zeros((CT.nrows, CT.ncols))
+    dense[rows, CT.crd] = CT.vals
+    np.testing.assert_allclose(dense, ref, rtol=1e-12, atol=1e-12)
