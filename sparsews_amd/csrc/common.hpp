@@ -74,7 +74,6 @@ struct KernelProfiler {
 
 struct spws_ctx {
   int device = 0;
-  int num_cus = 256;  // compute units of the device
   hipStream_t stream = nullptr;
   bool own_stream = false;
   // named scratch and output buffers (see the users for their layouts)

@@ -271,9 +271,6 @@ int spws_ctx_create(int device, void* stream, spws_ctx** out) {
   SPWS_HIP(hipSetDevice(device));
   spws_ctx* ctx = new spws_ctx();
   ctx->device = device;
-  int ncu = 0;
-  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
-    ctx->num_cus = ncu;
   if (stream) {
     ctx->stream = static_cast<hipStream_t>(stream);
   } else {

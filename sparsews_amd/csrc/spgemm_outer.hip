@@ -38,13 +38,18 @@
 //   k_bk_rowsize      bucket sizes, one wavefront per row, lanes over windows.
 //   scan              bucket starts (the digit's prefix sum).
 //   k_bk_etab         per entry and window: where its products start.
-//   k_bk_expand       each product goes straight to its bucket slot; inside a
-//                     bucket the products are in (j, k) order.
+//   k_bk_expand_lds   each product goes straight to its bucket slot (the
+//                     entry's window table in LDS; k_bk_expand reads it from
+//                     memory when a row has more than 1024 windows); inside
+//                     a bucket the products are in (j, k) order.
 //   k_bk_count        distinct columns per bucket (LDS bitmap).
 //   scan              output offsets of the buckets.
 //   k_bk_reduce       one wavefront per bucket: products accumulate in a dense
 //                     LDS window, entries with equal k in position (= j)
 //                     order; the window's columns are emitted in order.
+// The reduce of a chunk runs on the context's second stream while the main
+// stream prepares the next chunk (entries, bucket sizes and starts, window
+// table); the next chunk's expand waits for that reduce.
 #include <algorithm>
 #include <climits>
 
