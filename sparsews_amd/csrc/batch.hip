@@ -239,10 +239,12 @@ static void enqueue_batch(spws_batch* b) {
   if (c.max_cols > 0 && c.max_cols <= kBitmapCols) {
     // every request is at most max_cols wide: LDS bitmap over each row's own column range
     const int64_t v1 = std::min<int64_t>(b->pp.block_ub, c.max_cols), v2 = std::min<int64_t>(b->pp.block2_ub, c.max_cols);
-    hipLaunchKernelGGL((k_row_bitmap<true>), dim3(gblk), dim3(kBitmapBlock), bitmap_lds_bytes(c.max_cols, v1, true), s,
+    hipLaunchKernelGGL((k_row_bitmap_rows<kBitmapRowsK, kBitmapItems1>), dim3(gblk), dim3(kBitmapBlock),
+                       bitmap_lds_bytes(c.max_cols, v1, true, kBitmapBlock), s,
                        bin_rows, bmeta, (int)P_BLOCK, c.max_cols, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz,
                        coloff);
-    hipLaunchKernelGGL((k_row_bitmap<true>), dim3(gblk), dim3(kBitmapBlock), bitmap_lds_bytes(c.max_cols, v2, true), s,
+    hipLaunchKernelGGL((k_row_bitmap_rows<kBitmapRowsK, kBitmapItems2>), dim3(gblk), dim3(kBitmapBlock),
+                       bitmap_lds_bytes(c.max_cols, v2, true, kBitmapBlock), s,
                        bin_rows, bmeta, (int)P_BLOCK2, c.max_cols, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval,
                        row_nnz, coloff);
   } else {
@@ -252,9 +254,9 @@ static void enqueue_batch(spws_batch* b) {
                        (int)P_BLOCK2, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz);
   }
   if (c.max_cols > 0 && c.max_cols <= kBitmapCols) {
-    hipLaunchKernelGGL((k_row_bitmap<true, true>), dim3(gblk), dim3(kBitmapBlock), bitmap_lds_bytes(c.max_cols, 0, false),
-                       s, bin_rows, bmeta, (int)P_WIDE, c.max_cols, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval,
-                       row_nnz, coloff);
+    hipLaunchKernelGGL((k_row_bitmap<true, true, kBitmapLongBlock>), dim3(gblk), dim3(kBitmapLongBlock),
+                       bitmap_lds_bytes(c.max_cols, 0, false, kBitmapLongBlock), s, bin_rows, bmeta, (int)P_WIDE,
+                       c.max_cols, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz, coloff);
   } else {
     hipLaunchKernelGGL((k_row_wide<true>), dim3(b->wide_groups), dim3(kWideBlock), 0, s, bin_rows, bmeta,
                        (int)P_WIDE, Ap, Ai, Ax, Bp, Bi, Bx, row_ub, row_off, scrd, sval, row_nnz,

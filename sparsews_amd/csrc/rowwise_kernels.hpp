@@ -157,9 +157,10 @@ __global__ __launch_bounds__(256) void k_binscatter(int32_t m_host, const int32_
 // loads one A entry (j, a) and len(B_j); an inclusive wave scan gives the
 // product offsets, and each lane then finds, for its product index q, the
 // owning A entry by a binary search over the lanes' offsets (six
-// ds_bpermute rounds).  Every wavefront of the workgroup scans the same
-// chunk of A entries and takes its own strides of the chunk's products, so
-// rows with few A entries but long B rows still use all wavefronts.  Loop
+// ds_bpermute rounds).  Rows with fewer than two chunks of A entries per
+// wavefront: every wavefront scans the same chunk and takes its own strides
+// of the chunk's products, so rows with few A entries but long B rows still
+// use all wavefronts.  Longer rows: each wavefront takes whole chunks.  Loop
 // bounds are wave-uniform, so the shuffles see every lane.
 
 template <bool kValues, class F>
@@ -171,7 +172,17 @@ __device__ __forceinline__ void expand_products(int32_t a_beg, int32_t a_end, in
   // so their B loads are in flight together
   constexpr int kU = 4;
   const int lane = lane_id();
-  for (int32_t c = a_beg; c < a_end; c += kWave) {
+  // rows of at least two chunks per wavefront: each wavefront takes whole
+  // chunks of its own (chunk widx, widx + nwaves, ...), so the chunks' load
+  // chains run side by side instead of one after another
+  const bool own_chunks = a_end - a_beg >= 2 * kWave * nwaves;
+  const int32_t c_first = own_chunks ? a_beg + widx * kWave : a_beg;
+  const int32_t c_step = own_chunks ? nwaves * kWave : kWave;
+  if (own_chunks) {
+    widx = 0;
+    nwaves = 1;
+  }
+  for (int32_t c = c_first; c < a_end; c += c_step) {
     const int32_t p = c + lane;
     int32_t bstart = 0, blen = 0;
     double a = 0.0;
@@ -217,6 +228,87 @@ __device__ __forceinline__ void expand_products(int32_t a_beg, int32_t a_end, in
       for (int u = 0; u < kU; ++u) {
         if (u >= nu) break;
         if (q0 + u * kWave + lane < total) insert(col[u], kValues ? oa[u] * bv[u] : 0.0);
+      }
+    }
+  }
+}
+
+// The products of K rows of at most 64 A entries each, gathered into
+// registers level by level across the rows (A entries of every row, then
+// their B row bounds, then the B entries), so the K load chains overlap.
+// Product g of row k (in the order of expand_products) belongs to thread
+// g % BLOCK as item g / BLOCK; rows with take[k] false are skipped
+// (workgroup-uniform), and the caller guarantees that a taken row has at
+// most BLOCK * ITEMS products.  Unused items hold col = -1.
+template <int BLOCK, int K, int ITEMS>
+__device__ __forceinline__ void gather_rows_products(const bool (&take)[K], const int32_t (&a_beg)[K],
+                                                     const int32_t (&a_end)[K], const int32_t* __restrict__ Ai,
+                                                     const double* __restrict__ Ax, const int32_t* __restrict__ Bp,
+                                                     const int32_t* __restrict__ Bi, const double* __restrict__ Bx,
+                                                     int32_t (&col)[K][ITEMS], double (&val)[K][ITEMS]) {
+  const int lane = lane_id();
+  const int32_t wbase = (int32_t)threadIdx.x - lane;
+  int32_t j[K], bstart[K], blen[K];
+  double a[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    j[k] = -1;
+    a[k] = 0.0;
+    if (take[k] && a_beg[k] + lane < a_end[k]) {
+      j[k] = Ai[a_beg[k] + lane];
+      a[k] = Ax[a_beg[k] + lane];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    bstart[k] = 0;
+    blen[k] = 0;
+    if (j[k] >= 0) {
+      bstart[k] = Bp[j[k]];
+      blen[k] = Bp[j[k] + 1] - bstart[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+#pragma unroll
+    for (int u = 0; u < ITEMS; ++u) {
+      col[k][u] = -1;
+      val[k][u] = 0.0;
+    }
+    if (!take[k]) continue;
+    const int32_t incl = wave_incl_scan(blen[k]);
+    const int32_t excl = incl - blen[k];
+    const int32_t total = __shfl(incl, kWave - 1, kWave);
+    int32_t kb[ITEMS];
+    double oa[ITEMS];
+#pragma unroll
+    for (int u = 0; u < ITEMS; ++u) {
+      kb[u] = -1;
+      oa[u] = 0.0;
+      // this wave's item-u products are [g0, g0 + 64); skipped wave-uniformly
+      // when none of them exists
+      const int32_t g0 = u * BLOCK + wbase;
+      if (g0 >= total) continue;
+      const int32_t q = min(g0 + lane, total - 1);
+      int lo = 0, hi = kWave - 1;
+#pragma unroll
+      for (int it = 0; it < 6; ++it) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int32_t em = __shfl(excl, mid, kWave);
+        if (em <= q) lo = mid; else hi = mid - 1;
+      }
+      const int32_t kk = __shfl(bstart[k], lo, kWave) + (q - __shfl(excl, lo, kWave));
+      const double ao = __shfl(a[k], lo, kWave);
+      if (g0 + lane < total) {
+        kb[u] = kk;
+        oa[u] = ao;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < ITEMS; ++u) {
+      if (kb[u] >= 0) {
+        col[k][u] = Bi[kb[u]];
+        val[k][u] = oa[u] * Bx[kb[u]];
       }
     }
   }
@@ -409,7 +501,7 @@ __global__ __launch_bounds__(kCfBlock) void k_compact_fixed(int32_t m, const int
                                                             int32_t* __restrict__ C_pos, int32_t* __restrict__ C_crd,
                                                             double* __restrict__ C_val) {
   __shared__ int32_t sh[kCfBlock / kWave + 1];
-  __shared__ int32_t pre[kCfRows];
+  __shared__ int32_t pre[kCfRows], cnt[kCfRows];
   __shared__ int32_t s_base, s_total;
   const int lane = lane_id();
   const int w = threadIdx.x / kWave;
@@ -423,20 +515,35 @@ __global__ __launch_bounds__(kCfBlock) void k_compact_fixed(int32_t m, const int
     const int32_t c = (lane < kCfRows && r0 + lane < m) ? row_nnz[r0 + lane] : 0;
     const int32_t incl = wave_incl_scan(c);
     pre[lane] = incl - c;
+    cnt[lane] = c;
     if (lane == kWave - 1) s_total = incl;
     if (lane == 0) s_base = tot;
   }
   __syncthreads();
   const int32_t base = s_base;
-  for (int q = w; q < kCfRows; q += kCfBlock / kWave) {
+  // every row of this wavefront is loaded before any is stored
+  constexpr int kPer = kCfRows / (kCfBlock / kWave);
+  int32_t cc[kPer];
+  double vv[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int q = w + u * (kCfBlock / kWave);
+    const int32_t r = r0 + q;
+    if (kValues && r < m && lane < cnt[q]) {
+      cc[u] = slot_crd[(int64_t)r * kWave + lane];
+      vv[u] = slot_val[(int64_t)r * kWave + lane];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int q = w + u * (kCfBlock / kWave);
     const int32_t r = r0 + q;
     if (r >= m) break;
     const int32_t at = base + pre[q];
-    const int32_t n = row_nnz[r];
     if (lane == 0) C_pos[r] = at;
-    if (kValues && lane < n) {
-      C_crd[at + lane] = slot_crd[(int64_t)r * kWave + lane];
-      C_val[at + lane] = slot_val[(int64_t)r * kWave + lane];
+    if (kValues && lane < cnt[q]) {
+      C_crd[at + lane] = cc[u];
+      C_val[at + lane] = vv[u];
     }
   }
   if (threadIdx.x == 0 && r0 + kCfRows >= m) C_pos[m] = base + s_total;
@@ -519,89 +626,228 @@ __global__ __launch_bounds__(BLOCK) void k_row_block(const int32_t* __restrict__
 // bitmap covers that range.
 
 constexpr int kBitmapBlock = 256;
+constexpr int kBitmapLongBlock = 1024;  // rows beyond the block bins: 16 wavefronts per row
 constexpr int32_t kBitmapCols = 1 << 17;
 
-__host__ __device__ constexpr size_t bitmap_lds_bytes(int32_t ncols, int64_t vcap, bool values) {
-  return (size_t)((((ncols + 31) >> 5) + 1) & ~1) * 8 + (values ? (size_t)vcap * 8 : 0);
+// Each thread owns 2^s consecutive bitmap words (the least s with
+// BLOCK * 2^s >= words).  Word w is stored at w + (w >> s) when s > 0: one
+// pad word after every thread's range, so the threads' ranges start in
+// distinct LDS banks and the passes over owned words are conflict-free.
+__host__ __device__ constexpr int bitmap_own_log2(int32_t words, int block) {
+  int s = 0;
+  while (((int64_t)block << s) < words) ++s;
+  return s;
 }
 
-template <bool kValues, bool kGlobal = false>
-__global__ __launch_bounds__(kBitmapBlock) void k_row_bitmap(const int32_t* __restrict__ bin_rows,
-                                                             const int32_t* __restrict__ bmeta, int path,
-                                                             int32_t ncols, const int32_t* __restrict__ Ap,
-                                                             const int32_t* __restrict__ Ai,
-                                                             const double* __restrict__ Ax,
-                                                             const int32_t* __restrict__ Bp,
-                                                             const int32_t* __restrict__ Bi,
-                                                             const double* __restrict__ Bx,
-                                                             const int64_t* __restrict__ row_off,
-                                                             int32_t* __restrict__ slot_crd,
-                                                             double* __restrict__ slot_val,
-                                                             int32_t* __restrict__ row_nnz,
-                                                             const int32_t* __restrict__ row_colbase) {
+__host__ __device__ constexpr int32_t bitmap_padded_words(int32_t ncols, int block) {
+  const int32_t words = (ncols + 31) >> 5;
+  const int s = bitmap_own_log2(words, block);
+  const int32_t padded = s ? words + (words >> s) + 1 : words;
+  return (padded + 1) & ~1;
+}
+
+// bitmap and rank prefix (padded words each), then vcap FP64 accumulators
+__host__ __device__ constexpr size_t bitmap_lds_bytes(int32_t ncols, int64_t vcap, bool values, int block) {
+  return (size_t)bitmap_padded_words(ncols, block) * 8 + (values ? (size_t)vcap * 8 : 0);
+}
+
+// LDS layout of the bitmap kernels: bitmap and rank prefix (padded words
+// each), then the FP64 row accumulator.
+struct BitmapLds {
+  int32_t words, padded, pshift, w0, w1;
+  uint32_t *bm, *pre;
+  double* acc;
+  template <int BLOCK>
+  __device__ __forceinline__ void init(double* base, int32_t ncols) {
+    words = (ncols + 31) >> 5;
+    const int s = bitmap_own_log2(words, BLOCK);
+    padded = bitmap_padded_words(ncols, BLOCK);
+    pshift = s ? s : 31;
+    bm = reinterpret_cast<uint32_t*>(base);
+    pre = bm + padded;
+    acc = reinterpret_cast<double*>(pre + padded);
+    w0 = min((int32_t)threadIdx.x << s, words);
+    w1 = min(w0 + (1 << s), words);
+    // the bitmap is zero between rows: the emit pass clears the words it reads
+    for (int32_t w = threadIdx.x; w < padded; w += BLOCK) bm[w] = 0u;
+    __syncthreads();
+  }
+  __device__ __forceinline__ int32_t at(int32_t w) const { return w + (w >> pshift); }
+};
+
+// One row through the column bitmap (zero on entry and on return).
+// pass1(f) and pass2(f) call f(col, v) once for every product of the row;
+// pass 1 ignores v.  The sorted column list is staged in the rank-prefix
+// words (dead after pass 2) and stored coalesced when it fits there, else
+// each thread stores its own columns.  Ends with a barrier, so the next row
+// may reuse the LDS at once.
+template <int BLOCK, bool kValues, bool kGlobal, class P1, class P2>
+__device__ __forceinline__ void bitmap_row(int32_t i, int32_t cb, int64_t off, const BitmapLds& L, int32_t* sh,
+                                           int32_t* __restrict__ slot_crd, double* __restrict__ slot_val,
+                                           int32_t* __restrict__ row_nnz, P1&& pass1, P2&& pass2) {
+  pass1([&](int32_t col, double) {
+    const int32_t c = col - cb;
+    atomicOr(&L.bm[L.at(c >> 5)], 1u << (c & 31));
+  });
+  __syncthreads();
+  int32_t c = 0;
+  for (int32_t w = L.w0; w < L.w1; ++w) c += __popc(L.bm[L.at(w)]);
+  int32_t nnz;
+  int32_t run = block_excl_scan<int32_t, BLOCK>(c, sh, &nnz);
+  const int32_t first_rank = run;
+  if (!kValues) {
+    for (int32_t w = L.w0; w < L.w1; ++w) L.bm[L.at(w)] = 0u;
+    if (threadIdx.x == 0) row_nnz[i] = nnz;
+    __syncthreads();
+    return;
+  }
+  for (int32_t w = L.w0; w < L.w1; ++w) {
+    L.pre[L.at(w)] = run;
+    run += __popc(L.bm[L.at(w)]);
+  }
+  double* dst = kGlobal ? slot_val + off : L.acc;
+  for (int32_t e = threadIdx.x; e < nnz; e += BLOCK) dst[e] = 0.0;
+  __syncthreads();
+  pass2([&](int32_t col, double v) {
+    const int32_t cc = col - cb;
+    const int32_t wa = L.at(cc >> 5);
+    const int32_t rank = L.pre[wa] + __popc(L.bm[wa] & ((1u << (cc & 31)) - 1u));
+    atomicAdd(&dst[rank], v);
+  });
+  __syncthreads();
+  const bool staged = nnz <= L.padded;  // workgroup-uniform
+  int32_t* list = reinterpret_cast<int32_t*>(L.pre);
+  int32_t rk = first_rank;
+  for (int32_t w = L.w0; w < L.w1; ++w) {
+    const int32_t wa = L.at(w);
+    uint32_t bits = L.bm[wa];
+    L.bm[wa] = 0u;
+    while (bits) {
+      const int32_t col = cb + w * 32 + __builtin_ctz(bits);
+      if (staged) list[rk] = col;
+      else slot_crd[off + rk] = col;
+      ++rk;
+      bits &= bits - 1;
+    }
+  }
+  if (staged) __syncthreads();
+  for (int32_t e = threadIdx.x; e < nnz; e += BLOCK) {
+    if (staged) slot_crd[off + e] = list[e];
+    if (!kGlobal) slot_val[off + e] = L.acc[e];
+  }
+  if (threadIdx.x == 0) row_nnz[i] = nnz;
+  __syncthreads();
+}
+
+template <bool kValues, bool kGlobal = false, int BLOCK = kBitmapBlock>
+__global__ __launch_bounds__(BLOCK) void k_row_bitmap(const int32_t* __restrict__ bin_rows,
+                                                      const int32_t* __restrict__ bmeta, int path, int32_t ncols,
+                                                      const int32_t* __restrict__ Ap, const int32_t* __restrict__ Ai,
+                                                      const double* __restrict__ Ax, const int32_t* __restrict__ Bp,
+                                                      const int32_t* __restrict__ Bi, const double* __restrict__ Bx,
+                                                      const int64_t* __restrict__ row_off,
+                                                      int32_t* __restrict__ slot_crd, double* __restrict__ slot_val,
+                                                      int32_t* __restrict__ row_nnz,
+                                                      const int32_t* __restrict__ row_colbase) {
   // declared as double so the dynamic region (which follows the static
   // arrays) starts 8-byte aligned: acc is updated with 64-bit LDS atomics
   extern __shared__ double bml_d[];
-  __shared__ int32_t sh[kBitmapBlock / kWave + 1];
-  const int32_t words = (ncols + 31) >> 5;
-  const int32_t wpad = (words + 1) & ~1;
-  uint32_t* bm = reinterpret_cast<uint32_t*>(bml_d);
-  uint32_t* pre = bm + wpad;
-  double* acc = reinterpret_cast<double*>(pre + wpad);
+  __shared__ int32_t sh[BLOCK / kWave + 1];
+  BitmapLds L;
+  L.init<BLOCK>(bml_d, ncols);
   const int wave = threadIdx.x / kWave;
   const int32_t count = bmeta[kMetaCount + path];
   const int32_t* rows = bin_rows + bmeta[kMetaOff + path];
-  // each thread owns a contiguous range of bitmap words
-  const int32_t wpt = (words + kBitmapBlock - 1) / kBitmapBlock;
-  const int32_t w0 = min((int32_t)threadIdx.x * wpt, words), w1 = min(w0 + wpt, words);
   for (int32_t r = blockIdx.x; r < count; r += gridDim.x) {
     const int32_t i = rows[r];
     const int32_t cb = row_colbase ? row_colbase[i] : 0;
-    for (int32_t w = threadIdx.x; w < words; w += kBitmapBlock) bm[w] = 0u;
-    __syncthreads();
-    expand_products<false>(Ap[i], Ap[i + 1], wave, kBitmapBlock / kWave, Ai, Ax, Bp, Bi, Bx, [&](int32_t col, double) {
-      const int32_t c = col - cb;
-      atomicOr(&bm[c >> 5], 1u << (c & 31));
-    });
-    __syncthreads();
-    int32_t c = 0;
-    for (int32_t w = w0; w < w1; ++w) c += __popc(bm[w]);
-    int32_t nnz;
-    int32_t run = block_excl_scan<int32_t, kBitmapBlock>(c, sh, &nnz);
-    const int32_t first_rank = run;
-    for (int32_t w = w0; w < w1; ++w) {
-      pre[w] = run;
-      run += __popc(bm[w]);
-    }
-    if (!kValues) {
-      if (threadIdx.x == 0) row_nnz[i] = nnz;
-      __syncthreads();
-      continue;
-    }
-    const int64_t off = row_off[i];
-    double* dst = kGlobal ? slot_val + off : acc;
-    for (int32_t e = threadIdx.x; e < nnz; e += kBitmapBlock) dst[e] = 0.0;
-    __syncthreads();
-    expand_products<true>(Ap[i], Ap[i + 1], wave, kBitmapBlock / kWave, Ai, Ax, Bp, Bi, Bx,
-                          [&](int32_t col, double v) {
-                            const int32_t c = col - cb;
-                            const int32_t wd = c >> 5;
-                            const int32_t rank = pre[wd] + __popc(bm[wd] & ((1u << (c & 31)) - 1u));
-                            atomicAdd(&dst[rank], v);
-                          });
-    __syncthreads();
-    int32_t rk = first_rank;
-    for (int32_t w = w0; w < w1; ++w) {
-      uint32_t bits = bm[w];
-      while (bits) {
-        slot_crd[off + rk++] = cb + w * 32 + __builtin_ctz(bits);
-        bits &= bits - 1;
+    const int32_t a_beg = Ap[i], a_end = Ap[i + 1];
+    const int64_t off = kValues ? row_off[i] : 0;
+    bitmap_row<BLOCK, kValues, kGlobal>(
+        i, cb, off, L, sh, slot_crd, slot_val, row_nnz,
+        [&](auto&& f) { expand_products<false>(a_beg, a_end, wave, BLOCK / kWave, Ai, Ax, Bp, Bi, Bx, f); },
+        [&](auto&& f) { expand_products<true>(a_beg, a_end, wave, BLOCK / kWave, Ai, Ax, Bp, Bi, Bx, f); });
+  }
+}
+
+// register capacity of the two block bins (products per thread), rows per
+// step, and the largest grid of the numeric block-bin kernel.  The second
+// bin holds rows of up to kBlock2UbMax products but keeps 2048 of them in
+// registers, which keeps occupancy at the LDS limit.  One row per step:
+// two rows per step overlap their load chains but cost more occupancy
+// than they gain.
+constexpr int kBitmapItems1 = kBlockUbMax / kBitmapBlock;
+constexpr int kBitmapItems2 = 8;
+constexpr int kBitmapRowsK = 1;
+constexpr int kBitmapRowsGrid = 1 << 16;
+
+// Numeric block-bin rows, K rows per step of a grid-stride loop: the
+// products of every row with at most 64 A entries and at most
+// kBitmapBlock * ITEMS products are gathered for all K rows at once into
+// registers (their load chains overlap, and pass 2 reads no memory); the
+// rows then go through the bitmap one after another.  Other rows take the
+// two-gather expansion.
+template <int K, int ITEMS>
+__global__ __launch_bounds__(kBitmapBlock) void k_row_bitmap_rows(
+    const int32_t* __restrict__ bin_rows, const int32_t* __restrict__ bmeta, int path, int32_t ncols,
+    const int32_t* __restrict__ Ap, const int32_t* __restrict__ Ai, const double* __restrict__ Ax,
+    const int32_t* __restrict__ Bp, const int32_t* __restrict__ Bi, const double* __restrict__ Bx,
+    const int64_t* __restrict__ row_off, int32_t* __restrict__ slot_crd, double* __restrict__ slot_val,
+    int32_t* __restrict__ row_nnz, const int32_t* __restrict__ row_colbase) {
+  extern __shared__ double bml_d[];
+  __shared__ int32_t sh[kBitmapBlock / kWave + 1];
+  BitmapLds L;
+  L.init<kBitmapBlock>(bml_d, ncols);
+  const int wave = threadIdx.x / kWave;
+  const int32_t count = bmeta[kMetaCount + path];
+  const int32_t* rows = bin_rows + bmeta[kMetaOff + path];
+  for (int32_t r0 = blockIdx.x * K; r0 < count; r0 += gridDim.x * K) {
+    int32_t i[K], cb[K], a_beg[K], a_end[K];
+    int64_t off[K];
+    bool take[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) i[k] = r0 + k < count ? rows[r0 + k] : -1;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      a_beg[k] = a_end[k] = cb[k] = 0;
+      off[k] = 0;
+      take[k] = false;
+      if (i[k] >= 0) {
+        a_beg[k] = Ap[i[k]];
+        a_end[k] = Ap[i[k] + 1];
+        off[k] = row_off[i[k]];
+        // the row's upper bound is its slot-region length
+        take[k] = row_off[i[k] + 1] - off[k] <= (int64_t)kBitmapBlock * ITEMS && a_end[k] - a_beg[k] <= kWave;
+        cb[k] = row_colbase ? row_colbase[i[k]] : 0;
       }
     }
-    if (!kGlobal)
-      for (int32_t e = threadIdx.x; e < nnz; e += kBitmapBlock) slot_val[off + e] = acc[e];
-    if (threadIdx.x == 0) row_nnz[i] = nnz;
-    __syncthreads();
+    int32_t pc[K][ITEMS];
+    double pv[K][ITEMS];
+    gather_rows_products<kBitmapBlock, K, ITEMS>(take, a_beg, a_end, Ai, Ax, Bp, Bi, Bx, pc, pv);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (i[k] < 0) continue;
+      if (take[k]) {
+        bitmap_row<kBitmapBlock, true, false>(
+            i[k], cb[k], off[k], L, sh, slot_crd, slot_val, row_nnz,
+            [&](auto&& f) {
+#pragma unroll
+              for (int u = 0; u < ITEMS; ++u)
+                if (pc[k][u] >= 0) f(pc[k][u], 0.0);
+            },
+            [&](auto&& f) {
+#pragma unroll
+              for (int u = 0; u < ITEMS; ++u)
+                if (pc[k][u] >= 0) f(pc[k][u], pv[k][u]);
+            });
+      } else {
+        const int32_t ab = a_beg[k], ae = a_end[k];
+        bitmap_row<kBitmapBlock, true, false>(
+            i[k], cb[k], off[k], L, sh, slot_crd, slot_val, row_nnz,
+            [&](auto&& f) { expand_products<false>(ab, ae, wave, kBitmapBlock / kWave, Ai, Ax, Bp, Bi, Bx, f); },
+            [&](auto&& f) { expand_products<true>(ab, ae, wave, kBitmapBlock / kWave, Ai, Ax, Bp, Bi, Bx, f); });
+      }
+    }
   }
 }
 
@@ -838,6 +1084,8 @@ __global__ __launch_bounds__(kWideBlock) void k_row_wide(const int32_t* __restri
 
 // ------------------------------------------------------------------ compaction
 
+constexpr int kCompactUnroll = 8;
+
 __global__ __launch_bounds__(256) void k_compact(int32_t m_host, const int32_t* __restrict__ m_dev,
                                                  const int64_t* __restrict__ row_off,
                                                  const int32_t* __restrict__ C_pos,
@@ -853,9 +1101,27 @@ __global__ __launch_bounds__(256) void k_compact(int32_t m_host, const int32_t* 
     const int32_t n = C_pos[i + 1] - dst;
     const int64_t src = row_off[i];
     const int32_t shift = row_coloff ? row_coloff[i] : 0;
-    for (int32_t e = lane; e < n; e += kWave) {
-      C_crd[dst + e] = slot_crd[src + e] - shift;
-      C_val[dst + e] = slot_val[src + e];
+    // kCompactUnroll strides loaded before any is stored: the copy is
+    // bandwidth-bound, not bound by one load's latency per stride
+    for (int32_t e0 = 0; e0 < n; e0 += kWave * kCompactUnroll) {
+      int32_t c[kCompactUnroll];
+      double v[kCompactUnroll];
+#pragma unroll
+      for (int u = 0; u < kCompactUnroll; ++u) {
+        const int32_t e = e0 + u * kWave + lane;
+        if (e < n) {
+          c[u] = slot_crd[src + e];
+          v[u] = slot_val[src + e];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kCompactUnroll; ++u) {
+        const int32_t e = e0 + u * kWave + lane;
+        if (e < n) {
+          C_crd[dst + e] = c[u] - shift;
+          C_val[dst + e] = v[u];
+        }
+      }
     }
   }
 }

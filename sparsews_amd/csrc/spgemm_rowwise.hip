@@ -164,9 +164,20 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
     if (!counts[path]) continue;
     if (n <= kBitmapCols) {
       const int64_t vcap = std::min<int64_t>(maxub[path], n);
-      SPWS_LAUNCH(ctx, "row_bitmap", (k_row_bitmap<kValues>), dim3(grid_for(counts[path], 1 << 16)),
-                  dim3(kBitmapBlock), bitmap_lds_bytes(n, vcap, kValues), rows, bmeta, path, n, Ap, Ai, Ax, Bp, Bi,
-                  Bx, roff, scrd, sval, rnnz, (const int32_t*)nullptr);
+      const unsigned lds = (unsigned)bitmap_lds_bytes(n, vcap, kValues, kBitmapBlock);
+      const unsigned grid_k = grid_for(div_up(counts[path], kBitmapRowsK), kBitmapRowsGrid);
+      if (!kValues)
+        SPWS_LAUNCH(ctx, "row_bitmap", (k_row_bitmap<false>), dim3(grid_for(counts[path], 1 << 16)),
+                    dim3(kBitmapBlock), lds, rows, bmeta, path, n, Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz,
+                    (const int32_t*)nullptr);
+      else if (path == P_BLOCK)
+        SPWS_LAUNCH(ctx, "row_bitmap", (k_row_bitmap_rows<kBitmapRowsK, kBitmapItems1>), dim3(grid_k),
+                    dim3(kBitmapBlock), lds, rows, bmeta, path, n, Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz,
+                    (const int32_t*)nullptr);
+      else
+        SPWS_LAUNCH(ctx, "row_bitmap", (k_row_bitmap_rows<kBitmapRowsK, kBitmapItems2>), dim3(grid_k),
+                    dim3(kBitmapBlock), lds, rows, bmeta, path, n, Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz,
+                    (const int32_t*)nullptr);
     } else if (path == P_BLOCK) {
       SPWS_LAUNCH(ctx, "row_block", (k_row_block<2048, 1024, 256, kValues>), dim3(grid_for(counts[path], 1 << 16)),
                   dim3(256), 0, rows, bmeta, path, Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz);
@@ -181,9 +192,10 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
   if (n <= kBitmapCols) {
     for (int path : {(int)P_DENSE, (int)P_WIDE}) {
       if (!counts[path]) continue;
-      SPWS_LAUNCH(ctx, "row_bitmap_long", (k_row_bitmap<kValues, true>), dim3(grid_for(counts[path], 4096)),
-                  dim3(kBitmapBlock), bitmap_lds_bytes(n, 0, false), rows, bmeta, path, n, Ap, Ai, Ax, Bp, Bi, Bx,
-                  roff, scrd, sval, rnnz, (const int32_t*)nullptr);
+      SPWS_LAUNCH(ctx, "row_bitmap_long", (k_row_bitmap<kValues, true, kBitmapLongBlock>),
+                  dim3(grid_for(counts[path], 4096)), dim3(kBitmapLongBlock),
+                  bitmap_lds_bytes(n, 0, false, kBitmapLongBlock), rows, bmeta, path, n, Ap, Ai, Ax, Bp, Bi, Bx, roff,
+                  scrd, sval, rnnz, (const int32_t*)nullptr);
       counts[path] = 0;
     }
   }

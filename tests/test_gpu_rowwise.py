@@ -146,6 +146,29 @@ def test_block_bins_bitmap_and_hash(ctx, ncols, per_row):
     np.testing.assert_array_equal(spgemm_rowwise_symbolic(ctx, dA, dB), C.pos)
 
 
+@pytest.mark.parametrize("a_len,b_len,ncols,path", [
+    (30, 50, 5000, 3),       # second block bin, 1500 products held in registers
+    (700, 1, 5000, 2),       # block bin, 11 chunks of A entries into registers
+    (1800, 1, 5000, 3),      # second block bin, 29 chunks into registers
+    (40, 60, 5000, 3),       # 2400 products: beyond the register capacity, two gathers
+    (600, 1, 200_000, 2),    # LDS hash kernel, wavefronts on chunks of their own
+    (3000, 3, 100_000, 4),   # long rows: bitmap with HBM accumulation, 16 wavefronts on own chunks
+    (3000, 3, 400_000, 5),   # long rows: HBM hash table, wavefronts on own chunks
+])
+def test_rows_of_many_a_entries(ctx, a_len, b_len, ncols, path):
+    """Rows whose products span many 64-entry chunks of A entries, in every
+    kernel that enumerates them."""
+    print(f"case: many A entries a={a_len} b={b_len} ncols={ncols}", flush=True)
+    m, k = 40, 4000
+    A = random_csr(m, k, np.full(m, a_len), 30 + a_len, 0)
+    B = random_csr(k, ncols, np.full(k, b_len), 30 + a_len, 1)
+    C = run(ctx, A, B)
+    check(C, A, B)
+    assert ctx.stats()["rows_per_path"][path] == m
+    dA, dB = DeviceCSR.upload(ctx, A), DeviceCSR.upload(ctx, B)
+    np.testing.assert_array_equal(spgemm_rowwise_symbolic(ctx, dA, dB), C.pos)
+
+
 @pytest.mark.parametrize("path", [0, 1, 2, 3, 4, 5])
 def test_forced_paths(ctx, path):
     print(f"case: forced path {path}", flush=True)
