@@ -226,7 +226,9 @@ static void enqueue_batch(spws_batch* b) {
   double* Cx = (double*)P(BP_CVAL);
   const unsigned grows = (unsigned)std::min<int64_t>(div_up(c.max_rows, 256), 4096);
   hipLaunchKernelGGL(k_meta_clear, dim3(1), dim3(64), 0, s, bmeta, maxub);
-  hipLaunchKernelGGL(k_rowstat, dim3(grows), dim3(256), 0, s, 0, m_dev, 1, Ap, Ai, Bp, row_ub, bmeta, maxub, b->pp);
+  const unsigned gstat = (unsigned)std::min<int64_t>(div_up(c.max_rows, kRsRows), 16384);
+  hipLaunchKernelGGL(k_rowstat, dim3(gstat), dim3(kRsBlock), 0, s, 0, m_dev, 1, Ap, Ai, Bp, row_ub, bmeta, maxub,
+                     b->pp);
   dscan<int64_t, int64_t>(s, row_ub, m_dev, (int64_t*)P(BP_SCAN), row_off);
   hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(64), 0, s, bmeta);
   hipLaunchKernelGGL(k_binscatter, dim3(grows), dim3(256), 0, s, 0, m_dev, 1, row_ub, bmeta, bin_rows, row_nnz, b->pp);

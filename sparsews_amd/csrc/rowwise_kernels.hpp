@@ -42,68 +42,87 @@ constexpr int kMetaCount = 0, kMetaOff = 8, kMetaCursor = 16, kMetaInts = 24;
 
 __device__ __forceinline__ int32_t dev_or(const int32_t* p, int32_t v) { return p ? *p : v; }
 
-// One thread per row; rows longer than kRowstatShort entries are summed by
-// the whole wavefront, one at a time (A rows range from a few entries to
-// thousands).  Each row's path is counted in an LDS histogram.
-constexpr int32_t kRowstatShort = 32;
+// Tiles of kRsRows consecutive rows, one workgroup of kRsBlock threads per
+// tile (grid-stride).  The tile's A entries are walked as one flat range,
+// kRsUnroll strides of kRsBlock in flight per thread, so rows of any length
+// cost a few memory latencies: each entry's len(B_j) is added to its row's
+// LDS accumulator (the row found by a binary search over the tile's row
+// pointers in LDS; a wavefront whose entries all lie in one row adds once).
+// Each row's path is counted in an LDS histogram.
+constexpr int kRsBlock = 256;
+constexpr int kRsRows = 64;
+constexpr int kRsRowsLog2 = 6;
+constexpr int kRsUnroll = 8;
+static_assert(kRsRows == 1 << kRsRowsLog2 && kRsRows <= kRsBlock, "tile rows");
 
-__global__ __launch_bounds__(256) void k_rowstat(int32_t m_host, const int32_t* __restrict__ m_dev, int32_t ncols,
-                                                 const int32_t* __restrict__ Ap, const int32_t* __restrict__ Ai,
-                                                 const int32_t* __restrict__ Bp, int64_t* __restrict__ row_ub,
-                                                 int32_t* __restrict__ bmeta, unsigned long long* __restrict__ maxub,
-                                                 PathParams pp) {
+__global__ __launch_bounds__(kRsBlock) void k_rowstat(int32_t m_host, const int32_t* __restrict__ m_dev,
+                                                      int32_t ncols, const int32_t* __restrict__ Ap,
+                                                      const int32_t* __restrict__ Ai, const int32_t* __restrict__ Bp,
+                                                      int64_t* __restrict__ row_ub, int32_t* __restrict__ bmeta,
+                                                      unsigned long long* __restrict__ maxub, PathParams pp) {
   __shared__ int cnt[8];
   __shared__ unsigned long long mx[8];
-  if (threadIdx.x < 8) {
-    cnt[threadIdx.x] = 0;
-    mx[threadIdx.x] = 0;
+  __shared__ int32_t sAp[kRsRows + 1];
+  __shared__ unsigned long long sub[kRsRows];
+  const int t = threadIdx.x;
+  if (t < 8) {
+    cnt[t] = 0;
+    mx[t] = 0;
   }
-  __syncthreads();
   const int32_t m = dev_or(m_dev, m_host);
-  const int lane = lane_id();
-  const int32_t stride = gridDim.x * blockDim.x;
   unsigned long long tot = 0;
-  // wave-uniform trip count: the cooperative part needs every lane
-  for (int32_t base = blockIdx.x * blockDim.x; base < m; base += stride) {
-    const int32_t i = base + threadIdx.x;
-    const bool valid = i < m;
-    int32_t b = 0, e = 0;
-    if (valid) {
-      b = Ap[i];
-      e = Ap[i + 1];
-    }
-    const bool is_long = valid && e - b > kRowstatShort;
-    int64_t ub = 0;
-    if (valid && !is_long)
-      for (int32_t p = b; p < e; ++p) {
-        const int32_t j = Ai[p];
-        ub += Bp[j + 1] - Bp[j];
+  for (int32_t r0 = blockIdx.x * kRsRows; r0 < m; r0 += gridDim.x * kRsRows) {
+    const int32_t nr = min(kRsRows, m - r0);
+    __syncthreads();  // the previous tile's readers are done
+    if (t <= nr) sAp[t] = Ap[r0 + t];  // nr + 1 row pointers
+    if (t < kRsRows) sub[t] = 0;
+    __syncthreads();
+    const int32_t a0 = sAp[0], a1 = sAp[nr];
+    for (int32_t p0 = a0; p0 < a1; p0 += kRsBlock * kRsUnroll) {
+      int32_t j[kRsUnroll], len[kRsUnroll];
+#pragma unroll
+      for (int u = 0; u < kRsUnroll; ++u) {
+        const int32_t p = p0 + u * kRsBlock + t;
+        j[u] = p < a1 ? Ai[p] : -1;
       }
-    for (uint64_t lm = __ballot(is_long); lm; lm &= lm - 1) {
-      const int l = __builtin_ctzll(lm);
-      const int32_t lb = __shfl(b, l, kWave), le = __shfl(e, l, kWave);
-      int64_t part = 0;
-      for (int32_t p = lb + lane; p < le; p += kWave) {
-        const int32_t j = Ai[p];
-        part += Bp[j + 1] - Bp[j];
+#pragma unroll
+      for (int u = 0; u < kRsUnroll; ++u) len[u] = j[u] >= 0 ? Bp[j[u] + 1] - Bp[j[u]] : 0;
+#pragma unroll
+      for (int u = 0; u < kRsUnroll; ++u) {
+        if (p0 + u * kRsBlock >= a1) break;  // workgroup-uniform: no entries left
+        // the tile row holding entry p: the last q with sAp[q] <= p (p past
+        // the range maps to the last row and adds 0)
+        const int32_t p = min(p0 + u * kRsBlock + t, a1 - 1);
+        int lo = 0, hi = nr - 1;
+#pragma unroll
+        for (int it = 0; it < kRsRowsLog2; ++it) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (sAp[mid] <= p) lo = mid; else hi = mid - 1;
+        }
+        if (__all(lo == __shfl(lo, 0, kWave))) {
+          const int32_t wsum = wave_sum(len[u]);
+          if (lane_id() == 0 && wsum) atomicAdd(&sub[lo], (unsigned long long)wsum);
+        } else if (len[u]) {
+          atomicAdd(&sub[lo], (unsigned long long)len[u]);
+        }
       }
-      part = wave_sum(part);
-      if (lane == l) ub = part;
     }
-    if (valid) {
-      row_ub[i] = ub;
-      const int path = classify_row(ub, ncols, pp);
+    __syncthreads();
+    if (t < nr) {
+      const unsigned long long ub = sub[t];
+      row_ub[r0 + t] = (int64_t)ub;
+      const int path = classify_row((int64_t)ub, ncols, pp);
       atomicAdd(&cnt[path], 1);
-      atomicMax(&mx[path], (unsigned long long)ub);
-      tot += (unsigned long long)ub;
+      atomicMax(&mx[path], ub);
+      tot += ub;
     }
   }
   tot = wave_sum(tot);
-  if (lane == 0 && tot) atomicAdd(&maxub[8], tot);
+  if (lane_id() == 0 && tot) atomicAdd(&maxub[8], tot);
   __syncthreads();
-  if (threadIdx.x < kPaths && cnt[threadIdx.x]) {
-    atomicAdd(&bmeta[kMetaCount + threadIdx.x], cnt[threadIdx.x]);
-    atomicMax(&maxub[threadIdx.x], mx[threadIdx.x]);
+  if (t < kPaths && cnt[t]) {
+    atomicAdd(&bmeta[kMetaCount + t], cnt[t]);
+    atomicMax(&maxub[t], mx[t]);
   }
 }
 
@@ -125,29 +144,44 @@ __global__ void k_meta_clear(int32_t* __restrict__ bmeta, unsigned long long* __
   if (t < 9) maxub[t] = 0;  // [8]: sum of the upper bounds
 }
 
-__global__ __launch_bounds__(256) void k_binscatter(int32_t m_host, const int32_t* __restrict__ m_dev, int32_t ncols,
-                                                    const int64_t* __restrict__ row_ub, int32_t* __restrict__ bmeta,
-                                                    int32_t* __restrict__ bin_rows, int32_t* __restrict__ row_nnz,
-                                                    PathParams pp) {
+// Row ids grouped by path, a tile of kRsBlock rows per workgroup: ranks
+// within each wavefront from ballots, wavefront offsets within the tile in
+// LDS, and one global cursor atomic per path and tile (issued by separate
+// threads, so a tile waits for one atomic latency).
+__global__ __launch_bounds__(kRsBlock) void k_binscatter(int32_t m_host, const int32_t* __restrict__ m_dev,
+                                                         int32_t ncols, const int64_t* __restrict__ row_ub,
+                                                         int32_t* __restrict__ bmeta, int32_t* __restrict__ bin_rows,
+                                                         int32_t* __restrict__ row_nnz, PathParams pp) {
+  constexpr int NW = kRsBlock / kWave;
+  __shared__ int32_t woff[NW][8];
+  __shared__ int32_t gbase[8];
   const int32_t m = dev_or(m_dev, m_host);
-  const int lane = lane_id();
+  const int t = threadIdx.x, lane = lane_id(), w = t / kWave;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  // wave-uniform trip count: every lane of a wave runs the ballots together
-  const int32_t stride = gridDim.x * blockDim.x;
-  for (int32_t base = blockIdx.x * blockDim.x; base < m; base += stride) {
-    const int32_t i = base + threadIdx.x;
+  for (int32_t base = blockIdx.x * kRsBlock; base < m; base += gridDim.x * kRsBlock) {
+    const int32_t i = base + t;
     const int path = i < m ? classify_row(row_ub[i], ncols, pp) : -1;
     if (i < m && path == P_EMPTY) row_nnz[i] = 0;
+    int rank = 0;
 #pragma unroll
     for (int p = 0; p < P_EMPTY; ++p) {
       const uint64_t mask = __ballot(path == p);
-      if (mask == 0) continue;
-      const int leader = __ffsll((long long)mask) - 1;
-      int pos = 0;
-      if (lane == leader) pos = atomicAdd(&bmeta[kMetaCursor + p], __popcll(mask));
-      pos = __shfl(pos, leader, kWave);
-      if (path == p) bin_rows[bmeta[kMetaOff + p] + pos + __popcll(mask & lt)] = i;
+      if (path == p) rank = __popcll(mask & lt);
+      if (lane == 0) woff[w][p] = __popcll(mask);
     }
+    __syncthreads();
+    if (t < P_EMPTY) {
+      int32_t run = 0;
+      for (int v = 0; v < NW; ++v) {
+        const int32_t c = woff[v][t];
+        woff[v][t] = run;
+        run += c;
+      }
+      gbase[t] = run ? bmeta[kMetaOff + t] + atomicAdd(&bmeta[kMetaCursor + t], run) : 0;
+    }
+    __syncthreads();
+    if (path >= 0 && path < P_EMPTY) bin_rows[gbase[path] + woff[w][path] + rank] = i;
+    __syncthreads();  // woff and gbase are reused by the next tile
   }
 }
 

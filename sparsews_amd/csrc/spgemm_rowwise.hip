@@ -76,7 +76,8 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
   unsigned long long* maxub_d = reinterpret_cast<unsigned long long*>(bmeta + 32);
   SPWS_LAUNCH(ctx, "meta_clear", k_meta_clear, dim3(1), dim3(64), 0, bmeta, maxub_d);
   if (m > 0) {
-    SPWS_LAUNCH(ctx, "rowstat", k_rowstat, dim3((unsigned)std::min<int64_t>(div_up(m, 256), 65536)), dim3(256), 0,
+    SPWS_LAUNCH(ctx, "rowstat", k_rowstat, dim3((unsigned)std::min<int64_t>(div_up(m, kRsRows), 65536)),
+                dim3(kRsBlock), 0,
                 m, nullptr, n, Ap, Ai, Bp,
                 ctx->row_ub.as<int64_t>(), bmeta, maxub_d, pp);
   }
@@ -140,7 +141,7 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
       return rc;
   }
   if (m > 0) {
-    SPWS_LAUNCH(ctx, "binscatter", k_binscatter, dim3((unsigned)div_up(m, 256)), dim3(256), 0, m, nullptr, n,
+    SPWS_LAUNCH(ctx, "binscatter", k_binscatter, dim3((unsigned)div_up(m, kRsBlock)), dim3(kRsBlock), 0, m, nullptr, n,
                 ctx->row_ub.as<int64_t>(), bmeta, ctx->bin_rows.as<int32_t>(), ctx->row_nnz.as<int32_t>(), pp);
   }
   const int32_t* rows = ctx->bin_rows.as<int32_t>();
