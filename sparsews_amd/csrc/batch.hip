@@ -234,8 +234,8 @@ static void enqueue_batch(spws_batch* b) {
   hipLaunchKernelGGL(k_binscatter, dim3(grows), dim3(256), 0, s, 0, m_dev, 1, row_ub, bmeta, bin_rows, row_nnz, b->pp);
   const unsigned gwave = (unsigned)std::min<int64_t>(c.max_rows, 8192);
   const unsigned gblk = (unsigned)std::min<int64_t>(c.max_rows, 1024);
-  hipLaunchKernelGGL((k_row_wave<128, 1, true>), dim3(gwave), dim3(64), 0, s, bin_rows, bmeta, (int)P_WAVE, Ap, Ai, Ax,
-                     Bp, Bi, Bx, row_off, scrd, sval, row_nnz);
+  hipLaunchKernelGGL((k_row_wave<128, 1, true, kWaveRowsK>), dim3(gwave), dim3(64), 0, s, bin_rows, bmeta, (int)P_WAVE,
+                     Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz);
   hipLaunchKernelGGL((k_row_wave<512, 4, true>), dim3(gwave), dim3(64), 0, s, bin_rows, bmeta, (int)P_WAVE2, Ap, Ai,
                      Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz);
   if (c.max_cols > 0 && c.max_cols <= kBitmapCols) {

@@ -441,11 +441,19 @@ __device__ __forceinline__ void block_bitonic_lds(int32_t* list, int P) {
 
 // ------------------------------------------------------------------ wave-per-row kernel
 //
-// One 64-thread workgroup (one wavefront) per row, TS = 2 * max ub slots.
+// One 64-thread workgroup (one wavefront), K rows per step of a grid-stride
+// loop, TS = 2 * max ub slots per row.  The products of rows with at most 64
+// A entries (every row's ub is at most 64 * R by its bin) are gathered for
+// the K rows at once, one load level at a time, so their load chains
+// overlap; a row with more A entries (empty B rows) is expanded on its own.
+
+// rows per step of the first wave bin: two rows per step overlap their
+// load chains but measure slower than one (the per-row LDS work dominates)
+constexpr int kWaveRowsK = 1;
 
 // bin_rows == nullptr: rows 0 .. direct_m - 1 in order, each with a fixed
 // region of kWave slots at r * kWave (every row has ub <= 64).
-template <int TS, int R, bool kValues>
+template <int TS, int R, bool kValues, int K = 1>
 __global__ __launch_bounds__(64) void k_row_wave(const int32_t* __restrict__ bin_rows, const int32_t* __restrict__ bmeta,
                                                  int path,
                                                  const int32_t* __restrict__ Ap, const int32_t* __restrict__ Ai,
@@ -455,62 +463,89 @@ __global__ __launch_bounds__(64) void k_row_wave(const int32_t* __restrict__ bin
                                                  double* __restrict__ slot_val, int32_t* __restrict__ row_nnz,
                                                  int32_t direct_m = 0) {
   static_assert(TS >= 2 * kWave * R / 2, "table too small");
-  __shared__ int32_t keys[TS];
-  __shared__ double vals[kValues ? TS : 1];
-  __shared__ int32_t list[kWave * R];
+  __shared__ int32_t keys[K][TS];
+  __shared__ double vals[K][kValues ? TS : 1];
+  __shared__ int32_t list[K][kWave * R];
   constexpr uint32_t kMask = TS - 1;
   const uint32_t shift = 32 - log2_pow2(TS);
   const int lane = lane_id();
   const bool direct = bin_rows == nullptr;
   const int32_t count = direct ? direct_m : bmeta[kMetaCount + path];
   const int32_t* rows = direct ? nullptr : bin_rows + bmeta[kMetaOff + path];
-  for (int32_t r = blockIdx.x; r < count; r += gridDim.x) {
-    const int32_t i = direct ? r : rows[r];
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int32_t r0 = blockIdx.x * K; r0 < count; r0 += gridDim.x * K) {
+    int32_t i[K], a_beg[K], a_end[K];
+    bool take[K];
 #pragma unroll
-    for (int s = lane; s < TS; s += kWave) {
-      keys[s] = -1;
-      if (kValues) vals[s] = 0.0;
-    }
-    __syncthreads();
-    expand_products<kValues>(Ap[i], Ap[i + 1], 0, 1, Ai, Ax, Bp, Bi, Bx, [&](int32_t col, double v) {
-      lds_insert<kValues>(keys, vals, shift, kMask, col, v);
-    });
-    __syncthreads();
-    // compaction of the occupied slots (order irrelevant: sorted next)
-    int32_t nnz = 0;
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int k = 0; k < K; ++k) i[k] = r0 + k < count ? (direct ? r0 + k : rows[r0 + k]) : -1;
 #pragma unroll
-    for (int s0 = 0; s0 < TS; s0 += kWave) {
-      const int32_t key = keys[s0 + lane];
-      const uint64_t occ = __ballot(key != -1);
-      if (kValues && key != -1) list[nnz + __popcll(occ & lt)] = key;
-      nnz += __popcll(occ);
-    }
-    if (!kValues) {
-      if (lane == 0) row_nnz[i] = nnz;
-      __syncthreads();
-      continue;
-    }
-    __syncthreads();
-    int32_t v[R];
+    for (int k = 0; k < K; ++k) {
+      a_beg[k] = a_end[k] = 0;
+      if (i[k] >= 0) {
+        a_beg[k] = Ap[i[k]];
+        a_end[k] = Ap[i[k] + 1];
+      }
+      take[k] = i[k] >= 0 && a_end[k] - a_beg[k] <= kWave;
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int e = q * kWave + lane;
-      v[q] = e < nnz ? list[e] : INT_MAX;
-    }
-    int n_pow2 = 2;
-    while (n_pow2 < nnz) n_pow2 <<= 1;
-    wave_bitonic<R>(v, n_pow2);
-    const int64_t off = direct ? (int64_t)i * kWave : row_off[i];
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int e = q * kWave + lane;
-      if (e < nnz) {
-        slot_crd[off + e] = v[q];
-        slot_val[off + e] = lds_lookup(keys, vals, shift, kMask, v[q]);
+      for (int s = lane; s < TS; s += kWave) {
+        keys[k][s] = -1;
+        if (kValues) vals[k][s] = 0.0;
       }
     }
-    if (lane == 0) row_nnz[i] = nnz;
+    int32_t pc[K][R];
+    double pv[K][R];
+    gather_rows_products<kWave, K, R>(take, a_beg, a_end, Ai, Ax, Bp, Bi, Bx, pc, pv);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (take[k]) {
+#pragma unroll
+        for (int u = 0; u < R; ++u)
+          if (pc[k][u] >= 0) lds_insert<kValues>(keys[k], vals[k], shift, kMask, pc[k][u], pv[k][u]);
+      } else if (i[k] >= 0) {
+        expand_products<kValues>(a_beg[k], a_end[k], 0, 1, Ai, Ax, Bp, Bi, Bx, [&](int32_t col, double v) {
+          lds_insert<kValues>(keys[k], vals[k], shift, kMask, col, v);
+        });
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (i[k] < 0) continue;
+      // compaction of the occupied slots (order irrelevant: sorted next)
+      int32_t nnz = 0;
+#pragma unroll
+      for (int s0 = 0; s0 < TS; s0 += kWave) {
+        const int32_t key = keys[k][s0 + lane];
+        const uint64_t occ = __ballot(key != -1);
+        if (kValues && key != -1) list[k][nnz + __popcll(occ & lt)] = key;
+        nnz += __popcll(occ);
+      }
+      if (!kValues) {
+        if (lane == 0) row_nnz[i[k]] = nnz;
+        continue;
+      }
+      __syncthreads();
+      int32_t v[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int e = q * kWave + lane;
+        v[q] = e < nnz ? list[k][e] : INT_MAX;
+      }
+      int n_pow2 = 2;
+      while (n_pow2 < nnz) n_pow2 <<= 1;
+      wave_bitonic<R>(v, n_pow2);
+      const int64_t off = direct ? (int64_t)i[k] * kWave : row_off[i[k]];
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int e = q * kWave + lane;
+        if (e < nnz) {
+          slot_crd[off + e] = v[q];
+          slot_val[off + e] = lds_lookup(keys[k], vals[k], shift, kMask, v[q]);
+        }
+      }
+      if (lane == 0) row_nnz[i[k]] = nnz;
+    }
     __syncthreads();
   }
 }

@@ -114,8 +114,8 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
     if ((kValues && ((rc = ctx->slot_crd.ensure(fixed * 4)) || (rc = ctx->slot_val.ensure(fixed * 8)) ||
                      (rc = ctx->out_crd.ensure(cap * 4)) || (rc = ctx->out_val.ensure(cap * 8)))))
       return rc;
-    SPWS_LAUNCH(ctx, "row_wave", (k_row_wave<128, 1, kValues>), dim3((unsigned)std::min<int64_t>(m, 1 << 20)),
-                dim3(64), 0, (const int32_t*)nullptr, bmeta, (int)P_WAVE, Ap, Ai, Ax, Bp, Bi, Bx,
+    SPWS_LAUNCH(ctx, "row_wave", (k_row_wave<128, 1, kValues, kWaveRowsK>),
+                dim3((unsigned)std::min<int64_t>(div_up(m, kWaveRowsK), 1 << 20)), dim3(64), 0, (const int32_t*)nullptr, bmeta, (int)P_WAVE, Ap, Ai, Ax, Bp, Bi, Bx,
                 (const int64_t*)nullptr, kValues ? ctx->slot_crd.as<int32_t>() : nullptr,
                 kValues ? ctx->slot_val.as<double>() : nullptr, ctx->row_nnz.as<int32_t>(), m);
     SPWS_LAUNCH(ctx, "compact_fixed", (k_compact_fixed<kValues>), dim3((unsigned)div_up(m, kCfRows)),
@@ -152,8 +152,8 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
   auto grid_for = [](int64_t cnt, int64_t cap) { return (unsigned)(cnt < cap ? cnt : cap); };
 
   if (counts[P_WAVE]) {
-    SPWS_LAUNCH(ctx, "row_wave", (k_row_wave<128, 1, kValues>), dim3(grid_for(counts[P_WAVE], 1 << 20)), dim3(64), 0,
-                rows, bmeta, (int)P_WAVE, Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz);
+    SPWS_LAUNCH(ctx, "row_wave", (k_row_wave<128, 1, kValues, kWaveRowsK>),
+                dim3(grid_for(div_up(counts[P_WAVE], kWaveRowsK), 1 << 20)), dim3(64), 0, rows, bmeta, (int)P_WAVE, Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz);
   }
   if (counts[P_WAVE2]) {
     SPWS_LAUNCH(ctx, "row_wave", (k_row_wave<512, 4, kValues>), dim3(grid_for(counts[P_WAVE2], 1 << 20)), dim3(64),
