@@ -36,20 +36,33 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def timed_steps(ctx, step, steps, warmup):
+def timed_steps(ctx, step, steps, warmup, pick):
+    """`warmup` untimed steps; one more untimed step with every launch timed
+    (the per-kernel breakdown, from which `pick` names the dominant kernel);
+    then exactly `steps` timed steps in which only the dominant kernel's
+    launches carry HIP events (events on every launch would add their own
+    cost to a short step).  Returns (seconds per timed step, dominant kernel,
+    its (total ms, launches) over the timed steps, breakdown of one step)."""
     for _ in range(warmup):
         step()
     ctx.sync()
     ctx.set_profiling(True)
+    ctx.reset_kernel_times()
+    step()
+    ctx.sync()
+    breakdown = ctx.kernel_times()
+    dom = pick(breakdown)
+    ctx.set_profiling_only(dom)
     ctx.reset_kernel_times()
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
     ctx.sync()
     t1 = time.perf_counter()
-    times = ctx.kernel_times()
+    dom_times = ctx.kernel_times()[dom]
     ctx.set_profiling(False)
-    return (t1 - t0) / steps, times
+    ctx.set_profiling_only(None)
+    return (t1 - t0) / steps, dom, dom_times, breakdown
 
 
 def measured_traffic(workload, kernel):
@@ -80,8 +93,8 @@ def calibration_copy(ctx, nbytes):
     ctx.free(dst)
 
 
-def roofline(times, name, bytes_per_launch, workload):
-    ms, launches = times[name]
+def roofline(dom_times, name, bytes_per_launch, workload):
+    ms, launches = dom_times
     avg_s = ms / launches / 1e3
     achieved = bytes_per_launch / avg_s / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -141,17 +154,19 @@ def run_rowwise(args):
     def step():
         out["C"] = spgemm_rowwise(ctx, dA, dB)
 
-    sec, times = timed_steps(ctx, step, args.steps, args.warmup)
+    # dominant kernel: the numeric kernel of the short-row bin
+    sec, dom, dom_times, bd = timed_steps(
+        ctx, step, args.steps, args.warmup,
+        lambda bd: max((k for k in bd if k.startswith("row_")), key=lambda k: bd[k][0]))
     calibration_copy(ctx, args.calibrate_copy)
     C = out["C"]
     P = ctx.stats()["products"]
     nnz = C.nnz
     m = A.nrows
-    # dominant kernel: the numeric kernel of the short-row bin.  Algorithmic
-    # bytes per launch: A rows (pos 8 B/row + 12 B/entry), B_pos gathers (8 B per
-    # A entry), B entries gathered (12 B per product), slot writes (12 B per
-    # output entry), per-row metadata (row id 4 + offset 8 + nnz 4 B).
-    dom = max((k for k in times if k.startswith("row_")), key=lambda k: times[k][0])
+    # algorithmic bytes per launch of the dominant kernel: A rows (pos 8 B/row
+    # + 12 B/entry), B_pos gathers (8 B per A entry), B entries gathered (12 B
+    # per product), slot writes (12 B per output entry), per-row metadata (row
+    # id 4 + offset 8 + nnz 4 B).
     dom_bytes = m * 8 + A.nnz * 12 + A.nnz * 8 + P * 12 + nnz * 12 + m * 16
     compulsory = (4 * (m + 1) + 12 * A.nnz) + (4 * (B.nrows + 1) + 12 * B.nnz) + (4 * (m + 1) + 12 * nnz)
     res = {
@@ -162,10 +177,10 @@ def run_rowwise(args):
         "dtype": "f64",
         "config": {"workload": "rowwise_hash_spgemm_10k_r8", "m": m, "n": B.ncols, "nnz_per_row": 8,
                    "products": int(P), "nnz_C": int(nnz), "seed": 0},
-        "roofline": roofline(times, dom, dom_bytes, "rowwise"),
+        "roofline": roofline(dom_times, dom, dom_bytes, "rowwise"),
         "compulsory_bytes": int(compulsory),
         "compulsory_frac_of_hbm_roofline": round(compulsory / sec / 1e9 / HBM_PEAK_GBS, 5),
-        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(times.items())},
+        "kernel_ms_per_step": {k: round(v[0], 4) for k, v in sorted(bd.items())},
     }
     if not args.no_rocsparse:
         res["rocsparse"] = rocsparse_leg(ctx, dA, dB, C, P, args.steps, args.warmup)
@@ -212,7 +227,12 @@ def run_outer(args):
     def step():
         out["C"] = spgemm_outer(ctx, dA, dB, N, N, N)
 
-    sec, times = timed_steps(ctx, step, args.steps, args.warmup)
+    def pick(bd):
+        lsd = ("radix_scatter", "radix_hist", "outer_expand", "outer_reduce")
+        names = ("bk_expand", "bk_count", "bk_reduce") if ctx.stats()["outer_method"] == 2 else lsd
+        return max(names, key=lambda k: bd.get(k, (0, 1))[0])
+
+    sec, dom, dom_times, bd = timed_steps(ctx, step, args.steps, args.warmup, pick)
     calibration_copy(ctx, args.calibrate_copy)
     st = ctx.stats()
     P, nnz = st["products"], out["C"].nnz
@@ -227,7 +247,7 @@ def run_outer(args):
     #   bk_expand      10 B written (2 B window column + 8 B value) + 12 B of B gathered per product
     #   bk_count       2 B read per product
     #   bk_reduce      10 B read per product + 12 B written per distinct entry
-    passes = times["radix_scatter"][1] // (args.steps * st["chunks"]) if "radix_scatter" in times else 0
+    passes = bd["radix_scatter"][1] // st["chunks"] if "radix_scatter" in bd else 0
     if st["outer_method"] == 2:
         per_step = {"bk_expand": 22 * P, "bk_count": 2 * P, "bk_reduce": 10 * P + 12 * nnz}
     else:
@@ -237,8 +257,7 @@ def run_outer(args):
             "outer_expand": 24 * P,
             "outer_reduce": 12 * P + 12 * nnz,
         }
-    dom = max(per_step, key=lambda k: times.get(k, (0, 1))[0])
-    ms, launches = times[dom]
+    ms, launches = dom_times
     achieved = per_step[dom] * args.steps / (ms / 1e3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("outer", dom), "kernel": dom,
@@ -260,7 +279,7 @@ def run_outer(args):
         "roofline": roof,
         "compulsory_bytes": int(compulsory),
         "compulsory_frac_of_hbm_roofline": round(compulsory / sec / 1e9 / HBM_PEAK_GBS, 5),
-        "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in sorted(times.items())},
+        "kernel_ms_per_step": {k: round(v[0], 3) for k, v in sorted(bd.items())},
     }
     ctx.close()
     if not args.no_cpu_baseline:
@@ -319,9 +338,8 @@ def outer_cpu_baseline(B, seconds):
 
 # --------------------------------------------------------------------------- tensors
 
-def _dominant(times, per_step, steps, workload):
-    dom = max(per_step, key=lambda k: times.get(k, (0, 1))[0])
-    ms, launches = times[dom]
+def _dominant(dom, dom_times, per_step, steps, workload):
+    ms, launches = dom_times
     achieved = per_step[dom] * steps / (ms / 1e3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(workload, dom), "kernel": dom,
@@ -349,11 +367,13 @@ def run_mttkrp(args):
     def step():
         out["r"] = mttkrp(ctx, dX, I, J, K, R, dB, dC)
 
-    sec, times = timed_steps(ctx, step, args.steps, args.warmup)
+    names = ("mttkrp_accum", "radix_scatter", "mttkrp_keys")
+    sec, dom, dom_times, bd = timed_steps(ctx, step, args.steps, args.warmup,
+                                          lambda bd: max(names, key=lambda k: bd.get(k, (0, 1))[0]))
     calibration_copy(ctx, args.calibrate_copy)
     nnz = X.nnz
     nrows = out["r"][0].count
-    passes = times["radix_scatter"][1] // args.steps if "radix_scatter" in times else 0
+    passes = bd["radix_scatter"][1] if "radix_scatter" in bd else 0
     per_step = {
         # sorted (key, value) records + factor matrices once + the output rows
         "mttkrp_accum": 16 * nnz + 8 * R * (J + K) + 8 * R * nrows,
@@ -366,10 +386,10 @@ def run_mttkrp(args):
         "ms_per_step": round(sec * 1e3, 4), "dtype": "f64",
         "config": {"workload": "mttkrp_mode1_50k_40k_30k_20M_R32", "dims": [I, J, K], "nnz": nnz, "rank": R,
                    "zipf_s": 1.5, "rows_out": int(nrows), "seed": 0},
-        "roofline": _dominant(times, per_step, args.steps, "mttkrp"),
+        "roofline": _dominant(dom, dom_times, per_step, args.steps, "mttkrp"),
         "compulsory_bytes": int(compulsory),
         "compulsory_frac_of_hbm_roofline": round(compulsory / sec / 1e9 / HBM_PEAK_GBS, 5),
-        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(times.items())},
+        "kernel_ms_per_step": {k: round(v[0], 4) for k, v in sorted(bd.items())},
     }
     ctx.close()
     if not args.no_cpu_baseline:
@@ -409,7 +429,9 @@ def run_ttm(args):
     def step():
         out["r"] = ttm(ctx, dX, I, J, K, R, dU)
 
-    sec, times = timed_steps(ctx, step, args.steps, args.warmup)
+    names = ("ttm_accum", "radix_scatter", "ttm_keys")
+    sec, dom, dom_times, bd = timed_steps(ctx, step, args.steps, args.warmup,
+                                          lambda bd: max(names, key=lambda k: bd.get(k, (0, 1))[0]))
     calibration_copy(ctx, args.calibrate_copy)
     nnz = X.nnz
     F = out["r"][0].count
@@ -432,10 +454,10 @@ def run_ttm(args):
         "ms_per_step": round(sec * 1e3, 4), "dtype": "f64",
         "config": {"workload": "ttm_mode3_100k_80k_2k_10M_R16", "dims": [I, J, K], "nnz": nnz, "rank": R,
                    "fibers_out": int(F), "sort_bits": sort_bits, "radix_passes": passes, "seed": 0},
-        "roofline": _dominant(times, per_step, args.steps, "ttm"),
+        "roofline": _dominant(dom, dom_times, per_step, args.steps, "ttm"),
         "compulsory_bytes": int(compulsory),
         "compulsory_frac_of_hbm_roofline": round(compulsory / sec / 1e9 / HBM_PEAK_GBS, 5),
-        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(times.items())},
+        "kernel_ms_per_step": {k: round(v[0], 4) for k, v in sorted(bd.items())},
     }
     ctx.close()
     if not args.no_cpu_baseline:

@@ -78,10 +78,14 @@ int spws_ctx_set_rowwise_tuning(spws_ctx* ctx, const spws_rowwise_tuning* t);
 int spws_ctx_get_rowwise_tuning(spws_ctx* ctx, spws_rowwise_tuning* t);
 
 /* Per-kernel timing with HIP events recorded around every launch the
- * context makes (off by default).  spws_ctx_kernel_times returns the index-th
- * kernel name (alphabetical), its total milliseconds and launch count since
- * the last reset; an index past the end returns SPWS_ERR_INVALID. */
+ * context makes (off by default).  spws_ctx_set_profiling_only limits the
+ * timing to the launches of one name (as spws_ctx_kernel_times reports it;
+ * NULL or "" times every launch), so that a timed loop carries the events of
+ * one kernel only.  spws_ctx_kernel_times returns the index-th kernel name
+ * (alphabetical), its total milliseconds and launch count since the last
+ * reset; an index past the end returns SPWS_ERR_INVALID. */
 int spws_ctx_set_profiling(spws_ctx* ctx, int on);
+int spws_ctx_set_profiling_only(spws_ctx* ctx, const char* name);
 int spws_ctx_reset_kernel_times(spws_ctx* ctx);
 int spws_ctx_kernel_times(spws_ctx* ctx, int index, char* name, size_t name_len, double* total_ms,
                           int64_t* launches);

@@ -88,6 +88,7 @@ SIGNATURES = {
     "spws_ctx_set_rowwise_tuning": (C.c_int, [VP, C.POINTER(RowwiseTuning)]),
     "spws_ctx_get_rowwise_tuning": (C.c_int, [VP, C.POINTER(RowwiseTuning)]),
     "spws_ctx_set_profiling": (C.c_int, [VP, C.c_int]),
+    "spws_ctx_set_profiling_only": (C.c_int, [VP, C.c_char_p]),
     "spws_ctx_reset_kernel_times": (C.c_int, [VP]),
     "spws_ctx_kernel_times": (C.c_int, [VP, C.c_int, C.c_char_p, C.c_size_t, C.POINTER(C.c_double), PI64]),
     "spws_dev_alloc": (C.c_int, [VP, C.c_size_t, C.POINTER(VP)]),

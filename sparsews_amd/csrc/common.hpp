@@ -61,6 +61,7 @@ struct PinnedBuf {
 // Per-kernel HIP-event timing on the context's stream (off by default).
 struct KernelProfiler {
   bool on = false;
+  std::string only;  // when set, only launches of this name are timed
   std::vector<hipEvent_t> pool;
   struct Pending {
     const char* name;
@@ -113,15 +114,16 @@ int ctx_activate(spws_ctx* ctx);
 int ctx_aux_stream(spws_ctx* ctx, hipStream_t* out);
 
 // Kernel timing hooks: prof_begin records a start event (or returns null when
-// profiling is off), prof_end records the stop event; prof_collect turns the
+// profiling is off or the launch is not the one being timed), prof_end
+// records the stop event; prof_collect turns the
 // events recorded so far into totals (call it after the stream is idle).
-hipEvent_t prof_begin(spws_ctx* ctx);
+hipEvent_t prof_begin(spws_ctx* ctx, const char* name);
 void prof_end(spws_ctx* ctx, const char* name, hipEvent_t start);
 void prof_collect(spws_ctx* ctx);
 
 #define SPWS_LAUNCH(ctx, name, kern, grid, block, shmem, ...)                 \
   do {                                                                        \
-    hipEvent_t e0__ = ::spws::prof_begin(ctx);                                \
+    hipEvent_t e0__ = ::spws::prof_begin(ctx, name);                          \
     hipLaunchKernelGGL(kern, grid, block, shmem, (ctx)->stream, __VA_ARGS__); \
     SPWS_CHECK_LAUNCH();                                                      \
     ::spws::prof_end(ctx, name, e0__);                                        \

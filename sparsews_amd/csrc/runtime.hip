@@ -99,8 +99,9 @@ static hipEvent_t prof_event(spws_ctx* ctx) {
   return e;
 }
 
-hipEvent_t prof_begin(spws_ctx* ctx) {
+hipEvent_t prof_begin(spws_ctx* ctx, const char* name) {
   if (!ctx->prof.on) return nullptr;
+  if (!ctx->prof.only.empty() && ctx->prof.only != name) return nullptr;
   hipEvent_t e = prof_event(ctx);
   if (e) (void)hipEventRecord(e, ctx->stream);
   return e;
@@ -414,6 +415,13 @@ int spws_ctx_set_profiling(spws_ctx* ctx, int on) {
   SPWS_REQUIRE(ctx != nullptr, SPWS_ERR_INVALID, "null context");
   prof_collect(ctx);
   ctx->prof.on = on != 0;
+  return SPWS_OK;
+}
+
+int spws_ctx_set_profiling_only(spws_ctx* ctx, const char* name) {
+  SPWS_REQUIRE(ctx != nullptr, SPWS_ERR_INVALID, "null context");
+  prof_collect(ctx);
+  ctx->prof.only = name ? name : "";
   return SPWS_OK;
 }
 

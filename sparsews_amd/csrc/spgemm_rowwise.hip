@@ -81,11 +81,11 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
                 m, nullptr, n, Ap, Ai, Bp,
                 ctx->row_ub.as<int64_t>(), bmeta, maxub_d, pp);
   }
-  // one small read: counts, max ub per path, total products
-  int64_t* hm = ctx->host_meta.as<int64_t>();
-  int32_t* hc = reinterpret_cast<int32_t*>(hm + 24);
-  SPWS_HIP(hipMemcpyAsync(hc, bmeta, 8 * 4, hipMemcpyDeviceToHost, s));
-  SPWS_HIP(hipMemcpyAsync(hm, maxub_d, 9 * 8, hipMemcpyDeviceToHost, s));
+  // one small read of the metadata block: counts (ints [0, 8)), then the
+  // max ub per path and the total products (uint64 from int 32 on)
+  int32_t* hc = ctx->host_meta.as<int32_t>();
+  const int64_t* hm = reinterpret_cast<const int64_t*>(hc + 32);
+  SPWS_HIP(hipMemcpyAsync(hc, bmeta, 32 * 4 + 9 * 8, hipMemcpyDeviceToHost, s));
   SPWS_HIP(hipStreamSynchronize(s));
   int64_t counts[8], maxub[8];
   for (int p = 0; p < 8; ++p) {
@@ -115,14 +115,14 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
                      (rc = ctx->out_crd.ensure(cap * 4)) || (rc = ctx->out_val.ensure(cap * 8)))))
       return rc;
     SPWS_LAUNCH(ctx, "row_wave", (k_row_wave<128, 1, kValues, kWaveRowsK>),
-                dim3((unsigned)std::min<int64_t>(div_up(m, kWaveRowsK), 1 << 20)), dim3(64), 0, (const int32_t*)nullptr, bmeta, (int)P_WAVE, Ap, Ai, Ax, Bp, Bi, Bx,
-                (const int64_t*)nullptr, kValues ? ctx->slot_crd.as<int32_t>() : nullptr,
+                dim3((unsigned)std::min<int64_t>(div_up(m, kWaveRowsK), 1 << 20)), dim3(64), 0,
+                (const int32_t*)nullptr, bmeta, (int)P_WAVE, Ap, Ai, Ax, Bp, Bi, Bx, (const int64_t*)nullptr, kValues ? ctx->slot_crd.as<int32_t>() : nullptr,
                 kValues ? ctx->slot_val.as<double>() : nullptr, ctx->row_nnz.as<int32_t>(), m);
     SPWS_LAUNCH(ctx, "compact_fixed", (k_compact_fixed<kValues>), dim3((unsigned)div_up(m, kCfRows)),
                 dim3(kCfBlock), 0, m, ctx->row_nnz.as<int32_t>(), kValues ? ctx->slot_crd.as<int32_t>() : nullptr,
                 kValues ? ctx->slot_val.as<double>() : nullptr, ctx->out_pos.as<int32_t>(),
                 kValues ? ctx->out_crd.as<int32_t>() : nullptr, kValues ? ctx->out_val.as<double>() : nullptr);
-    int32_t* h32 = reinterpret_cast<int32_t*>(hm + 20);
+    int32_t* h32 = hc + 100;  // past the metadata block
     SPWS_HIP(hipMemcpyAsync(h32, ctx->out_pos.as<int32_t>() + m, 4, hipMemcpyDeviceToHost, s));
     SPWS_HIP(hipStreamSynchronize(s));
     *nnz_out = *h32;
@@ -153,11 +153,12 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
 
   if (counts[P_WAVE]) {
     SPWS_LAUNCH(ctx, "row_wave", (k_row_wave<128, 1, kValues, kWaveRowsK>),
-                dim3(grid_for(div_up(counts[P_WAVE], kWaveRowsK), 1 << 20)), dim3(64), 0, rows, bmeta, (int)P_WAVE, Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz);
+                dim3(grid_for(div_up(counts[P_WAVE], kWaveRowsK), 1 << 20)), dim3(64), 0, rows, bmeta, (int)P_WAVE,
+                Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz, 0);
   }
   if (counts[P_WAVE2]) {
     SPWS_LAUNCH(ctx, "row_wave", (k_row_wave<512, 4, kValues>), dim3(grid_for(counts[P_WAVE2], 1 << 20)), dim3(64),
-                0, rows, bmeta, (int)P_WAVE2, Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz);
+                0, rows, bmeta, (int)P_WAVE2, Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz, 0);
   }
   // block bins: the LDS bitmap kernel while all columns fit the bitmap, else
   // the LDS hash table and bitonic sort
@@ -238,7 +239,7 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
     SPWS_LAUNCH(ctx, "compact", k_compact, dim3(g), dim3(256), 0, m, nullptr, roff, ctx->out_pos.as<int32_t>(), scrd,
                 sval, nullptr, ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>());
   }
-  int32_t* h32 = reinterpret_cast<int32_t*>(hm + 20);
+  int32_t* h32 = hc + 100;  // past the metadata block
   SPWS_HIP(hipMemcpyAsync(h32, ctx->out_pos.as<int32_t>() + m, 4, hipMemcpyDeviceToHost, s));
   SPWS_HIP(hipStreamSynchronize(s));
   *nnz_out = *h32;

@@ -116,6 +116,10 @@ class Context:
     def set_profiling(self, on: bool):
         check(lib().spws_ctx_set_profiling(self.h, 1 if on else 0))
 
+    def set_profiling_only(self, name=None):
+        """Time only the launches called `name` (None: every launch)."""
+        check(lib().spws_ctx_set_profiling_only(self.h, name.encode() if name else None))
+
     def reset_kernel_times(self):
         check(lib().spws_ctx_reset_kernel_times(self.h))
 

@@ -22,6 +22,8 @@ import re
 
 
 def short_name(name: str) -> str:
+    """Kernel name without return type, namespaces and parameter list."""
+    name = name.replace("(anonymous namespace)::", "")
     name = re.sub(r"\(.*\)$", "", name)
     name = name.replace("void ", "").replace("spws::", "")
     return name
