@@ -153,13 +153,15 @@ __global__ __launch_bounds__(kDScanThreads) void k_dscan_apply(const Tin* __rest
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = part[kDScanBlocks];
 }
 
-// operands of every request into the pools, indices rebased
+// operands of every request into the pools, indices rebased; per row, the
+// request's first column (row_coloff[row]) and width (row_coloff[max_rows +
+// row])
 __global__ __launch_bounds__(256) void k_batch_gather(const spws_batch_req* __restrict__ reqs,
                                                       const spws_batch_desc* __restrict__ desc,
                                                       int32_t* __restrict__ A_pos, int32_t* __restrict__ A_crd,
                                                       double* __restrict__ A_val, int32_t* __restrict__ B_pos,
                                                       int32_t* __restrict__ B_crd, double* __restrict__ B_val,
-                                                      int32_t* __restrict__ row_coloff) {
+                                                      int32_t* __restrict__ row_coloff, int32_t max_rows) {
   const int32_t r = blockIdx.y;
   if (r >= desc->nreq) return;
   const spws_batch_req q = reqs[r];
@@ -167,7 +169,10 @@ __global__ __launch_bounds__(256) void k_batch_gather(const spws_batch_req* __re
   const int64_t st = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = t0; i <= q.m; i += st) {
     A_pos[q.row0 + i] = (int32_t)(q.annz0 + q.A_pos[i]);
-    if (i < q.m) row_coloff[q.row0 + i] = q.col0;
+    if (i < q.m) {
+      row_coloff[q.row0 + i] = q.col0;
+      row_coloff[max_rows + q.row0 + i] = q.n;  // the row's request width
+    }
   }
   for (int64_t i = t0; i <= q.k; i += st) B_pos[q.brow0 + i] = (int32_t)(q.bnnz0 + q.B_pos[i]);
   for (int64_t p = t0; p < q.annz; p += st) {
@@ -211,7 +216,8 @@ static void enqueue_batch(spws_batch* b) {
   double* Bx = (double*)P(BP_BVAL);
   int32_t* coloff = (int32_t*)P(BP_COLOFF);
   hipLaunchKernelGGL(k_batch_gather, dim3(64, c.max_requests), dim3(256), 0, s, d_req, d_desc, Ap, Ai, Ax, Bp, Bi, Bx,
-                     coloff);
+                     coloff, (int32_t)c.max_rows);
+  const int32_t* row_ncols = coloff + c.max_rows;
   const int32_t* m_dev = &d_desc->rows;
   int64_t* row_ub = (int64_t*)P(BP_ROWUB);
   int64_t* row_off = (int64_t*)P(BP_ROWOFF);
@@ -244,11 +250,11 @@ static void enqueue_batch(spws_batch* b) {
     hipLaunchKernelGGL((k_row_bitmap_rows<kBitmapRowsK, kBitmapItems1>), dim3(gblk), dim3(kBitmapBlock),
                        bitmap_lds_bytes(c.max_cols, v1, true, kBitmapBlock), s,
                        bin_rows, bmeta, (int)P_BLOCK, c.max_cols, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz,
-                       coloff);
+                       coloff, row_ncols);
     hipLaunchKernelGGL((k_row_bitmap_rows<kBitmapRowsK, kBitmapItems2>), dim3(gblk), dim3(kBitmapBlock),
                        bitmap_lds_bytes(c.max_cols, v2, true, kBitmapBlock), s,
                        bin_rows, bmeta, (int)P_BLOCK2, c.max_cols, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval,
-                       row_nnz, coloff);
+                       row_nnz, coloff, row_ncols);
   } else {
     hipLaunchKernelGGL((k_row_block<2048, 1024, 256, true>), dim3(gblk), dim3(256), 0, s, bin_rows, bmeta,
                        (int)P_BLOCK, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz);
@@ -258,7 +264,7 @@ static void enqueue_batch(spws_batch* b) {
   if (c.max_cols > 0 && c.max_cols <= kBitmapCols) {
     hipLaunchKernelGGL((k_row_bitmap<true, true, kBitmapLongBlock>), dim3(gblk), dim3(kBitmapLongBlock),
                        bitmap_lds_bytes(c.max_cols, 0, false, kBitmapLongBlock), s, bin_rows, bmeta, (int)P_WIDE,
-                       c.max_cols, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz, coloff);
+                       c.max_cols, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz, coloff, row_ncols);
   } else {
     hipLaunchKernelGGL((k_row_wide<true>), dim3(b->wide_groups), dim3(kWideBlock), 0, s, bin_rows, bmeta,
                        (int)P_WIDE, Ap, Ai, Ax, Bp, Bi, Bx, row_ub, row_off, scrd, sval, row_nnz,
@@ -290,7 +296,7 @@ static int batch_build(spws_batch* b) {
   b->wide_groups = 64;
   const size_t sizes[BP_COUNT] = {
       sizeof(spws_batch_req) * (size_t)c.max_requests, sizeof(spws_batch_desc), (rows + 1) * 4, annz * 4, annz * 8,
-      (brows + 1) * 4, bnnz * 4, bnnz * 8, rows * 4, rows * 8, (rows + 1) * 8, rows * 4, rows * 4, 64 * 8, ub * 4,
+      (brows + 1) * 4, bnnz * 4, bnnz * 8, rows * 8, rows * 8, (rows + 1) * 8, rows * 4, rows * 4, 64 * 8, ub * 4,
       ub * 8, (rows + 1) * 4, ub * 4, ub * 8, (size_t)b->wide_groups * b->wide_slots * 4,
       (size_t)b->wide_groups * b->wide_slots * 8, (kDScanBlocks + 1) * 8, 8};
   int rc;

@@ -691,8 +691,8 @@ __global__ __launch_bounds__(BLOCK) void k_row_block(const int32_t* __restrict__
 // rows accumulate in LDS (ds_add_f64, kGlobal = false); longer rows
 // accumulate straight into their slot region in HBM (global FP64 atomics,
 // kGlobal = true), which holds at least nnz entries.  In a stacked batch a
-// row's columns lie in [row_colbase[i], row_colbase[i] + ncols), and the
-// bitmap covers that range.
+// row's columns lie in [row_colbase[i], row_colbase[i] + row_ncols[i]) (its
+// request's width), and the passes over the bitmap cover that range only.
 
 constexpr int kBitmapBlock = 256;
 constexpr int kBitmapLongBlock = 1024;  // rows beyond the block bins: 16 wavefronts per row
@@ -723,7 +723,7 @@ __host__ __device__ constexpr size_t bitmap_lds_bytes(int32_t ncols, int64_t vca
 // LDS layout of the bitmap kernels: bitmap and rank prefix (padded words
 // each), then the FP64 row accumulator.
 struct BitmapLds {
-  int32_t words, padded, pshift, w0, w1;
+  int32_t words, padded, pshift;
   uint32_t *bm, *pre;
   double* acc;
   template <int BLOCK>
@@ -735,13 +735,19 @@ struct BitmapLds {
     bm = reinterpret_cast<uint32_t*>(base);
     pre = bm + padded;
     acc = reinterpret_cast<double*>(pre + padded);
-    w0 = min((int32_t)threadIdx.x << s, words);
-    w1 = min(w0 + (1 << s), words);
     // the bitmap is zero between rows: the emit pass clears the words it reads
     for (int32_t w = threadIdx.x; w < padded; w += BLOCK) bm[w] = 0u;
     __syncthreads();
   }
   __device__ __forceinline__ int32_t at(int32_t w) const { return w + (w >> pshift); }
+  // this thread's words of a row of ncols_row <= ncols columns
+  template <int BLOCK>
+  __device__ __forceinline__ void row_words(int32_t ncols_row, int32_t* rw0, int32_t* rw1) const {
+    const int32_t wr = (ncols_row + 31) >> 5;
+    const int s = bitmap_own_log2(wr, BLOCK);
+    *rw0 = min((int32_t)threadIdx.x << s, wr);
+    *rw1 = min(*rw0 + (1 << s), wr);
+  }
 };
 
 // One row through the column bitmap (zero on entry and on return).
@@ -751,7 +757,8 @@ struct BitmapLds {
 // each thread stores its own columns.  Ends with a barrier, so the next row
 // may reuse the LDS at once.
 template <int BLOCK, bool kValues, bool kGlobal, class P1, class P2>
-__device__ __forceinline__ void bitmap_row(int32_t i, int32_t cb, int64_t off, const BitmapLds& L, int32_t* sh,
+__device__ __forceinline__ void bitmap_row(int32_t i, int32_t cb, int32_t w0, int32_t w1, int64_t off,
+                                           const BitmapLds& L, int32_t* sh,
                                            int32_t* __restrict__ slot_crd, double* __restrict__ slot_val,
                                            int32_t* __restrict__ row_nnz, P1&& pass1, P2&& pass2) {
   pass1([&](int32_t col, double) {
@@ -760,17 +767,17 @@ __device__ __forceinline__ void bitmap_row(int32_t i, int32_t cb, int64_t off, c
   });
   __syncthreads();
   int32_t c = 0;
-  for (int32_t w = L.w0; w < L.w1; ++w) c += __popc(L.bm[L.at(w)]);
+  for (int32_t w = w0; w < w1; ++w) c += __popc(L.bm[L.at(w)]);
   int32_t nnz;
   int32_t run = block_excl_scan<int32_t, BLOCK>(c, sh, &nnz);
   const int32_t first_rank = run;
   if (!kValues) {
-    for (int32_t w = L.w0; w < L.w1; ++w) L.bm[L.at(w)] = 0u;
+    for (int32_t w = w0; w < w1; ++w) L.bm[L.at(w)] = 0u;
     if (threadIdx.x == 0) row_nnz[i] = nnz;
     __syncthreads();
     return;
   }
-  for (int32_t w = L.w0; w < L.w1; ++w) {
+  for (int32_t w = w0; w < w1; ++w) {
     L.pre[L.at(w)] = run;
     run += __popc(L.bm[L.at(w)]);
   }
@@ -787,7 +794,7 @@ __device__ __forceinline__ void bitmap_row(int32_t i, int32_t cb, int64_t off, c
   const bool staged = nnz <= L.padded;  // workgroup-uniform
   int32_t* list = reinterpret_cast<int32_t*>(L.pre);
   int32_t rk = first_rank;
-  for (int32_t w = L.w0; w < L.w1; ++w) {
+  for (int32_t w = w0; w < w1; ++w) {
     const int32_t wa = L.at(w);
     uint32_t bits = L.bm[wa];
     L.bm[wa] = 0u;
@@ -817,7 +824,8 @@ __global__ __launch_bounds__(BLOCK) void k_row_bitmap(const int32_t* __restrict_
                                                       const int64_t* __restrict__ row_off,
                                                       int32_t* __restrict__ slot_crd, double* __restrict__ slot_val,
                                                       int32_t* __restrict__ row_nnz,
-                                                      const int32_t* __restrict__ row_colbase) {
+                                                      const int32_t* __restrict__ row_colbase,
+                                                      const int32_t* __restrict__ row_ncols) {
   // declared as double so the dynamic region (which follows the static
   // arrays) starts 8-byte aligned: acc is updated with 64-bit LDS atomics
   extern __shared__ double bml_d[];
@@ -830,10 +838,12 @@ __global__ __launch_bounds__(BLOCK) void k_row_bitmap(const int32_t* __restrict_
   for (int32_t r = blockIdx.x; r < count; r += gridDim.x) {
     const int32_t i = rows[r];
     const int32_t cb = row_colbase ? row_colbase[i] : 0;
+    int32_t w0, w1;
+    L.row_words<BLOCK>(row_ncols ? row_ncols[i] : ncols, &w0, &w1);
     const int32_t a_beg = Ap[i], a_end = Ap[i + 1];
     const int64_t off = kValues ? row_off[i] : 0;
     bitmap_row<BLOCK, kValues, kGlobal>(
-        i, cb, off, L, sh, slot_crd, slot_val, row_nnz,
+        i, cb, w0, w1, off, L, sh, slot_crd, slot_val, row_nnz,
         [&](auto&& f) { expand_products<false>(a_beg, a_end, wave, BLOCK / kWave, Ai, Ax, Bp, Bi, Bx, f); },
         [&](auto&& f) { expand_products<true>(a_beg, a_end, wave, BLOCK / kWave, Ai, Ax, Bp, Bi, Bx, f); });
   }
@@ -862,7 +872,7 @@ __global__ __launch_bounds__(kBitmapBlock) void k_row_bitmap_rows(
     const int32_t* __restrict__ Ap, const int32_t* __restrict__ Ai, const double* __restrict__ Ax,
     const int32_t* __restrict__ Bp, const int32_t* __restrict__ Bi, const double* __restrict__ Bx,
     const int64_t* __restrict__ row_off, int32_t* __restrict__ slot_crd, double* __restrict__ slot_val,
-    int32_t* __restrict__ row_nnz, const int32_t* __restrict__ row_colbase) {
+    int32_t* __restrict__ row_nnz, const int32_t* __restrict__ row_colbase, const int32_t* __restrict__ row_ncols) {
   extern __shared__ double bml_d[];
   __shared__ int32_t sh[kBitmapBlock / kWave + 1];
   BitmapLds L;
@@ -871,7 +881,7 @@ __global__ __launch_bounds__(kBitmapBlock) void k_row_bitmap_rows(
   const int32_t count = bmeta[kMetaCount + path];
   const int32_t* rows = bin_rows + bmeta[kMetaOff + path];
   for (int32_t r0 = blockIdx.x * K; r0 < count; r0 += gridDim.x * K) {
-    int32_t i[K], cb[K], a_beg[K], a_end[K];
+    int32_t i[K], cb[K], nc[K], a_beg[K], a_end[K];
     int64_t off[K];
     bool take[K];
 #pragma unroll
@@ -879,6 +889,7 @@ __global__ __launch_bounds__(kBitmapBlock) void k_row_bitmap_rows(
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       a_beg[k] = a_end[k] = cb[k] = 0;
+      nc[k] = ncols;
       off[k] = 0;
       take[k] = false;
       if (i[k] >= 0) {
@@ -888,6 +899,7 @@ __global__ __launch_bounds__(kBitmapBlock) void k_row_bitmap_rows(
         // the row's upper bound is its slot-region length
         take[k] = row_off[i[k] + 1] - off[k] <= (int64_t)kBitmapBlock * ITEMS && a_end[k] - a_beg[k] <= kWave;
         cb[k] = row_colbase ? row_colbase[i[k]] : 0;
+        if (row_ncols) nc[k] = row_ncols[i[k]];
       }
     }
     int32_t pc[K][ITEMS];
@@ -896,9 +908,11 @@ __global__ __launch_bounds__(kBitmapBlock) void k_row_bitmap_rows(
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       if (i[k] < 0) continue;
+      int32_t w0, w1;
+      L.row_words<kBitmapBlock>(nc[k], &w0, &w1);
       if (take[k]) {
         bitmap_row<kBitmapBlock, true, false>(
-            i[k], cb[k], off[k], L, sh, slot_crd, slot_val, row_nnz,
+            i[k], cb[k], w0, w1, off[k], L, sh, slot_crd, slot_val, row_nnz,
             [&](auto&& f) {
 #pragma unroll
               for (int u = 0; u < ITEMS; ++u)
@@ -912,7 +926,7 @@ __global__ __launch_bounds__(kBitmapBlock) void k_row_bitmap_rows(
       } else {
         const int32_t ab = a_beg[k], ae = a_end[k];
         bitmap_row<kBitmapBlock, true, false>(
-            i[k], cb[k], off[k], L, sh, slot_crd, slot_val, row_nnz,
+            i[k], cb[k], w0, w1, off[k], L, sh, slot_crd, slot_val, row_nnz,
             [&](auto&& f) { expand_products<false>(ab, ae, wave, kBitmapBlock / kWave, Ai, Ax, Bp, Bi, Bx, f); },
             [&](auto&& f) { expand_products<true>(ab, ae, wave, kBitmapBlock / kWave, Ai, Ax, Bp, Bi, Bx, f); });
       }

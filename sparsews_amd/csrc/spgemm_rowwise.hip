@@ -116,8 +116,9 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
       return rc;
     SPWS_LAUNCH(ctx, "row_wave", (k_row_wave<128, 1, kValues, kWaveRowsK>),
                 dim3((unsigned)std::min<int64_t>(div_up(m, kWaveRowsK), 1 << 20)), dim3(64), 0,
-                (const int32_t*)nullptr, bmeta, (int)P_WAVE, Ap, Ai, Ax, Bp, Bi, Bx, (const int64_t*)nullptr, kValues ? ctx->slot_crd.as<int32_t>() : nullptr,
-                kValues ? ctx->slot_val.as<double>() : nullptr, ctx->row_nnz.as<int32_t>(), m);
+                (const int32_t*)nullptr, bmeta, (int)P_WAVE, Ap, Ai, Ax, Bp, Bi, Bx, (const int64_t*)nullptr,
+                kValues ? ctx->slot_crd.as<int32_t>() : nullptr, kValues ? ctx->slot_val.as<double>() : nullptr,
+                ctx->row_nnz.as<int32_t>(), m);
     SPWS_LAUNCH(ctx, "compact_fixed", (k_compact_fixed<kValues>), dim3((unsigned)div_up(m, kCfRows)),
                 dim3(kCfBlock), 0, m, ctx->row_nnz.as<int32_t>(), kValues ? ctx->slot_crd.as<int32_t>() : nullptr,
                 kValues ? ctx->slot_val.as<double>() : nullptr, ctx->out_pos.as<int32_t>(),
@@ -171,15 +172,15 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
       if (!kValues)
         SPWS_LAUNCH(ctx, "row_bitmap", (k_row_bitmap<false>), dim3(grid_for(counts[path], 1 << 16)),
                     dim3(kBitmapBlock), lds, rows, bmeta, path, n, Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz,
-                    (const int32_t*)nullptr);
+                    (const int32_t*)nullptr, (const int32_t*)nullptr);
       else if (path == P_BLOCK)
         SPWS_LAUNCH(ctx, "row_bitmap", (k_row_bitmap_rows<kBitmapRowsK, kBitmapItems1>), dim3(grid_k),
                     dim3(kBitmapBlock), lds, rows, bmeta, path, n, Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz,
-                    (const int32_t*)nullptr);
+                    (const int32_t*)nullptr, (const int32_t*)nullptr);
       else
         SPWS_LAUNCH(ctx, "row_bitmap", (k_row_bitmap_rows<kBitmapRowsK, kBitmapItems2>), dim3(grid_k),
                     dim3(kBitmapBlock), lds, rows, bmeta, path, n, Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz,
-                    (const int32_t*)nullptr);
+                    (const int32_t*)nullptr, (const int32_t*)nullptr);
     } else if (path == P_BLOCK) {
       SPWS_LAUNCH(ctx, "row_block", (k_row_block<2048, 1024, 256, kValues>), dim3(grid_for(counts[path], 1 << 16)),
                   dim3(256), 0, rows, bmeta, path, Ap, Ai, Ax, Bp, Bi, Bx, roff, scrd, sval, rnnz);
@@ -197,7 +198,7 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
       SPWS_LAUNCH(ctx, "row_bitmap_long", (k_row_bitmap<kValues, true, kBitmapLongBlock>),
                   dim3(grid_for(counts[path], 4096)), dim3(kBitmapLongBlock),
                   bitmap_lds_bytes(n, 0, false, kBitmapLongBlock), rows, bmeta, path, n, Ap, Ai, Ax, Bp, Bi, Bx, roff,
-                  scrd, sval, rnnz, (const int32_t*)nullptr);
+                  scrd, sval, rnnz, (const int32_t*)nullptr, (const int32_t*)nullptr);
       counts[path] = 0;
     }
   }

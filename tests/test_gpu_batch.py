@@ -111,6 +111,23 @@ def test_long_rows_take_the_hbm_hash_path_in_a_batch(ctx, engine):
         check_result(C, a, b)
 
 
+def test_request_widths_in_one_batch(ctx, engine):
+    """Rows of requests from 64 to 65,536 columns wide share the bitmap
+    kernels' workgroups; each row's passes cover its own request's columns
+    (block bins in registers and with two gathers, and rows beyond them)."""
+    print("case: request widths 64..65536 in one batch", flush=True)
+    pairs = []
+    for t, (n, a_len, b_len) in enumerate([(64, 20, 30), (65536, 30, 40), (700, 25, 35), (30000, 40, 60),
+                                           (5000, 400, 12), (96, 10, 50)]):
+        k = max(2 * a_len, 800)
+        A = random_csr(60, k, np.full(60, a_len), 70 + t, 0)
+        B = random_csr(k, n, np.full(k, min(b_len, n)), 70 + t, 1)
+        pairs.append((A, B))
+    results, _ = run_batch(engine, ctx, pairs)
+    for (A, B), C in zip(pairs, results):
+        check_result(C, A, B)
+
+
 def test_any_width_engine_and_width_limit(ctx):
     """max_cols = 0: the block bins hash and sort; max_cols > 0: the LDS
     column bitmap, and a wider request is refused (it runs plain)."""
