@@ -262,9 +262,14 @@ static void enqueue_batch(spws_batch* b) {
                        (int)P_BLOCK2, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz);
   }
   if (c.max_cols > 0 && c.max_cols <= kBitmapCols) {
-    hipLaunchKernelGGL((k_row_bitmap<true, true, kBitmapLongBlock>), dim3(gblk), dim3(kBitmapLongBlock),
-                       bitmap_lds_bytes(c.max_cols, 0, false, kBitmapLongBlock), s, bin_rows, bmeta, (int)P_WIDE,
-                       c.max_cols, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz, coloff, row_ncols);
+    // products staged in the output pools, which the compaction fills later;
+    // rows of at most acc_cap entries accumulate in LDS
+    const size_t bm_bytes = bitmap_lds_bytes(c.max_cols, 0, false, kBitmapLongBlock);
+    const int32_t acc_cap = (int32_t)std::min<int64_t>(c.max_cols, (int64_t)((kLdsBudget - bm_bytes) / 8));
+    hipLaunchKernelGGL(k_row_bitmap_staged<kBitmapLongBlock>, dim3(gblk), dim3(kBitmapLongBlock),
+                       bitmap_lds_bytes(c.max_cols, acc_cap, true, kBitmapLongBlock), s, bin_rows, bmeta,
+                       (int)P_WIDE, c.max_cols, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz, coloff,
+                       row_ncols, Ci, Cx, acc_cap);
   } else {
     hipLaunchKernelGGL((k_row_wide<true>), dim3(b->wide_groups), dim3(kWideBlock), 0, s, bin_rows, bmeta,
                        (int)P_WIDE, Ap, Ai, Ax, Bp, Bi, Bx, row_ub, row_off, scrd, sval, row_nnz,

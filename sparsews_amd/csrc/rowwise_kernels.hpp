@@ -695,6 +695,7 @@ __global__ __launch_bounds__(BLOCK) void k_row_block(const int32_t* __restrict__
 // request's width), and the passes over the bitmap cover that range only.
 
 constexpr int kBitmapBlock = 256;
+constexpr size_t kLdsBudget = 156 * 1024;  // dynamic LDS a workgroup may take (of 160 KiB)
 constexpr int kBitmapLongBlock = 1024;  // rows beyond the block bins: 16 wavefronts per row
 constexpr int32_t kBitmapCols = 1 << 17;
 
@@ -752,13 +753,15 @@ struct BitmapLds {
 
 // One row through the column bitmap (zero on entry and on return).
 // pass1(f) and pass2(f) call f(col, v) once for every product of the row;
-// pass 1 ignores v.  The sorted column list is staged in the rank-prefix
+// pass 1 ignores v.  Values accumulate in the LDS accumulator, except with
+// kGlobal when the row has more than acc_cap entries: then in the row's slot
+// region with FP64 atomics.  The sorted column list is staged in the rank-prefix
 // words (dead after pass 2) and stored coalesced when it fits there, else
 // each thread stores its own columns.  Ends with a barrier, so the next row
 // may reuse the LDS at once.
 template <int BLOCK, bool kValues, bool kGlobal, class P1, class P2>
 __device__ __forceinline__ void bitmap_row(int32_t i, int32_t cb, int32_t w0, int32_t w1, int64_t off,
-                                           const BitmapLds& L, int32_t* sh,
+                                           int32_t acc_cap, const BitmapLds& L, int32_t* sh,
                                            int32_t* __restrict__ slot_crd, double* __restrict__ slot_val,
                                            int32_t* __restrict__ row_nnz, P1&& pass1, P2&& pass2) {
   pass1([&](int32_t col, double) {
@@ -781,7 +784,8 @@ __device__ __forceinline__ void bitmap_row(int32_t i, int32_t cb, int32_t w0, in
     L.pre[L.at(w)] = run;
     run += __popc(L.bm[L.at(w)]);
   }
-  double* dst = kGlobal ? slot_val + off : L.acc;
+  const bool in_lds = !kGlobal || nnz <= acc_cap;  // workgroup-uniform
+  double* dst = in_lds ? L.acc : slot_val + off;
   for (int32_t e = threadIdx.x; e < nnz; e += BLOCK) dst[e] = 0.0;
   __syncthreads();
   pass2([&](int32_t col, double v) {
@@ -809,7 +813,7 @@ __device__ __forceinline__ void bitmap_row(int32_t i, int32_t cb, int32_t w0, in
   if (staged) __syncthreads();
   for (int32_t e = threadIdx.x; e < nnz; e += BLOCK) {
     if (staged) slot_crd[off + e] = list[e];
-    if (!kGlobal) slot_val[off + e] = L.acc[e];
+    if (in_lds) slot_val[off + e] = L.acc[e];
   }
   if (threadIdx.x == 0) row_nnz[i] = nnz;
   __syncthreads();
@@ -843,9 +847,87 @@ __global__ __launch_bounds__(BLOCK) void k_row_bitmap(const int32_t* __restrict_
     const int32_t a_beg = Ap[i], a_end = Ap[i + 1];
     const int64_t off = kValues ? row_off[i] : 0;
     bitmap_row<BLOCK, kValues, kGlobal>(
-        i, cb, w0, w1, off, L, sh, slot_crd, slot_val, row_nnz,
+        i, cb, w0, w1, off, 0, L, sh, slot_crd, slot_val, row_nnz,
         [&](auto&& f) { expand_products<false>(a_beg, a_end, wave, BLOCK / kWave, Ai, Ax, Bp, Bi, Bx, f); },
         [&](auto&& f) { expand_products<true>(a_beg, a_end, wave, BLOCK / kWave, Ai, Ax, Bp, Bi, Bx, f); });
+  }
+}
+
+// Numeric rows beyond the block bins (at most kBitmapCols columns): pass 1
+// gathers every product once, sets its column bit and stages (column,
+// value) in the row's region [off, off + ub) of a scratch pair (the final
+// output arrays, unused until the compaction), each wavefront taking slots
+// from an LDS cursor; pass 2 reads the staged products back coalesced
+// instead of gathering B again (B rows of power-law inputs are mostly 1-3
+// entries long, so a second gather costs a cache line per product).  Values
+// accumulate in LDS when the row has at most acc_cap entries (the dynamic
+// LDS holds acc_cap accumulators after the bitmap), else with FP64 atomics
+// in the row's slot region.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_row_bitmap_staged(
+    const int32_t* __restrict__ bin_rows, const int32_t* __restrict__ bmeta, int path, int32_t ncols,
+    const int32_t* __restrict__ Ap, const int32_t* __restrict__ Ai, const double* __restrict__ Ax,
+    const int32_t* __restrict__ Bp, const int32_t* __restrict__ Bi, const double* __restrict__ Bx,
+    const int64_t* __restrict__ row_off, int32_t* __restrict__ slot_crd, double* __restrict__ slot_val,
+    int32_t* __restrict__ row_nnz, const int32_t* __restrict__ row_colbase, const int32_t* __restrict__ row_ncols,
+    int32_t* __restrict__ stage_crd, double* __restrict__ stage_val, int32_t acc_cap) {
+  extern __shared__ double bml_d[];
+  __shared__ int32_t sh[BLOCK / kWave + 1];
+  __shared__ int32_t cursor;
+  if (threadIdx.x == 0) cursor = 0;
+  BitmapLds L;
+  L.init<BLOCK>(bml_d, ncols);  // ends with a barrier
+  const int wave = threadIdx.x / kWave;
+  const int lane = lane_id();
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int32_t count = bmeta[kMetaCount + path];
+  const int32_t* rows = bin_rows + bmeta[kMetaOff + path];
+  for (int32_t r = blockIdx.x; r < count; r += gridDim.x) {
+    const int32_t i = rows[r];
+    const int32_t cb = row_colbase ? row_colbase[i] : 0;
+    int32_t w0, w1;
+    L.row_words<BLOCK>(row_ncols ? row_ncols[i] : ncols, &w0, &w1);
+    const int32_t a_beg = Ap[i], a_end = Ap[i + 1];
+    const int64_t off = row_off[i];
+    const int32_t ub = (int32_t)(row_off[i + 1] - off);
+    int32_t* sc = stage_crd + off;
+    double* sv = stage_val + off;
+    bitmap_row<BLOCK, true, true>(
+        i, cb, w0, w1, off, acc_cap, L, sh, slot_crd, slot_val, row_nnz,
+        [&](auto&& f) {
+          expand_products<true>(a_beg, a_end, wave, BLOCK / kWave, Ai, Ax, Bp, Bi, Bx, [&](int32_t col, double v) {
+            // the active lanes of this wavefront take consecutive slots
+            const uint64_t act = __ballot(1);
+            const int leader = __ffsll((long long)act) - 1;
+            int32_t base = 0;
+            if (lane == leader) base = atomicAdd(&cursor, __popcll(act));
+            base = __shfl(base, leader, kWave);
+            const int32_t slot = base + __popcll(act & lt);
+            sc[slot] = col;
+            sv[slot] = v;
+            f(col, v);
+          });
+          __threadfence_block();  // the staged products are read by other wavefronts in pass 2
+        },
+        [&](auto&& f) {
+          if (threadIdx.x == 0) cursor = 0;  // pass 1 of every wavefront is done
+          constexpr int kU = 4;
+          for (int32_t q0 = 0; q0 < ub; q0 += BLOCK * kU) {
+            int32_t c[kU];
+            double v[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+              const int32_t q = q0 + u * BLOCK + (int32_t)threadIdx.x;
+              if (q < ub) {
+                c[u] = sc[q];
+                v[u] = sv[q];
+              }
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+              if (q0 + u * BLOCK + (int32_t)threadIdx.x < ub) f(c[u], v[u]);
+          }
+        });
   }
 }
 
@@ -912,7 +994,7 @@ __global__ __launch_bounds__(kBitmapBlock) void k_row_bitmap_rows(
       L.row_words<kBitmapBlock>(nc[k], &w0, &w1);
       if (take[k]) {
         bitmap_row<kBitmapBlock, true, false>(
-            i[k], cb[k], w0, w1, off[k], L, sh, slot_crd, slot_val, row_nnz,
+            i[k], cb[k], w0, w1, off[k], 0, L, sh, slot_crd, slot_val, row_nnz,
             [&](auto&& f) {
 #pragma unroll
               for (int u = 0; u < ITEMS; ++u)
@@ -926,7 +1008,7 @@ __global__ __launch_bounds__(kBitmapBlock) void k_row_bitmap_rows(
       } else {
         const int32_t ab = a_beg[k], ae = a_end[k];
         bitmap_row<kBitmapBlock, true, false>(
-            i[k], cb[k], w0, w1, off[k], L, sh, slot_crd, slot_val, row_nnz,
+            i[k], cb[k], w0, w1, off[k], 0, L, sh, slot_crd, slot_val, row_nnz,
             [&](auto&& f) { expand_products<false>(ab, ae, wave, kBitmapBlock / kWave, Ai, Ax, Bp, Bi, Bx, f); },
             [&](auto&& f) { expand_products<true>(ab, ae, wave, kBitmapBlock / kWave, Ai, Ax, Bp, Bi, Bx, f); });
       }

@@ -195,10 +195,22 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
   if (n <= kBitmapCols) {
     for (int path : {(int)P_DENSE, (int)P_WIDE}) {
       if (!counts[path]) continue;
-      SPWS_LAUNCH(ctx, "row_bitmap_long", (k_row_bitmap<kValues, true, kBitmapLongBlock>),
-                  dim3(grid_for(counts[path], 4096)), dim3(kBitmapLongBlock),
-                  bitmap_lds_bytes(n, 0, false, kBitmapLongBlock), rows, bmeta, path, n, Ap, Ai, Ax, Bp, Bi, Bx, roff,
-                  scrd, sval, rnnz, (const int32_t*)nullptr, (const int32_t*)nullptr);
+      // staged in the output arrays, which the compaction fills later; rows
+      // of at most acc_cap entries accumulate in LDS
+      const size_t bm_bytes = bitmap_lds_bytes(n, 0, false, kBitmapLongBlock);
+      const int32_t acc_cap = (int32_t)std::min<int64_t>(std::min<int64_t>(maxub[path], n),
+                                                         (int64_t)((kLdsBudget - bm_bytes) / 8));
+      if (kValues)
+        SPWS_LAUNCH(ctx, "row_bitmap_long", k_row_bitmap_staged<kBitmapLongBlock>,
+                    dim3(grid_for(counts[path], 4096)), dim3(kBitmapLongBlock),
+                    bitmap_lds_bytes(n, acc_cap, true, kBitmapLongBlock), rows, bmeta, path, n, Ap, Ai, Ax, Bp, Bi,
+                    Bx, roff, scrd, sval, rnnz, (const int32_t*)nullptr, (const int32_t*)nullptr,
+                    ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>(), acc_cap);
+      else
+        SPWS_LAUNCH(ctx, "row_bitmap_long", (k_row_bitmap<false, true, kBitmapLongBlock>),
+                    dim3(grid_for(counts[path], 4096)), dim3(kBitmapLongBlock),
+                    bitmap_lds_bytes(n, 0, false, kBitmapLongBlock), rows, bmeta, path, n, Ap, Ai, Ax, Bp, Bi, Bx,
+                    roff, scrd, sval, rnnz, (const int32_t*)nullptr, (const int32_t*)nullptr);
       counts[path] = 0;
     }
   }

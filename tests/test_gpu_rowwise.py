@@ -47,9 +47,9 @@ def test_rowwise_workload_10k(ctx):
 
 @pytest.mark.parametrize("m,seed", [(1, 11), (63, 12), (64, 13), (65, 14), (3000, 15), (20000, 16)])
 def test_single_pass_short_rows(ctx, m, seed):
-    """Every row has ub <= 64: the single-pass kernel (rows claimed in order,
-    decoupled look-back for the output offsets) writes C directly.  Includes
-    empty rows and row counts around the 64-row look-back window."""
+    """Every row has ub <= 64: k_row_wave writes each row into a fixed
+    64-entry slot region and one compaction (k_compact_fixed, tiles of 64
+    rows) forms C.  Includes empty rows and row counts around the tile size."""
     print(f"case: single-pass m={m} seed={seed}", flush=True)
     A = uniform_rows(m, 700, 0, 8, seed, 0)
     B = uniform_rows(700, 900, 0, 8, seed, 1)
@@ -152,7 +152,8 @@ def test_block_bins_bitmap_and_hash(ctx, ncols, per_row):
     (1800, 1, 5000, 3),      # second block bin, 29 chunks into registers
     (40, 60, 5000, 3),       # 2400 products: beyond the register capacity, two gathers
     (600, 1, 200_000, 2),    # LDS hash kernel, wavefronts on chunks of their own
-    (3000, 3, 100_000, 4),   # long rows: bitmap with HBM accumulation, 16 wavefronts on own chunks
+    (3000, 3, 100_000, 4),   # long rows: staged products, LDS accumulation, 16 wavefronts on own chunks
+    (2000, 15, 100_000, 4),  # long rows of ~26k entries: more than the LDS accumulator holds, HBM atomics
     (3000, 3, 400_000, 5),   # long rows: HBM hash table, wavefronts on own chunks
 ])
 def test_rows_of_many_a_entries(ctx, a_len, b_len, ncols, path):
