@@ -525,7 +525,7 @@ def run_serve(args):
     arrivals at 80% of the plain path's capacity for both, and at 3x that
     rate for the batched path, with p50/p99 latency (finish - arrival).
     Last, the latency-bound rate of each path: the highest Poisson arrival
-    rate (bisection over the rate, the first 1,024 requests) at which the
+    rate (bisection over the rate, the first 2,048 requests) at which the
     p99 latency stays within 2 ms."""
     from sparsews_amd import gen
     from sparsews_amd.device import Context
@@ -562,9 +562,9 @@ def run_serve(args):
     f_bl, _, st_bl = serve(ctx, engines, reqs, arr, "batched", max_batch_products=cap)
     arr3 = arr / 3.0
     f_b3, _, st_b3 = serve(ctx, engines, reqs, arr3, "batched", max_batch_products=cap)
-    bound = {"p99_target_ms": P99_TARGET_MS, "requests": min(N, 1024)}
+    bound = {"p99_target_ms": P99_TARGET_MS, "requests": min(N, 2048)}
     for mode, sat in (("plain", sat_plain), ("batched", sat_batch)):
-        rate, p99 = latency_bound_rate(ctx, engines if mode == "batched" else [], reqs[:1024], mode, sat, cap)
+        rate, p99 = latency_bound_rate(ctx, engines if mode == "batched" else [], reqs[:2048], mode, sat, cap)
         bound[mode] = {"req_per_s": round(rate, 2), "p99_ms": round(p99, 4)}
         log(f"serve: {mode} sustains {rate:.1f} req/s at p99 {p99:.3f} ms")
     bound["speedup"] = round(bound["batched"]["req_per_s"] / max(bound["plain"]["req_per_s"], 1e-9), 3)

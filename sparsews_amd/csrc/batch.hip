@@ -153,6 +153,36 @@ __global__ __launch_bounds__(kDScanThreads) void k_dscan_apply(const Tin* __rest
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = part[kDScanBlocks];
 }
 
+// dst_crd[p] = crd[p] + shift, dst_val[p] = val[p] for p = t0, t0 + st, ...:
+// kGatherUnroll strides loaded before any is stored, so a large request's
+// copy is bound by bandwidth rather than by one load latency per stride
+constexpr int kGatherUnroll = 8;
+
+__device__ __forceinline__ void copy_rebased(const int32_t* __restrict__ crd, const double* __restrict__ val,
+                                             int64_t n, int32_t shift, int32_t* __restrict__ dst_crd,
+                                             double* __restrict__ dst_val, int64_t t0, int64_t st) {
+  for (int64_t p0 = t0; p0 < n; p0 += st * kGatherUnroll) {
+    int32_t c[kGatherUnroll];
+    double v[kGatherUnroll];
+#pragma unroll
+    for (int u = 0; u < kGatherUnroll; ++u) {
+      const int64_t p = p0 + u * st;
+      if (p < n) {
+        c[u] = crd[p];
+        v[u] = val[p];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kGatherUnroll; ++u) {
+      const int64_t p = p0 + u * st;
+      if (p < n) {
+        dst_crd[p] = c[u] + shift;
+        dst_val[p] = v[u];
+      }
+    }
+  }
+}
+
 // operands of every request into the pools, indices rebased; per row, the
 // request's first column (row_coloff[row]) and width (row_coloff[max_rows +
 // row])
@@ -175,14 +205,8 @@ __global__ __launch_bounds__(256) void k_batch_gather(const spws_batch_req* __re
     }
   }
   for (int64_t i = t0; i <= q.k; i += st) B_pos[q.brow0 + i] = (int32_t)(q.bnnz0 + q.B_pos[i]);
-  for (int64_t p = t0; p < q.annz; p += st) {
-    A_crd[q.annz0 + p] = q.A_crd[p] + q.brow0;
-    A_val[q.annz0 + p] = q.A_val[p];
-  }
-  for (int64_t p = t0; p < q.bnnz; p += st) {
-    B_crd[q.bnnz0 + p] = q.B_crd[p] + q.col0;
-    B_val[q.bnnz0 + p] = q.B_val[p];
-  }
+  copy_rebased(q.A_crd, q.A_val, q.annz, q.brow0, A_crd + q.annz0, A_val + q.annz0, t0, st);
+  copy_rebased(q.B_crd, q.B_val, q.bnnz, q.col0, B_crd + q.bnnz0, B_val + q.bnnz0, t0, st);
 }
 
 __global__ void k_batch_total(const int32_t* __restrict__ C_pos, const spws_batch_desc* __restrict__ desc,
@@ -240,6 +264,8 @@ static void enqueue_batch(spws_batch* b) {
   hipLaunchKernelGGL(k_binscatter, dim3(grows), dim3(256), 0, s, 0, m_dev, 1, row_ub, bmeta, bin_rows, row_nnz, b->pp);
   const unsigned gwave = (unsigned)std::min<int64_t>(c.max_rows, 8192);
   const unsigned gblk = (unsigned)std::min<int64_t>(c.max_rows, 1024);
+  // the block bins' row-per-workgroup kernel: a wider grid keeps more rows in flight
+  const unsigned gbin = (unsigned)std::min<int64_t>(c.max_rows, 8192);
   hipLaunchKernelGGL((k_row_wave<128, 1, true, kWaveRowsK>), dim3(gwave), dim3(64), 0, s, bin_rows, bmeta, (int)P_WAVE,
                      Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz);
   hipLaunchKernelGGL((k_row_wave<512, 4, true>), dim3(gwave), dim3(64), 0, s, bin_rows, bmeta, (int)P_WAVE2, Ap, Ai,
@@ -247,11 +273,11 @@ static void enqueue_batch(spws_batch* b) {
   if (c.max_cols > 0 && c.max_cols <= kBitmapCols) {
     // every request is at most max_cols wide: LDS bitmap over each row's own column range
     const int64_t v1 = std::min<int64_t>(b->pp.block_ub, c.max_cols), v2 = std::min<int64_t>(b->pp.block2_ub, c.max_cols);
-    hipLaunchKernelGGL((k_row_bitmap_rows<kBitmapRowsK, kBitmapItems1>), dim3(gblk), dim3(kBitmapBlock),
+    hipLaunchKernelGGL((k_row_bitmap_rows<kBitmapRowsK, kBitmapItems1>), dim3(gbin), dim3(kBitmapBlock),
                        bitmap_lds_bytes(c.max_cols, v1, true, kBitmapBlock), s,
                        bin_rows, bmeta, (int)P_BLOCK, c.max_cols, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval, row_nnz,
                        coloff, row_ncols);
-    hipLaunchKernelGGL((k_row_bitmap_rows<kBitmapRowsK, kBitmapItems2>), dim3(gblk), dim3(kBitmapBlock),
+    hipLaunchKernelGGL((k_row_bitmap_rows<kBitmapRowsK, kBitmapItems2>), dim3(gbin), dim3(kBitmapBlock),
                        bitmap_lds_bytes(c.max_cols, v2, true, kBitmapBlock), s,
                        bin_rows, bmeta, (int)P_BLOCK2, c.max_cols, Ap, Ai, Ax, Bp, Bi, Bx, row_off, scrd, sval,
                        row_nnz, coloff, row_ncols);
