@@ -303,7 +303,7 @@ static void enqueue_batch(spws_batch* b) {
   }
   dscan<int32_t, int32_t>(s, row_nnz, m_dev, (int32_t*)P(BP_SCAN), Cp);
   hipLaunchKernelGGL(k_compact, dim3((unsigned)std::min<int64_t>(div_up(c.max_rows, 4), 16384)), dim3(256), 0, s, 0,
-                     m_dev, row_off, Cp, scrd, sval, coloff, Ci, Cx);
+                     m_dev, row_off, Cp, scrd, sval, coloff, Ci, Cx, nullptr);
   hipLaunchKernelGGL(k_batch_total, dim3(1), dim3(64), 0, s, Cp, d_desc, (int64_t*)P(BP_TOTAL));
   (void)hipMemcpyAsync(b->h_total, P(BP_TOTAL), 8, hipMemcpyDeviceToHost, s);
 }

@@ -137,6 +137,10 @@ __global__ void k_bin_offsets(int32_t* __restrict__ bmeta) {
   }
 }
 
+// ints of the metadata block: bmeta [0, kMetaInts), then 9 uint64 maxub
+// from int 32 on
+constexpr int kMetaBlockInts = 32 + 2 * 9;
+
 // zero the bin metadata (a kernel, so the sequence can be graph-captured)
 __global__ void k_meta_clear(int32_t* __restrict__ bmeta, unsigned long long* __restrict__ maxub) {
   const int t = threadIdx.x;
@@ -568,16 +572,29 @@ __global__ __launch_bounds__(kCfBlock) void k_compact_fixed(int32_t m, const int
                                                             const int32_t* __restrict__ slot_crd,
                                                             const double* __restrict__ slot_val,
                                                             int32_t* __restrict__ C_pos, int32_t* __restrict__ C_crd,
-                                                            double* __restrict__ C_val) {
+                                                            double* __restrict__ C_val, int32_t* __restrict__ meta_zero) {
   __shared__ int32_t sh[kCfBlock / kWave + 1];
+  // the product's metadata block, read by the host before this launch, is
+  // left zero for the next product
+  if (meta_zero && blockIdx.x == 0 && threadIdx.x < kMetaBlockInts) meta_zero[threadIdx.x] = 0;
   __shared__ int32_t pre[kCfRows], cnt[kCfRows];
   __shared__ int32_t s_base, s_total;
   const int lane = lane_id();
   const int w = threadIdx.x / kWave;
   const int32_t r0 = blockIdx.x * kCfRows;
   // counts of all earlier rows
+  // eight independent partial sums: the loads are in flight together
+  int32_t part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int32_t i0 = threadIdx.x; i0 < r0; i0 += 8 * kCfBlock) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int32_t i = i0 + u * kCfBlock;
+      if (i < r0) part[u] += row_nnz[i];
+    }
+  }
   int32_t before = 0;
-  for (int32_t i = threadIdx.x; i < r0; i += kCfBlock) before += row_nnz[i];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) before += part[u];
   int32_t tot;
   (void)block_excl_scan<int32_t, kCfBlock>(before, sh, &tot);
   if (w == 0) {
@@ -1257,7 +1274,10 @@ __global__ __launch_bounds__(256) void k_compact(int32_t m_host, const int32_t* 
                                                  const int32_t* __restrict__ slot_crd,
                                                  const double* __restrict__ slot_val,
                                                  const int32_t* __restrict__ row_coloff, int32_t* __restrict__ C_crd,
-                                                 double* __restrict__ C_val) {
+                                                 double* __restrict__ C_val, int32_t* __restrict__ meta_zero) {
+  // the product's metadata block, read by the host before this launch, is
+  // left zero for the next product
+  if (meta_zero && blockIdx.x == 0 && threadIdx.x < kMetaBlockInts) meta_zero[threadIdx.x] = 0;
   const int32_t m = dev_or(m_dev, m_host);
   const int lane = lane_id();
   const int32_t waves = gridDim.x * (blockDim.x / kWave);

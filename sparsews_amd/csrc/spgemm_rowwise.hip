@@ -74,7 +74,9 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
     return rc;
   int32_t* bmeta = ctx->bin_meta.as<int32_t>();
   unsigned long long* maxub_d = reinterpret_cast<unsigned long long*>(bmeta + 32);
-  SPWS_LAUNCH(ctx, "meta_clear", k_meta_clear, dim3(1), dim3(64), 0, bmeta, maxub_d);
+  // the last kernel of a completed product leaves the block zero
+  if (!ctx->rw_meta_clean) SPWS_LAUNCH(ctx, "meta_clear", k_meta_clear, dim3(1), dim3(64), 0, bmeta, maxub_d);
+  ctx->rw_meta_clean = false;
   if (m > 0) {
     SPWS_LAUNCH(ctx, "rowstat", k_rowstat, dim3((unsigned)std::min<int64_t>(div_up(m, kRsRows), 65536)),
                 dim3(kRsBlock), 0,
@@ -122,10 +124,11 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
     SPWS_LAUNCH(ctx, "compact_fixed", (k_compact_fixed<kValues>), dim3((unsigned)div_up(m, kCfRows)),
                 dim3(kCfBlock), 0, m, ctx->row_nnz.as<int32_t>(), kValues ? ctx->slot_crd.as<int32_t>() : nullptr,
                 kValues ? ctx->slot_val.as<double>() : nullptr, ctx->out_pos.as<int32_t>(),
-                kValues ? ctx->out_crd.as<int32_t>() : nullptr, kValues ? ctx->out_val.as<double>() : nullptr);
+                kValues ? ctx->out_crd.as<int32_t>() : nullptr, kValues ? ctx->out_val.as<double>() : nullptr, bmeta);
     int32_t* h32 = hc + 100;  // past the metadata block
     SPWS_HIP(hipMemcpyAsync(h32, ctx->out_pos.as<int32_t>() + m, 4, hipMemcpyDeviceToHost, s));
     SPWS_HIP(hipStreamSynchronize(s));
+    ctx->rw_meta_clean = true;
     *nnz_out = *h32;
     ctx->stats.nnz = *h32;
     return SPWS_OK;
@@ -250,11 +253,12 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
     unsigned g = (unsigned)div_up(m, 4);
     if (g > 65536) g = 65536;
     SPWS_LAUNCH(ctx, "compact", k_compact, dim3(g), dim3(256), 0, m, nullptr, roff, ctx->out_pos.as<int32_t>(), scrd,
-                sval, nullptr, ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>());
+                sval, nullptr, ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>(), bmeta);
   }
   int32_t* h32 = hc + 100;  // past the metadata block
   SPWS_HIP(hipMemcpyAsync(h32, ctx->out_pos.as<int32_t>() + m, 4, hipMemcpyDeviceToHost, s));
   SPWS_HIP(hipStreamSynchronize(s));
+  ctx->rw_meta_clean = kValues && m > 0;
   *nnz_out = *h32;
   ctx->stats.nnz = *h32;
   return SPWS_OK;
