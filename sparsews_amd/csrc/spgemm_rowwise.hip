@@ -1,29 +1,38 @@
-// Row-wise CSR x CSR SpGEMM with per-row hash workspaces (Gustavson order).
+// Row-wise CSR x CSR SpGEMM with per-row workspaces (Gustavson order).
 //
 // Pipeline (one stream; two host synchronisations, both tiny reads):
 //   1. k_rowstat      per-row upper bound ub_i = sum_{j in A_i} len(B_j) and
-//                     the path of each row (bin histogram + max ub per bin).
+//                     the path of each row (bin histogram, max ub per bin,
+//                     total); the metadata block starts zero (the last
+//                     kernel of the previous product leaves it so, else a
+//                     clear kernel runs first).
+//   -- sync: read the metadata block
+//   If every row has ub <= 64: k_row_wave over the rows in order into fixed
+//   64-entry slots, then k_compact_fixed forms C (no bins, no scans).
+//   Otherwise:
 //   2. scan(ub)       int64 offsets of each row's slot region (ub_i entries).
-//   -- sync: read sum(ub), bin counts, per-bin max ub
 //   3. k_binscatter   row ids grouped by path.
 //   4. numeric        one kernel per non-empty path; each row accumulates in
-//                     its workspace, then its distinct columns are compacted
-//                     and sorted (bitonic) into the row's slot region:
+//                     its workspace and is written in column order into its
+//                     slot region:
 //        wave   / wave2   one wavefront per row, LDS open-addressing table
-//                         (2*ub slots), LDS atomicCAS keys + ds_add_f64 values,
-//                         register bitonic sort across the wavefront;
-//        block  / block2  one workgroup per row: with at most 2^17 columns an
-//                         LDS bitmap of the columns (rank = popcount prefix)
-//                         and ds_add_f64 at each column's rank, written in
-//                         order with no sort; wider outputs use an LDS table
-//                         and an LDS bitonic sort;
-//        dense / wide     longer rows: with at most 2^17 columns, the LDS
-//                         column bitmap with FP64 atomics into the row's slot
-//                         region; wider outputs: the densest rows use a dense
-//                         FP64 accumulator + flag array per workgroup in HBM,
-//                         emitted in column order, the others a per-row HBM
-//                         hash table (2*ub slots), then an LDS or global
-//                         bitonic sort.
+//                         (2*ub slots), LDS atomicCAS keys + ds_add_f64
+//                         values, register bitonic sort across the wavefront;
+//        block  / block2  one workgroup per row: with at most 2^17 columns,
+//                         the row's products gathered once into registers,
+//                         an LDS bitmap of the columns (rank = popcount
+//                         prefix) and ds_add_f64 at each column's rank, no
+//                         sort; wider outputs use an LDS table and an LDS
+//                         bitonic sort;
+//        dense / wide     longer rows: with at most 2^17 columns, the column
+//                         bitmap with the products staged in the output
+//                         arrays and read back coalesced, accumulated in LDS
+//                         (or with FP64 atomics into the slot region when the
+//                         row has too many entries for LDS); wider outputs:
+//                         the densest rows use a dense FP64 accumulator +
+//                         flag array per workgroup in HBM, the others a
+//                         per-row HBM hash table (2*ub slots), then an LDS or
+//                         global bitonic sort.
 //   5. scan(nnz)      C_pos.
 //   6. k_compact      slot regions -> contiguous C_crd / C_vals.
 //   -- sync: read nnz(C)
