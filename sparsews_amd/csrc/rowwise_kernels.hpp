@@ -741,13 +741,12 @@ __host__ __device__ constexpr size_t bitmap_lds_bytes(int32_t ncols, int64_t vca
 // LDS layout of the bitmap kernels: bitmap and rank prefix (padded words
 // each), then the FP64 row accumulator.
 struct BitmapLds {
-  int32_t words, padded, pshift;
+  int32_t padded, pshift;
   uint32_t *bm, *pre;
   double* acc;
   template <int BLOCK>
   __device__ __forceinline__ void init(double* base, int32_t ncols) {
-    words = (ncols + 31) >> 5;
-    const int s = bitmap_own_log2(words, BLOCK);
+    const int s = bitmap_own_log2((ncols + 31) >> 5, BLOCK);
     padded = bitmap_padded_words(ncols, BLOCK);
     pshift = s ? s : 31;
     bm = reinterpret_cast<uint32_t*>(base);

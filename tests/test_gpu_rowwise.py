@@ -253,3 +253,29 @@ def test_taco_entry_points():
     C2 = taco.taco_to_csr(C2t)
     taco.free_outputs(C2t)
     check(C2, A, B)
+
+
+def test_kernel_timing_filter(ctx):
+    """Every launch is timed by default; spws_ctx_set_profiling_only limits
+    the events to one kernel name.  Products alternate between the short-row
+    path and the binned path, which also checks that each leaves the bin
+    metadata ready for the next."""
+    print("case: kernel timing filter", flush=True)
+    short = (uniform_rows(400, 300, 0, 8, 44, 0), uniform_rows(300, 350, 0, 8, 44, 1))
+    binned = (uniform_rows(300, 400, 10, 30, 45, 0), uniform_rows(400, 600, 10, 30, 45, 1))
+    try:
+        ctx.set_profiling(True)
+        ctx.reset_kernel_times()
+        check(run(ctx, *short), *short)
+        every = ctx.kernel_times()
+        assert {"rowstat", "row_wave", "compact_fixed"} <= set(every), every
+        ctx.set_profiling_only("row_wave")
+        ctx.reset_kernel_times()
+        for A, B in (short, binned, short):
+            check(run(ctx, A, B), A, B)
+        only = ctx.kernel_times()
+        assert set(only) == {"row_wave"}, only
+        assert only["row_wave"][1] >= 3 and only["row_wave"][0] > 0.0
+    finally:
+        ctx.set_profiling_only(None)
+        ctx.set_profiling(False)
