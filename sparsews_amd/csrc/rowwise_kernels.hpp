@@ -704,10 +704,10 @@ __global__ __launch_bounds__(BLOCK) void k_row_block(const int32_t* __restrict__
 // Pass 1 over the row's products sets each column's bit in an LDS bitmap of
 // all columns; a prefix of the word popcounts gives every column its rank in
 // the sorted row; pass 2 adds each product at its column's rank.  The row is
-// then written in column order with no hash probing and no sort.  Block-bin
-// rows accumulate in LDS (ds_add_f64, kGlobal = false); longer rows
-// accumulate straight into their slot region in HBM (global FP64 atomics,
-// kGlobal = true), which holds at least nnz entries.  In a stacked batch a
+// then written in column order with no hash probing and no sort.  Values
+// accumulate in LDS (ds_add_f64); a long row (kGlobal) whose entries do not
+// fit there accumulates straight into its slot region in HBM (global FP64
+// atomics), which holds at least nnz entries.  In a stacked batch a
 // row's columns lie in [row_colbase[i], row_colbase[i] + row_ncols[i]) (its
 // request's width), and the passes over the bitmap cover that range only.
 
