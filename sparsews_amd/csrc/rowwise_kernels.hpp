@@ -78,11 +78,12 @@ __global__ __launch_bounds__(kRsBlock) void k_rowstat(int32_t m_host, const int3
     if (t < kRsRows) sub[t] = 0;
     __syncthreads();
     const int32_t a0 = sAp[0], a1 = sAp[nr];
-    for (int32_t p0 = a0; p0 < a1; p0 += kRsBlock * kRsUnroll) {
+    // int64 positions: a0 + kRsBlock * kRsUnroll may pass INT32_MAX
+    for (int64_t p0 = a0; p0 < a1; p0 += kRsBlock * kRsUnroll) {
       int32_t j[kRsUnroll], len[kRsUnroll];
 #pragma unroll
       for (int u = 0; u < kRsUnroll; ++u) {
-        const int32_t p = p0 + u * kRsBlock + t;
+        const int64_t p = p0 + u * kRsBlock + t;
         j[u] = p < a1 ? Ai[p] : -1;
       }
 #pragma unroll
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(kRsBlock) void k_rowstat(int32_t m_host, const int3
         if (p0 + u * kRsBlock >= a1) break;  // workgroup-uniform: no entries left
         // the tile row holding entry p: the last q with sAp[q] <= p (p past
         // the range maps to the last row and adds 0)
-        const int32_t p = min(p0 + u * kRsBlock + t, a1 - 1);
+        const int32_t p = (int32_t)min(p0 + u * kRsBlock + t, (int64_t)a1 - 1);
         int lo = 0, hi = nr - 1;
 #pragma unroll
         for (int it = 0; it < kRsRowsLog2; ++it) {
