@@ -272,14 +272,17 @@ __device__ __forceinline__ void expand_products(int32_t a_beg, int32_t a_end, in
   }
 }
 
-// The products of K rows of at most 64 A entries each, gathered into
-// registers level by level across the rows (A entries of every row, then
-// their B row bounds, then the B entries), so the K load chains overlap.
-// Product g of row k (in the order of expand_products) belongs to thread
-// g % BLOCK as item g / BLOCK; rows with take[k] false are skipped
-// (workgroup-uniform), and the caller guarantees that a taken row has at
-// most BLOCK * ITEMS products.  Unused items hold col = -1.
-template <int BLOCK, int K, int ITEMS>
+// The products of K rows gathered into registers: product g of row k (in
+// the order of expand_products) belongs to thread g % BLOCK as item
+// g / BLOCK.  A row's A entries are taken 64 at a time (kMultiChunk false:
+// the caller guarantees at most 64); with K > 1 each level of the first
+// chunk (A entries, B row bounds, B entries) is loaded for all K rows
+// together, so their load chains overlap.  Rows with take[k] false are
+// skipped (workgroup-uniform), and the caller guarantees that a taken row
+// has at most BLOCK * ITEMS products.  Unused items hold col = -1.  The
+// chunk loop costs registers: the block-bin kernel, whose occupancy
+// depends on them, takes single-chunk rows only.
+template <int BLOCK, int K, int ITEMS, bool kMultiChunk>
 __device__ __forceinline__ void gather_rows_products(const bool (&take)[K], const int32_t (&a_beg)[K],
                                                      const int32_t (&a_end)[K], const int32_t* __restrict__ Ai,
                                                      const double* __restrict__ Ax, const int32_t* __restrict__ Bp,
@@ -289,6 +292,7 @@ __device__ __forceinline__ void gather_rows_products(const bool (&take)[K], cons
   const int32_t wbase = (int32_t)threadIdx.x - lane;
   int32_t j[K], bstart[K], blen[K];
   double a[K];
+  // first chunk of every row: A entries, then B row bounds
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     j[k] = -1;
@@ -315,40 +319,58 @@ __device__ __forceinline__ void gather_rows_products(const bool (&take)[K], cons
       val[k][u] = 0.0;
     }
     if (!take[k]) continue;
-    const int32_t incl = wave_incl_scan(blen[k]);
-    const int32_t excl = incl - blen[k];
-    const int32_t total = __shfl(incl, kWave - 1, kWave);
-    int32_t kb[ITEMS];
-    double oa[ITEMS];
-#pragma unroll
-    for (int u = 0; u < ITEMS; ++u) {
-      kb[u] = -1;
-      oa[u] = 0.0;
-      // this wave's item-u products are [g0, g0 + 64); skipped wave-uniformly
-      // when none of them exists
-      const int32_t g0 = u * BLOCK + wbase;
-      if (g0 >= total) continue;
-      const int32_t q = min(g0 + lane, total - 1);
-      int lo = 0, hi = kWave - 1;
-#pragma unroll
-      for (int it = 0; it < 6; ++it) {
-        const int mid = (lo + hi + 1) >> 1;
-        const int32_t em = __shfl(excl, mid, kWave);
-        if (em <= q) lo = mid; else hi = mid - 1;
+    int32_t base = 0;  // products of the row's earlier chunks
+    const int32_t c_end = kMultiChunk ? a_end[k] : min(a_end[k], a_beg[k] + kWave);
+    for (int32_t c = a_beg[k]; c < c_end; c += kWave) {
+      if (c != a_beg[k]) {  // later chunks: their A entries and B row bounds
+        const int32_t p = c + lane;
+        j[k] = -1;
+        a[k] = 0.0;
+        bstart[k] = blen[k] = 0;
+        if (p < a_end[k]) {
+          j[k] = Ai[p];
+          a[k] = Ax[p];
+          bstart[k] = Bp[j[k]];
+          blen[k] = Bp[j[k] + 1] - bstart[k];
+        }
       }
-      const int32_t kk = __shfl(bstart[k], lo, kWave) + (q - __shfl(excl, lo, kWave));
-      const double ao = __shfl(a[k], lo, kWave);
-      if (g0 + lane < total) {
-        kb[u] = kk;
-        oa[u] = ao;
-      }
-    }
+      const int32_t incl = wave_incl_scan(blen[k]);
+      const int32_t excl = incl - blen[k];
+      const int32_t total = __shfl(incl, kWave - 1, kWave);
+      int32_t kb[ITEMS];
+      double oa[ITEMS];
 #pragma unroll
-    for (int u = 0; u < ITEMS; ++u) {
-      if (kb[u] >= 0) {
-        col[k][u] = Bi[kb[u]];
-        val[k][u] = oa[u] * Bx[kb[u]];
+      for (int u = 0; u < ITEMS; ++u) {
+        kb[u] = -1;
+        oa[u] = 0.0;
+        // this wave's item-u products are [g0, g0 + 64); skipped
+        // wave-uniformly when none of them lies in this chunk
+        const int32_t g0 = u * BLOCK + wbase;
+        if (g0 + kWave <= base || g0 >= base + total) continue;
+        const int32_t qr = g0 + lane - base;
+        const int32_t q = min(max(qr, 0), total - 1);
+        int lo = 0, hi = kWave - 1;
+#pragma unroll
+        for (int it = 0; it < 6; ++it) {
+          const int mid = (lo + hi + 1) >> 1;
+          const int32_t em = __shfl(excl, mid, kWave);
+          if (em <= q) lo = mid; else hi = mid - 1;
+        }
+        const int32_t kk = __shfl(bstart[k], lo, kWave) + (q - __shfl(excl, lo, kWave));
+        const double ao = __shfl(a[k], lo, kWave);
+        if (qr >= 0 && qr < total) {
+          kb[u] = kk;
+          oa[u] = ao;
+        }
       }
+#pragma unroll
+      for (int u = 0; u < ITEMS; ++u) {
+        if (kb[u] >= 0) {
+          col[k][u] = Bi[kb[u]];
+          val[k][u] = oa[u] * Bx[kb[u]];
+        }
+      }
+      base += total;
     }
   }
 }
@@ -447,10 +469,9 @@ __device__ __forceinline__ void block_bitonic_lds(int32_t* list, int P) {
 // ------------------------------------------------------------------ wave-per-row kernel
 //
 // One 64-thread workgroup (one wavefront), K rows per step of a grid-stride
-// loop, TS = 2 * max ub slots per row.  The products of rows with at most 64
-// A entries (every row's ub is at most 64 * R by its bin) are gathered for
-// the K rows at once, one load level at a time, so their load chains
-// overlap; a row with more A entries (empty B rows) is expanded on its own.
+// loop, TS = 2 * max ub slots per row.  Every row's products (at most 64 * R
+// by its bin) are gathered into registers, the first chunk of A entries of
+// the K rows one load level at a time, so their load chains overlap.
 
 // rows per step of the first wave bin: two rows per step overlap their
 // load chains but measure slower than one (the per-row LDS work dominates)
@@ -490,7 +511,7 @@ __global__ __launch_bounds__(64) void k_row_wave(const int32_t* __restrict__ bin
         a_beg[k] = Ap[i[k]];
         a_end[k] = Ap[i[k] + 1];
       }
-      take[k] = i[k] >= 0 && a_end[k] - a_beg[k] <= kWave;
+      take[k] = i[k] >= 0;  // every row of these bins has ub <= 64 * R
 #pragma unroll
       for (int s = lane; s < TS; s += kWave) {
         keys[k][s] = -1;
@@ -499,19 +520,13 @@ __global__ __launch_bounds__(64) void k_row_wave(const int32_t* __restrict__ bin
     }
     int32_t pc[K][R];
     double pv[K][R];
-    gather_rows_products<kWave, K, R>(take, a_beg, a_end, Ai, Ax, Bp, Bi, Bx, pc, pv);
+    gather_rows_products<kWave, K, R, true>(take, a_beg, a_end, Ai, Ax, Bp, Bi, Bx, pc, pv);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      if (take[k]) {
 #pragma unroll
-        for (int u = 0; u < R; ++u)
-          if (pc[k][u] >= 0) lds_insert<kValues>(keys[k], vals[k], shift, kMask, pc[k][u], pv[k][u]);
-      } else if (i[k] >= 0) {
-        expand_products<kValues>(a_beg[k], a_end[k], 0, 1, Ai, Ax, Bp, Bi, Bx, [&](int32_t col, double v) {
-          lds_insert<kValues>(keys[k], vals[k], shift, kMask, col, v);
-        });
-      }
+      for (int u = 0; u < R; ++u)
+        if (pc[k][u] >= 0) lds_insert<kValues>(keys[k], vals[k], shift, kMask, pc[k][u], pv[k][u]);
     }
     __syncthreads();
 #pragma unroll
@@ -961,10 +976,9 @@ constexpr int kBitmapRowsGrid = 1 << 16;
 
 // Numeric block-bin rows, K rows per step of a grid-stride loop: the
 // products of every row with at most 64 A entries and at most
-// kBitmapBlock * ITEMS products are gathered for all K rows at once into
-// registers (their load chains overlap, and pass 2 reads no memory); the
-// rows then go through the bitmap one after another.  Other rows take the
-// two-gather expansion.
+// kBitmapBlock * ITEMS products are gathered into registers (pass 2 reads
+// no memory); the rows then go through the bitmap one after another.  Other
+// rows take the two-gather expansion.
 template <int K, int ITEMS>
 __global__ __launch_bounds__(kBitmapBlock) void k_row_bitmap_rows(
     const int32_t* __restrict__ bin_rows, const int32_t* __restrict__ bmeta, int path, int32_t ncols,
@@ -1003,7 +1017,7 @@ __global__ __launch_bounds__(kBitmapBlock) void k_row_bitmap_rows(
     }
     int32_t pc[K][ITEMS];
     double pv[K][ITEMS];
-    gather_rows_products<kBitmapBlock, K, ITEMS>(take, a_beg, a_end, Ai, Ax, Bp, Bi, Bx, pc, pv);
+    gather_rows_products<kBitmapBlock, K, ITEMS, false>(take, a_beg, a_end, Ai, Ax, Bp, Bi, Bx, pc, pv);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       if (i[k] < 0) continue;

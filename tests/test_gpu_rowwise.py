@@ -170,6 +170,23 @@ def test_rows_of_many_a_entries(ctx, a_len, b_len, ncols, path):
     np.testing.assert_array_equal(spgemm_rowwise_symbolic(ctx, dA, dB), C.pos)
 
 
+@pytest.mark.parametrize("every,path", [(10, 0), (2, 1)])
+def test_wave_bins_with_many_a_entries(ctx, every, path):
+    """Rows of 300 A entries whose B rows are mostly empty: at most 64 (or
+    256) products over five chunks of A entries, gathered chunk by chunk by
+    the wavefront-per-row kernels."""
+    print(f"case: wave bin {path}, 300 A entries, B rows nonempty every {every}", flush=True)
+    m, k = 50, 3000
+    A = random_csr(m, k, np.full(m, 300), 60 + every, 0)
+    lengths = np.zeros(k, dtype=np.int64)
+    lengths[::every] = 1
+    B = random_csr(k, 700, lengths, 60 + every, 1)
+    assert oracle.spgemm_row_ub(A, B).max() <= (64 if path == 0 else 256)
+    C = run(ctx, A, B)
+    check(C, A, B)
+    assert ctx.stats()["rows_per_path"][path] == m
+
+
 @pytest.mark.parametrize("path", [0, 1, 2, 3, 4, 5])
 def test_forced_paths(ctx, path):
     print(f"case: forced path {path}", flush=True)
