@@ -158,6 +158,16 @@ int spws_ctx_set_outer_chunking(spws_ctx* ctx, int64_t max_products, int32_t max
  * j and give identical results. */
 int spws_ctx_set_outer_method(spws_ctx* ctx, int32_t method, int32_t window_log2);
 
+/* CSR transpose on the device, the device form of transpose() in
+ * benchmark/CSC_CSR_T/nonfuse/CSC_CSR_T_hash.h:352: the m x n matrix
+ * (pos[m+1], crd[nnz], vals[nnz]) becomes its n x m transpose in
+ * out_pos[n+1], out_crd[nnz], out_vals[nnz] (device arrays the caller
+ * provides), each output row's columns in increasing order.  Asynchronous
+ * on the context's stream. */
+int spws_csr_transpose_dev(spws_ctx* ctx, int32_t m, int32_t n, int64_t nnz, const int32_t* pos,
+                           const int32_t* crd, const double* vals, int32_t* out_pos, int32_t* out_crd,
+                           double* out_vals);
+
 /* ------------------------------------------------------ sparse tensor kernels */
 
 /* A 3-level CSF is passed as nroots root coordinates l1_crd (the compressed
@@ -330,6 +340,20 @@ int spws_dcsc_dcsr_coord_compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor
  * Sets C->indices[1][0], C->indices[1][1], C->vals (SPWS_ERR_UNSUPPORTED
  * when nnz(C) >= 2^31). */
 int spws_csc_csr_t_hash_compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B, int32_t w_cap);
+
+/* int compute_nofuse(C, A, B, w_cap) of
+ * benchmark/CSC_CSR_T/nonfuse/CSC_CSR_T_hash.h:232: the same product
+ * without the transposition, C = A B with A CSC (m x k), B CSR (k x n) and
+ * C CSR (m x n), each entry summed over j in increasing order.  Sets
+ * C->indices[1][0], C->indices[1][1], C->vals (SPWS_ERR_UNSUPPORTED when
+ * nnz(C) >= 2^31). */
+int spws_csc_csr_nofuse_compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B, int32_t w_cap);
+
+/* int transpose(A, B) of benchmark/CSC_CSR_T/nonfuse/CSC_CSR_T_hash.h:352:
+ * A = B^T for a CSR B with B->dimensions[0] rows.  A, initialised as a
+ * 2-level tensor whose A->dimensions[0] covers B's columns, receives
+ * malloc'd pos/crd/vals, each row's columns in increasing order. */
+int spws_csr_transpose(taco_tensor_t* A, taco_tensor_t* B);
 
 /* int compute(A, B, C, D, w_cap) of
  * benchmark/MTTKRP/COO_CSF_DCSR_DCSR_hash_free.h:233: B a 3-level CSF with

@@ -106,6 +106,7 @@ SIGNATURES = {
                                     PI64, PVP, PVP, PVP]),
     "spws_ctx_set_outer_chunking": (C.c_int, [VP, I64, I32]),
     "spws_ctx_set_outer_method": (C.c_int, [VP, I32, I32]),
+    "spws_csr_transpose_dev": (C.c_int, [VP, I32, I32, I64, VP, VP, VP, VP, VP, VP]),
     "spws_mttkrp": (C.c_int, [VP, I32, I32, I32, I32, I32, VP, VP, VP, I32, VP, VP, VP, I64, VP, VP,
                               C.POINTER(I32), PVP, PVP]),
     "spws_ttm": (C.c_int, [VP, I32, I32, I32, I32, I32, VP, VP, VP, I32, VP, VP, VP, I64, VP,
@@ -130,6 +131,8 @@ SIGNATURES = {
     "spws_csr_csr_compute": (C.c_int, [C.POINTER(TacoTensor)] * 3),
     "spws_dcsc_dcsr_coord_compute": (C.c_int, [C.POINTER(TacoTensor)] * 3 + [I32]),
     "spws_csc_csr_t_hash_compute": (C.c_int, [C.POINTER(TacoTensor)] * 3 + [I32]),
+    "spws_csc_csr_nofuse_compute": (C.c_int, [C.POINTER(TacoTensor)] * 3 + [I32]),
+    "spws_csr_transpose": (C.c_int, [C.POINTER(TacoTensor)] * 2),
     "spws_mttkrp_hash_compute": (C.c_int, [C.POINTER(TacoTensor)] * 4 + [I32]),
     "spws_ttm_hash_il_compute": (C.c_int, [C.POINTER(TacoTensor)] * 3 + [I32]),
 }

@@ -84,6 +84,7 @@ struct spws_ctx {
   spws::DevBuf slot_crd, slot_val, wide_tab, dense_val, dense_flag;
   spws::DevBuf out_pos, out_crd, out_val, out_aux;
   spws::DevBuf keys[2], vals[2], chunk_a, chunk_b;
+  spws::DevBuf tr_keys, tr_vals, tr_counts, tr_out[3];  // CSR transpose: sort pairs, column counts, outputs
   spws::DevBuf stage[12];  // device copies of host inputs (TACO entry points)
   spws::DevBuf radix_hist;
   spws::DevBuf obuf[8];    // outer-product per-column metadata

@@ -300,7 +300,8 @@ int spws_ctx_destroy(spws_ctx* ctx) {
                     &ctx->bin_meta, &ctx->slot_crd, &ctx->slot_val,   &ctx->wide_tab,  &ctx->dense_val,
                     &ctx->dense_flag, &ctx->out_pos, &ctx->out_crd,   &ctx->out_val,   &ctx->out_aux,
                     &ctx->keys[0],  &ctx->keys[1],  &ctx->vals[0],    &ctx->vals[1],   &ctx->chunk_a,
-                    &ctx->chunk_b};
+                    &ctx->chunk_b,  &ctx->tr_keys,  &ctx->tr_vals,    &ctx->tr_counts, &ctx->tr_out[0],
+                    &ctx->tr_out[1], &ctx->tr_out[2]};
   if (ctx->aux_stream) (void)hipStreamSynchronize(ctx->aux_stream);
   for (DevBuf* b : bufs) b->release();
   for (DevBuf& b : ctx->stage) b.release();

@@ -334,6 +334,37 @@ int spws_dcsc_dcsr_coord_compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor
   return SPWS_OK;
 }
 
+// An outer-product result (int64 row pointers on the device) as the
+// malloc'd int32 CSR arrays of C.
+static int outer_result_to_taco(spws_ctx* ctx, taco_tensor_t* C, int32_t rows, int64_t nnz, const int64_t* dpos,
+                                const int32_t* dcrd, const double* dval) {
+  int rc;
+  SPWS_REQUIRE(nnz <= INT32_MAX, SPWS_ERR_UNSUPPORTED, "nnz(C) = %lld does not fit the int32 CSR of taco_tensor_t",
+               (long long)nnz);
+  int64_t* p64 = nullptr;
+  int32_t* hc = nullptr;
+  double* hv = nullptr;
+  if ((rc = d2h_alloc(ctx, dpos, (int64_t)rows + 1, &p64))) return rc;
+  int32_t* hp = host_alloc<int32_t>((int64_t)rows + 1);
+  if (!hp) {
+    free(p64);
+    set_error("host allocation failed");
+    return SPWS_ERR_OOM;
+  }
+  for (int32_t i = 0; i <= rows; ++i) hp[i] = (int32_t)p64[i];
+  free(p64);
+  if ((rc = d2h_alloc(ctx, dcrd, nnz, &hc)) || (rc = d2h_alloc(ctx, dval, nnz, &hv))) {
+    free(hp);
+    free(hc);
+    return rc;
+  }
+  C->indices[1][0] = hp;
+  C->indices[1][1] = hc;
+  C->vals = hv;
+  C->vals_size = (int32_t)nnz;
+  return SPWS_OK;
+}
+
 int spws_csc_csr_t_hash_compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B, int32_t w_cap) {
   (void)w_cap;
   SPWS_REQUIRE(C && C->dimensions && C->indices && C->indices[1], SPWS_ERR_INVALID,
@@ -366,29 +397,79 @@ int spws_csc_csr_t_hash_compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_
   rc = spws_spgemm_outer(ctx, n, k, m, k, nullptr, bpos, bcrd, bval, k, nullptr, apos, acrd, aval, &nnz, &dpos, &dcrd,
                          &dval);
   if (rc) return rc;
-  SPWS_REQUIRE(nnz <= INT32_MAX, SPWS_ERR_UNSUPPORTED, "nnz(C) = %lld does not fit the int32 CSR of taco_tensor_t",
-               (long long)nnz);
-  int64_t* p64 = nullptr;
-  int32_t* hc = nullptr;
+  return outer_result_to_taco(ctx, C, n, nnz, dpos, dcrd, dval);
+}
+
+int spws_csc_csr_nofuse_compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B, int32_t w_cap) {
+  (void)w_cap;
+  SPWS_REQUIRE(C && C->dimensions && C->indices && C->indices[1], SPWS_ERR_INVALID,
+               "C must be initialised as a 2-level tensor (init_taco_tensor_DC)");
+  SPWS_REQUIRE(A && B && A->dimensions && B->dimensions && A->indices && B->indices && A->indices[1] &&
+                   B->indices[1] && A->indices[1][0] && B->indices[1][0],
+               SPWS_ERR_INVALID, "A must be CSC and B CSR taco tensors");
+  // C (m x n) = A B: A's columns j (its stored level) are the stored
+  // columns of the left operand, B's rows j those of the right; both outer
+  // levels are dense (every j)
+  const int32_t m = C->dimensions[0], n = C->dimensions[1], k = B->dimensions[0];
+  SPWS_REQUIRE(B->dimensions[1] == n && A->dimensions[0] == m && A->dimensions[1] == k, SPWS_ERR_INVALID,
+               "dimensions: A %d x %d (CSC), B %d x %d, C must be %d x %d", A->dimensions[0], A->dimensions[1],
+               B->dimensions[0], B->dimensions[1], A->dimensions[0], B->dimensions[1]);
+  int rc;
+  spws_ctx* ctx;
+  if ((rc = default_ctx(&ctx)) || (rc = ctx_activate(ctx))) return rc;
+  const int64_t annz = A->indices[1][0][k], bnnz = B->indices[1][0][k];
+  const int32_t *apos, *acrd, *bpos, *bcrd;
+  const double *aval, *bval;
+  if ((rc = stage_in(ctx, 0, A->indices[1][0], (int64_t)k + 1, &apos)) ||
+      (rc = stage_in(ctx, 1, A->indices[1][1], annz, &acrd)) || (rc = stage_in(ctx, 2, A->vals, annz, &aval)) ||
+      (rc = stage_in(ctx, 3, B->indices[1][0], (int64_t)k + 1, &bpos)) ||
+      (rc = stage_in(ctx, 4, B->indices[1][1], bnnz, &bcrd)) || (rc = stage_in(ctx, 5, B->vals, bnnz, &bval)))
+    return rc;
+  int64_t nnz = 0;
+  int64_t* dpos;
+  int32_t* dcrd;
+  double* dval;
+  rc = spws_spgemm_outer(ctx, m, k, n, k, nullptr, apos, acrd, aval, k, nullptr, bpos, bcrd, bval, &nnz, &dpos, &dcrd,
+                         &dval);
+  if (rc) return rc;
+  return outer_result_to_taco(ctx, C, m, nnz, dpos, dcrd, dval);
+}
+
+int spws_csr_transpose(taco_tensor_t* A, taco_tensor_t* B) {
+  // A = B^T: A has B's columns as rows (A->dimensions[0] of them)
+  int rc = check_csr(B, "B");
+  if (rc) return rc;
+  SPWS_REQUIRE(A && A->dimensions && A->indices && A->indices[1], SPWS_ERR_INVALID,
+               "A must be initialised as a 2-level tensor (init_taco_tensor_DC)");
+  spws_ctx* ctx;
+  if ((rc = default_ctx(&ctx)) || (rc = ctx_activate(ctx))) return rc;
+  const int32_t m = B->dimensions[0], n = A->dimensions[0];
+  const int64_t nnz = B->indices[1][0][m];
+  const int32_t *bpos, *bcrd;
+  const double* bval;
+  if ((rc = stage_in(ctx, 0, B->indices[1][0], (int64_t)m + 1, &bpos)) ||
+      (rc = stage_in(ctx, 1, B->indices[1][1], nnz, &bcrd)) || (rc = stage_in(ctx, 2, B->vals, nnz, &bval)))
+    return rc;
+  const size_t nn = (size_t)(nnz > 0 ? nnz : 1);
+  if ((rc = ctx->tr_out[0].ensure(((size_t)n + 1) * 4)) || (rc = ctx->tr_out[1].ensure(nn * 4)) ||
+      (rc = ctx->tr_out[2].ensure(nn * 8)))
+    return rc;
+  int32_t* dpos = ctx->tr_out[0].as<int32_t>();
+  int32_t* dcrd = ctx->tr_out[1].as<int32_t>();
+  double* dval = ctx->tr_out[2].as<double>();
+  if ((rc = spws_csr_transpose_dev(ctx, m, n, nnz, bpos, bcrd, bval, dpos, dcrd, dval))) return rc;
+  int32_t *hp = nullptr, *hc = nullptr;
   double* hv = nullptr;
-  if ((rc = d2h_alloc(ctx, dpos, (int64_t)n + 1, &p64))) return rc;
-  int32_t* hp = host_alloc<int32_t>((int64_t)n + 1);
-  if (!hp) {
-    free(p64);
-    set_error("host allocation failed");
-    return SPWS_ERR_OOM;
-  }
-  for (int32_t i = 0; i <= n; ++i) hp[i] = (int32_t)p64[i];
-  free(p64);
-  if ((rc = d2h_alloc(ctx, dcrd, nnz, &hc)) || (rc = d2h_alloc(ctx, dval, nnz, &hv))) {
+  if ((rc = d2h_alloc(ctx, (const int32_t*)dpos, (int64_t)n + 1, &hp))) return rc;
+  if ((rc = d2h_alloc(ctx, (const int32_t*)dcrd, nnz, &hc)) || (rc = d2h_alloc(ctx, (const double*)dval, nnz, &hv))) {
     free(hp);
     free(hc);
     return rc;
   }
-  C->indices[1][0] = hp;
-  C->indices[1][1] = hc;
-  C->vals = hv;
-  C->vals_size = (int32_t)nnz;
+  A->indices[1][0] = hp;
+  A->indices[1][1] = hc;
+  A->vals = hv;
+  A->vals_size = (int32_t)nnz;
   return SPWS_OK;
 }
 

@@ -267,6 +267,19 @@ def spgemm_outer(ctx: Context, A: DeviceDCSR, B: DeviceDCSR, m: int, k: int, n: 
                        DeviceArray.view(ctx, pc.value, n_, np.int32), DeviceArray.view(ctx, pv.value, n_, np.float64))
 
 
+def csr_transpose(ctx: Context, A: DeviceCSR) -> DeviceCSR:
+    """A^T as a new device CSR (arrays owned by the result), each row's
+    columns in increasing order (the device form of the reference's
+    transpose(), nonfuse/CSC_CSR_T_hash.h:352)."""
+    pos = DeviceArray(ctx, A.ncols + 1, np.int32)
+    crd = DeviceArray(ctx, A.nnz, np.int32)
+    vals = DeviceArray(ctx, A.nnz, np.float64)
+    check(lib().spws_csr_transpose_dev(ctx.h, int(A.nrows), int(A.ncols), int(A.nnz), A.pos.ptr, A.crd.ptr,
+                                       A.vals.ptr, pos.ptr, crd.ptr, vals.ptr))
+    ctx.sync()
+    return DeviceCSR(A.ncols, A.nrows, A.nnz, pos, crd, vals)
+
+
 @dataclass
 class DeviceCSF:
     """A 3-level CSF on the device (see formats.CSF3)."""

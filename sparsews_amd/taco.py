@@ -217,6 +217,20 @@ def csc_csr_t_compute(Ct: Taco, At: Taco, Bt: Taco, w_cap: int) -> int:
     return 0
 
 
+def csc_csr_nofuse_compute(Ct: Taco, At: Taco, Bt: Taco, w_cap: int) -> int:
+    """C = A B without the transposition, A CSC, B CSR
+    (nonfuse/CSC_CSR_T_hash.h:232, compute_nofuse)."""
+    check(lib().spws_csc_csr_nofuse_compute(Ct.ref, At.ref, Bt.ref, int(w_cap)))
+    return 0
+
+
+def transpose(At: Taco, Bt: Taco) -> int:
+    """A = B^T for CSR B (nonfuse/CSC_CSR_T_hash.h:352); A receives malloc'd
+    arrays (free_outputs)."""
+    check(lib().spws_csr_transpose(At.ref, Bt.ref))
+    return 0
+
+
 def mttkrp_compute(At: Taco, Bt: Taco, Ct: Taco, Dt: Taco, w_cap: int) -> int:
     """MTTKRP hash workspace (COO_CSF_DCSR_DCSR_hash_free.h:233)."""
     check(lib().spws_mttkrp_hash_compute(At.ref, Bt.ref, Ct.ref, Dt.ref, int(w_cap)))

@@ -263,3 +263,55 @@ def test_csc_csr_transposed_output(w_cap):
     dense = np.zeros((CT.nrows, CT.ncols))
     dense[rows, CT.crd] = CT.vals
     np.testing.assert_allclose(dense, ref, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("w_cap", [1 << 10, 1 << 16])
+def test_csc_csr_nonfused_transposed_output(w_cap):
+    """The reference's non-fused variant (nonfuse/CSC_CSR_T_hash.h:232 and
+    :352): C_noT = A B equals the oracle's Gustavson product bit for bit
+    (entries summed over j in increasing order), and transpose(C_noT)
+    equals the fused entry's C = (A B)^T bit for bit."""
+    print(f"case: CSC x CSR non-fused transposed output w_cap={w_cap}", flush=True)
+    from sparsews_amd import taco
+    A = powerlaw_rows(300, 250, 1.8, 120, 73, 0)   # m x k
+    B = powerlaw_rows(250, 400, 1.8, 150, 73, 1)   # k x n
+    At = taco.DC_to_taco_tensor_csc(A.transpose())
+    Bt = taco.DC_to_taco_tensor(B)
+    Cn = taco.init_taco_tensor_DC(A.nrows, B.ncols)
+    taco.csc_csr_nofuse_compute(Cn, At, Bt, w_cap)
+    CN = taco.taco_to_csr(Cn)
+    G = oracle.spgemm_gustavson(A, B)
+    np.testing.assert_array_equal(np.asarray(CN.pos, np.int64), np.asarray(G.pos, np.int64))
+    np.testing.assert_array_equal(CN.crd, G.crd)
+    np.testing.assert_array_equal(CN.vals, G.vals)
+    Ct = taco.init_taco_tensor_DC(B.ncols, A.nrows)
+    taco.transpose(Ct, Cn)
+    CT = taco.taco_to_csr(Ct)
+    Cf = taco.init_taco_tensor_DC(B.ncols, A.nrows)
+    taco.csc_csr_t_compute(Cf, At, Bt, w_cap)
+    CF = taco.taco_to_csr(Cf)
+    for t in (Cn, Ct, Cf):
+        taco.free_outputs(t)
+    np.testing.assert_array_equal(np.asarray(CT.pos, np.int64), np.asarray(CF.pos, np.int64))
+    np.testing.assert_array_equal(CT.crd, CF.crd)
+    np.testing.assert_array_equal(CT.vals, CF.vals)
+
+
+@pytest.mark.parametrize("m,n,lo,hi,seed", [(500, 700, 0, 30, 1), (1, 1, 1, 1, 2), (300, 70_000, 0, 50, 3),
+                                            (40, 5, 0, 5, 4), (200, 300, 0, 0, 5), (0, 10, 0, 0, 6)])
+def test_device_transpose(ctx, m, n, lo, hi, seed):
+    """The device CSR transpose equals scipy's (counting-sort) transpose bit
+    for bit: row pointers, columns in increasing order, values; empty rows,
+    columns and matrices included."""
+    print(f"case: device transpose m={m} n={n} per row {lo}..{hi}", flush=True)
+    import scipy.sparse as sp
+    from sparsews_amd.device import DeviceCSR, csr_transpose
+    A = uniform_rows(m, n, lo, hi, seed, 0) if m > 0 else CSR(0, n, np.zeros(1, np.int32), np.zeros(0, np.int32),
+                                                                np.zeros(0))
+    T = csr_transpose(ctx, DeviceCSR.upload(ctx, A)).download()
+    ref = sp.csr_matrix((A.vals, A.crd, A.pos), shape=(m, n)).T.tocsr()
+    ref.sort_indices()
+    assert (T.nrows, T.ncols, T.nnz) == (n, m, A.nnz)
+    np.testing.assert_array_equal(np.asarray(T.pos, np.int64), ref.indptr.astype(np.int64))
+    np.testing.assert_array_equal(T.crd, ref.indices)
+    np.testing.assert_array_equal(T.vals, ref.data)
