@@ -20,7 +20,7 @@ def test_header_compiles_alone(header, tmp_path):
                     str(src)], check=True)
 
 
-@pytest.mark.parametrize("name", ["drv_csr", "drv_coord"])
+@pytest.mark.parametrize("name", ["drv_csr", "drv_coord", "drv_cscT"])
 def test_drivers_link(name, tmp_path):
     from sparsews_amd import build as B
     if not os.path.exists(B.LIB):

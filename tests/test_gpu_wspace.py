@@ -79,7 +79,7 @@ def test_merge_rejects_short_capacity(ctx):
 @pytest.fixture(scope="module")
 def drivers(tmp_path_factory):
     d = tmp_path_factory.mktemp("compat")
-    return {name: build_compat_driver(name, d) for name in ("drv_csr", "drv_coord")}
+    return {name: build_compat_driver(name, d) for name in ("drv_csr", "drv_coord", "drv_cscT")}
 
 
 def run_driver(args, timeout=120):
@@ -135,3 +135,21 @@ def test_coord_driver(drivers, tmp_path):
     run_driver([drivers["drv_coord"], str(tmp_path / "in"), str(tmp_path / "out")])
     pos, crd, vals = read_frames(tmp_path / "out", [np.int32, np.int32, np.float64])
     check_csr(pos, crd, vals, A, B)
+
+
+def test_nonfused_transposed_driver(drivers, tmp_path):
+    """The reference's non-fused transposed-output driver call
+    (CSC_CSR_T_hash_nonfuse: compute_nofuse, then transpose) through the
+    drop-in header: C = (A B)^T equals the oracle's Gustavson B^T A^T bit
+    for bit."""
+    print("case: drop-in driver non-fused transposed output", flush=True)
+    A, B = powerlaw_rows(400, 300, 1.8, 150, 74, 0), uniform_rows(300, 500, 0, 20, 74, 1)
+    Ac = A.transpose()  # the columns of A as rows: its CSC arrays
+    write_frames(tmp_path / "in", [np.array([A.nrows, A.ncols, B.ncols], np.int32), Ac.pos, Ac.crd, Ac.vals,
+                                   B.pos, B.crd, B.vals])
+    run_driver([drivers["drv_cscT"], str(tmp_path / "in"), str(tmp_path / "out")])
+    pos, crd, vals = read_frames(tmp_path / "out", [np.int32, np.int32, np.float64])
+    G = oracle.spgemm_gustavson(B.transpose(), A.transpose())
+    np.testing.assert_array_equal(np.asarray(pos, np.int64), np.asarray(G.pos, np.int64))
+    np.testing.assert_array_equal(crd, G.crd)
+    np.testing.assert_array_equal(vals, G.vals)
