@@ -342,6 +342,7 @@ __global__ __launch_bounds__(256) void k_ttm_heads(int64_t nnz, const uint64_t* 
 constexpr int kTtmChunk = 128;
 constexpr int kTtmLanes = 16;
 constexpr int kTtmU = 16;
+constexpr int kTtmBlocksPerCu = 4;  // workgroups per CU in the accumulation grid (4 waves per SIMD)
 
 __global__ __launch_bounds__(256) void k_ttm_chunks(int64_t F, const int32_t* __restrict__ foff,
                                                     int32_t* __restrict__ nchunk) {
@@ -349,61 +350,132 @@ __global__ __launch_bounds__(256) void k_ttm_chunks(int64_t F, const int32_t* __
     nchunk[q] = (foff[q + 1] - foff[q] + kTtmChunk - 1) / kTtmChunk;
 }
 
-// item -> fiber map from the item offsets
-__global__ __launch_bounds__(256) void k_ttm_itemmap(int64_t F, const int32_t* __restrict__ coff,
-                                                     int32_t* __restrict__ item_fiber) {
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < F; q += (int64_t)gridDim.x * blockDim.x)
-    for (int32_t c = coff[q]; c < coff[q + 1]; ++c) item_fiber[c] = (int32_t)q;
+// Work item records {first nonzero, end, destination}: the destination is
+// the fiber q for a fiber of one item (its vector goes to Y), else ~c for
+// item c (its partial sum goes to the partial-sum rows).
+__global__ __launch_bounds__(256) void k_ttm_itemmap(int64_t F, const int32_t* __restrict__ foff,
+                                                     const int32_t* __restrict__ coff, int4* __restrict__ items) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < F; q += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t c0 = coff[q], c1 = coff[q + 1], f1 = foff[q + 1];
+    for (int32_t c = c0; c < c1; ++c) {
+      const int32_t p0 = foff[q] + (c - c0) * kTtmChunk;
+      items[c] = make_int4(p0, min(p0 + kTtmChunk, f1), c1 - c0 == 1 ? (int32_t)q : ~c, 0);
+    }
+  }
+}
+
+// acc + the in-order sum over nonzeros [base, p1) of an item, loaded
+// kTtmLanes at a time (only items of more than kTtmLanes nonzeros take it).
+template <bool kOff32>
+__device__ __forceinline__ double ttm_load_u(const double* __restrict__ U, uint32_t k, int R, int rr) {
+  // with kOff32 (K * R * 8 < 2^32) the byte offset is a 32-bit value: the
+  // load takes U's address from scalar registers and a one-register offset
+  if (kOff32)
+    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(U) +
+                                            (k * (uint32_t)R + (uint32_t)rr) * 8u);
+  return U[(int64_t)k * R + rr];
+}
+
+template <bool kOff32>
+__device__ __forceinline__ double ttm_item_tail(double acc, int32_t base, int32_t p1, int l, int rr, int R, uint64_t km,
+                                             const uint64_t* __restrict__ keys, const double* __restrict__ vals,
+                                             const double* __restrict__ U) {
+  for (; base < p1; base += kTtmLanes) {
+    const int cnt2 = p1 - base < kTtmLanes ? p1 - base : kTtmLanes;
+    const int32_t m2 = base + (l < cnt2 ? l : cnt2 - 1);
+    const uint32_t k2 = (uint32_t)(keys[m2] & km);
+    const double v2 = vals[m2];
+    double uv[kTtmU];
+#pragma unroll
+    for (int u = 0; u < kTtmU; ++u) {
+      const uint32_t k = __shfl(k2, u < cnt2 ? u : cnt2 - 1, kTtmLanes);
+      uv[u] = ttm_load_u<kOff32>(U, k, R, rr);
+    }
+#pragma unroll
+    for (int u = 0; u < kTtmU; ++u) {
+      const double x = __shfl(v2, u, kTtmLanes);
+      if (u < cnt2) acc += x * uv[u];
+    }
+  }
+  return acc;
 }
 
 // One group of kTtmLanes lanes per work item (ranks r = lane, lane +
-// kTtmLanes, ...): the group loads kTtmLanes sorted keys/values at once and
-// the rows U(k,:) of all of them, then sums in order.  A single-item fiber
-// writes its vector; the items of a longer fiber write partial sums, which
-// k_ttm_combine adds in item order (so the result does not depend on timing).
-__global__ __launch_bounds__(256) void k_ttm_accum(int64_t F, int64_t nitems, int R,
+// kTtmLanes, ...), each group walking items c, c + G, ... of a grid of G
+// groups.  The group holds its item's first kTtmLanes sorted keys/values (a
+// lane each), gathers the rows U(k,:) of all of them, then sums in order;
+// longer items load their further nonzeros kTtmLanes at a time.  While an
+// item's rows of U are in flight, the next item's keys/values and the record
+// after it load too, so each item costs about one memory latency instead of
+// three dependent ones.  A single-item fiber writes its vector; the items of
+// a longer fiber write partial sums, which k_ttm_combine adds in item order
+// (so the result does not depend on timing).
+template <bool kOff32>
+__global__ __launch_bounds__(256) void k_ttm_accum(int64_t nitems, int R, const int4* __restrict__ items,
                                                    const uint64_t* __restrict__ keys, const double* __restrict__ vals,
-                                                   const int32_t* __restrict__ foff,
-                                                   const int32_t* __restrict__ coff,
-                                                   const int32_t* __restrict__ item_fiber, int bk,
-                                                   const double* __restrict__ U, double* __restrict__ Y,
+                                                   int bk, const double* __restrict__ U, double* __restrict__ Y,
                                                    double* __restrict__ part) {
   const int groups = 256 / kTtmLanes;
-  const int g = threadIdx.x / kTtmLanes;
   const int l = threadIdx.x % kTtmLanes;
   const uint64_t km = (1ull << bk) - 1;
-  for (int64_t c = (int64_t)blockIdx.x * groups + g; c < nitems; c += (int64_t)gridDim.x * groups) {
-    const int64_t q = item_fiber[c];
-    const int32_t sub = (int32_t)(c - coff[q]);
-    const int32_t single = coff[q + 1] - coff[q] == 1;
-    const int32_t p0 = foff[q] + sub * kTtmChunk;
-    const int32_t p1 = min(p0 + kTtmChunk, foff[q + 1]);
+  const uint32_t km32 = (uint32_t)km;  // k < 2^31: the key's low word holds it
+  const uint32_t* __restrict__ keys_lo = reinterpret_cast<const uint32_t*>(keys);  // little-endian low words
+  const int64_t stride = (int64_t)gridDim.x * groups;
+  int64_t c = (int64_t)blockIdx.x * groups + threadIdx.x / kTtmLanes;
+  if (c >= nitems) return;  // uniform per group
+  // current item: record and first keys/values; next item: record
+  int4 cur = items[c];
+  int4 nxt = make_int4(0, 1, 0, 0);
+  if (c + stride < nitems) nxt = items[c + stride];
+  int32_t me = cur.x + min(l, cur.y - cur.x - 1);
+  uint32_t kraw = keys_lo[2 * (int64_t)me];
+  double myval = vals[me];
+  for (;;) {
+    const int64_t cn = c + stride;
+    const bool has_next = cn < nitems;  // uniform per group
+    // prefetches, left unused until the next iteration (no wait here)
+    uint32_t nkraw = 0;
+    double nval = 0.0;
+    int4 nn = nxt;
+    if (has_next) {
+      const int32_t nme = nxt.x + min(l, nxt.y - nxt.x - 1);
+      nkraw = keys_lo[2 * (int64_t)nme];
+      nval = vals[nme];
+      if (cn + stride < nitems) nn = items[cn + stride];
+    }
+    const uint32_t myk = kraw & km32;
+    const int32_t p0 = cur.x, p1 = cur.y;
+    const int cnt = min(p1 - p0, kTtmLanes);
     for (int r0 = 0; r0 < R; r0 += kTtmLanes) {
       const int r = r0 + l;
       const int rr = r < R ? r : R - 1;
       double acc = 0.0;
-      for (int32_t base = p0; base < p1; base += kTtmLanes) {
-        const int cnt = p1 - base < kTtmLanes ? p1 - base : kTtmLanes;
-        const int32_t me = base + (l < cnt ? l : cnt - 1);
-        const uint64_t mykey = keys[me];
-        const double myval = vals[me];
-        double xv[kTtmU], uv[kTtmU];
+      {
+        double uv[kTtmU];
 #pragma unroll
         for (int u = 0; u < kTtmU; ++u) {
-          const int src = u < cnt ? u : cnt - 1;
-          const uint64_t key = __shfl(mykey, src, kTtmLanes);
-          xv[u] = __shfl(myval, src, kTtmLanes);
-          uv[u] = U[(int64_t)(key & km) * R + rr];
+          const uint32_t k = __shfl(myk, u < cnt ? u : cnt - 1, kTtmLanes);
+          uv[u] = ttm_load_u<kOff32>(U, k, R, rr);
         }
 #pragma unroll
-        for (int u = 0; u < kTtmU; ++u)
-          if (u < cnt) acc += xv[u] * uv[u];
+        for (int u = 0; u < kTtmU; ++u) {
+          const double x = __shfl(myval, u, kTtmLanes);
+          if (u < cnt) acc += x * uv[u];
+        }
       }
+      // the item's further nonzeros (items of more than kTtmLanes)
+      if (p1 - p0 > kTtmLanes) acc = ttm_item_tail<kOff32>(acc, p0 + kTtmLanes, p1, l, rr, R, km, keys, vals, U);
       if (r < R) {
-        if (single) Y[q * R + r] = acc;
-        else part[c * R + r] = acc;
+        if (cur.z >= 0) Y[(int64_t)cur.z * R + r] = acc;
+        else part[(int64_t)(~cur.z) * R + r] = acc;
       }
     }
+    if (!has_next) break;
+    c = cn;
+    cur = nxt;
+    nxt = nn;
+    kraw = nkraw;
+    myval = nval;
   }
 }
 
@@ -576,16 +648,25 @@ static int ttm_accumulate(spws_ctx* ctx, int64_t nnz, int64_t F, int32_t R, int 
   if ((rc = exclusive_scan<int32_t, int32_t>(ctx, nchunk, coff, F))) return rc;
   // items <= F + nnz / kTtmChunk
   const int64_t max_items = F + nnz / kTtmChunk + 1;
-  if ((rc = ctx->obuf[3].ensure((size_t)max_items * R * 8)) || (rc = ctx->obuf[4].ensure((size_t)max_items * 4)))
+  if ((rc = ctx->obuf[3].ensure((size_t)max_items * R * 8)) || (rc = ctx->obuf[4].ensure((size_t)max_items * 16)))
     return rc;
   SPWS_HIP(hipMemcpyAsync(hm, coff + F, 4, hipMemcpyDeviceToHost, s));
   SPWS_HIP(hipStreamSynchronize(s));
   const int64_t nitems = *hm;
-  int32_t* item_fiber = ctx->obuf[4].as<int32_t>();
-  SPWS_LAUNCH(ctx, "ttm_itemmap", k_ttm_itemmap, dim3(gf2), dim3(256), 0, F, coff, item_fiber);
-  const unsigned ga = (unsigned)std::min<int64_t>(div_up(nitems, 256 / kTtmLanes), 65536);
-  SPWS_LAUNCH(ctx, "ttm_accum", k_ttm_accum, dim3(ga), dim3(256), 0, F, nitems, R, nkeys, nvals, foff, coff, item_fiber,
-              bk, U, Y, ctx->obuf[3].as<double>());
+  int4* items = ctx->obuf[4].as<int4>();
+  SPWS_LAUNCH(ctx, "ttm_itemmap", k_ttm_itemmap, dim3(gf2), dim3(256), 0, F, foff, coff, items);
+  // one workgroup per slot of a few per CU: each group walks many items
+  int cus = 0;
+  SPWS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  const unsigned ga = (unsigned)std::max<int64_t>(1, std::min<int64_t>(div_up(nitems, 256 / kTtmLanes),
+                                                                        (int64_t)std::max(cus, 1) * kTtmBlocksPerCu));
+  const bool off32 = ((int64_t)1 << bk) * R * 8 <= (int64_t)UINT32_MAX;  // every U offset fits in 32 bits
+  if (nitems > 0 && off32)
+    SPWS_LAUNCH(ctx, "ttm_accum", k_ttm_accum<true>, dim3(ga), dim3(256), 0, nitems, R, items, nkeys, nvals, bk, U,
+                Y, ctx->obuf[3].as<double>());
+  else if (nitems > 0)
+    SPWS_LAUNCH(ctx, "ttm_accum", k_ttm_accum<false>, dim3(ga), dim3(256), 0, nitems, R, items, nkeys, nvals, bk, U,
+                Y, ctx->obuf[3].as<double>());
   const unsigned gc = (unsigned)std::min<int64_t>(div_up(F, 256 / kTtmLanes), 65536);
   SPWS_LAUNCH(ctx, "ttm_combine", k_ttm_combine, dim3(gc), dim3(256), 0, F, R, coff, ctx->obuf[3].as<double>(), Y);
   return SPWS_OK;
