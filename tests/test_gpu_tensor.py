@@ -74,7 +74,7 @@ def test_mttkrp_taco_entry():
     assert oracle.max_value_error(gv, ov, absM[oi, orr]) <= TOL
 
 
-@pytest.mark.parametrize("R", [16, 5])
+@pytest.mark.parametrize("R", [16, 5, 40])
 def test_ttm_small(ctx, R):
     print(f"case: ttm small R={R}", flush=True)
     X = gen.zipf_coo3((500, 400, 60), 20000, (1.5, 1.5, None), 13, 5, -1.0, 1.0)
@@ -86,6 +86,45 @@ def test_ttm_small(ctx, R):
     np.testing.assert_array_equal(np.repeat(fj, R), oj)
     absY = np.einsum("ijk,kr->ijr", np.abs(dense(X)), np.abs(U))
     assert oracle.max_value_error(Y.ravel(), ov, absY[fi, fj].ravel()) <= TOL
+
+
+@pytest.mark.parametrize("R", [16, 40])
+def test_ttm_long_fibers(ctx, R):
+    """Fibers of 139-200 nonzeros: two 128-nonzero work items per fiber
+    (partial sums combined in item order), items longer than one 16-lane
+    load, ranks in three 16-lane slices for R = 40."""
+    print(f"case: ttm long fibers R={R}", flush=True)
+    X = gen.zipf_coo3((20, 30, 400), 100000, (None, None, None), 21, 5, -1.0, 1.0)
+    U = np.random.default_rng(30 + R).uniform(-1, 1, (400, R))
+    fi, fj, Y = run_ttm(ctx, X, U)
+    csf = formats.csf3_from_coo(X, (0, 2, 1))
+    ok, oj, ol, ov = oracle.ttm_hash_il(csf, formats.dense_as_dcsr(U), 20, 30, R)
+    np.testing.assert_array_equal(np.repeat(fi, R), ok)
+    np.testing.assert_array_equal(np.repeat(fj, R), oj)
+    absY = np.einsum("ijk,kr->ijr", np.abs(dense(X)), np.abs(U))
+    assert oracle.max_value_error(Y.ravel(), ov, absY[fi, fj].ravel()) <= TOL
+
+
+def test_ttm_many_items_per_group(ctx):
+    """~400k short fibers: each 16-lane group of the accumulation grid walks
+    about two dozen items, with the next item's keys and record loaded
+    ahead."""
+    print("case: ttm many items per group", flush=True)
+    X = gen.zipf_coo3((3000, 3000, 50), 400000, (None, None, None), 22, 5, -1.0, 1.0)
+    R = 16
+    U = np.random.default_rng(40).uniform(-1, 1, (50, R))
+    fi, fj, Y = run_ttm(ctx, X, U)
+    csf = formats.csf3_from_coo(X, (0, 2, 1))
+    ok, oj, ol, ov = oracle.ttm_hash_il(csf, formats.dense_as_dcsr(U), 3000, 3000, R)
+    np.testing.assert_array_equal(np.repeat(fi, R), ok)
+    np.testing.assert_array_equal(np.repeat(fj, R), oj)
+    # |X| |U| per fiber from the sorted coordinates (no dense 3000 x 3000 x 50 copy)
+    order = np.lexsort((X.idx[:, 2], X.idx[:, 1], X.idx[:, 0]))
+    xi, xj, xk = (X.idx[order, m].astype(np.int64) for m in range(3))
+    fid = np.searchsorted(fi.astype(np.int64) * 3000 + fj, xi * 3000 + xj)
+    absY = np.zeros((len(fi), R))
+    np.add.at(absY, fid, np.abs(X.vals[order])[:, None] * np.abs(U[xk]))
+    assert oracle.max_value_error(Y.ravel(), ov, absY.ravel()) <= TOL
 
 
 def test_ttm_wide_coordinates(ctx):
