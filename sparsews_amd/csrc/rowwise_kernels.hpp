@@ -579,8 +579,9 @@ __global__ __launch_bounds__(64) void k_row_wave(const int32_t* __restrict__ bin
 // thread), scans its own rows' counts and copies the rows, writing C_pos.
 // Two launches, no upper-bound scan, no bins and no count scan.
 
-constexpr int kCfRows = 64;
+constexpr int kCfRows = 64;  // a multiple of 4: the earlier rows' counts are read as int4
 constexpr int kCfBlock = 256;
+constexpr int kCfPart = 8;  // partial sums (16-byte loads in flight per thread) of the row-count prefix
 constexpr int32_t kCfMaxRows = 1 << 16;
 
 template <bool kValues>
@@ -598,19 +599,27 @@ __global__ __launch_bounds__(kCfBlock) void k_compact_fixed(int32_t m, const int
   const int lane = lane_id();
   const int w = threadIdx.x / kWave;
   const int32_t r0 = blockIdx.x * kCfRows;
-  // counts of all earlier rows
-  // eight independent partial sums: the loads are in flight together
-  int32_t part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int32_t i0 = threadIdx.x; i0 < r0; i0 += 8 * kCfBlock) {
+  // counts of all earlier rows, four per 16-byte load (r0 is a multiple of
+  // kCfRows, so r0 / 4 whole quads); kCfPart independent partial sums: the
+  // loads are in flight together (one round per 8,192 earlier rows)
+  const int4* __restrict__ nnz4 = reinterpret_cast<const int4*>(row_nnz);
+  const int32_t q0 = r0 / 4;
+  int32_t part[kCfPart];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+  for (int u = 0; u < kCfPart; ++u) part[u] = 0;
+  for (int32_t i0 = threadIdx.x; i0 < q0; i0 += kCfPart * kCfBlock) {
+#pragma unroll
+    for (int u = 0; u < kCfPart; ++u) {
       const int32_t i = i0 + u * kCfBlock;
-      if (i < r0) part[u] += row_nnz[i];
+      if (i < q0) {
+        const int4 c4 = nnz4[i];
+        part[u] += (c4.x + c4.y) + (c4.z + c4.w);
+      }
     }
   }
   int32_t before = 0;
 #pragma unroll
-  for (int u = 0; u < 8; ++u) before += part[u];
+  for (int u = 0; u < kCfPart; ++u) before += part[u];
   int32_t tot;
   (void)block_excl_scan<int32_t, kCfBlock>(before, sh, &tot);
   if (w == 0) {
