@@ -81,6 +81,7 @@ struct spws_ctx {
   spws::DevBuf scan_tmp;
   spws::DevBuf row_ub, row_off, row_nnz, bin_rows, bin_meta;
   bool rw_meta_clean = false;  // bin_meta is zero (left so by the last completed product)
+  bool rw_fixed_last = false;  // the last row-wise product took the fixed-slot path
   spws::DevBuf slot_crd, slot_val, wide_tab, dense_val, dense_flag;
   spws::DevBuf out_pos, out_crd, out_val, out_aux;
   spws::DevBuf keys[2], vals[2], chunk_a, chunk_b;

@@ -487,7 +487,7 @@ __global__ __launch_bounds__(64) void k_row_wave(const int32_t* __restrict__ bin
                                                  const int32_t* __restrict__ Bi, const double* __restrict__ Bx,
                                                  const int64_t* __restrict__ row_off, int32_t* __restrict__ slot_crd,
                                                  double* __restrict__ slot_val, int32_t* __restrict__ row_nnz,
-                                                 int32_t direct_m = 0) {
+                                                 int32_t direct_m = 0, const int64_t* __restrict__ direct_ub = nullptr) {
   static_assert(TS >= 2 * kWave * R / 2, "table too small");
   __shared__ int32_t keys[K][TS];
   __shared__ double vals[K][kValues ? TS : 1];
@@ -511,7 +511,9 @@ __global__ __launch_bounds__(64) void k_row_wave(const int32_t* __restrict__ bin
         a_beg[k] = Ap[i[k]];
         a_end[k] = Ap[i[k] + 1];
       }
-      take[k] = i[k] >= 0;  // every row of these bins has ub <= 64 * R
+      // every row of the bins has ub <= 64 * R; over all rows (direct_ub
+      // given), a row beyond that is skipped with a count of 0
+      take[k] = i[k] >= 0 && (direct_ub == nullptr || direct_ub[i[k]] <= (int64_t)kWave * R);
 #pragma unroll
       for (int s = lane; s < TS; s += kWave) {
         keys[k][s] = -1;

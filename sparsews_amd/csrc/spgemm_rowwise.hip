@@ -96,7 +96,40 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
   // max ub per path and the total products (uint64 from int 32 on)
   int32_t* hc = ctx->host_meta.as<int32_t>();
   const int64_t* hm = reinterpret_cast<const int64_t*>(hc + 32);
+  int32_t* h32 = hc + 100;  // nnz(C) of the fixed-slot path, past the metadata block
   SPWS_HIP(hipMemcpyAsync(hc, bmeta, 32 * 4 + 9 * 8, hipMemcpyDeviceToHost, s));
+
+  // The fixed-slot path (every row fits one wavefront: rows straight into
+  // fixed 64-entry slots, then one compaction that also forms C_pos; no
+  // bins, no scans).  When the last product on this context took it, it is
+  // launched now, before the metadata reaches the host: the wave kernel
+  // skips (count 0) any row whose upper bound exceeds 64, and the result is
+  // kept only if the metadata shows that every row fits, which is exactly
+  // when the path would have been chosen; otherwise the general path below
+  // recomputes C, after the host's copy of the metadata block is put back
+  // (the compaction left it zero for the next product).
+  auto launch_fixed = [&](size_t cap, int32_t* meta_zero) -> int {
+    const size_t fixed = (size_t)m * kWave;
+    int rc2;
+    if ((kValues && ((rc2 = ctx->slot_crd.ensure(fixed * 4)) || (rc2 = ctx->slot_val.ensure(fixed * 8)) ||
+                     (rc2 = ctx->out_crd.ensure(cap * 4)) || (rc2 = ctx->out_val.ensure(cap * 8)))))
+      return rc2;
+    SPWS_LAUNCH(ctx, "row_wave", (k_row_wave<128, 1, kValues, kWaveRowsK>),
+                dim3((unsigned)std::min<int64_t>(div_up(m, kWaveRowsK), 1 << 20)), dim3(64), 0,
+                (const int32_t*)nullptr, bmeta, (int)P_WAVE, Ap, Ai, Ax, Bp, Bi, Bx, (const int64_t*)nullptr,
+                kValues ? ctx->slot_crd.as<int32_t>() : nullptr, kValues ? ctx->slot_val.as<double>() : nullptr,
+                ctx->row_nnz.as<int32_t>(), m, (const int64_t*)ctx->row_ub.as<int64_t>());
+    SPWS_LAUNCH(ctx, "compact_fixed", (k_compact_fixed<kValues>), dim3((unsigned)div_up(m, kCfRows)),
+                dim3(kCfBlock), 0, m, ctx->row_nnz.as<int32_t>(), kValues ? ctx->slot_crd.as<int32_t>() : nullptr,
+                kValues ? ctx->slot_val.as<double>() : nullptr, ctx->out_pos.as<int32_t>(),
+                kValues ? ctx->out_crd.as<int32_t>() : nullptr, kValues ? ctx->out_val.as<double>() : nullptr,
+                meta_zero);
+    SPWS_HIP(hipMemcpyAsync(h32, ctx->out_pos.as<int32_t>() + m, 4, hipMemcpyDeviceToHost, s));
+    return SPWS_OK;
+  };
+  const bool spec = ctx->rw_fixed_last && t.force_path < 0 && m > 0 && m <= kCfMaxRows;
+  // at most 64 entries per row whichever rows the wave kernel skips
+  if (spec && (rc = launch_fixed((size_t)m * kWave, bmeta))) return rc;
   SPWS_HIP(hipStreamSynchronize(s));
   int64_t counts[8], maxub[8];
   for (int p = 0; p < 8; ++p) {
@@ -116,32 +149,20 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
                    (long long)maxub[t.force_path]);
   }
 
-  // every row fits one wavefront: rows straight into fixed 64-entry slots,
-  // then one compaction that also forms C_pos (no bins, no scans)
-  if (t.force_path < 0 && m > 0 && m <= kCfMaxRows && counts[P_WAVE] + counts[P_EMPTY] == m &&
-      maxub[P_WAVE] <= kWaveUbMax) {
-    const size_t cap = (size_t)(total_ub > 0 ? total_ub : 1);
-    const size_t fixed = (size_t)m * kWave;
-    if ((kValues && ((rc = ctx->slot_crd.ensure(fixed * 4)) || (rc = ctx->slot_val.ensure(fixed * 8)) ||
-                     (rc = ctx->out_crd.ensure(cap * 4)) || (rc = ctx->out_val.ensure(cap * 8)))))
-      return rc;
-    SPWS_LAUNCH(ctx, "row_wave", (k_row_wave<128, 1, kValues, kWaveRowsK>),
-                dim3((unsigned)std::min<int64_t>(div_up(m, kWaveRowsK), 1 << 20)), dim3(64), 0,
-                (const int32_t*)nullptr, bmeta, (int)P_WAVE, Ap, Ai, Ax, Bp, Bi, Bx, (const int64_t*)nullptr,
-                kValues ? ctx->slot_crd.as<int32_t>() : nullptr, kValues ? ctx->slot_val.as<double>() : nullptr,
-                ctx->row_nnz.as<int32_t>(), m);
-    SPWS_LAUNCH(ctx, "compact_fixed", (k_compact_fixed<kValues>), dim3((unsigned)div_up(m, kCfRows)),
-                dim3(kCfBlock), 0, m, ctx->row_nnz.as<int32_t>(), kValues ? ctx->slot_crd.as<int32_t>() : nullptr,
-                kValues ? ctx->slot_val.as<double>() : nullptr, ctx->out_pos.as<int32_t>(),
-                kValues ? ctx->out_crd.as<int32_t>() : nullptr, kValues ? ctx->out_val.as<double>() : nullptr, bmeta);
-    int32_t* h32 = hc + 100;  // past the metadata block
-    SPWS_HIP(hipMemcpyAsync(h32, ctx->out_pos.as<int32_t>() + m, 4, hipMemcpyDeviceToHost, s));
-    SPWS_HIP(hipStreamSynchronize(s));
+  const bool fixed_ok = t.force_path < 0 && m > 0 && m <= kCfMaxRows && counts[P_WAVE] + counts[P_EMPTY] == m &&
+                        maxub[P_WAVE] <= kWaveUbMax;
+  ctx->rw_fixed_last = fixed_ok;
+  if (fixed_ok) {
+    if (!spec) {
+      if ((rc = launch_fixed((size_t)(total_ub > 0 ? total_ub : 1), bmeta))) return rc;
+      SPWS_HIP(hipStreamSynchronize(s));
+    }
     ctx->rw_meta_clean = true;
     *nnz_out = *h32;
     ctx->stats.nnz = *h32;
     return SPWS_OK;
   }
+  if (spec) SPWS_HIP(hipMemcpyAsync(bmeta, hc, 32 * 4 + 9 * 8, hipMemcpyHostToDevice, s));
 
   rc = exclusive_scan<int64_t, int64_t>(ctx, ctx->row_ub.as<int64_t>(), ctx->row_off.as<int64_t>(), m);
   if (rc) return rc;
@@ -264,7 +285,6 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
     SPWS_LAUNCH(ctx, "compact", k_compact, dim3(g), dim3(256), 0, m, nullptr, roff, ctx->out_pos.as<int32_t>(), scrd,
                 sval, nullptr, ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>(), bmeta);
   }
-  int32_t* h32 = hc + 100;  // past the metadata block
   SPWS_HIP(hipMemcpyAsync(h32, ctx->out_pos.as<int32_t>() + m, 4, hipMemcpyDeviceToHost, s));
   SPWS_HIP(hipStreamSynchronize(s));
   ctx->rw_meta_clean = kValues && m > 0;

@@ -62,6 +62,30 @@ def test_single_pass_short_rows(ctx, m, seed):
     np.testing.assert_array_equal(spgemm_rowwise_symbolic(ctx, dA, dB), C.pos)
 
 
+def test_fixed_slot_path_launched_ahead(ctx):
+    """After a product that took the fixed-slot path, the next product on the
+    context launches that path before its metadata reaches the host (rows
+    over 64 products skipped); the result is kept only when every row fits,
+    else the general path recomputes C.  A sequence of fitting and
+    non-fitting products, numeric and symbolic, against the oracle."""
+    print("case: fixed-slot path launched ahead", flush=True)
+    short = [(uniform_rows(m, 700, 0, 8, 80 + m % 7, 0), uniform_rows(700, 900, 0, 8, 80 + m % 7, 1))
+             for m in (3000, 64, 20000)]
+    # one row of 20 x 8 = 160 products among rows of 32: not fitting
+    longA = random_csr(2000, 700, np.where(np.arange(2000) == 1234, 20, 4), 91, 0)
+    longB = random_csr(700, 900, np.full(700, 8), 91, 1)
+    assert oracle.spgemm_row_ub(longA, longB).max() > 64
+    seq = [short[0], (longA, longB), short[1], short[2], (longA, longB), (longA, longB), short[0]]
+    for t, (A, B) in enumerate(seq):
+        C = run(ctx, A, B)
+        check(C, A, B)
+        dA, dB = DeviceCSR.upload(ctx, A), DeviceCSR.upload(ctx, B)
+        np.testing.assert_array_equal(spgemm_rowwise_symbolic(ctx, dA, dB), C.pos)
+        fits = oracle.spgemm_row_ub(A, B).max() <= 64
+        st = ctx.stats()
+        assert (st["rows_per_path"][0] + st["rows_per_path"][6] == A.nrows) == fits, t
+
+
 def test_matrix_market_file(ctx, tmp_path):
     """A symmetric Matrix Market file (lower triangle stored) read with its
     values, squared on the device."""
