@@ -8,7 +8,10 @@
 //                     clear kernel runs first).
 //   -- sync: read the metadata block
 //   If every row has ub <= 64: k_row_wave over the rows in order into fixed
-//   64-entry slots, then k_compact_fixed forms C (no bins, no scans).
+//   64-entry slots, then k_compact_fixed forms C (no bins, no scans).  When
+//   the previous product on the context took this path, both kernels are
+//   launched before the sync (rows over 64 products skipped) and their
+//   result is kept if the metadata allows the path: one sync in all.
 //   Otherwise:
 //   2. scan(ub)       int64 offsets of each row's slot region (ub_i entries).
 //   3. k_binscatter   row ids grouped by path.
