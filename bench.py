@@ -653,7 +653,7 @@ def main(argv=None):
                     help="outer workload, bucket method: log2 of the column window (0 = automatic)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+        print(f"bench.py measures one GPU: the library has no multi-GPU path (asked for --gpus {args.gpus}); "
               "run with --gpus 1", file=sys.stderr)
         return 2
     if int(os.environ.get("RANK", "0")) != 0:
