@@ -15,6 +15,7 @@ Workloads (sizes in the docstring of each runner):
   rowwise   row-wise CSR x CSR hash SpGEMM, 10,000 x 10,000, 8 per row
   mttkrp    mode-1 MTTKRP, 50k x 40k x 30k, 20M nonzeros, rank 32
   ttm       mode-3 TTM, 100k x 80k x 2k, 10M nonzeros, U 2,000 x 16
+  transpose CSR transpose on the device, 200k x 200k, 32 columns per row
   serve     4,096 SpGEMM requests, plain vs continuously batched
 """
 from __future__ import annotations
@@ -626,9 +627,71 @@ def serve_cpu_baseline(stream, seconds):
                       f"(CSR_CSR_taco_parallel.h), {busy:.1f} s"}
 
 
-WORKLOADS = {"rowwise": run_rowwise, "outer": run_outer, "mttkrp": run_mttkrp, "ttm": run_ttm, "serve": run_serve}
+def run_transpose(args):
+    """CSR transpose on the device (the transpose pass of the non-fused
+    transposed-output product), 200,000 x 200,000 with 32 uniform columns per
+    row (6.4M entries), FP64.  Input and preallocated outputs are resident;
+    a step is one spws_csr_transpose_dev call.  Value = algorithmic bytes per
+    second: every entry read (12 B) and written (12 B) once, both row-pointer
+    arrays once."""
+    import ctypes as C
+    from sparsews_amd import gen
+    from sparsews_amd._lib import check, lib
+    from sparsews_amd.device import Context, DeviceArray, DeviceCSR
+    m = n = 200_000
+    A = gen.uniform_csr(m, n, 32, 3, 0)
+    nnz = int(A.pos[-1])
+    ctx = Context(0)
+    dA = DeviceCSR.upload(ctx, A)
+    out_pos = DeviceArray(ctx, n + 1, np.int32)
+    out_crd = DeviceArray(ctx, nnz, np.int32)
+    out_val = DeviceArray(ctx, nnz, np.float64)
+
+    def step():
+        check(lib().spws_csr_transpose_dev(ctx.h, m, n, nnz, dA.pos.ptr, dA.crd.ptr, dA.vals.ptr, out_pos.ptr,
+                                           out_crd.ptr, out_val.ptr))
+
+    names = ("radix_scatter", "tr_keys", "tr_out")
+    sec, dom, dom_times, bd = timed_steps(ctx, step, args.steps, args.warmup,
+                                          lambda bd: max(names, key=lambda k: bd.get(k, (0, 1))[0]))
+    calibration_copy(ctx, args.calibrate_copy)
+    # spot check against the host transpose (the first 1,000 output rows)
+    T = DeviceCSR(n, m, nnz, out_pos, out_crd, out_val).download()
+    cols = A.crd.astype(np.int64)
+    order = np.argsort(cols, kind="stable")
+    rows = np.repeat(np.arange(m, dtype=np.int32), np.diff(A.pos))
+    lim = int(np.searchsorted(cols[order], 1000))
+    assert np.array_equal(T.crd[:lim], rows[order][:lim]) and np.array_equal(T.vals[:lim], A.vals[order][:lim])
+    passes = -(-max(1, (n - 1).bit_length()) // 8)
+    per_step = {
+        # per entry and pass: key + value read and written
+        "radix_scatter": nnz * 32 * passes,
+        # per entry: column and value read, key and value written; row pointers
+        "tr_keys": nnz * (4 + 8 + 8 + 8) + 4 * (m + 1),
+        # per entry: key and value read, row index and value written
+        "tr_out": nnz * (8 + 8 + 4 + 8),
+    }
+    algorithmic = 24 * nnz + 4 * (m + 1) + 4 * (n + 1)
+    res = {
+        "metric": "CSR transpose GB/s", "value": round(algorithmic / sec / 1e9, 2), "unit": "GB/s",
+        "ms_per_step": round(sec * 1e3, 4), "dtype": "f64",
+        "config": {"workload": "csr_transpose_200k_32_per_row", "rows": m, "cols": n, "nnz": nnz,
+                   "radix_passes": passes, "seed": 3},
+        "roofline": _dominant(dom, dom_times, per_step, args.steps, "transpose"),
+        "compulsory_bytes": int(algorithmic),
+        "compulsory_frac_of_hbm_roofline": round(algorithmic / sec / 1e9 / HBM_PEAK_GBS, 5),
+        "kernel_ms_per_step": {k: round(v[0], 4) for k, v in sorted(bd.items())},
+        "cpu_baseline": None,
+    }
+    ctx.close()
+    return res
+
+
+WORKLOADS = {"rowwise": run_rowwise, "outer": run_outer, "mttkrp": run_mttkrp, "ttm": run_ttm, "serve": run_serve,
+             "transpose": run_transpose}
 DEFAULT_WORKLOAD = "outer"
-DEFAULT_STEPS = {"rowwise": (50, 5), "outer": (5, 1), "mttkrp": (20, 3), "ttm": (20, 3), "serve": (1, 1)}
+DEFAULT_STEPS = {"rowwise": (50, 5), "outer": (5, 1), "mttkrp": (20, 3), "ttm": (20, 3), "serve": (1, 1),
+                 "transpose": (20, 3)}
 
 
 def main(argv=None):
