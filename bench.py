@@ -681,9 +681,18 @@ def run_transpose(args):
         "compulsory_bytes": int(algorithmic),
         "compulsory_frac_of_hbm_roofline": round(algorithmic / sec / 1e9 / HBM_PEAK_GBS, 5),
         "kernel_ms_per_step": {k: round(v[0], 4) for k, v in sorted(bd.items())},
-        "cpu_baseline": None,
     }
     ctx.close()
+    if not args.no_cpu_baseline:
+        import oracle
+        reps, t0 = 0, time.perf_counter()
+        while reps < 3 or time.perf_counter() - t0 < 3.0:
+            oracle.csr_transpose(A)
+            reps += 1
+        dt = (time.perf_counter() - t0) / reps
+        res["cpu_baseline"] = {"value": round(algorithmic / dt / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+                               "sample": f"full workload, transpose() restatement (nonfuse/CSC_CSR_T_hash.h:352), "
+                                         f"{reps} runs"}
     return res
 
 

@@ -75,6 +75,8 @@ def lib():
         L.orc_wspace_merge.argtypes = [i32p, i32p, f64p, C.c_int64, C.c_void_p, C.c_int64, C.c_int]
         L.orc_free.restype = None
         L.orc_free.argtypes = [C.c_void_p]
+        L.orc_csr_transpose.restype = None
+        L.orc_csr_transpose.argtypes = [C.c_int32, C.c_int32, i32p, i32p, f64p, i32p, i32p, f64p]
         _lib = L
     return _lib
 
@@ -137,6 +139,18 @@ def spgemm_row_ub(A, B):
     ub = np.zeros(A.nrows, dtype=np.int64)
     lib().orc_spgemm_row_ub(A.nrows, _c32(A.pos), _c32(A.crd), _c32(B.pos), ub)
     return ub
+
+
+def csr_transpose(A):
+    """A^T as a CSR, each row's columns increasing (transpose() of
+    benchmark/CSC_CSR_T/nonfuse/CSC_CSR_T_hash.h:352-400)."""
+    from sparsews_amd.gen import CSR
+    nnz = int(A.pos[-1])
+    pos = np.zeros(A.ncols + 1, dtype=np.int32)
+    crd = np.zeros(nnz, dtype=np.int32)
+    vals = np.zeros(nnz, dtype=np.float64)
+    lib().orc_csr_transpose(A.nrows, A.ncols, _c32(A.pos), _c32(A.crd), _cf(A.vals), pos, crd, vals)
+    return CSR(A.ncols, A.nrows, pos, crd, vals)
 
 
 def spgemm_refsum(A, B, Cpos, Ccrd, nthreads: int = 0):

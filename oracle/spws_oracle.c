@@ -26,6 +26,7 @@
  *   orc_ttm_hash_il        benchmark/TTM/COO_CSF_DCSR_hash_il.h:247-377
  *   orc_wspace_sort        utils/wspace.h:206-217 (Sort; esc_cmp :24-31, esc_cmp_rev :32-39)
  *   orc_wspace_merge       utils/wspace.h:62-204 (Merge, both directions)
+ *   orc_csr_transpose      benchmark/CSC_CSR_T/nonfuse/CSC_CSR_T_hash.h:352-400 (transpose)
  */
 #include <limits.h>
 #include <stdint.h>
@@ -790,3 +791,24 @@ ORC_API int64_t orc_wspace_merge(int32_t* c0, int32_t* c1, double* v, int64_t co
 }
 
 ORC_API void orc_free(void* p) { free(p); }
+
+/* transpose(A, B) of benchmark/CSC_CSR_T/nonfuse/CSC_CSR_T_hash.h:352-400,
+ * At = A^T: the entries of each column of A are counted (:366-371), the
+ * counts prefixed into At's row pointers (:372-376), and A's rows scattered
+ * in row order through one cursor per column (:380-388), so each row of At
+ * lists its columns (A's rows) in increasing order. */
+ORC_API void orc_csr_transpose(int32_t m, int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax,
+                               int32_t* Atp, int32_t* Ati, double* Atx) {
+  int32_t* cur = (int32_t*)calloc((size_t)n + 1, sizeof(int32_t));
+  for (int32_t i = 0; i < m; i++)
+    for (int32_t p = Ap[i]; p < Ap[i + 1]; p++) cur[Ai[p] + 1]++;
+  for (int32_t c = 0; c < n; c++) cur[c + 1] += cur[c];
+  memcpy(Atp, cur, ((size_t)n + 1) * sizeof(int32_t));
+  for (int32_t i = 0; i < m; i++)
+    for (int32_t p = Ap[i]; p < Ap[i + 1]; p++) {
+      const int32_t q = cur[Ai[p]]++;
+      Ati[q] = i;
+      Atx[q] = Ax[p];
+    }
+  free(cur);
+}

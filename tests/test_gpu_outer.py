@@ -315,3 +315,7 @@ def test_device_transpose(ctx, m, n, lo, hi, seed):
     np.testing.assert_array_equal(np.asarray(T.pos, np.int64), ref.indptr.astype(np.int64))
     np.testing.assert_array_equal(T.crd, ref.indices)
     np.testing.assert_array_equal(T.vals, ref.data)
+    O = oracle.csr_transpose(A)  # the restatement of the reference's transpose()
+    np.testing.assert_array_equal(T.pos, O.pos)
+    np.testing.assert_array_equal(T.crd, O.crd)
+    np.testing.assert_array_equal(T.vals, O.vals)

@@ -160,3 +160,25 @@ def test_value_error_metric_catches_a_dropped_term():
     if len(q):
         bad[0] -= A.vals[A.pos[0]] * B.vals[B.pos[j] + q[0]]
         assert oracle.max_value_error(bad, exact, absum) > 1e-12
+
+
+@pytest.mark.parametrize("m,n,r,seed", [(300, 500, 7, 1), (1, 40, 40, 2), (50, 1, 1, 3), (200, 200, 0, 4)])
+def test_csr_transpose_oracle(m, n, r, seed):
+    """The transpose() restatement against scipy's transpose (row pointers,
+    increasing columns, values); transposing twice gives A back; empty rows
+    and columns (a transposed operand or a dropped count fails these)."""
+    import scipy.sparse as sp
+    from sparsews_amd.gen import uniform_csr
+    A = uniform_csr(m, n, r, seed, 0)
+    T = oracle.csr_transpose(A)
+    S = sp.csr_matrix((A.vals, A.crd, A.pos), shape=(m, n)).T.tocsr()
+    S.sort_indices()
+    assert (T.nrows, T.ncols) == (n, m)
+    np.testing.assert_array_equal(T.pos, S.indptr)
+    np.testing.assert_array_equal(T.crd, S.indices)
+    np.testing.assert_array_equal(T.vals, S.data)
+    TT = oracle.csr_transpose(T)
+    np.testing.assert_array_equal(TT.pos, A.pos)
+    np.testing.assert_array_equal(TT.crd, A.crd)
+    np.testing.assert_array_equal(TT.vals, A.vals)
+
