@@ -10,7 +10,9 @@ on the context's stream, recorded inside the timed region) and the CPU
 baseline (the oracle on the host cores, on a bounded sample).
 
 Workloads (sizes in the docstring of each runner):
-  outer     (default) outer-product DCSC x DCSR SpGEMM, 200,000 x 200,000,
+  all       (default) the outer product as the headline, with a record of
+            each of rowwise, mttkrp, ttm and serve under "workloads"
+  outer     outer-product DCSC x DCSR SpGEMM, 200,000 x 200,000,
             Zipf 1.8 row lengths, A = B^T
   rowwise   row-wise CSR x CSR hash SpGEMM, 10,000 x 10,000, 8 per row
   mttkrp    mode-1 MTTKRP, 50k x 40k x 30k, 20M nonzeros, rank 32
@@ -94,14 +96,24 @@ def calibration_copy(ctx, nbytes):
     ctx.free(dst)
 
 
-def roofline(dom_times, name, bytes_per_launch, workload):
+def roofline(dom, dom_times, steps, compulsory_per_step, kernel_io_per_step, workload):
+    """Roofline object of the dominant kernel `dom`.  `achieved` is the
+    workload's compulsory bytes (inputs read once, outputs written once) per
+    launch of `dom` divided by its average launch time from HIP events over
+    the timed steps.  `kernel_io` is the same kernel measured against the
+    bytes it reads and writes itself (workspace round trips included), and
+    `traffic` the HBM bytes per launch that rocprofv3 counted (committed
+    traffic summary), or None."""
     ms, launches = dom_times
     avg_s = ms / launches / 1e3
-    achieved = bytes_per_launch / avg_s / 1e9
+    alg = compulsory_per_step * steps / launches
+    achieved = alg / avg_s / 1e9
+    io = kernel_io_per_step * steps / launches
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(workload, name),
-            "kernel": name, "avg_launch_us": round(avg_s * 1e6, 3),
-            "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(workload, dom), "kernel": dom,
+            "avg_launch_us": round(avg_s * 1e6, 3), "algorithmic_bytes_per_launch": int(alg),
+            "kernel_io": {"bytes_per_launch": int(io), "achieved": round(io / avg_s / 1e9, 2),
+                          "frac": round(io / avg_s / 1e9 / HBM_PEAK_GBS, 4)}}
 
 
 def rocsparse_leg(ctx, dA, dB, C, products, steps, warmup):
@@ -169,6 +181,7 @@ def run_rowwise(args):
     # per product), slot writes (12 B per output entry), per-row metadata (row
     # id 4 + offset 8 + nnz 4 B).
     dom_bytes = m * 8 + A.nnz * 12 + A.nnz * 8 + P * 12 + nnz * 12 + m * 16
+    # compulsory bytes: A and B read once, C written once (int32 pointers)
     compulsory = (4 * (m + 1) + 12 * A.nnz) + (4 * (B.nrows + 1) + 12 * B.nnz) + (4 * (m + 1) + 12 * nnz)
     res = {
         "metric": "SpGEMM GFLOP/s",
@@ -178,7 +191,7 @@ def run_rowwise(args):
         "dtype": "f64",
         "config": {"workload": "rowwise_hash_spgemm_10k_r8", "m": m, "n": B.ncols, "nnz_per_row": 8,
                    "products": int(P), "nnz_C": int(nnz), "seed": 0},
-        "roofline": roofline(dom_times, dom, dom_bytes, "rowwise"),
+        "roofline": roofline(dom, dom_times, args.steps, compulsory, dom_bytes, "rowwise"),
         "compulsory_bytes": int(compulsory),
         "compulsory_frac_of_hbm_roofline": round(compulsory / sec / 1e9 / HBM_PEAK_GBS, 5),
         "kernel_ms_per_step": {k: round(v[0], 4) for k, v in sorted(bd.items())},
@@ -258,14 +271,11 @@ def run_outer(args):
             "outer_expand": 24 * P,
             "outer_reduce": 12 * P + 12 * nnz,
         }
-    ms, launches = dom_times
-    achieved = per_step[dom] * args.steps / (ms / 1e3) / 1e9
-    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("outer", dom), "kernel": dom,
-            "avg_launch_us": round(ms / launches * 1e3, 3),
-            "algorithmic_bytes_per_launch": int(per_step[dom] * args.steps / launches)}
+    # compulsory bytes: both operands read once, int64 row pointers and
+    # (column, value) of every output entry written once
     n_in = 2 * (4 * (N + 1) + 12 * B.nnz)
     compulsory = n_in + 8 * (N + 1) + 12 * nnz
+    roof = roofline(dom, dom_times, args.steps, compulsory, per_step[dom], "outer")
     res = {
         "metric": "SpGEMM GFLOP/s",
         "value": round(2 * P / sec / 1e9, 3),
@@ -339,15 +349,6 @@ def outer_cpu_baseline(B, seconds):
 
 # --------------------------------------------------------------------------- tensors
 
-def _dominant(dom, dom_times, per_step, steps, workload):
-    ms, launches = dom_times
-    achieved = per_step[dom] * steps / (ms / 1e3) / 1e9
-    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(workload, dom), "kernel": dom,
-            "avg_launch_us": round(ms / launches * 1e3, 3),
-            "algorithmic_bytes_per_launch": int(per_step[dom] * steps / launches)}
-
-
 def run_mttkrp(args):
     """Mode-1 MTTKRP, X 50,000 x 40,000 x 30,000 with 20M distinct nonzeros
     (per-mode Zipf 1.5 indices), FP64 U[0,1); dense factors of rank 32.  The
@@ -381,13 +382,15 @@ def run_mttkrp(args):
         "radix_scatter": 2 * 16 * nnz * passes,
         "mttkrp_keys": 4 * nnz + 4 * nnz + 8 * nnz,
     }
+    # compulsory bytes: X as COO (three int32 coordinates + FP64 value per
+    # nonzero), both factor matrices once, the dense output once
     compulsory = 20 * nnz + 8 * R * (J + K) + 8 * R * I
     res = {
         "metric": "MTTKRP GFLOP/s", "value": round(3 * R * nnz / sec / 1e9, 3), "unit": "GFLOP/s",
         "ms_per_step": round(sec * 1e3, 4), "dtype": "f64",
         "config": {"workload": "mttkrp_mode1_50k_40k_30k_20M_R32", "dims": [I, J, K], "nnz": nnz, "rank": R,
                    "zipf_s": 1.5, "rows_out": int(nrows), "seed": 0},
-        "roofline": _dominant(dom, dom_times, per_step, args.steps, "mttkrp"),
+        "roofline": roofline(dom, dom_times, args.steps, compulsory, per_step[dom], "mttkrp"),
         "compulsory_bytes": int(compulsory),
         "compulsory_frac_of_hbm_roofline": round(compulsory / sec / 1e9 / HBM_PEAK_GBS, 5),
         "kernel_ms_per_step": {k: round(v[0], 4) for k, v in sorted(bd.items())},
@@ -449,13 +452,15 @@ def run_ttm(args):
         # and value read; key + value written
         "ttm_keys": nnz * (4 + 4 + 4 + 4 + 4 + 8 + 16),
     }
+    # compulsory bytes: X as COO once, U once, every output fiber's (i, j)
+    # and R values written once
     compulsory = 20 * nnz + 8 * R * K + F * (8 + 8 * R)
     res = {
         "metric": "TTM GFLOP/s", "value": round(2 * R * nnz / sec / 1e9, 3), "unit": "GFLOP/s",
         "ms_per_step": round(sec * 1e3, 4), "dtype": "f64",
         "config": {"workload": "ttm_mode3_100k_80k_2k_10M_R16", "dims": [I, J, K], "nnz": nnz, "rank": R,
                    "fibers_out": int(F), "sort_bits": sort_bits, "radix_passes": passes, "seed": 0},
-        "roofline": _dominant(dom, dom_times, per_step, args.steps, "ttm"),
+        "roofline": roofline(dom, dom_times, args.steps, compulsory, per_step[dom], "ttm"),
         "compulsory_bytes": int(compulsory),
         "compulsory_frac_of_hbm_roofline": round(compulsory / sec / 1e9 / HBM_PEAK_GBS, 5),
         "kernel_ms_per_step": {k: round(v[0], 4) for k, v in sorted(bd.items())},
@@ -483,19 +488,18 @@ def run_ttm(args):
 P99_TARGET_MS = 2.0
 
 
-def latency_bound_rate(ctx, engines, reqs, mode, sat, cap, steps=7):
-    """Highest Poisson arrival rate at which `mode` keeps the p99 latency of
-    `reqs` within P99_TARGET_MS: bisection (geometric) between 2% and 100%
-    of the saturation rate `sat`, the same unit-rate arrival pattern scaled.
-    Returns (rate, p99 at that rate); rate 0 when even the lowest fails."""
-    from sparsews_amd.serve import serve
-    n = len(reqs)
+def latency_bound_rate(run, n, sat, steps=7):
+    """Highest Poisson arrival rate at which run(arrivals) keeps the p99
+    latency of its n requests within P99_TARGET_MS: bisection (geometric)
+    between 2% and 100% of the saturation rate `sat`, the same unit-rate
+    arrival pattern scaled.  Returns (rate, p99 at that rate); rate 0 when
+    even the lowest fails."""
     gaps = np.random.default_rng(777).exponential(1.0, n)
     unit = np.cumsum(gaps) - gaps[0]
 
     def p99_at(rate):
         arr = unit / rate
-        fin, _, _ = serve(ctx, engines, reqs, arr, mode, max_batch_products=cap)
+        fin = run(arr)
         return float(np.percentile((fin - arr) * 1e3, 99))
 
     lo, hi = 0.02 * sat, sat
@@ -519,10 +523,13 @@ def run_serve(args):
     """Batched / served SpGEMM: a stream of requests A_b * B_b, square n_b
     log-uniform in [256, 65,536], half with uniform 2..64 nonzeros per row
     and half Zipf(1.8) capped at 4,096, FP64 U[-1,1], seed 0.  Operands are
-    resident in HBM before the run.  Measured: saturation throughput (all
-    requests queued at t = 0) of the plain path (one row-wise call per
-    request) and of the continuously batched path (4 engines, one hipGraph
-    launch per batch, at most 2^26 products per batch); then Poisson
+    resident in HBM before the run; results stay on the device (in the
+    engines' pools, or the context's output for the plain path).  Measured:
+    saturation throughput (all requests queued at t = 0) of the plain path
+    (one row-wise call per request) and of the continuously batched path
+    (engines with one hipGraph launch per batch, a products cap per batch,
+    and a large lane: requests of at least 2^serve_large_log2 products run
+    one per engine on serve_large_engines reserved engines); then Poisson
     arrivals at 80% of the plain path's capacity for both, and at 3x that
     rate for the batched path, with p50/p99 latency (finish - arrival).
     Last, the latency-bound rate of each path: the highest Poisson arrival
@@ -535,23 +542,32 @@ def run_serve(args):
     ctx = Context(0)
     t0 = time.perf_counter()
     reqs, products = [], 0
+    in_bytes = 0
     for r in stream:
         A, B = gen.request_operands(r)
         d = DeviceRequest.upload(ctx, A, B)
         reqs.append(d)
         products += d.products
+        in_bytes += 4 * (A.nrows + 1) + 12 * A.nnz + 4 * (B.nrows + 1) + 12 * B.nnz
         del A, B
-        if len(reqs) % 512 == 0:
+        if len(reqs) % 1024 == 0:
             log(f"serve: {len(reqs)} requests generated and uploaded ({time.perf_counter() - t0:.0f} s)")
     N = len(reqs)
     n_engines, cap = args.serve_engines, 1 << args.serve_cap_log2
+    large = (1 << args.serve_large_log2) if args.serve_large_engines > 0 else 0
+    lanes = dict(max_batch_products=cap, large_products=large, large_engines=args.serve_large_engines)
     engines = [Engine(ctx) for _ in range(n_engines)]
     zero = np.zeros(N)
+
+    def batched(reqs_, arr_, **kw):
+        return serve(ctx, engines, reqs_, arr_, "batched", **lanes, **kw)
+
     # warm-up: one pass of each mode on a prefix
     serve(ctx, [], reqs[:32], zero[:32], "plain")
-    serve(ctx, engines, reqs[:32], zero[:32], "batched", max_batch_products=cap)
-    f_p, _, st_p = serve(ctx, [], reqs, zero, "plain")
-    f_b, bid, st_b = serve(ctx, engines, reqs, zero, "batched", max_batch_products=cap)
+    batched(reqs[:32], zero[:32])
+    f_p, _, st_p, nnz_p = serve(ctx, [], reqs, zero, "plain", with_nnz=True)
+    f_b, bid, st_b, nnz_b = batched(reqs, zero, with_nnz=True)
+    assert np.array_equal(nnz_p, nnz_b), "plain and batched output sizes differ"
     sat_plain = N / st_p["wall_seconds"]
     sat_batch = N / st_b["wall_seconds"]
     log(f"serve: saturation plain {sat_plain:.1f} req/s, batched {sat_batch:.1f} req/s")
@@ -560,28 +576,44 @@ def run_serve(args):
     gaps = rng.exponential(1.0 / lam, N)
     arr = np.cumsum(gaps) - gaps[0]
     f_pl, _, st_pl = serve(ctx, [], reqs, arr, "plain")
-    f_bl, _, st_bl = serve(ctx, engines, reqs, arr, "batched", max_batch_products=cap)
+    f_bl, _, st_bl = batched(reqs, arr)
     arr3 = arr / 3.0
-    f_b3, _, st_b3 = serve(ctx, engines, reqs, arr3, "batched", max_batch_products=cap)
-    bound = {"p99_target_ms": P99_TARGET_MS, "requests": min(N, 2048)}
+    f_b3, _, st_b3 = batched(reqs, arr3)
+    nb = min(N, 2048)
+    bound = {"p99_target_ms": P99_TARGET_MS, "requests": nb}
+    runs = {"plain": lambda a: serve(ctx, [], reqs[:nb], a, "plain")[0],
+            "batched": lambda a: batched(reqs[:nb], a)[0]}
     for mode, sat in (("plain", sat_plain), ("batched", sat_batch)):
-        rate, p99 = latency_bound_rate(ctx, engines if mode == "batched" else [], reqs[:2048], mode, sat, cap)
+        rate, p99 = latency_bound_rate(runs[mode], nb, sat)
         bound[mode] = {"req_per_s": round(rate, 2), "p99_ms": round(p99, 4)}
         log(f"serve: {mode} sustains {rate:.1f} req/s at p99 {p99:.3f} ms")
     bound["speedup"] = round(bound["batched"]["req_per_s"] / max(bound["plain"]["req_per_s"], 1e-9), 3)
     for e in engines:
         e.close()
-    ctx.close()
     sizes = np.array([r.products for r in reqs])
+    del reqs[:], runs
+    ctx.close()
     big = sizes >= np.percentile(sizes, 99)
     lat_b3 = (f_b3 - arr3) * 1e3
+    # compulsory bytes of the stream: every request's operands read once and
+    # its output (int32 row pointers, column and value per entry) written
+    # once; over the batched run's wall time at saturation (the batch
+    # pipeline's kernels run inside graphs, on several streams)
+    out_bytes = int(sum(4 * (r.n + 1) for r in stream)) + 12 * int(nnz_b.sum())
+    compulsory = in_bytes + out_bytes
+    achieved = compulsory / st_b["wall_seconds"] / 1e9
     res = {
         "metric": "batched SpGEMM req/s", "value": round(sat_batch, 2), "unit": "req/s",
         "ms_per_step": round(st_b["wall_seconds"] * 1e3, 3), "dtype": "f64",
         "config": {"workload": "served_spgemm_4096_requests", "requests": N, "n_range": [256, 65536],
-                   "products_total": int(products), "products_median": int(np.median(sizes)),
-                   "products_q99": int(np.percentile(sizes, 99)), "engines": n_engines,
-                   "max_batch_products": cap, "seed": 0},
+                   "products_total": int(products), "nnz_total": int(nnz_b.sum()),
+                   "products_median": int(np.median(sizes)), "products_q99": int(np.percentile(sizes, 99)),
+                   "engines": n_engines, "max_batch_products": cap, "large_lane_products": large,
+                   "large_lane_engines": args.serve_large_engines, "seed": 0},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "batch pipeline (all graphs, wall time of the saturated batched run)",
+                     "algorithmic_bytes_per_launch": None, "compulsory_bytes_stream": int(compulsory)},
         "saturation": {"plain_req_per_s": round(sat_plain, 2), "batched_req_per_s": round(sat_batch, 2),
                        "speedup": round(sat_batch / sat_plain, 3), "batches": st_b["batches"],
                        "fallback_requests": st_b["fallback_requests"],
@@ -677,7 +709,7 @@ def run_transpose(args):
         "ms_per_step": round(sec * 1e3, 4), "dtype": "f64",
         "config": {"workload": "csr_transpose_200k_32_per_row", "rows": m, "cols": n, "nnz": nnz,
                    "radix_passes": passes, "seed": 3},
-        "roofline": _dominant(dom, dom_times, per_step, args.steps, "transpose"),
+        "roofline": roofline(dom, dom_times, args.steps, algorithmic, per_step[dom], "transpose"),
         "compulsory_bytes": int(algorithmic),
         "compulsory_frac_of_hbm_roofline": round(algorithmic / sec / 1e9 / HBM_PEAK_GBS, 5),
         "kernel_ms_per_step": {k: round(v[0], 4) for k, v in sorted(bd.items())},
@@ -698,9 +730,30 @@ def run_transpose(args):
 
 WORKLOADS = {"rowwise": run_rowwise, "outer": run_outer, "mttkrp": run_mttkrp, "ttm": run_ttm, "serve": run_serve,
              "transpose": run_transpose}
-DEFAULT_WORKLOAD = "outer"
 DEFAULT_STEPS = {"rowwise": (50, 5), "outer": (5, 1), "mttkrp": (20, 3), "ttm": (20, 3), "serve": (1, 1),
                  "transpose": (20, 3)}
+# the default run: the outer product (headline) and one record per other
+# workload of the measured set, each with its own roofline and CPU baseline
+ALL_HEAD = "outer"
+ALL_SUB = ("rowwise", "mttkrp", "ttm", "serve")
+
+
+def run_one(name, args):
+    """One workload with its own step counts (those given on the command line
+    win); returns its record with steps and warmup filled in."""
+    import copy
+    a = copy.copy(args)
+    ds, dw = DEFAULT_STEPS[name]
+    a.steps = ds if args.steps is None else args.steps
+    a.warmup = dw if args.warmup is None else args.warmup
+    t0 = time.perf_counter()
+    res = WORKLOADS[name](a)
+    log(f"bench: {name} done in {time.perf_counter() - t0:.0f} s")
+    rec = {"metric": res.pop("metric"), "value": res.pop("value"), "unit": res.pop("unit"),
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": res.pop("ms_per_step"), "dtype": res.pop("dtype"),
+           "config": res.pop("config")}
+    rec.update(res)
+    return rec
 
 
 def main(argv=None):
@@ -708,14 +761,21 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="all", choices=sorted(WORKLOADS) + ["all"],
+                    help="all (default): the outer product as the headline line with a record for each of "
+                         + ", ".join(ALL_SUB) + " under 'workloads'")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-rocsparse", action="store_true", help="row-wise workload: skip the rocSPARSE comparison")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=None,
+                    help="CPU baseline seconds per workload (default 10; 5 in the combined run)")
     ap.add_argument("--requests", type=int, default=4096, help="serve workload: number of requests")
     ap.add_argument("--serve-engines", type=int, default=4, help="serve workload: batch engines")
     ap.add_argument("--serve-cap-log2", type=int, default=26,
                     help="serve workload: log2 of the products (output upper bound) per batch")
+    ap.add_argument("--serve-large-log2", type=int, default=24,
+                    help="serve workload: requests of at least 2^this many products form the large lane")
+    ap.add_argument("--serve-large-engines", type=int, default=1,
+                    help="serve workload: engines reserved for the large lane (0 = one lane)")
     ap.add_argument("--outer-method", type=int, default=0, choices=[0, 1, 2],
                     help="outer workload: 0 automatic, 1 LSD radix sort, 2 row/window buckets")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -730,15 +790,17 @@ def main(argv=None):
         return 2
     if int(os.environ.get("RANK", "0")) != 0:
         return 0
-    ds, dw = DEFAULT_STEPS[args.workload]
-    args.steps = ds if args.steps is None else args.steps
-    args.warmup = dw if args.warmup is None else args.warmup
-    res = WORKLOADS[args.workload](args)
-    line = {"metric": res.pop("metric"), "value": res.pop("value"), "unit": res.pop("unit"), "n_gpus": 1,
-            "steps": args.steps, "warmup": args.warmup, "ms_per_step": res.pop("ms_per_step"),
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": res.pop("dtype"),
-            "data": "synthetic", "config": res.pop("config")}
-    line.update(res)
+    combined = args.workload == "all"
+    if args.cpu_seconds is None:
+        args.cpu_seconds = 5.0 if combined else 10.0
+    head = run_one(ALL_HEAD if combined else args.workload, args)
+    line = {"metric": head.pop("metric"), "value": head.pop("value"), "unit": head.pop("unit"), "n_gpus": 1,
+            "steps": head.pop("steps"), "warmup": head.pop("warmup"), "ms_per_step": head.pop("ms_per_step"),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": head.pop("dtype"),
+            "data": "synthetic", "config": head.pop("config")}
+    line.update(head)
+    if combined:
+        line["workloads"] = {name: run_one(name, args) for name in ALL_SUB}
     print(json.dumps(line), flush=True)
     return 0
 

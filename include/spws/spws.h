@@ -253,10 +253,24 @@ int spws_batch_reset(spws_batch* b);
  * SPWS_ERR_UNSUPPORTED when a row exceeds max_row_products or the request
  * is wider than max_cols.  *slot names the
  * request in spws_batch_result.  The operand arrays must stay valid until
- * the batch completes. */
+ * the batch completes, and must be consistent: A_pos[m] = annz,
+ * B_pos[k] = bnnz, every A column < k and every B column < n.
+ * products must be at least the true sum over A's entries of the matching B
+ * row lengths, and max_row_products at least the largest row's sum: admission
+ * sizes the pools by them.  The batch recomputes both on the device before
+ * writing; when either was under-reported nothing is computed and
+ * spws_batch_nnz / spws_batch_result / spws_batch_result_nnz return
+ * SPWS_ERR_INVALID. */
 int spws_batch_add(spws_batch* b, int32_t m, int32_t k, int32_t n, const int32_t* A_pos, const int32_t* A_crd,
                    const double* A_vals, int64_t annz, const int32_t* B_pos, const int32_t* B_crd,
                    const double* B_vals, int64_t bnnz, int64_t products, int64_t max_row_products, int32_t* slot);
+/* Give request `slot` of the batch being filled an output buffer (device):
+ * C_pos[m + 1], C_crd and C_vals of `capacity` entries.  The batch writes the
+ * request's result (row pointers from 0) there instead of into its pools;
+ * spws_batch_result_nnz then gives its nnz, or -1 when it exceeds capacity
+ * (nothing copied).  C_pos = NULL removes the buffer. */
+int spws_batch_set_output(spws_batch* b, int32_t slot, int32_t* C_pos, int32_t* C_crd, double* C_vals,
+                          int64_t capacity);
 /* Launch the batch's graph on the engine's stream (asynchronous). */
 int spws_batch_launch(spws_batch* b);
 /* *done = 1 once the launched batch has completed. */
@@ -266,12 +280,18 @@ int spws_batch_wait(spws_batch* b);
 int spws_batch_nnz(spws_batch* b, int64_t* nnz);
 /* Result of one request of a completed batch: its row pointers rebased to 0
  * (host array of m+1, may be NULL), nnz, and device pointers to its columns
- * and values (engine-owned, valid until the engine's next launch). */
+ * and values: in the request's output buffer when it has one that holds the
+ * result, else engine-owned (valid until the engine's next launch). */
 int spws_batch_result(spws_batch* b, int32_t slot, int64_t* nnz, int32_t* pos_host, int32_t** crd, double** vals);
+/* Output nnz of request `slot` of the completed batch (-1: it exceeded the
+ * capacity given to spws_batch_set_output). */
+int spws_batch_result_nnz(spws_batch* b, int32_t slot, int64_t* nnz);
 void* spws_batch_stream(spws_batch* b);
 
 /* One request of the serving loop: device operands, the upper bounds used by
- * admission, and the arrival time in seconds from the start of the run. */
+ * admission, the arrival time in seconds from the start of the run, and an
+ * optional device output buffer (C_pos[m + 1]; C_crd, C_vals of C_cap
+ * entries; C_pos = NULL: the result is not kept). */
 typedef struct {
   int32_t m, k, n, reserved;
   const int32_t* A_pos;
@@ -285,6 +305,10 @@ typedef struct {
   int64_t products;
   int64_t max_row_products;
   double arrival;
+  int32_t* C_pos;
+  int32_t* C_crd;
+  double* C_vals;
+  int64_t C_cap;
 } spws_request;
 
 typedef struct {
@@ -293,6 +317,11 @@ typedef struct {
   int64_t max_batch_products;  /* batched: products (output upper bound) per batch, so a
                                   batch stays short and latency bounded; a single request
                                   larger than this still runs alone (0 = engine capacity) */
+  int64_t large_products;      /* batched: requests with at least this many products form the
+                                  large lane (0 = one lane for all) */
+  int32_t large_engines;       /* batched: engines reserved for the large lane, one request
+                                  each (< nengines) */
+  int32_t reserved;
 } spws_serve_options;
 
 typedef struct {
@@ -303,10 +332,13 @@ typedef struct {
 
 /* Run the request loop in real time: requests (sorted by arrival) arrive at
  * their arrival times; finish[r] receives the time (s) the loop saw request
- * r complete; batch_of[r] (optional) the batch that carried it (-1 = plain).
+ * r complete, its result already in the request's output buffer; batch_of[r]
+ * (optional) the batch that carried it (-1 = plain); result_nnz[r]
+ * (optional) its output nnz (-1: it exceeded C_cap, nothing copied).
  * Batched mode uses the nengines engines.  No counterpart in the reference. */
 int spws_serve(spws_ctx* ctx, spws_batch** engines, int32_t nengines, int32_t nreq, const spws_request* reqs,
-               const spws_serve_options* opt, double* finish, int32_t* batch_of, spws_serve_stats* stats);
+               const spws_serve_options* opt, double* finish, int32_t* batch_of, int64_t* result_nnz,
+               spws_serve_stats* stats);
 
 /* Host TACO entry points (host taco_tensor_t in, malloc'd host arrays out). */
 

@@ -43,6 +43,8 @@ def lib():
                                       C.c_int32, i32p, C.POINTER(PI32), C.POINTER(PF64)]
         L.orc_spgemm_symbolic.restype = C.c_int64
         L.orc_spgemm_symbolic.argtypes = [C.c_int32, C.c_int32, i32p, i32p, i32p, i32p, i32p, C.c_int]
+        L.orc_spgemm_row_counts.restype = C.c_int64
+        L.orc_spgemm_row_counts.argtypes = [C.c_int32, C.c_int32, i32p, i32p, i32p, i32p, i32p, C.c_int]
         L.orc_spgemm_gustavson.restype = C.c_int64
         L.orc_spgemm_gustavson.argtypes = [C.c_int32, C.c_int32, i32p, i32p, f64p, i32p, i32p, f64p,
                                            i32p, C.POINTER(PI32), C.POINTER(PF64), C.c_int]
@@ -133,6 +135,15 @@ def spgemm_symbolic(A, B, nthreads: int = 0):
     if nnz < 0:
         raise OverflowError("product has more than 2^31 - 1 entries (int32 CSR)")
     return pos
+
+
+def spgemm_row_counts(A, B, nthreads: int = 0):
+    """Entries per output row of A*B (int32 each) and their int64 total; no
+    int32 limit on the total."""
+    cnt = np.zeros(A.nrows, dtype=np.int32)
+    total = lib().orc_spgemm_row_counts(A.nrows, B.ncols, _c32(A.pos), _c32(A.crd), _c32(B.pos), _c32(B.crd),
+                                        cnt, int(nthreads))
+    return cnt, int(total)
 
 
 def spgemm_row_ub(A, B):

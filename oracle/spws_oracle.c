@@ -254,6 +254,40 @@ static int int_cmp(const void* a, const void* b) {
   return x < y ? -1 : (x > y ? 1 : 0);
 }
 
+/* The per-row counts of the symbolic phase, C2_nnz[i] = |U_{j in A_i}
+ * cols(B_j)| (seperate/CSR_CSR_1.h:69-108: a dense flag array and an index
+ * list per row), into cnt[m]; returns their int64 total, so products with
+ * more than 2^31 - 1 entries are counted too. */
+ORC_API int64_t orc_spgemm_row_counts(int32_t m, int32_t n, const int32_t* Ap, const int32_t* Ai,
+                                      const int32_t* Bp, const int32_t* Bi, int32_t* cnt, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+  int64_t total = 0;
+#pragma omp parallel reduction(+ : total)
+  {
+    unsigned char* seen = (unsigned char*)calloc((size_t)n + 1, 1);
+    int32_t* list = (int32_t*)malloc(((size_t)n + 1) * sizeof(int32_t));
+#pragma omp for schedule(dynamic, 64)
+    for (int32_t i = 0; i < m; i++) {
+      int32_t L = 0;
+      for (int32_t p = Ap[i]; p < Ap[i + 1]; p++) {
+        int32_t j = Ai[p];
+        for (int32_t q = Bp[j]; q < Bp[j + 1]; q++) {
+          int32_t k = Bi[q];
+          if (!seen[k]) { seen[k] = 1; list[L++] = k; }
+        }
+      }
+      for (int32_t t = 0; t < L; t++) seen[list[t]] = 0;
+      cnt[i] = L;
+      total += L;
+    }
+    free(seen);
+    free(list);
+  }
+  return total;
+}
+
 /* Symbolic phase (assemble): Cp[i+1]-Cp[i] = |U_{j in A_i} cols(B_j)|. */
 ORC_API int64_t orc_spgemm_symbolic(int32_t m, int32_t n, const int32_t* Ap, const int32_t* Ai,
                                     const int32_t* Bp, const int32_t* Bi, int32_t* Cp, int nthreads) {

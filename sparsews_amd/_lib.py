@@ -54,11 +54,13 @@ class Request(C.Structure):
     _fields_ = [("m", C.c_int32), ("k", C.c_int32), ("n", C.c_int32), ("reserved", C.c_int32),
                 ("A_pos", C.c_void_p), ("A_crd", C.c_void_p), ("A_vals", C.c_void_p), ("annz", C.c_int64),
                 ("B_pos", C.c_void_p), ("B_crd", C.c_void_p), ("B_vals", C.c_void_p), ("bnnz", C.c_int64),
-                ("products", C.c_int64), ("max_row_products", C.c_int64), ("arrival", C.c_double)]
+                ("products", C.c_int64), ("max_row_products", C.c_int64), ("arrival", C.c_double),
+                ("C_pos", C.c_void_p), ("C_crd", C.c_void_p), ("C_vals", C.c_void_p), ("C_cap", C.c_int64)]
 
 
 class ServeOptions(C.Structure):
-    _fields_ = [("mode", C.c_int32), ("max_batch_requests", C.c_int32), ("max_batch_products", C.c_int64)]
+    _fields_ = [("mode", C.c_int32), ("max_batch_requests", C.c_int32), ("max_batch_products", C.c_int64),
+                ("large_products", C.c_int64), ("large_engines", C.c_int32), ("reserved", C.c_int32)]
 
 
 class ServeStats(C.Structure):
@@ -123,8 +125,10 @@ SIGNATURES = {
     "spws_batch_wait": (C.c_int, [VP]),
     "spws_batch_nnz": (C.c_int, [VP, PI64]),
     "spws_batch_result": (C.c_int, [VP, I32, PI64, VP, PVP, PVP]),
+    "spws_batch_set_output": (C.c_int, [VP, I32, VP, VP, VP, I64]),
+    "spws_batch_result_nnz": (C.c_int, [VP, I32, PI64]),
     "spws_batch_stream": (VP, [VP]),
-    "spws_serve": (C.c_int, [VP, VP, I32, I32, C.POINTER(Request), C.POINTER(ServeOptions), VP, VP,
+    "spws_serve": (C.c_int, [VP, VP, I32, I32, C.POINTER(Request), C.POINTER(ServeOptions), VP, VP, VP,
                              C.POINTER(ServeStats)]),
     "spws_csr_csr_hash_compute": (C.c_int, [C.POINTER(TacoTensor)] * 3 + [I32]),
     "spws_csr_csr_assemble": (C.c_int, [C.POINTER(TacoTensor)] * 3),

@@ -19,11 +19,21 @@
 //   (persistent grids; each reads its row count from the bin metadata; the
 //   block bins use the LDS bitmap kernel over each row's own column range
 //   when the engine bounds the request width by max_cols)
-//   device-sized scan -> C_pos, k_compact (columns shifted back per row)
-//   k_batch_total, D2H of the batch nnz
+//   device-sized scan -> C_pos
+//   k_batch_req_meta   each request's nnz (and whether its output buffer holds it)
+//   k_batch_compact    rows into their request's output buffer when it has
+//                      one, into the engine's pools otherwise (columns
+//                      shifted back per row)
+//   k_batch_total, D2H of the batch nnz and of each request's nnz
 // No memset is recorded into the graph: clears are kernels.  The dense path
 // is not used in batches: the stacked column space is the sum of the
 // requests' widths, so long rows take the HBM hash path instead.
+//
+// Admission trusts the caller's upper bounds (products, max_row_products).
+// The graph recomputes them on the device (k_rowstat) and k_batch_guard
+// compares them with the engine's pools before any kernel writes into them:
+// when a bound was under-reported, the batch's row count is set to zero, so
+// every later kernel has nothing to do, and the batch reports an error.
 #include <algorithm>
 #include <chrono>
 #include <thread>
@@ -41,17 +51,24 @@ struct spws_batch_req {  // device-visible per-request descriptor
   int32_t m, k, n;
   int32_t row0, brow0, col0;
   int64_t annz0, bnnz0, annz, bnnz;
+  // optional output (device): row pointers [m + 1], columns and values
+  // [out_cap]; null out_pos = the result stays in the engine's pools
+  int32_t* out_pos;
+  int32_t* out_crd;
+  double* out_val;
+  int64_t out_cap;
 };
 
 struct spws_batch_desc {
   int32_t nreq, rows, brows, cols;
   int64_t annz, bnnz;
+  int32_t overflow, pad;  // set on the device when an upper bound was under-reported
 };
 
 enum BatchPool : int {
   BP_REQ = 0, BP_DESC, BP_APOS, BP_ACRD, BP_AVAL, BP_BPOS, BP_BCRD, BP_BVAL, BP_COLOFF, BP_ROWUB, BP_ROWOFF,
   BP_ROWNNZ, BP_BINROWS, BP_META, BP_SCRD, BP_SVAL, BP_CPOS, BP_CCRD, BP_CVAL, BP_WKEY, BP_WVAL, BP_SCAN, BP_TOTAL,
-  BP_COUNT
+  BP_REQNNZ, BP_COUNT
 };
 
 struct spws_batch {
@@ -62,6 +79,7 @@ struct spws_batch {
   spws_batch_req* h_req = nullptr;
   spws_batch_desc* h_desc = nullptr;
   int64_t* h_total = nullptr;
+  int64_t* h_req_nnz = nullptr;  // per request: output nnz (-1: out_cap too small, -2: overflow)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipEvent_t done = nullptr;
@@ -184,8 +202,8 @@ __device__ __forceinline__ void copy_rebased(const int32_t* __restrict__ crd, co
 }
 
 // operands of every request into the pools, indices rebased; per row, the
-// request's first column (row_coloff[row]) and width (row_coloff[max_rows +
-// row])
+// request's first column (row_coloff[row]), width (row_coloff[max_rows +
+// row]) and index (row_coloff[2 * max_rows + row])
 __global__ __launch_bounds__(256) void k_batch_gather(const spws_batch_req* __restrict__ reqs,
                                                       const spws_batch_desc* __restrict__ desc,
                                                       int32_t* __restrict__ A_pos, int32_t* __restrict__ A_crd,
@@ -202,6 +220,7 @@ __global__ __launch_bounds__(256) void k_batch_gather(const spws_batch_req* __re
     if (i < q.m) {
       row_coloff[q.row0 + i] = q.col0;
       row_coloff[max_rows + q.row0 + i] = q.n;  // the row's request width
+      row_coloff[2 * max_rows + q.row0 + i] = r;
     }
   }
   for (int64_t i = t0; i <= q.k; i += st) B_pos[q.brow0 + i] = (int32_t)(q.bnnz0 + q.B_pos[i]);
@@ -211,7 +230,105 @@ __global__ __launch_bounds__(256) void k_batch_gather(const spws_batch_req* __re
 
 __global__ void k_batch_total(const int32_t* __restrict__ C_pos, const spws_batch_desc* __restrict__ desc,
                               int64_t* __restrict__ total) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *total = C_pos[desc->rows];
+  if (threadIdx.x == 0 && blockIdx.x == 0) *total = desc->overflow ? -2 : C_pos[desc->rows];
+}
+
+// The true upper bounds (k_rowstat) against the pools: the sum of all row
+// bounds must fit the slot and output pools, and no row may exceed the
+// per-row capacity the hash-table pool was sized for.  On overflow the
+// batch's row count becomes zero (every later kernel reads it), the bins
+// are emptied and the overflow flag is set.
+__global__ void k_batch_guard(spws_batch_desc* __restrict__ desc, int32_t* __restrict__ bmeta,
+                              const unsigned long long* __restrict__ maxub, int64_t cap_products,
+                              int64_t cap_row_products) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  unsigned long long mx = 0;
+  for (int p = 0; p < P_EMPTY; ++p) mx = maxub[p] > mx ? maxub[p] : mx;
+  if (maxub[8] > (unsigned long long)cap_products || mx > (unsigned long long)cap_row_products) {
+    desc->rows = 0;
+    desc->overflow = 1;
+    for (int p = 0; p < 8; ++p) bmeta[kMetaCount + p] = 0;
+  }
+}
+
+// Per request, after the output row pointers are known: req_nnz[r] = its
+// nnz, -1 when it has an output buffer smaller than that (its rows then stay
+// in the pools), -2 when the batch overflowed.  A request with no rows gets
+// its single row pointer here.
+__global__ __launch_bounds__(256) void k_batch_req_meta(const spws_batch_req* __restrict__ reqs,
+                                                        const spws_batch_desc* __restrict__ desc,
+                                                        const int32_t* __restrict__ C_pos,
+                                                        int64_t* __restrict__ req_nnz) {
+  for (int32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < desc->nreq; r += gridDim.x * blockDim.x) {
+    if (desc->overflow) {
+      req_nnz[r] = -2;
+      continue;
+    }
+    const spws_batch_req& q = reqs[r];
+    const int64_t nnz = (int64_t)C_pos[q.row0 + q.m] - C_pos[q.row0];
+    const bool fits = nnz <= q.out_cap;
+    req_nnz[r] = (q.out_pos && !fits) ? -1 : nnz;
+    if (q.out_pos && fits && q.m == 0) q.out_pos[0] = 0;
+  }
+}
+
+// Slot regions -> contiguous rows, a wavefront per row (k_compact's loop),
+// with each row written straight into its request's output buffer when it
+// has one that holds the request (row pointers rebased to the request's
+// first entry; the request's last row also writes the closing pointer), and
+// into the engine's output pools otherwise.
+__global__ __launch_bounds__(256) void k_batch_compact(const spws_batch_desc* __restrict__ desc,
+                                                       const int64_t* __restrict__ row_off,
+                                                       const int32_t* __restrict__ C_pos,
+                                                       const int32_t* __restrict__ slot_crd,
+                                                       const double* __restrict__ slot_val,
+                                                       const int32_t* __restrict__ row_coloff,
+                                                       const int32_t* __restrict__ row_req,
+                                                       const spws_batch_req* __restrict__ reqs,
+                                                       const int64_t* __restrict__ req_nnz,
+                                                       int32_t* __restrict__ C_crd, double* __restrict__ C_val) {
+  const int32_t m = desc->rows;
+  const int lane = lane_id();
+  const int32_t waves = gridDim.x * (blockDim.x / kWave);
+  for (int32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < m; i += waves) {
+    const int32_t dst = C_pos[i];
+    const int32_t n = C_pos[i + 1] - dst;
+    const int64_t src = row_off[i];
+    const int32_t shift = row_coloff[i];
+    const int32_t r = row_req[i];
+    int32_t* oc = C_crd + dst;
+    double* ov = C_val + dst;
+    const spws_batch_req& q = reqs[r];
+    if (q.out_pos && req_nnz[r] >= 0) {
+      const int32_t start = C_pos[q.row0];
+      oc = q.out_crd + (dst - start);
+      ov = q.out_val + (dst - start);
+      if (lane == 0) {
+        q.out_pos[i - q.row0] = dst - start;
+        if (i == q.row0 + q.m - 1) q.out_pos[q.m] = dst + n - start;
+      }
+    }
+    for (int32_t e0 = 0; e0 < n; e0 += kWave * kCompactUnroll) {
+      int32_t c[kCompactUnroll];
+      double v[kCompactUnroll];
+#pragma unroll
+      for (int u = 0; u < kCompactUnroll; ++u) {
+        const int32_t e = e0 + u * kWave + lane;
+        if (e < n) {
+          c[u] = slot_crd[src + e];
+          v[u] = slot_val[src + e];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kCompactUnroll; ++u) {
+        const int32_t e = e0 + u * kWave + lane;
+        if (e < n) {
+          oc[e] = c[u] - shift;
+          ov[e] = v[u];
+        }
+      }
+    }
+  }
 }
 
 }  // namespace
@@ -259,6 +376,8 @@ static void enqueue_batch(spws_batch* b) {
   const unsigned gstat = (unsigned)std::min<int64_t>(div_up(c.max_rows, kRsRows), 16384);
   hipLaunchKernelGGL(k_rowstat, dim3(gstat), dim3(kRsBlock), 0, s, 0, m_dev, 1, Ap, Ai, Bp, row_ub, bmeta, maxub,
                      b->pp);
+  hipLaunchKernelGGL(k_batch_guard, dim3(1), dim3(64), 0, s, d_desc, bmeta, maxub, c.max_products,
+                     c.max_row_products);
   dscan<int64_t, int64_t>(s, row_ub, m_dev, (int64_t*)P(BP_SCAN), row_off);
   hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(64), 0, s, bmeta);
   hipLaunchKernelGGL(k_binscatter, dim3(grows), dim3(256), 0, s, 0, m_dev, 1, row_ub, bmeta, bin_rows, row_nnz, b->pp);
@@ -302,10 +421,15 @@ static void enqueue_batch(spws_batch* b) {
                        (int32_t*)P(BP_WKEY), (double*)P(BP_WVAL), b->wide_slots);
   }
   dscan<int32_t, int32_t>(s, row_nnz, m_dev, (int32_t*)P(BP_SCAN), Cp);
-  hipLaunchKernelGGL(k_compact, dim3((unsigned)std::min<int64_t>(div_up(c.max_rows, 4), 16384)), dim3(256), 0, s, 0,
-                     m_dev, row_off, Cp, scrd, sval, coloff, Ci, Cx, nullptr);
+  int64_t* req_nnz = (int64_t*)P(BP_REQNNZ);
+  hipLaunchKernelGGL(k_batch_req_meta, dim3((unsigned)div_up(c.max_requests, 256)), dim3(256), 0, s, d_req, d_desc,
+                     Cp, req_nnz);
+  hipLaunchKernelGGL(k_batch_compact, dim3((unsigned)std::min<int64_t>(div_up(c.max_rows, 4), 16384)), dim3(256), 0,
+                     s, d_desc, row_off, Cp, scrd, sval, coloff, coloff + 2 * (size_t)c.max_rows, d_req, req_nnz, Ci,
+                     Cx);
   hipLaunchKernelGGL(k_batch_total, dim3(1), dim3(64), 0, s, Cp, d_desc, (int64_t*)P(BP_TOTAL));
   (void)hipMemcpyAsync(b->h_total, P(BP_TOTAL), 8, hipMemcpyDeviceToHost, s);
+  (void)hipMemcpyAsync(b->h_req_nnz, req_nnz, 8 * (size_t)c.max_requests, hipMemcpyDeviceToHost, s);
 }
 
 static void batch_release(spws_batch* b) {
@@ -316,6 +440,7 @@ static void batch_release(spws_batch* b) {
   if (b->h_req) (void)hipHostFree(b->h_req);
   if (b->h_desc) (void)hipHostFree(b->h_desc);
   if (b->h_total) (void)hipHostFree(b->h_total);
+  if (b->h_req_nnz) (void)hipHostFree(b->h_req_nnz);
   if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
@@ -327,15 +452,16 @@ static int batch_build(spws_batch* b) {
   b->wide_groups = 64;
   const size_t sizes[BP_COUNT] = {
       sizeof(spws_batch_req) * (size_t)c.max_requests, sizeof(spws_batch_desc), (rows + 1) * 4, annz * 4, annz * 8,
-      (brows + 1) * 4, bnnz * 4, bnnz * 8, rows * 8, rows * 8, (rows + 1) * 8, rows * 4, rows * 4, 64 * 8, ub * 4,
+      (brows + 1) * 4, bnnz * 4, bnnz * 8, rows * 12, rows * 8, (rows + 1) * 8, rows * 4, rows * 4, 64 * 8, ub * 4,
       ub * 8, (rows + 1) * 4, ub * 4, ub * 8, (size_t)b->wide_groups * b->wide_slots * 4,
-      (size_t)b->wide_groups * b->wide_slots * 8, (kDScanBlocks + 1) * 8, 8};
+      (size_t)b->wide_groups * b->wide_slots * 8, (kDScanBlocks + 1) * 8, 8, (size_t)c.max_requests * 8};
   int rc;
   for (int i = 0; i < BP_COUNT; ++i)
     if ((rc = b->pool[i].ensure(sizes[i] > 0 ? sizes[i] : 8))) return rc;
   SPWS_HIP(hipHostMalloc((void**)&b->h_req, sizeof(spws_batch_req) * (size_t)c.max_requests, hipHostMallocDefault));
   SPWS_HIP(hipHostMalloc((void**)&b->h_desc, sizeof(spws_batch_desc), hipHostMallocDefault));
   SPWS_HIP(hipHostMalloc((void**)&b->h_total, 8, hipHostMallocDefault));
+  SPWS_HIP(hipHostMalloc((void**)&b->h_req_nnz, 8 * (size_t)c.max_requests, hipHostMallocDefault));
   memset(b->h_req, 0, sizeof(spws_batch_req) * (size_t)c.max_requests);
   memset(b->h_desc, 0, sizeof(spws_batch_desc));
   SPWS_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
@@ -442,6 +568,10 @@ int spws_batch_add(spws_batch* b, int32_t m, int32_t k, int32_t n, const int32_t
   q.bnnz0 = d.bnnz;
   q.annz = annz;
   q.bnnz = bnnz;
+  q.out_pos = nullptr;
+  q.out_crd = nullptr;
+  q.out_val = nullptr;
+  q.out_cap = 0;
   if (slot) *slot = d.nreq;
   d.nreq += 1;
   d.rows += m;
@@ -450,6 +580,22 @@ int spws_batch_add(spws_batch* b, int32_t m, int32_t k, int32_t n, const int32_t
   d.annz += annz;
   d.bnnz += bnnz;
   b->cur_ub += products;
+  return SPWS_OK;
+}
+
+int spws_batch_set_output(spws_batch* b, int32_t slot, int32_t* C_pos, int32_t* C_crd, double* C_vals,
+                          int64_t capacity) {
+  SPWS_REQUIRE(b != nullptr, SPWS_ERR_INVALID, "null batch");
+  SPWS_REQUIRE(!b->in_flight, SPWS_ERR_INVALID, "batch is still running");
+  SPWS_REQUIRE(slot >= 0 && slot < b->cur.nreq, SPWS_ERR_INVALID, "slot %d out of range", slot);
+  SPWS_REQUIRE(capacity >= 0, SPWS_ERR_INVALID, "negative output capacity");
+  SPWS_REQUIRE(C_pos == nullptr || capacity == 0 || (C_crd && C_vals), SPWS_ERR_INVALID,
+               "an output buffer with capacity needs columns and values");
+  spws_batch_req& q = b->h_req[slot];
+  q.out_pos = C_pos;
+  q.out_crd = C_crd;
+  q.out_val = C_vals;
+  q.out_cap = C_pos ? capacity : 0;
   return SPWS_OK;
 }
 
@@ -491,10 +637,26 @@ int spws_batch_wait(spws_batch* b) {
   return SPWS_OK;
 }
 
+static int batch_overflow_error() {
+  set_error("an upper bound given at admission (products or max_row_products) was below the true one; "
+            "the batch was not computed");
+  return SPWS_ERR_INVALID;
+}
+
 int spws_batch_nnz(spws_batch* b, int64_t* nnz) {
   SPWS_REQUIRE(b && nnz, SPWS_ERR_INVALID, "null argument");
   SPWS_REQUIRE(!b->in_flight, SPWS_ERR_INVALID, "batch is still running");
+  if (*b->h_total == -2) return batch_overflow_error();
   *nnz = *b->h_total;
+  return SPWS_OK;
+}
+
+int spws_batch_result_nnz(spws_batch* b, int32_t slot, int64_t* nnz) {
+  SPWS_REQUIRE(b && nnz, SPWS_ERR_INVALID, "null argument");
+  SPWS_REQUIRE(!b->in_flight, SPWS_ERR_INVALID, "batch is still running");
+  SPWS_REQUIRE(slot >= 0 && slot < b->cur.nreq, SPWS_ERR_INVALID, "slot %d out of range", slot);
+  if (*b->h_total == -2) return batch_overflow_error();
+  *nnz = b->h_req_nnz[slot];
   return SPWS_OK;
 }
 
@@ -502,6 +664,7 @@ int spws_batch_result(spws_batch* b, int32_t slot, int64_t* nnz, int32_t* pos_ho
   SPWS_REQUIRE(b != nullptr, SPWS_ERR_INVALID, "null batch");
   SPWS_REQUIRE(!b->in_flight, SPWS_ERR_INVALID, "batch is still running");
   SPWS_REQUIRE(slot >= 0 && slot < b->cur.nreq, SPWS_ERR_INVALID, "slot %d out of range", slot);
+  if (*b->h_total == -2) return batch_overflow_error();
   int rc = ctx_activate(b->ctx);
   if (rc) return rc;
   const spws_batch_req& q = b->h_req[slot];
@@ -513,8 +676,10 @@ int spws_batch_result(spws_batch* b, int32_t slot, int64_t* nnz, int32_t* pos_ho
   if (pos_host)
     for (int32_t i = 0; i <= q.m; ++i) pos_host[i] = tmp[i] - base;
   if (nnz) *nnz = tmp[q.m] - base;
-  if (crd) *crd = b->pool[BP_CCRD].as<int32_t>() + base;
-  if (vals) *vals = b->pool[BP_CVAL].as<double>() + base;
+  // a request with an output buffer that holds it was written there
+  const bool delivered = q.out_pos && b->h_req_nnz[slot] >= 0;
+  if (crd) *crd = delivered ? q.out_crd : b->pool[BP_CCRD].as<int32_t>() + base;
+  if (vals) *vals = delivered ? q.out_val : b->pool[BP_CVAL].as<double>() + base;
   return SPWS_OK;
 }
 

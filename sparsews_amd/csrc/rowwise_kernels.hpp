@@ -1293,7 +1293,9 @@ __global__ __launch_bounds__(kWideBlock) void k_row_wide(const int32_t* __restri
 
 constexpr int kCompactUnroll = 8;
 
-__global__ __launch_bounds__(256) void k_compact(int32_t m_host, const int32_t* __restrict__ m_dev,
+// (the batch engine compacts with its own kernel, which writes into request
+// output buffers, so this one is unused there)
+[[maybe_unused]] __global__ __launch_bounds__(256) void k_compact(int32_t m_host, const int32_t* __restrict__ m_dev,
                                                  const int64_t* __restrict__ row_off,
                                                  const int32_t* __restrict__ C_pos,
                                                  const int32_t* __restrict__ slot_crd,

@@ -441,6 +441,9 @@ int spws_csr_transpose(taco_tensor_t* A, taco_tensor_t* B) {
   if (rc) return rc;
   SPWS_REQUIRE(A && A->dimensions && A->indices && A->indices[1], SPWS_ERR_INVALID,
                "A must be initialised as a 2-level tensor (init_taco_tensor_DC)");
+  SPWS_REQUIRE(A->dimensions[0] == B->dimensions[1] && A->dimensions[1] == B->dimensions[0], SPWS_ERR_INVALID,
+               "A (%d x %d) must have B's transposed shape (%d x %d)", A->dimensions[0], A->dimensions[1],
+               B->dimensions[1], B->dimensions[0]);
   spws_ctx* ctx;
   if ((rc = default_ctx(&ctx)) || (rc = ctx_activate(ctx))) return rc;
   const int32_t m = B->dimensions[0], n = A->dimensions[0];

@@ -184,15 +184,22 @@ def test_admission_limits(ctx):
 
 
 def test_serve_plain_and_batched(ctx):
+    """Both loop modes hand back every request's product (copied into the
+    request's own output buffer), equal to the oracle's and to the same
+    request run alone; the batched loop runs two engines and a large lane."""
     print("case: serve loop", flush=True)
     pairs = make_requests(8, 48)
-    reqs = [DeviceRequest.upload(ctx, A, B) for A, B in pairs]
+    reqs = [DeviceRequest.upload(ctx, A, B, output=True) for A, B in pairs]
     arrivals = np.cumsum(np.random.default_rng(8).exponential(2e-4, len(reqs)))
     arrivals -= arrivals[0]
-    engines = [Engine(ctx, default_caps(**SMALL_CAPS)) for _ in range(2)]
+    engines = [Engine(ctx, default_caps(**SMALL_CAPS)) for _ in range(3)]
+    large_at = int(np.percentile([r.products for r in reqs], 80))
     try:
-        f_plain, b_plain, st_plain = serve(ctx, [], reqs, arrivals, "plain")
-        f_bat, b_bat, st_bat = serve(ctx, engines, reqs, arrivals, "batched")
+        f_plain, b_plain, st_plain, n_plain = serve(ctx, [], reqs, arrivals, "plain", with_nnz=True)
+        plain_results = [r.result(int(k)) for r, k in zip(reqs, n_plain)]
+        f_bat, b_bat, st_bat, n_bat = serve(ctx, engines, reqs, arrivals, "batched", large_products=large_at,
+                                            large_engines=1, with_nnz=True)
+        batch_results = [r.result(int(k)) for r, k in zip(reqs, n_bat)]
     finally:
         for e in engines:
             e.close()
@@ -200,4 +207,111 @@ def test_serve_plain_and_batched(ctx):
     assert np.all(b_plain == -1)
     assert st_bat["batches"] >= 1 and st_bat["batches"] <= len(reqs)
     assert np.all(b_bat >= 0)
+    for (A, B), r, Cp, Cb in zip(pairs, reqs, plain_results, batch_results):
+        _, absum = check_result(Cp, A, B)
+        check_result(Cb, A, B)
+        alone = spgemm_rowwise(ctx, r.A, r.B).download()
+        np.testing.assert_array_equal(alone.crd, Cb.crd)
+        assert oracle.max_value_error(Cb.vals, alone.vals, absum) <= TOL
     print("plain", latency_summary(arrivals, f_plain), "batched", latency_summary(arrivals, f_bat), st_bat)
+
+
+def test_output_capacity_too_small(ctx):
+    """A request whose output buffer is smaller than its product reports -1
+    and the rest of its batch is delivered."""
+    print("case: output capacity", flush=True)
+    pairs = make_requests(11, 6)
+    reqs = [DeviceRequest.upload(ctx, A, B, output=True) for A, B in pairs]
+    nnz_true = [oracle.spgemm_gustavson(A, B).nnz for A, B in pairs]
+    reqs[2].alloc_output(ctx, capacity=max(nnz_true[2] - 1, 0))
+    engines = [Engine(ctx, default_caps(**SMALL_CAPS))]
+    try:
+        _, _, _, n_bat = serve(ctx, engines, reqs, np.zeros(len(reqs)), "batched", with_nnz=True)
+        _, _, _, n_plain = serve(ctx, [], reqs, np.zeros(len(reqs)), "plain", with_nnz=True)
+    finally:
+        engines[0].close()
+    for n_out in (n_bat, n_plain):
+        assert n_out[2] == -1
+        for t in (0, 1, 3, 4, 5):
+            assert n_out[t] == nnz_true[t]
+            check_result(reqs[t].result(int(n_out[t])), *pairs[t])
+
+
+def test_underreported_bounds_are_refused_on_the_device(ctx):
+    """Admission trusts the caller's products; the graph recomputes them and
+    refuses a batch whose true bounds exceed the pools (nothing is written),
+    and the engine computes the next batch correctly."""
+    print("case: under-reported upper bound", flush=True)
+    caps = default_caps(max_requests=4, max_rows=1 << 12, max_nnz=1 << 16, max_products=1 << 14,
+                        max_row_products=1 << 10)
+    e = Engine(ctx, caps)
+    try:
+        A, B = uniform_rows(400, 400, 8, 12, 801, 0), uniform_rows(400, 400, 8, 12, 801, 1)
+        r = DeviceRequest.upload(ctx, A, B)
+        assert r.products > caps.max_products
+        r.products = 1000  # the caller's claim
+        e.reset()
+        e.add(r)
+        e.launch()
+        e.wait()
+        with pytest.raises(SpwsError) as ex:
+            e.nnz()
+        assert ex.value.code == 1
+        with pytest.raises(SpwsError):
+            e.result_nnz(0)
+        a, b = uniform_rows(60, 60, 0, 5, 802, 0), uniform_rows(60, 60, 0, 5, 802, 1)
+        results, _ = run_batch(e, ctx, [(a, b)])
+        check_result(results[0], a, b)
+    finally:
+        e.close()
+
+
+def config4_selection(stream, prefix, largest):
+    """Indices of the first `prefix` requests of the stream and of its
+    `largest` largest requests of each kind (by rows; the uniform ones hold
+    the stream's q99-and-above product counts)."""
+    pick = set(range(prefix))
+    for kind in ("uniform", "powerlaw"):
+        ranked = sorted((r for r in stream if r.kind == kind), key=lambda r: -r.n)
+        pick.update(r.rid for r in ranked[:largest])
+    return sorted(pick)
+
+
+def test_config4_stream_served_continuously(ctx):
+    """The served-SpGEMM request mix itself (n log-uniform in 256..65,536,
+    half uniform 2..64 entries per row, half Zipf(1.8) capped at 4,096; seed
+    0): its first 120 requests and the largest ones (n up to 65,517, about
+    7e7 products), arriving as a Poisson stream fast enough that batches
+    overlap, through the continuously batched loop with four engines and a
+    large lane.  Every served product is checked against the oracle
+    (structure bit-exact, values within 1e-12) and against the same request
+    computed alone."""
+    stream = gen.request_stream(count=4096, seed=0)
+    idx = config4_selection(stream, 120, 4)
+    print(f"case: config-4 stream, {len(idx)} requests", flush=True)
+    pairs = [gen.request_operands(stream[i]) for i in idx]
+    reqs = [DeviceRequest.upload(ctx, A, B, output=True) for A, B in pairs]
+    products = np.array([r.products for r in reqs])
+    assert max(r.n for r in reqs) >= 65_000 and products.max() >= 6e7
+    gaps = np.random.default_rng(44).exponential(1.0 / 20_000.0, len(reqs))
+    arrivals = np.cumsum(gaps) - gaps[0]
+    engines = [Engine(ctx) for _ in range(4)]
+    try:
+        fin, bid, st, nnz = serve(ctx, engines, reqs, arrivals, "batched", max_batch_products=1 << 26,
+                                  large_products=1 << 24, large_engines=1, with_nnz=True)
+    finally:
+        for e in engines:
+            e.close()
+    print("served:", st, latency_summary(arrivals, fin), flush=True)
+    assert st["fallback_requests"] == 0
+    # batches carried several requests each
+    assert st["batches"] < len(reqs)
+    for t, ((A, B), r) in enumerate(zip(pairs, reqs)):
+        C = r.result(int(nnz[t]))
+        _, absum = check_result(C, A, B)
+        alone = spgemm_rowwise(ctx, r.A, r.B).download()
+        np.testing.assert_array_equal(alone.pos, C.pos)
+        np.testing.assert_array_equal(alone.crd, C.crd)
+        assert oracle.max_value_error(C.vals, alone.vals, absum) <= TOL
+        for arr in (r.C_pos, r.C_crd, r.C_vals):
+            arr.free()

@@ -14,7 +14,7 @@ import pytest
 import oracle
 from helpers import powerlaw_rows, random_csr, uniform_rows
 from sparsews_amd import formats, gen
-from sparsews_amd.device import DeviceDCSR, set_outer_chunking, set_outer_method, spgemm_outer
+from sparsews_amd.device import DeviceArray, DeviceDCSR, set_outer_chunking, set_outer_method, spgemm_outer
 from sparsews_amd.gen import CSR
 
 pytestmark = pytest.mark.gpu
@@ -202,9 +202,32 @@ def test_outer_workload_scaled(ctx, method):
     check(C, A, B)
 
 
+def row_sums_chunked(C, chunk=1 << 27):
+    """Row sums of a device CSR with int64 row pointers, its values copied
+    to the host a chunk at a time (a row split by a chunk boundary sums its
+    parts)."""
+    pos = C.pos.to_host().astype(np.int64)
+    out = np.zeros(C.nrows)
+    for c0 in range(0, C.nnz, chunk):
+        c1 = min(c0 + chunk, C.nnz)
+        v = DeviceArray.view(C.vals.ctx, C.vals.ptr + 8 * c0, c1 - c0, np.float64).to_host()
+        r_lo = int(np.searchsorted(pos, c0, "right")) - 1
+        r_hi = int(np.searchsorted(pos, c1, "left"))
+        rows = np.arange(r_lo, r_hi)
+        beg = np.maximum(pos[rows], c0) - c0
+        end = np.minimum(pos[rows + 1], c1) - c0
+        keep = end > beg
+        out[rows[keep]] += np.add.reduceat(v, beg[keep])
+    return pos, out
+
+
 def test_outer_workload_full(ctx):
-    """The 200,000 x 200,000, Zipf 1.8 workload at full size: products count,
-    row-pointer invariants, and 64 sampled rows exactly against the oracle."""
+    """The 200,000 x 200,000, Zipf 1.8 workload at full size (1.16e10
+    products, 1.0e10 entries): every row's entry count against the oracle's
+    int64 symbolic count (bit-exact), every row sum against A (B 1) (the sum
+    of a row of C is the row of A times B's row sums; within the rounding
+    bound n eps sum|terms| of reordering n terms, 1e-10 sum|terms| here), the
+    total, and 64 sampled rows entry by entry against the oracle."""
     print("case: outer full workload", flush=True)
     B = gen.outer_operand()
     L = B.row_lengths()
@@ -214,12 +237,21 @@ def test_outer_workload_full(ctx):
     C = spgemm_outer(ctx, dA, dB, N, N, N)
     st = ctx.stats()
     assert st["products"] == int((L * L).sum())
+    AT = B.transpose()
+    cnt, total = oracle.spgemm_row_counts(AT, B)
+    assert C.nnz == total
+    pos, rs = row_sums_chunked(C)
+    assert pos[0] == 0 and pos[-1] == C.nnz
+    np.testing.assert_array_equal(np.diff(pos), cnt.astype(np.int64))
+    Sa, Sb = AT.to_scipy(), B.to_scipy()
+    ref = Sa @ (Sb @ np.ones(N))
+    absum = abs(Sa) @ (abs(Sb) @ np.ones(N))
+    assert np.all(np.abs(rs - ref) <= 1e-10 * absum)
+    assert abs(rs.sum() - ref.sum()) <= 1e-10 * absum.sum()
     rng = np.random.default_rng(123)
     rows = np.sort(rng.choice(N, 64, replace=False))
-    pos, got = C.rows(rows)
-    assert pos[0] == 0 and pos[-1] == C.nnz and np.all(np.diff(pos) >= 0)
+    _, got = C.rows(rows)
     # oracle on the sampled rows of A = B^T
-    AT = B.transpose()
     lens = AT.row_lengths()[rows]
     sub_pos = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
     sub_crd = np.concatenate([AT.crd[AT.pos[r]:AT.pos[r + 1]] for r in rows])

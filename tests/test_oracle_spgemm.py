@@ -57,6 +57,17 @@ def test_symbolic_and_upper_bound():
     assert np.all(ub >= np.diff(pos))
 
 
+@pytest.mark.parametrize("name,make", CASES)
+def test_row_counts_match_the_pattern_product(name, make):
+    """Per-row entry counts (int64 total, no int32 prefix) equal the row
+    lengths of the integer pattern product."""
+    A, B = make()
+    pos, _, _ = pattern_product(A, B)
+    cnt, total = oracle.spgemm_row_counts(A, B)
+    np.testing.assert_array_equal(cnt, np.diff(pos))
+    assert total == int(pos[-1])
+
+
 def test_identity_gives_b_exactly():
     B = uniform_rows(120, 90, 0, 15, 4, 1)
     n = B.nrows
