@@ -41,6 +41,10 @@ def lib():
         L.orc_spgemm_hash.restype = C.c_int64
         L.orc_spgemm_hash.argtypes = [C.c_int32, C.c_int32, i32p, i32p, f64p, i32p, i32p, f64p,
                                       C.c_int32, i32p, C.POINTER(PI32), C.POINTER(PF64)]
+        L.orc_spgemm_hash_parallel.restype = C.c_int64
+        L.orc_spgemm_hash_parallel.argtypes = [C.c_int32, C.c_int32, i32p, i32p, f64p, i32p, i32p, f64p,
+                                               C.c_int32, C.c_int32, i32p, C.POINTER(PI32), C.POINTER(PF64),
+                                               C.POINTER(C.c_int64)]
         L.orc_spgemm_symbolic.restype = C.c_int64
         L.orc_spgemm_symbolic.argtypes = [C.c_int32, C.c_int32, i32p, i32p, i32p, i32p, i32p, C.c_int]
         L.orc_spgemm_row_counts.restype = C.c_int64
@@ -113,6 +117,20 @@ def spgemm_hash(A, B, w_cap: int):
                                 _c32(B.pos), _c32(B.crd), _cf(B.vals), int(w_cap), pos,
                                 C.byref(ci), C.byref(cx))
     return CSR(A.nrows, B.ncols, pos, _take(ci, nnz, np.int32), _take(cx, nnz, np.float64))
+
+
+def spgemm_hash_parallel(A, B, w_cap: int, cap: int):
+    """CAP-parallel hash workspace (CSR_CSR_hash_parallel.h:235-408): `cap`
+    workers over rows i = t mod cap and one merger.  Returns (CSR, number of
+    mid-run flushes)."""
+    from sparsews_amd.gen import CSR
+    pos = np.zeros(A.nrows + 1, dtype=np.int32)
+    ci, cx = PI32(), PF64()
+    fl = C.c_int64()
+    nnz = lib().orc_spgemm_hash_parallel(A.nrows, B.ncols, _c32(A.pos), _c32(A.crd), _cf(A.vals),
+                                         _c32(B.pos), _c32(B.crd), _cf(B.vals), int(w_cap), int(cap), pos,
+                                         C.byref(ci), C.byref(cx), C.byref(fl))
+    return CSR(A.nrows, B.ncols, pos, _take(ci, nnz, np.int32), _take(cx, nnz, np.float64)), int(fl.value)
 
 
 def spgemm_gustavson(A, B, nthreads: int = 0):

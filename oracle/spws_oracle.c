@@ -15,7 +15,10 @@
  *   orc_spgemm_hash        benchmark/Parallel/CSR_CSR_hash_sequential.h:232-353
  *                          (TryInsert_hash :151-190, flush/Merge_hash :55-123,
  *                           refresh_wspace growth :202-230)
+ *   orc_spgemm_hash_parallel benchmark/Parallel/CSR_CSR_hash_parallel.h:235-408 (CAP
+ *                          workers + a merger; the racy flush handshake serialised)
  *   orc_spgemm_gustavson   benchmark/CSR_CSR/CSR_CSR_taco_parallel.h:4-113
+ *   orc_spgemm_row_counts  seperate/CSR_CSR_1.h:69-108 (C2_nnz per row, int64 total)
  *   orc_spgemm_symbolic    seperate/CSR_CSR_1.h:69-167 (assemble: C2_nnz -> C2_pos)
  *   orc_spgemm_row_ub      evaluate/CSR_CSR_1.h:153-171 (C2_nnz_upper_bound)
  *   orc_spgemm_refsum      FP64 sorted-summation oracle (no reference counterpart):
@@ -247,6 +250,103 @@ ORC_API int64_t orc_spgemm_hash(int32_t m, int32_t n,
   *Ci = ci;
   *Cx = cx;
   return nnz;
+}
+
+/* CAP-parallel hash SpGEMM (benchmark/Parallel/CSR_CSR_hash_parallel.h:
+ * 235-408): `cap` worker threads, worker t inserting the rows i = t, t + cap,
+ * ... into its own bounded hash workspace (same flush schedule as the
+ * sequential path, :280-298), and one merger that merges each worker's
+ * sorted epochs into the global sorted array with add-on-equal (Merge_hash
+ * :55-123).  The reference hands a full buffer to the merger through a spin
+ * on a plain int (racy: it can hang or merge a buffer being refilled); here
+ * a worker's mid-run epochs go into its own sorted array, and the merger
+ * merges the workers' arrays in worker order after they finish.  Workers own
+ * disjoint rows, so equal keys only ever meet within one worker, and every
+ * value is that worker's epoch partials added in epoch order, as in the
+ * reference.  *flushes receives the number of mid-run epochs (0 whenever
+ * every worker's entries fit 2 w_cap, e.g. the 10,000 x 10,000 8-per-row
+ * workload with cap >= 5 and w_cap 2^16).  Returns nnz(C). */
+ORC_API int64_t orc_spgemm_hash_parallel(int32_t m, int32_t n, const int32_t* Ap, const int32_t* Ai,
+                                         const double* Ax, const int32_t* Bp, const int32_t* Bi,
+                                         const double* Bx, int32_t w_cap, int32_t cap, int32_t* Cp, int32_t** Ci,
+                                         double** Cx, int64_t* flushes) {
+  (void)n;
+  if (cap < 1) cap = 1;
+  orc_hws* ws = (orc_hws*)calloc((size_t)cap, sizeof(orc_hws));
+  int64_t* nfl = (int64_t*)calloc((size_t)cap, sizeof(int64_t));
+#pragma omp parallel num_threads(cap)
+  {
+#ifdef _OPENMP
+    const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+#else
+    const int t = 0, nt = 1;
+#endif
+    /* a worker per thread; with fewer threads than cap, a thread runs several */
+    for (int wkr = t; wkr < cap; wkr += nt) {
+      orc_hws* w = &ws[wkr];
+      hws_init(w, w_cap, 0);
+      for (int32_t i = wkr; i < m; i += cap) {
+        for (int32_t p = Ap[i]; p < Ap[i + 1]; p++) {
+          int32_t j = Ai[p];
+          double a = Ax[p];
+          for (int32_t q = Bp[j]; q < Bp[j + 1]; q++) {
+            uint64_t key = ((uint64_t)(uint32_t)i << 32) | (uint32_t)Bi[q];
+            if (w->numel == w->cap) nfl[wkr]++;
+            hws_insert(w, key, a * Bx[q]);
+          }
+        }
+      }
+      hws_flush(w);
+    }
+  }
+  /* the merger: each worker's sorted array into the global one, in order */
+  uint64_t* gk = (uint64_t*)malloc(sizeof(uint64_t));
+  double* gv = (double*)malloc(sizeof(double));
+  int64_t gs = 0, tot_fl = 0;
+  for (int wkr = 0; wkr < cap; wkr++) {
+    orc_hws* w = &ws[wkr];
+    tot_fl += nfl[wkr];
+    int64_t need = gs + w->asize;
+    uint64_t* nk = (uint64_t*)malloc((size_t)(need > 0 ? need : 1) * sizeof(uint64_t));
+    double* nv = (double*)malloc((size_t)(need > 0 ? need : 1) * sizeof(double));
+    int64_t a = 0, c = 0, o = 0;
+    while (a < w->asize && c < gs) {
+      if (w->akey[a] == gk[c]) {
+        nk[o] = gk[c];
+        nv[o] = w->aval[a] + gv[c];
+        a++; c++; o++;
+      } else if (w->akey[a] < gk[c]) {
+        nk[o] = w->akey[a]; nv[o] = w->aval[a]; a++; o++;
+      } else {
+        nk[o] = gk[c]; nv[o] = gv[c]; c++; o++;
+      }
+    }
+    while (a < w->asize) { nk[o] = w->akey[a]; nv[o] = w->aval[a]; a++; o++; }
+    while (c < gs) { nk[o] = gk[c]; nv[o] = gv[c]; c++; o++; }
+    free(gk);
+    free(gv);
+    gk = nk;
+    gv = nv;
+    gs = o;
+    hws_free(w);
+  }
+  free(ws);
+  free(nfl);
+  int32_t* ci = (int32_t*)malloc((size_t)(gs > 0 ? gs : 1) * sizeof(int32_t));
+  double* cx = (double*)malloc((size_t)(gs > 0 ? gs : 1) * sizeof(double));
+  memset(Cp, 0, (size_t)(m + 1) * sizeof(int32_t));
+  for (int64_t t = 0; t < gs; t++) {
+    Cp[(int32_t)(gk[t] >> 32) + 1]++;
+    ci[t] = (int32_t)(uint32_t)(gk[t] & 0xffffffffu);
+    cx[t] = gv[t];
+  }
+  for (int32_t r = 0; r < m; r++) Cp[r + 1] += Cp[r];
+  free(gk);
+  free(gv);
+  *Ci = ci;
+  *Cx = cx;
+  if (flushes) *flushes = tot_fl;
+  return gs;
 }
 
 static int int_cmp(const void* a, const void* b) {

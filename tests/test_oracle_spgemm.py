@@ -120,6 +120,33 @@ def test_hash_without_flush_is_left_to_right():
     np.testing.assert_array_equal(H.vals, G.vals)
 
 
+@pytest.mark.parametrize("cap,w_cap", [(4, 1 << 20), (7, 1 << 20), (3, 64), (5, 16)])
+def test_hash_parallel_is_each_workers_sequential_hash(cap, w_cap):
+    """The CAP-parallel workspace (CSR_CSR_hash_parallel.h): worker t owns
+    rows t, t + cap, ..., so each of its rows must equal, bit for bit, the
+    sequential hash path run on A restricted to worker t's rows (the same
+    flush schedule over the same inserts); without flushes every row is the
+    left-to-right (Gustavson) sum."""
+    A, B = powerlaw_rows(300, 250, 1.8, 120, 11, 0), powerlaw_rows(250, 280, 1.8, 120, 11, 1)
+    P, flushes = oracle.spgemm_hash_parallel(A, B, w_cap, cap)
+    G = oracle.spgemm_gustavson(A, B)
+    assert_same_structure(P, G.pos, G.crd)
+    if w_cap >= 1 << 20:
+        assert flushes == 0
+        np.testing.assert_array_equal(P.vals, G.vals)
+    else:
+        assert flushes > 0
+    for t in range(cap):
+        lens = A.row_lengths().copy()
+        lens[np.arange(A.nrows) % cap != t] = 0
+        keep = np.repeat(np.arange(A.nrows) % cap == t, A.row_lengths())
+        At = CSR(A.nrows, A.ncols, np.concatenate([[0], np.cumsum(lens)]).astype(np.int32), A.crd[keep],
+                 A.vals[keep])
+        H = oracle.spgemm_hash(At, B, w_cap)
+        for i in range(t, A.nrows, cap):
+            np.testing.assert_array_equal(P.vals[P.pos[i]:P.pos[i + 1]], H.vals[H.pos[i]:H.pos[i + 1]])
+
+
 def test_hash_flush_grouping_known_answer():
     """Hand-derived from CSR_CSR_hash_sequential.h:151-190, 280-298.
 
