@@ -168,6 +168,21 @@ int spws_csr_transpose_dev(spws_ctx* ctx, int32_t m, int32_t n, int64_t nnz, con
                            const int32_t* crd, const double* vals, int32_t* out_pos, int32_t* out_crd,
                            double* out_vals);
 
+/* spws_csr_transpose_dev without a context and without allocating anything
+ * (no hipMalloc/hipFree, no host allocation): the same kernels and the same
+ * result, on `stream` (a hipStream_t; NULL = the null stream) of the current
+ * device, with every workspace taken from a fixed 48 MiB device array that
+ * each GPU holds for the life of the process.  About 33 bytes per entry plus
+ * 4 per output row must fit it (e.g. nnz 1,000,000 with n up to 3,000,000);
+ * larger inputs return SPWS_ERR_UNSUPPORTED before any launch.  Calls that
+ * overlap on one GPU share the array and race: serialise them (one stream,
+ * or events).  No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation). */
+int spws_csr_transpose_dev_noalloc(void* stream, int32_t m, int32_t n, int64_t nnz, const int32_t* pos,
+                                   const int32_t* crd, const double* vals, int32_t* out_pos, int32_t* out_crd,
+                                   double* out_vals);
+
 /* ------------------------------------------------------ sparse tensor kernels */
 
 /* A 3-level CSF is passed as nroots root coordinates l1_crd (the compressed

@@ -40,11 +40,19 @@ int fail_hip(hipError_t e, const char* what, const char* file, int line);
 
 // A device buffer that only grows.  Growing frees the old allocation, which
 // synchronises the device; steady-state calls of the same size never allocate.
+// A fixed buffer (borrow()) views memory owned elsewhere: ensure() never
+// allocates and fails when the view is too small, release() only forgets it.
 struct DevBuf {
   void* ptr = nullptr;
   size_t bytes = 0;
+  bool fixed = false;
   int ensure(size_t want);
   void release();
+  void borrow(void* p, size_t n) {
+    ptr = p;
+    bytes = n;
+    fixed = true;
+  }
   template <class T>
   T* as() const { return static_cast<T*>(ptr); }
 };
@@ -138,6 +146,12 @@ void prof_collect(spws_ctx* ctx);
 // Tin/Tout in {int32_t, int64_t}.  Uses ctx->scan_tmp.
 template <class Tin, class Tout>
 int exclusive_scan(spws_ctx* ctx, const Tin* in, Tout* out, int64_t n);
+// Bytes of ctx->scan_tmp that exclusive_scan of n values of elem_bytes uses.
+size_t scan_scratch_bytes(int64_t n, size_t elem_bytes);
+
+// Resolves the no-allocation entry point's scratch address on `device`
+// (current), once per device; spws_ctx_create calls it for its device.
+int noalloc_scratch_init(int device);
 
 // ---------------------------------------------------------------- device helpers
 

@@ -25,6 +25,10 @@ int fail_hip(hipError_t e, const char* what, const char* file, int line) {
 
 int DevBuf::ensure(size_t want) {
   if (want <= bytes && ptr) return SPWS_OK;
+  if (fixed) {
+    set_error("a fixed scratch region of %zu bytes cannot hold %zu", bytes, want);
+    return SPWS_ERR_UNSUPPORTED;
+  }
   release();
   size_t sz = want < 256 ? 256 : want;
   // grow geometrically so a slowly growing workload does not reallocate each call
@@ -42,9 +46,10 @@ int DevBuf::ensure(size_t want) {
 }
 
 void DevBuf::release() {
-  if (ptr) (void)hipFree(ptr);
+  if (ptr && !fixed) (void)hipFree(ptr);
   ptr = nullptr;
   bytes = 0;
+  fixed = false;
 }
 
 int PinnedBuf::ensure(size_t want) {
@@ -222,6 +227,11 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_tile_apply(const Tin* __res
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = tile_off[gridDim.x];
 }
 
+size_t scan_scratch_bytes(int64_t n, size_t elem_bytes) {
+  if (n <= 4 * (int64_t)kScanTile) return 0;
+  return (size_t)(div_up(n, kScanTile) + 1) * 2 * elem_bytes;
+}
+
 template <class Tin, class Tout>
 int exclusive_scan(spws_ctx* ctx, const Tin* in, Tout* out, int64_t n) {
   if (n <= 4 * (int64_t)kScanTile) {
@@ -283,6 +293,7 @@ int spws_ctx_create(int device, void* stream, spws_ctx** out) {
     ctx->own_stream = true;
   }
   int rc = ctx->host_meta.ensure(4096);
+  if (!rc) rc = noalloc_scratch_init(device);
   if (rc) {
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
