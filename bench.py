@@ -217,6 +217,23 @@ def run_rowwise(args):
         res["cpu_baseline_hash_sequential"] = {"value": round(2 * P / dt1 / 1e9, 3), "unit": "GFLOP/s",
                                                "cores": 1, "kind": "port",
                                                "sample": "full workload, w_cap 2^16, 1 run"}
+        # the CAP-parallel hash (CSR_CSR_hash_parallel.h): cap = cores - 1
+        # workers and the merger on the last core; at least 5 workers keep
+        # each one's entries under 2 w_cap, so no flush runs
+        cap = max(5, cores - 1)
+        G = oracle.spgemm_gustavson(A, B, cores)
+        oracle.spgemm_hash_parallel(A, B, 1 << 16, cap)
+        reps, t0 = 0, time.perf_counter()
+        while reps < 3 or time.perf_counter() - t0 < min(args.cpu_seconds, 3.0):
+            Hp, flushes = oracle.spgemm_hash_parallel(A, B, 1 << 16, cap)
+            reps += 1
+        dtp = (time.perf_counter() - t0) / reps
+        same = bool(np.array_equal(Hp.pos, G.pos) and np.array_equal(Hp.crd, G.crd) and np.array_equal(Hp.vals, G.vals))
+        res["cpu_baseline_hash_parallel"] = {"value": round(2 * P / dtp / 1e9, 3), "unit": "GFLOP/s",
+                                             "cores": cap + 1, "kind": "port", "flushes": flushes,
+                                             "same_as_gustavson": same,
+                                             "sample": f"full workload, {cap} workers + merger, w_cap 2^16, "
+                                                       f"{reps} runs"}
     ctx.close()
     return res
 
@@ -242,8 +259,9 @@ def run_outer(args):
         out["C"] = spgemm_outer(ctx, dA, dB, N, N, N)
 
     def pick(bd):
-        lsd = ("radix_scatter", "radix_hist", "outer_expand", "outer_reduce")
-        names = ("bk_expand", "bk_count", "bk_reduce") if ctx.stats()["outer_method"] == 2 else lsd
+        names = {1: ("radix_scatter", "radix_hist", "outer_expand", "outer_reduce"),
+                 2: ("bk_expand", "bk_count", "bk_reduce"),
+                 3: ("fz_count", "fz_reduce")}[ctx.stats()["outer_method"]]
         return max(names, key=lambda k: bd.get(k, (0, 1))[0])
 
     sec, dom, dom_times, bd = timed_steps(ctx, step, args.steps, args.warmup, pick)
@@ -262,7 +280,12 @@ def run_outer(args):
     #   bk_count       2 B read per product
     #   bk_reduce      10 B read per product + 12 B written per distinct entry
     passes = bd["radix_scatter"][1] // st["chunks"] if "radix_scatter" in bd else 0
-    if st["outer_method"] == 2:
+    # fused method
+    #   fz_count       4 B of B columns gathered per product
+    #   fz_reduce      12 B of B gathered per product + 12 B written per distinct entry
+    if st["outer_method"] == 3:
+        per_step = {"fz_count": 4 * P, "fz_reduce": 12 * P + 12 * nnz}
+    elif st["outer_method"] == 2:
         per_step = {"bk_expand": 22 * P, "bk_count": 2 * P, "bk_reduce": 10 * P + 12 * nnz}
     else:
         per_step = {
@@ -284,7 +307,8 @@ def run_outer(args):
         "dtype": "f64",
         "config": {"workload": "outer_product_spgemm_200k_zipf1.8", "n": N, "zipf_s": 1.8, "row_cap": 20000,
                    "nnz_B": int(B.nnz), "products": int(P), "nnz_C": int(nnz), "chunks": st["chunks"],
-                   "sort_method": {1: "lsd_radix", 2: "row_window_buckets"}[st["outer_method"]],
+                   "sort_method": {1: "lsd_radix", 2: "row_window_buckets",
+                                   3: "fused_row_windows"}[st["outer_method"]],
                    "window_bits": st["window_bits"], "sort_key_bits": st["sort_bits"],
                    "radix_passes": int(passes), "seed": 1},
         "roofline": roof,
@@ -776,8 +800,9 @@ def main(argv=None):
                     help="serve workload: requests of at least 2^this many products form the large lane")
     ap.add_argument("--serve-large-engines", type=int, default=1,
                     help="serve workload: engines reserved for the large lane (0 = one lane)")
-    ap.add_argument("--outer-method", type=int, default=0, choices=[0, 1, 2],
-                    help="outer workload: 0 automatic, 1 LSD radix sort, 2 row/window buckets")
+    ap.add_argument("--outer-method", type=int, default=0, choices=[0, 1, 2, 3],
+                    help="outer workload: 0 automatic, 1 LSD radix sort, 2 row/window buckets, "
+                         "3 fused row windows")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="after the timed steps, one plain device copy of this many bytes (the reference "
                          "point for rocprofv3 FETCH_SIZE / WRITE_SIZE runs)")

@@ -152,10 +152,14 @@ int spws_ctx_set_outer_chunking(spws_ctx* ctx, int64_t max_products, int32_t max
  *      (row, column window) bucket at a position computed from the operands'
  *      structure (exact digit counts, no histogram pass over the products),
  *      then each bucket is ordered by k within the window and reduced in LDS;
+ *   3  fused row windows: no product workspace; one wavefront per output row
+ *      walks its column windows, gathering each window's products straight
+ *      from B (symbolic pass for the exact row pointers, then the numeric
+ *      pass writing every entry in place);
  *   0  automatic (2 unless the windows would be too sparse or too many).
- * window_log2 sets the bucket method's column window (6..11; 0 = chosen from
- * the average products per row).  Both methods sum each entry in increasing
- * j and give identical results. */
+ * window_log2 sets the column window of methods 2 and 3 (6..11; 0 = chosen
+ * from the average products per row).  All methods sum each entry in
+ * increasing j and give identical results. */
 int spws_ctx_set_outer_method(spws_ctx* ctx, int32_t method, int32_t window_log2);
 
 /* CSR transpose on the device, the device form of transpose() in
@@ -211,6 +215,31 @@ int spws_ttm(spws_ctx* ctx, int32_t I, int32_t J, int32_t K, int32_t R,
              int32_t nroots, const int32_t* l1_crd, const int32_t* l2_pos, const int32_t* l2_crd,
              int32_t nfibers, const int32_t* l3_pos, const int32_t* l3_crd, const double* vals, int64_t nnz,
              const double* U, int64_t* nfib, int32_t** fib_i, int32_t** fib_j, double** Y);
+
+/* Hypersparse (DCSR) factors: the reference co-iterates its factors' stored
+ * rows and columns with the tensor's levels (hash_free.h:277-300: k over
+ * B1 and C1, l over B2 and D1, the rank over C2 and D2; hash_il.h:294-303:
+ * the contracted index over B2 and C1, the rank over C2).  Here a factor is
+ * passed dense (zero where not stored) with a mask per row: bit r set when
+ * column r of that row is stored (a missing row has mask 0), R <= 64.  A
+ * nonzero whose rows share no stored column takes no part; an output row
+ * (MTTKRP) or fiber (TTM) exists when one of its nonzeros takes part, and
+ * *row_mask / *fib_mask (context-owned, one uint64 per listed row or fiber)
+ * give its stored output columns: the OR over its nonzeros of the shared
+ * columns.  Values of columns not stored are 0.  Bmask/Cmask: both or
+ * neither (NULL: dense, as spws_mttkrp); Umask NULL: as spws_ttm.  The TTM
+ * mask needs the root, leaf and contracted coordinates to fit 64 bits
+ * together.  Same references as spws_mttkrp / spws_ttm. */
+int spws_mttkrp_masked(spws_ctx* ctx, int32_t I, int32_t J, int32_t K, int32_t R,
+                       int32_t nroots, const int32_t* l1_crd, const int32_t* l2_pos, const int32_t* l2_crd,
+                       int32_t nfibers, const int32_t* l3_pos, const int32_t* l3_crd, const double* vals,
+                       int64_t nnz, const double* Bf, const double* Cf, const uint64_t* Bmask,
+                       const uint64_t* Cmask, int32_t* nrows, int32_t** rows, uint64_t** row_mask, double** M);
+int spws_ttm_masked(spws_ctx* ctx, int32_t I, int32_t J, int32_t K, int32_t R,
+                    int32_t nroots, const int32_t* l1_crd, const int32_t* l2_pos, const int32_t* l2_crd,
+                    int32_t nfibers, const int32_t* l3_pos, const int32_t* l3_crd, const double* vals, int64_t nnz,
+                    const double* U, const uint64_t* Umask, int64_t* nfib, int32_t** fib_i, int32_t** fib_j,
+                    uint64_t** fib_mask, double** Y);
 
 /* ------------------------------------------------ workspace API (wspace.h) */
 

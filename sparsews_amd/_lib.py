@@ -114,6 +114,10 @@ SIGNATURES = {
                               C.POINTER(I32), PVP, PVP]),
     "spws_ttm": (C.c_int, [VP, I32, I32, I32, I32, I32, VP, VP, VP, I32, VP, VP, VP, I64, VP,
                            PI64, PVP, PVP, PVP]),
+    "spws_mttkrp_masked": (C.c_int, [VP, I32, I32, I32, I32, I32, VP, VP, VP, I32, VP, VP, VP, I64, VP, VP,
+                                     VP, VP, C.POINTER(I32), PVP, PVP, PVP]),
+    "spws_ttm_masked": (C.c_int, [VP, I32, I32, I32, I32, I32, VP, VP, VP, I32, VP, VP, VP, I64, VP, VP,
+                                  PI64, PVP, PVP, PVP, PVP]),
     "spws_wspace_sort": (C.c_int, [VP, VP, I64, C.c_int, PI64]),
     "spws_wspace_merge": (C.c_int, [VP, VP, VP, VP, I64, VP, I64, PI64]),
     "spws_batch_create": (C.c_int, [VP, C.POINTER(BatchCaps), C.POINTER(VP)]),

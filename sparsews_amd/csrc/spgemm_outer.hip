@@ -886,6 +886,304 @@ __global__ __launch_bounds__(kBkRedWaves * kWave) void k_bk_reduce(int32_t nb, i
   }
 }
 
+// ------------------------------------------------------------ fused method
+//
+// Expand and reduce in one walk, with no product workspace: one wavefront
+// per output row (rows handed out by a work counter), the row's column
+// windows of 2^s columns in increasing order.  The products of row i in
+// window w are the segments B(j, [w 2^s, (w+1) 2^s)) of its entries A(i, j),
+// taken in increasing j (bseg gives each segment); concatenated they are the
+// products of bucket (i, w) in the bucket method's order, gathered straight
+// from B (whose rows stay in the caches) instead of from an expanded copy.
+// Each stride of 64 products finds every lane's entry by a binary search
+// over the lanes' segment offsets (wavefront shuffles).
+//   symbolic pass: distinct columns per window in an LDS bitmap, summed per
+//                  row (-> row pointers);
+//   numeric pass:  the bucket reduce's LDS window accumulation (entries with
+//                  equal k summed in position = increasing-j order, from
+//                  -0.0), and the window's columns emitted in order at the
+//                  row's running output offset.
+// The loads of the next batch of strides (across window boundaries) are
+// issued before the current batch is applied, and each lane's next segment
+// end is loaded a window ahead.  Rows with more than 64 entries take their
+// entries 64 at a time within each window, in order.
+
+// One record per entry of a row (entries grouped by row, increasing j):
+// where the matched B row starts, the A column (for bseg), the A value.
+__global__ __launch_bounds__(256) void k_fz_records(int32_t En, const double* __restrict__ order,
+                                                    const int32_t* __restrict__ ent_ja,
+                                                    const int32_t* __restrict__ ent_ai,
+                                                    const int32_t* __restrict__ match,
+                                                    const int32_t* __restrict__ B2_pos, const double* __restrict__ Ax,
+                                                    int32_t* __restrict__ rec_b, int32_t* __restrict__ rec_ja,
+                                                    double* __restrict__ rec_a) {
+  for (int32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < En; q += gridDim.x * blockDim.x) {
+    const int32_t e = (int32_t)__double_as_longlong(order[q]);
+    const int32_t ja = ent_ja[e];
+    rec_b[q] = B2_pos[match[ja]];
+    rec_ja[q] = ja;
+    rec_a[q] = Ax[ent_ai[e]];
+  }
+}
+
+constexpr int kFzU = 16;  // strides of 64 products per batch
+
+// A batch: up to kFzU strides of one (window, entry chunk) of the row.
+struct FzBatch {
+  int32_t w;      // window (-1: the row is done)
+  int32_t c;      // first entry of the chunk (relative to the row)
+  int32_t q0;     // first product of the batch within the (window, chunk)
+  int32_t total;  // products of the (window, chunk)
+  bool win_end;   // last batch of the window
+};
+
+template <bool kNumeric>
+__global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, const int32_t* __restrict__ rp_off,
+                                                   const int32_t* __restrict__ rec_b,
+                                                   const int32_t* __restrict__ rec_ja,
+                                                   const double* __restrict__ rec_a,
+                                                   const int32_t* __restrict__ bseg,
+                                                   const int32_t* __restrict__ B2_crd,
+                                                   const double* __restrict__ Bx, int32_t* __restrict__ row_cnt,
+                                                   const int64_t* __restrict__ C_pos, int32_t* __restrict__ C_crd,
+                                                   double* __restrict__ C_val, int32_t* __restrict__ work) {
+  extern __shared__ double fz_lds[];
+  const int S = 1 << s, words = S >> 5;  // words <= 64: one per lane
+  const int lane = lane_id();
+  double* acc = fz_lds;                                          // numeric: 2^s values
+  uint32_t* seen = kNumeric ? reinterpret_cast<uint32_t*>(acc + S) : reinterpret_cast<uint32_t*>(fz_lds);
+  uint32_t* sb = seen + words;                                   // numeric: this stride's columns
+  uint16_t* list = reinterpret_cast<uint16_t*>(sb + words);      // numeric: emitted columns
+  if (lane < words) {
+    seen[lane] = 0u;
+    if (kNumeric) sb[lane] = 0u;
+  }
+  if (kNumeric)
+    for (int q = lane; q < S; q += kWave) acc[q] = -0.0;
+  __builtin_amdgcn_wave_barrier();
+
+  for (;;) {
+    int32_t r = 0;
+    if (lane == 0) r = atomicAdd(work, 1);
+    r = __builtin_amdgcn_readfirstlane(__shfl(r, 0, kWave));
+    if (r >= m) break;
+    const int32_t q_beg = __builtin_amdgcn_readfirstlane(rp_off[r]);
+    const int32_t E = __builtin_amdgcn_readfirstlane(rp_off[r + 1]) - q_beg;
+    const int64_t out0 = kNumeric ? C_pos[r] : 0;
+    // the symbolic pass's count bounds the numeric pass's writes (the two
+    // walk the same segments, so a window never passes it)
+    const int64_t out_end = kNumeric ? C_pos[r + 1] : 0;
+    int64_t run = 0;  // entries emitted (numeric) or counted (symbolic) so far
+    if (E == 0) {
+      if (!kNumeric && lane == 0) row_cnt[r] = 0;
+      continue;
+    }
+    const bool one_chunk = E <= kWave;
+    // per lane: its entry of the current chunk
+    int32_t b_start = 0;
+    double a_val = 0.0;
+    const int32_t* bs_row = bseg;
+    int32_t seg_lo = 0, seg_hi = 0, nxt_hi = 0;
+    auto load_chunk = [&](int32_t c, int32_t w) {
+      const int32_t q = c + lane;
+      if (q < E) {
+        b_start = rec_b[q_beg + q];
+        a_val = kNumeric ? rec_a[q_beg + q] : 0.0;
+        bs_row = bseg + (int64_t)rec_ja[q_beg + q] * (W + 1);
+        seg_lo = bs_row[w];
+        seg_hi = bs_row[w + 1];
+        nxt_hi = (one_chunk && w + 2 <= W) ? bs_row[w + 2] : seg_hi;
+      } else {
+        b_start = 0;
+        a_val = 0.0;
+        bs_row = bseg;
+        seg_lo = seg_hi = nxt_hi = 0;
+      }
+    };
+    // the segment scan of the current (window, chunk): base + q = B index of
+    // product q for the lane owning it
+    int32_t excl = 0, base = 0;
+    auto scan = [&](FzBatch& b) {
+      const int32_t len = seg_hi - seg_lo;
+      const int32_t incl = wave_incl_scan(len);
+      excl = incl - len;
+      base = b_start + seg_lo - excl;
+      b.total = __builtin_amdgcn_readlane(incl, kWave - 1);
+    };
+    // the batch after b (advancing chunks and windows; empty ones skipped)
+    auto next = [&](const FzBatch& b) -> FzBatch {
+      FzBatch n = b;
+      if (b.q0 + kWave * kFzU < b.total) {
+        n.q0 = b.q0 + kWave * kFzU;
+        n.win_end = (n.q0 + kWave * kFzU >= n.total) && (n.c + kWave >= E);
+        return n;
+      }
+      for (;;) {
+        if (n.c + kWave < E) {
+          n.c += kWave;  // next chunk of the same window
+          load_chunk(n.c, n.w);
+        } else {
+          n.w += 1;
+          n.c = 0;
+          if (n.w >= W) {
+            n.w = -1;
+            return n;
+          }
+          if (one_chunk) {
+            seg_lo = seg_hi;
+            seg_hi = nxt_hi;
+            // the next window's end, a window ahead
+            if (lane < E && n.w + 2 <= W) nxt_hi = bs_row[n.w + 2];
+          } else {
+            load_chunk(0, n.w);
+          }
+        }
+        n.q0 = 0;
+        scan(n);
+        n.win_end = (kWave * kFzU >= n.total) && (n.c + kWave >= E);
+        // an empty (window, chunk) still closes its window when it is the last chunk
+        if (n.total > 0 || n.c + kWave >= E) return n;
+      }
+    };
+    // loads of one batch: column and value (numeric) of each lane's product
+    uint32_t kc[2][kFzU];
+    double bv[2][kFzU], av[2][kFzU];
+    auto load = [&](const FzBatch& b, int slot) {
+      const int32_t len = seg_hi - seg_lo;
+#pragma unroll
+      for (int u = 0; u < kFzU; ++u) {
+        const int32_t s0 = b.q0 + u * kWave;  // the stride's first product (uniform)
+        if (s0 >= b.total) break;             // apply stops at the same stride
+        const int32_t q = min(s0 + lane, b.total - 1);
+        // a lane whose segment covers the whole stride owns every product of
+        // it (the common case: most products lie in long segments); else
+        // each lane finds its owner, the last lane t with excl_t <= q
+        const int32_t s_end = min(s0 + kWave, b.total);
+        const uint64_t cov = __ballot(len > 0 && excl <= s0 && excl + len >= s_end);
+        int32_t idx;
+        double a = 0.0;
+        if (cov) {
+          const int o = __builtin_ctzll(cov);
+          idx = __builtin_amdgcn_readlane(base, o) + q;
+          if (kNumeric) a = readlane_f64(a_val, o);
+        } else {
+          int lo = 0, hi = kWave - 1;
+#pragma unroll
+          for (int it = 0; it < 6; ++it) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (__shfl(excl, mid, kWave) <= q) lo = mid; else hi = mid - 1;
+          }
+          idx = __shfl(base, lo, kWave) + q;
+          if (kNumeric) a = __shfl(a_val, lo, kWave);
+        }
+        kc[slot][u] = (uint32_t)B2_crd[idx];
+        if (kNumeric) {
+          bv[slot][u] = Bx[idx];
+          av[slot][u] = a;
+        }
+      }
+    };
+    auto apply = [&](const FzBatch& b, int slot) {
+      const uint32_t wmask = (uint32_t)S - 1;
+#pragma unroll
+      for (int u = 0; u < kFzU; ++u) {
+        if (b.q0 + u * kWave >= b.total) break;
+        const bool ok = b.q0 + u * kWave + lane < b.total;
+        const uint32_t k = kc[slot][u] & wmask;
+        const uint32_t wi = k >> 5, bit = 1u << (k & 31);
+        if (!kNumeric) {
+          if (ok) atomicOr(&seen[wi], bit);
+          continue;
+        }
+        const double v = av[slot][u] * bv[slot][u];
+        uint32_t old = 0;
+        double a = 0.0;
+        if (ok) {
+          old = atomicOr(&sb[wi], bit);
+          a = acc[k];
+        }
+        double sum = a + v;
+        bool lead = ok;
+        uint64_t todo = __ballot(ok && (old & bit) != 0);
+        while (todo) {
+          const int l = __builtin_ctzll(todo);
+          const uint32_t kl = (uint32_t)__builtin_amdgcn_readlane((int)k, l);
+          const uint64_t grp = __ballot(ok && k == kl);
+          todo &= ~grp;
+          const int ld = __builtin_ctzll(grp);
+          for (uint64_t mm = grp & (grp - 1); mm; mm &= mm - 1) {
+            const double vl = readlane_f64(v, __builtin_ctzll(mm));
+            if (lane == ld) sum = sum + vl;
+          }
+          if (lane != ld && ((grp >> lane) & 1)) lead = false;
+        }
+        if (lead) {
+          acc[k] = sum;
+          atomicOr(&seen[wi], bit);
+        }
+        if (ok) atomicAnd(&sb[wi], ~bit);
+        __builtin_amdgcn_wave_barrier();
+      }
+      if (!b.win_end) return;
+      __builtin_amdgcn_wave_barrier();
+      const uint32_t myw = lane < words ? seen[lane] : 0u;
+      __builtin_amdgcn_wave_barrier();
+      if (lane < words) seen[lane] = 0u;
+      const int cnt = __popc(myw);
+      const int incl = wave_incl_scan(cnt);
+      const int total = __builtin_amdgcn_readlane(incl, kWave - 1);
+      if (kNumeric && total > 0 && out0 + run + total <= out_end) {
+        int rr = incl - cnt;
+        for (uint32_t bits = myw; bits; bits &= bits - 1) list[rr++] = (uint16_t)(lane * 32 + __builtin_ctz(bits));
+        __builtin_amdgcn_wave_barrier();
+        int32_t* oc = C_crd + out0 + run;
+        double* ov = C_val + out0 + run;
+        const int32_t wb = b.w << s;
+        for (int r0 = 0; r0 < total; r0 += 4 * kWave) {
+          uint32_t kq[4];
+          double aq[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) kq[i] = list[min(r0 + i * kWave + lane, total - 1)];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) aq[i] = acc[kq[i]];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int q = r0 + i * kWave + lane;
+            if (q < total) {
+              oc[q] = wb + (int32_t)kq[i];
+              ov[q] = aq[i];
+              acc[kq[i]] = -0.0;
+            }
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+      run += total;
+    };
+
+    FzBatch c0;
+    c0.w = 0;
+    c0.c = 0;
+    c0.q0 = 0;
+    load_chunk(0, 0);
+    scan(c0);
+    c0.win_end = (kWave * kFzU >= c0.total) && (kWave >= E);
+    load(c0, 0);
+    while (c0.w >= 0) {
+      vm_drain();
+      FzBatch c1 = next(c0);
+      if (c1.w >= 0) load(c1, 1);
+      apply(c0, 0);
+      if (c1.w < 0) break;
+      vm_drain();
+      c0 = next(c1);
+      if (c0.w >= 0) load(c0, 0);
+      apply(c1, 1);
+    }
+    if (!kNumeric && lane == 0) row_cnt[r] = (int32_t)run;
+  }
+}
+
 static int bits_for(int64_t v) {  // bits to represent values in [0, v)
   int b = 0;
   while (b < 63 && ((int64_t)1 << b) < v) ++b;
@@ -1262,6 +1560,120 @@ static int outer_chunks_bucket(spws_ctx* ctx, const OuterArgs& a, const std::vec
   return SPWS_OK;
 }
 
+constexpr int kFzSymWavesPerCu = 16;  // k_fz_rows<false>: one wave per workgroup, tiny LDS
+constexpr int kFzNumWavesPerCu = 8;   // k_fz_rows<true>: bounded by the LDS window
+
+// Fused method over all rows at once (see the fused-method comment): A's
+// matched entries grouped by row (sorted by row, stable, so increasing j
+// within a row), one record per entry, the symbolic pass and its prefix
+// (exact row pointers, and an output allocated to the exact size), then the
+// numeric pass writing every entry in place.
+static int outer_fused(spws_ctx* ctx, const OuterArgs& a, const std::vector<int64_t>& pair_prefix,
+                       const int32_t* row_pairs, int s, int64_t* nnz_out) {
+  hipStream_t st = ctx->stream;
+  int rc;
+  const int32_t m = a.m;
+  const int32_t na = a.a_ncols;
+  const size_t nac = (size_t)(na > 0 ? na : 1);
+  const int32_t W = (int32_t)std::max<int64_t>(1, div_up((int64_t)a.n, (int64_t)1 << s));
+  const int64_t En64 = pair_prefix[m];
+  SPWS_REQUIRE(En64 < ((int64_t)1 << 31), SPWS_ERR_UNSUPPORTED, "%lld matched A entries exceed the int32 range",
+               (long long)En64);
+  const int32_t En = (int32_t)En64;
+  const size_t ecap = (size_t)(En > 0 ? En : 1);
+  const size_t mm = (size_t)(m > 0 ? m : 1);
+  int32_t* match = ctx->obuf[0].as<int32_t>();
+  int32_t* blen = ctx->obuf[1].as<int32_t>();
+  if ((rc = ctx->obuf[2].ensure(nac * 4 * 3)) || (rc = ctx->obuf[3].ensure((nac + 1) * 4)) ||
+      (rc = ctx->obuf[4].ensure(nac * 4 * 5)) || (rc = ctx->obuf[5].ensure((nac + 1) * 8 * 2)) ||
+      (rc = ctx->obuf[6].ensure((nac + 1) * 4)) || (rc = ctx->bk[0].ensure(nac * (size_t)(W + 1) * 4)) ||
+      (rc = ctx->bk[1].ensure(ecap * 4)) || (rc = ctx->bk[2].ensure(ecap * 4)) ||
+      (rc = ctx->bk[3].ensure(ecap * 8)) || (rc = ctx->bk[4].ensure(ecap * 8)) ||
+      (rc = ctx->bk[5].ensure(ecap * 4)) || (rc = ctx->bk[9].ensure(ecap * 4)) ||
+      (rc = ctx->bk[6].ensure(ecap * 16)) || (rc = ctx->bk[7].ensure((mm + 1) * 4)) ||
+      (rc = ctx->bk[8].ensure((mm + 64) * 4)) || (rc = ctx->out_pos.ensure((mm + 1) * 8)))
+    return rc;
+  int32_t* col_lo = ctx->obuf[2].as<int32_t>();
+  int32_t* col_m = col_lo + nac;
+  int32_t* flag = col_m + nac;
+  int32_t* flag_off = ctx->obuf[3].as<int32_t>();
+  int32_t* act_astart = ctx->obuf[4].as<int32_t>();
+  int32_t* act_L = act_astart + nac;
+  int32_t* act_bstart = act_L + nac;
+  int32_t* act_ja = act_bstart + nac;
+  int32_t* act_m = act_ja + nac;
+  int64_t* act_cnt = ctx->obuf[5].as<int64_t>();
+  int32_t* act_eoff = ctx->obuf[6].as<int32_t>();
+  int32_t* bseg = ctx->bk[0].as<int32_t>();
+  int32_t* rp_off = ctx->bk[7].as<int32_t>();
+  int32_t* row_cnt = ctx->bk[8].as<int32_t>();
+  int32_t* work = row_cnt + mm + 16;  // 256-byte apart from the counts
+  int32_t* rec_b = ctx->bk[6].as<int32_t>();
+  int32_t* rec_ja = rec_b + ecap;
+  double* rec_a = reinterpret_cast<double*>(rec_ja + ecap);
+  int64_t* out_pos = ctx->out_pos.as<int64_t>();
+  ctx->stats.outer_method = 3;
+  ctx->stats.window_bits = s;
+  ctx->stats.chunks = 1;
+  const unsigned gcol = (unsigned)div_up(na > 0 ? na : 1, 256);
+  if (na > 0) {
+    const unsigned g = (unsigned)std::min<int64_t>(div_up((int64_t)na * (W + 1), 256), 16384);
+    SPWS_LAUNCH(ctx, "bk_bseg", k_bk_bseg, dim3(g), dim3(256), 0, na, W, s, match, blen, a.B2_pos, a.B2_crd, bseg);
+  }
+  // A's matched entries of every row, grouped by row (increasing j within a row)
+  if (En > 0) {
+    SPWS_LAUNCH(ctx, "outer_colrange", k_outer_colrange, dim3(gcol), dim3(256), 0, na, 0, m, a.A2_pos, a.A2_crd, blen,
+                col_lo, col_m, flag);
+    if ((rc = exclusive_scan<int32_t, int32_t>(ctx, flag, flag_off, na))) return rc;
+    SPWS_HIP(hipMemsetAsync(act_cnt, 0, nac * 8, st));
+    SPWS_HIP(hipMemsetAsync(act_m, 0, nac * 4, st));
+    SPWS_LAUNCH(ctx, "outer_compact", k_outer_compact, dim3(gcol), dim3(256), 0, na, flag_off, col_lo, col_m, match,
+                blen, a.B2_pos, act_astart, act_L, act_bstart, act_cnt, act_ja, act_m);
+    if ((rc = exclusive_scan<int32_t, int32_t>(ctx, act_m, act_eoff, na))) return rc;
+    uint32_t* pkey = ctx->bk[1].as<uint32_t>();
+    double* pval = ctx->bk[3].as<double>();
+    const unsigned ge = (unsigned)std::min<int64_t>(div_up(En, 256), 16384);
+    SPWS_LAUNCH(ctx, "bk_entries", k_bk_entries, dim3(ge), dim3(256), 0, En, flag_off + na, act_eoff, act_astart,
+                act_ja, a.A2_crd, 0, pkey, pval, ctx->bk[5].as<int32_t>(), ctx->bk[9].as<int32_t>());
+    bool alt = false;
+    if ((rc = radix_sort_pairs<uint32_t>(ctx, pkey, pval, ctx->bk[2].as<uint32_t>(), ctx->bk[4].as<double>(), En, 0,
+                                         std::max(1, bits_for(m)), &alt)))
+      return rc;
+    if (alt) pval = ctx->bk[4].as<double>();
+    SPWS_LAUNCH(ctx, "fz_records", k_fz_records, dim3(ge), dim3(256), 0, En, pval, ctx->bk[5].as<int32_t>(),
+                ctx->bk[9].as<int32_t>(), match, a.B2_pos, a.Ax, rec_b, rec_ja, rec_a);
+  }
+  if ((rc = exclusive_scan<int32_t, int32_t>(ctx, row_pairs, rp_off, m))) return rc;
+  int dev_cus = 256;
+  (void)hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+  // symbolic pass: entries per row
+  SPWS_HIP(hipMemsetAsync(work, 0, 4, st));
+  const unsigned gsym = (unsigned)std::max<int64_t>(1, std::min<int64_t>(m, (int64_t)dev_cus * kFzSymWavesPerCu));
+  SPWS_LAUNCH(ctx, "fz_count", k_fz_rows<false>, dim3(gsym), dim3(kWave), (((size_t)1 << s) >> 3), m, W, s, rp_off,
+              rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, row_cnt, (const int64_t*)nullptr, (int32_t*)nullptr,
+              (double*)nullptr, work);
+  if ((rc = exclusive_scan<int32_t, int64_t>(ctx, row_cnt, out_pos, m))) return rc;
+  int64_t* hm = ctx->host_meta.as<int64_t>();
+  SPWS_HIP(hipMemcpyAsync(hm, out_pos + m, 8, hipMemcpyDeviceToHost, st));
+  SPWS_HIP(hipStreamSynchronize(st));
+  const int64_t total = *hm;
+  const size_t cap = (size_t)(total > 0 ? total : 1);
+  if (outer_output_capacity(ctx) < cap) {
+    ctx->out_crd.release();
+    ctx->out_val.release();
+    if ((rc = ctx->out_crd.ensure(cap * 4)) || (rc = ctx->out_val.ensure(cap * 8))) return rc;
+  }
+  // numeric pass: every entry written at its final position
+  SPWS_HIP(hipMemsetAsync(work, 0, 4, st));
+  const unsigned gnum = (unsigned)std::max<int64_t>(1, std::min<int64_t>(m, (int64_t)dev_cus * kFzNumWavesPerCu));
+  SPWS_LAUNCH(ctx, "fz_reduce", k_fz_rows<true>, dim3(gnum), dim3(kWave), bk_reduce_lds_per_wave(s), m, W, s, rp_off,
+              rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, (int32_t*)nullptr, (const int64_t*)out_pos,
+              ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>(), work);
+  SPWS_HIP(hipStreamSynchronize(st));
+  *nnz_out = total;
+  return SPWS_OK;
+}
+
 static int outer_run(spws_ctx* ctx, const OuterArgs& a, int64_t* nnz_out) {
   int rc = ctx_activate(ctx);
   if (rc) return rc;
@@ -1326,7 +1738,9 @@ static int outer_run(spws_ctx* ctx, const OuterArgs& a, int64_t* nnz_out) {
     for (int32_t i = 0; i < m; ++i) max_pairs = std::max(max_pairs, pairs[i]);
     method = (per_bucket >= 64.0 && W <= 4096 && (int64_t)max_pairs * W <= ((int64_t)1 << 28)) ? 2 : 1;
   }
-  if (method == 2) {
+  if (method == 3) {
+    rc = outer_fused(ctx, a, pair_prefix, row_pairs, wlog, nnz_out);
+  } else if (method == 2) {
     rc = outer_chunks_bucket(ctx, a, prefix, pair_prefix, row_pairs, wlog, max_products,
                              (int32_t)std::min<int64_t>(user_rows, INT32_MAX), nnz_out);
   } else {
@@ -1362,7 +1776,7 @@ int spws_ctx_set_outer_chunking(spws_ctx* ctx, int64_t max_products, int32_t max
 
 int spws_ctx_set_outer_method(spws_ctx* ctx, int32_t method, int32_t window_log2) {
   SPWS_REQUIRE(ctx != nullptr, SPWS_ERR_INVALID, "null context");
-  SPWS_REQUIRE(method >= 0 && method <= 2, SPWS_ERR_INVALID, "outer method must be 0, 1 or 2");
+  SPWS_REQUIRE(method >= 0 && method <= 3, SPWS_ERR_INVALID, "outer method must be 0, 1, 2 or 3");
   SPWS_REQUIRE(window_log2 == 0 || (window_log2 >= kBkMinLog2 && window_log2 <= kBkMaxLog2), SPWS_ERR_INVALID,
                "window_log2 must be 0 or in [%d, %d]", kBkMinLog2, kBkMaxLog2);
   ctx->outer_method = method;

@@ -4,6 +4,7 @@
 // device named by SPWS_DEVICE (default 0).
 #include <climits>
 #include <cstdlib>
+#include <vector>
 
 #include "common.hpp"
 
@@ -127,23 +128,49 @@ static int stage_csf(spws_ctx* ctx, const taco_tensor_t* t, const char* name, Ho
   return SPWS_OK;
 }
 
-// A dense factor stored as DCSR: every row present, each row exactly the
-// columns 0..R-1 in order.  Returns the row-major values (host pointer).
-static int dense_factor(const taco_tensor_t* t, const char* name, int32_t R, const double** vals) {
-  SPWS_REQUIRE(t && t->indices && t->indices[0] && t->indices[1] && t->indices[0][0] && t->indices[1][0] && t->vals,
+// A DCSR factor (rows x R): when every row and column is stored, *vals is
+// its value array and *full is set; otherwise it is copied into `dense`
+// (rows x R, zero where not stored) with `mask` holding each row's stored
+// columns as bits (a missing row has mask 0), and *vals = dense.data().
+static int factor_dense(const taco_tensor_t* t, const char* name, int32_t R, std::vector<double>& dense,
+                        std::vector<uint64_t>& mask, bool* full, const double** vals) {
+  SPWS_REQUIRE(t && t->dimensions && t->indices && t->indices[0] && t->indices[1] && t->indices[0][0] &&
+                   t->indices[1][0] && (t->vals || t->indices[0][0][1] == t->indices[0][0][0]),
                SPWS_ERR_INVALID, "%s is not a compressed-compressed taco tensor", name);
+  SPWS_REQUIRE(t->dimensions[1] == R, SPWS_ERR_INVALID, "%s has %d columns, not R = %d", name, t->dimensions[1], R);
+  const int32_t rows_dim = t->dimensions[0];
   const int32_t nr = t->indices[0][0][1] - t->indices[0][0][0];
-  SPWS_REQUIRE(nr == t->dimensions[0], SPWS_ERR_UNSUPPORTED, "%s: only dense factor matrices are supported", name);
   const int32_t* rows = t->indices[0][1];
   const int32_t* pos = t->indices[1][0];
   const int32_t* crd = t->indices[1][1];
-  for (int32_t r = 0; r < nr; ++r) {
-    SPWS_REQUIRE(rows[r] == r && pos[r + 1] - pos[r] == R, SPWS_ERR_UNSUPPORTED,
-                 "%s: only dense factor matrices are supported", name);
-    for (int32_t q = 0; q < R; ++q)
-      SPWS_REQUIRE(crd[pos[r] + q] == q, SPWS_ERR_UNSUPPORTED, "%s: only dense factor matrices are supported", name);
+  SPWS_REQUIRE(nr >= 0 && nr <= rows_dim && (nr == 0 || (rows && crd)), SPWS_ERR_INVALID, "%s: malformed levels",
+               name);
+  bool dense_all = nr == rows_dim;
+  for (int32_t r = 0; r < nr && dense_all; ++r) {
+    dense_all = rows[r] == r && pos[r + 1] - pos[r] == R;
+    for (int32_t q = 0; q < R && dense_all; ++q) dense_all = crd[pos[r] + q] == q;
   }
-  *vals = t->vals;
+  *full = dense_all;
+  if (dense_all) {
+    *vals = t->vals;
+    return SPWS_OK;
+  }
+  SPWS_REQUIRE(R <= 64, SPWS_ERR_UNSUPPORTED, "%s: a factor with rows or columns not stored needs R <= 64", name);
+  dense.assign((size_t)rows_dim * R, 0.0);
+  mask.assign((size_t)(rows_dim > 0 ? rows_dim : 1), 0);
+  for (int32_t r = 0; r < nr; ++r) {
+    const int32_t row = rows[r];
+    SPWS_REQUIRE(row >= 0 && row < rows_dim && (r == 0 || rows[r - 1] < row), SPWS_ERR_INVALID,
+                 "%s: stored rows must be increasing and within the dimension", name);
+    for (int32_t q = pos[r]; q < pos[r + 1]; ++q) {
+      const int32_t c = crd[q];
+      SPWS_REQUIRE(c >= 0 && c < R && (q == pos[r] || crd[q - 1] < c), SPWS_ERR_INVALID,
+                   "%s: a row's columns must be increasing and below R", name);
+      dense[(size_t)row * R + c] = t->vals[q];
+      mask[row] |= (uint64_t)1 << c;
+    }
+  }
+  *vals = dense.data();
   return SPWS_OK;
 }
 
@@ -485,45 +512,70 @@ int spws_mttkrp_hash_compute(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* 
   const int32_t J = C->dimensions[0], K = D->dimensions[0];
   SPWS_REQUIRE(C->dimensions[1] == R && D->dimensions[1] == R, SPWS_ERR_INVALID, "factor ranks differ from A");
   int rc;
+  std::vector<double> denseC, denseD;
+  std::vector<uint64_t> maskC, maskD;
+  bool fullC, fullD;
   const double *hC, *hD;
-  if ((rc = dense_factor(C, "C", R, &hC)) || (rc = dense_factor(D, "D", R, &hD))) return rc;
+  if ((rc = factor_dense(C, "C", R, denseC, maskC, &fullC, &hC)) ||
+      (rc = factor_dense(D, "D", R, denseD, maskD, &fullD, &hD)))
+    return rc;
+  const bool masked = !fullC || !fullD;
+  const uint64_t all = R == 64 ? ~(uint64_t)0 : (((uint64_t)1 << R) - 1);
+  if (masked) {  // a dense factor next to a sparse one: every row stores every column
+    if (fullC) maskC.assign((size_t)(J > 0 ? J : 1), all);
+    if (fullD) maskD.assign((size_t)(K > 0 ? K : 1), all);
+  }
   spws_ctx* ctx;
   if ((rc = default_ctx(&ctx)) || (rc = ctx_activate(ctx))) return rc;
   HostCsf h;
   const int32_t *l1, *l2p, *l2c, *l3p, *l3c;
   const double *v, *dC, *dD;
+  const uint64_t *dmC = nullptr, *dmD = nullptr;
   if ((rc = stage_csf(ctx, B, "B", &h, &l1, &l2p, &l2c, &l3p, &l3c, &v)) ||
       (rc = stage_in(ctx, 6, hC, (int64_t)J * R, &dC)) || (rc = stage_in(ctx, 7, hD, (int64_t)K * R, &dD)))
     return rc;
+  if (masked && ((rc = stage_in(ctx, 8, maskC.data(), (int64_t)maskC.size(), &dmC)) ||
+                 (rc = stage_in(ctx, 9, maskD.data(), (int64_t)maskD.size(), &dmD))))
+    return rc;
   int32_t nrows = 0;
   int32_t* drows;
+  uint64_t* dmask;
   double* dM;
-  rc = spws_mttkrp(ctx, I, J, K, R, h.nroots, l1, l2p, l2c, h.nfibers, l3p, l3c, v, h.nnz, dC, dD, &nrows, &drows,
-                   &dM);
+  rc = spws_mttkrp_masked(ctx, I, J, K, R, h.nroots, l1, l2p, l2c, h.nfibers, l3p, l3c, v, h.nnz, dC, dD, dmC, dmD,
+                          &nrows, &drows, &dmask, &dM);
   if (rc) return rc;
   int32_t* rows = nullptr;
+  uint64_t* rmask = nullptr;
   double* M = nullptr;
-  if ((rc = d2h_alloc(ctx, drows, nrows, &rows)) || (rc = d2h_alloc(ctx, dM, (int64_t)I * R, &M))) {
+  if ((rc = d2h_alloc(ctx, drows, nrows, &rows)) || (masked && (rc = d2h_alloc(ctx, (const uint64_t*)dmask, nrows,
+                                                                              &rmask))) ||
+      (rc = d2h_alloc(ctx, dM, (int64_t)I * R, &M))) {
     free(rows);
+    free(rmask);
     return rc;
   }
-  const int64_t n = (int64_t)nrows * R;
+  // COO (i, r) in row order, the row's stored columns in increasing r
+  int64_t n = 0;
+  for (int32_t t = 0; t < nrows; ++t) n += masked ? __builtin_popcountll(rmask[t]) : R;
   int32_t* oi = host_alloc<int32_t>(n);
   int32_t* orr = host_alloc<int32_t>(n);
   double* ov = host_alloc<double>(n);
   if (!oi || !orr || !ov) {
-    free(rows); free(M); free(oi); free(orr); free(ov);
+    free(rows); free(rmask); free(M); free(oi); free(orr); free(ov);
     set_error("host allocation failed");
     return SPWS_ERR_OOM;
   }
+  int64_t q = 0;
   for (int32_t t = 0; t < nrows; ++t)
     for (int32_t r = 0; r < R; ++r) {
-      const int64_t q = (int64_t)t * R + r;
+      if (masked && !((rmask[t] >> r) & 1)) continue;
       oi[q] = rows[t];
       orr[q] = r;
       ov[q] = M[(int64_t)rows[t] * R + r];
+      ++q;
     }
   free(rows);
+  free(rmask);
   free(M);
   A->indices[1][0] = oi;
   A->indices[2][0] = orr;
@@ -541,50 +593,71 @@ int spws_ttm_hash_il_compute(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* 
   const int32_t K = C->dimensions[0];
   SPWS_REQUIRE(C->dimensions[1] == R, SPWS_ERR_INVALID, "factor columns differ from A's last dimension");
   int rc;
+  std::vector<double> denseU;
+  std::vector<uint64_t> maskU;
+  bool fullU;
   const double* hU;
-  if ((rc = dense_factor(C, "C", R, &hU))) return rc;
+  if ((rc = factor_dense(C, "C", R, denseU, maskU, &fullU, &hU))) return rc;
   spws_ctx* ctx;
   if ((rc = default_ctx(&ctx)) || (rc = ctx_activate(ctx))) return rc;
   HostCsf h;
   const int32_t *l1, *l2p, *l2c, *l3p, *l3c;
   const double *v, *dU;
+  const uint64_t* dmU = nullptr;
   if ((rc = stage_csf(ctx, B, "B", &h, &l1, &l2p, &l2c, &l3p, &l3c, &v)) ||
       (rc = stage_in(ctx, 6, hU, (int64_t)K * R, &dU)))
     return rc;
+  if (!fullU && (rc = stage_in(ctx, 8, maskU.data(), (int64_t)maskU.size(), &dmU))) return rc;
   int64_t F = 0;
   int32_t *fi, *fj;
+  uint64_t* fm;
   double* Y;
-  rc = spws_ttm(ctx, I, J, K, R, h.nroots, l1, l2p, l2c, h.nfibers, l3p, l3c, v, h.nnz, dU, &F, &fi, &fj, &Y);
+  rc = spws_ttm_masked(ctx, I, J, K, R, h.nroots, l1, l2p, l2c, h.nfibers, l3p, l3c, v, h.nnz, dU, dmU, &F, &fi, &fj,
+                       &fm, &Y);
   if (rc) return rc;
-  SPWS_REQUIRE(F * R <= INT32_MAX, SPWS_ERR_UNSUPPORTED, "TTM output exceeds the int32 vals_size");
   int32_t *hi = nullptr, *hj = nullptr;
+  uint64_t* hm = nullptr;
   double* hy = nullptr;
-  if ((rc = d2h_alloc(ctx, fi, F, &hi)) || (rc = d2h_alloc(ctx, fj, F, &hj)) || (rc = d2h_alloc(ctx, Y, F * R, &hy))) {
+  if ((rc = d2h_alloc(ctx, fi, F, &hi)) || (rc = d2h_alloc(ctx, fj, F, &hj)) ||
+      (!fullU && (rc = d2h_alloc(ctx, (const uint64_t*)fm, F, &hm))) || (rc = d2h_alloc(ctx, Y, F * R, &hy))) {
     free(hi);
     free(hj);
+    free(hm);
     return rc;
   }
-  const int64_t n = F * R;
+  // COO (i, j, r) in fiber order, each fiber's stored columns in increasing r
+  int64_t n = 0;
+  for (int64_t f = 0; f < F; ++f) n += fullU ? R : __builtin_popcountll(hm[f]);
+  SPWS_REQUIRE(n <= INT32_MAX, SPWS_ERR_UNSUPPORTED, "TTM output exceeds the int32 vals_size");
   int32_t* o1 = host_alloc<int32_t>(n);
   int32_t* o2 = host_alloc<int32_t>(n);
   int32_t* o3 = host_alloc<int32_t>(n);
-  if (!o1 || !o2 || !o3) {
-    free(hi); free(hj); free(hy); free(o1); free(o2); free(o3);
+  double* ov = fullU ? hy : host_alloc<double>(n);
+  if (!o1 || !o2 || !o3 || !ov) {
+    free(hi); free(hj); free(hm); free(o1); free(o2); free(o3);
+    if (ov != hy) free(ov);
+    free(hy);
     set_error("host allocation failed");
     return SPWS_ERR_OOM;
   }
+  int64_t q = 0;
   for (int64_t f = 0; f < F; ++f)
     for (int32_t r = 0; r < R; ++r) {
-      o1[f * R + r] = hi[f];
-      o2[f * R + r] = hj[f];
-      o3[f * R + r] = r;
+      if (!fullU && !((hm[f] >> r) & 1)) continue;
+      o1[q] = hi[f];
+      o2[q] = hj[f];
+      o3[q] = r;
+      if (!fullU) ov[q] = hy[f * R + r];
+      ++q;
     }
   free(hi);
   free(hj);
+  free(hm);
+  if (ov != hy) free(hy);
   A->indices[1][0] = o1;
   A->indices[2][0] = o2;
   A->indices[3][0] = o3;
-  A->vals = hy;
+  A->vals = ov;
   A->vals_size = (int32_t)n;
   return SPWS_OK;
 }

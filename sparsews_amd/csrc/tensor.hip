@@ -124,18 +124,34 @@ static int bits_of(int64_t v) {  // bits for values in [0, v)
 
 // ------------------------------------------------------------------ MTTKRP
 
+// With factor row masks (bit r: column r of the factor row is stored), a
+// nonzero whose two rows share no stored column takes no part (the
+// reference's co-iteration never reaches it): it gets the row key `drop_i`,
+// past every output row, so the sort moves it to the end, and is counted.
+template <bool kMask>
 __global__ __launch_bounds__(256) void k_mttkrp_keys(int64_t nnz, const int32_t* __restrict__ leaf_fiber,
                                                      const int32_t* __restrict__ fiber_root,
                                                      const int32_t* __restrict__ l1_crd,
                                                      const int32_t* __restrict__ l2_crd,
                                                      const int32_t* __restrict__ l3_crd, int bj, int bk,
-                                                     uint64_t* __restrict__ keys) {
+                                                     uint64_t* __restrict__ keys, const uint64_t* __restrict__ mB,
+                                                     const uint64_t* __restrict__ mC, uint64_t drop_i,
+                                                     unsigned long long* __restrict__ dropped) {
+  unsigned long long nd = 0;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
     const int32_t f = leaf_fiber[e];
     const uint64_t j = (uint32_t)l1_crd[fiber_root[f]];
     const uint64_t k = (uint32_t)l2_crd[f];
-    const uint64_t i = (uint32_t)l3_crd[e];
+    uint64_t i = (uint32_t)l3_crd[e];
+    if (kMask && (mB[j] & mC[k]) == 0) {
+      i = drop_i;
+      ++nd;
+    }
     keys[e] = (i << (bj + bk)) | (j << bk) | k;
+  }
+  if (kMask) {
+    nd = wave_sum(nd);
+    if (lane_id() == 0 && nd) atomicAdd(dropped, nd);
   }
 }
 
@@ -148,12 +164,17 @@ __global__ __launch_bounds__(256) void k_mttkrp_keys(int64_t nnz, const int32_t*
 constexpr int kAccBlock = 256;
 constexpr int kAccU = 8;
 
-template <bool kTwo>
+// kMask: per row, the OR of its nonzeros' shared stored factor columns
+// (mB[j] & mC[k]) is or-ed into rmask[i], the row's stored output columns.
+template <bool kTwo, bool kMask>
 __global__ __launch_bounds__(kAccBlock) void k_mttkrp_accum(int64_t nnz, int64_t seg, const uint64_t* __restrict__ keys,
                                                             const double* __restrict__ vals, int bj, int bk, int R,
                                                             const double* __restrict__ Bf,
                                                             const double* __restrict__ Cf, double* __restrict__ M,
-                                                            int32_t* __restrict__ present) {
+                                                            int32_t* __restrict__ present,
+                                                            const uint64_t* __restrict__ mB,
+                                                            const uint64_t* __restrict__ mC,
+                                                            unsigned long long* __restrict__ rmask) {
   const int half = threadIdx.x >> 5;
   const int r0 = threadIdx.x & 31;
   const int64_t hid = (int64_t)blockIdx.x * (kAccBlock / 32) + half;
@@ -166,6 +187,7 @@ __global__ __launch_bounds__(kAccBlock) void k_mttkrp_accum(int64_t nnz, int64_t
   const int rr0 = r0 < R ? r0 : 0, rr1 = r1 < R ? r1 : 0;  // in-range rank for the loads
   int64_t cur = -1;
   double acc0 = 0.0, acc1 = 0.0;
+  uint64_t rowm = 0;
   // Bf(j,:) of the last nonzero: within a row the nonzeros come in (j, k)
   // order, so runs of equal j reuse it instead of gathering it again
   int64_t jprev = -1;
@@ -218,10 +240,13 @@ __global__ __launch_bounds__(kAccBlock) void k_mttkrp_accum(int64_t nnz, int64_t
             if (r0 < R) atomicAdd(&M[cur * R + r0], acc0);
             if (two && r1 < R) atomicAdd(&M[cur * R + r1], acc1);
             if (r0 == 0) present[cur] = 1;
+            if (kMask && r0 == 0) atomicOr(&rmask[cur], (unsigned long long)rowm);
           }
           cur = i;
           acc0 = acc1 = 0.0;
+          rowm = 0;
         }
+        if (kMask) rowm |= mB[(kk[u] >> bk) & jm] & mC[kk[u] & km];
         acc0 += (xv[u] * b0[u]) * c0[u];
         if (two) acc1 += (xv[u] * b1[u]) * c1[u];
       }
@@ -231,13 +256,19 @@ __global__ __launch_bounds__(kAccBlock) void k_mttkrp_accum(int64_t nnz, int64_t
     if (r0 < R) atomicAdd(&M[cur * R + r0], acc0);
     if (two && r1 < R) atomicAdd(&M[cur * R + r1], acc1);
     if (r0 == 0) present[cur] = 1;
+    if (kMask && r0 == 0) atomicOr(&rmask[cur], (unsigned long long)rowm);
   }
 }
 
 __global__ __launch_bounds__(256) void k_rows_compact(int32_t n, const int32_t* __restrict__ present,
-                                                      const int32_t* __restrict__ off, int32_t* __restrict__ rows) {
+                                                      const int32_t* __restrict__ off, int32_t* __restrict__ rows,
+                                                      const unsigned long long* __restrict__ rmask,
+                                                      uint64_t* __restrict__ rows_mask) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && present[i]) rows[off[i]] = i;
+  if (i < n && present[i]) {
+    rows[off[i]] = i;
+    if (rmask) rows_mask[off[i]] = rmask[i];
+  }
 }
 
 // ------------------------------------------------------------------ TTM
@@ -515,20 +546,48 @@ struct Csf3 {
 // (i, j) order, with its nonzeros in increasing k (the CSF order (i, k, j)
 // has them so within a root) -- the order hash_il accumulates them
 // (COO_CSF_DCSR_hash_il.h:247-377).
+// With a U row mask (bit r: U(k, r) stored), a nonzero whose U row stores
+// nothing takes no part (the reference's co-iteration of the contracted
+// level with U's rows skips it): it gets the fiber key of row `drop_i`, past
+// every root, so the sort moves it to the end, and is counted.
+template <bool kMask>
 __global__ __launch_bounds__(256) void k_ttm_direct_keys(int64_t nnz, const int32_t* __restrict__ leaf_fiber,
                                                          const int32_t* __restrict__ fiber_root,
                                                          const int32_t* __restrict__ l1_crd,
                                                          const int32_t* __restrict__ l2_crd,
                                                          const int32_t* __restrict__ l3_crd,
                                                          const double* __restrict__ xvals, int bj, int bk,
-                                                         uint64_t* __restrict__ keys, double* __restrict__ vals) {
+                                                         uint64_t* __restrict__ keys, double* __restrict__ vals,
+                                                         const uint64_t* __restrict__ umask, uint64_t drop_i,
+                                                         unsigned long long* __restrict__ dropped) {
+  unsigned long long nd = 0;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
     const int32_t f = leaf_fiber[e];
-    const uint64_t i = (uint32_t)l1_crd[fiber_root[f]];
+    uint64_t i = (uint32_t)l1_crd[fiber_root[f]];
     const uint64_t k = (uint32_t)l2_crd[f];
     const uint64_t j = (uint32_t)l3_crd[e];
+    if (kMask && umask[k] == 0) {
+      i = drop_i;
+      ++nd;
+    }
     keys[e] = (i << (bj + bk)) | (j << bk) | k;
     vals[e] = xvals[e];
+  }
+  if (kMask) {
+    nd = wave_sum(nd);
+    if (lane_id() == 0 && nd) atomicAdd(dropped, nd);
+  }
+}
+
+// per fiber: the OR of its nonzeros' U row masks (its stored output columns)
+__global__ __launch_bounds__(256) void k_ttm_fiber_mask(int64_t F, const int32_t* __restrict__ foff,
+                                                        const uint64_t* __restrict__ keys, int bk,
+                                                        const uint64_t* __restrict__ umask, uint64_t* __restrict__ fmask) {
+  const uint64_t km = (1ull << bk) - 1;
+  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < F; f += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t m = 0;
+    for (int32_t p = foff[f]; p < foff[f + 1]; ++p) m |= umask[keys[p] & km];
+    fmask[f] = m;
   }
 }
 
@@ -569,13 +628,21 @@ static int csf_owners(spws_ctx* ctx, const Csf3& x, int32_t** leaf_fiber, int32_
   return seg_owner(ctx, x.l2_pos, x.nroots, x.nfibers, *fiber_root);
 }
 
+// mB / mC (device, J and K entries; both null or both set): per factor row,
+// its stored columns as bits (the factors are given dense, zero where not
+// stored).  With masks, the rows' stored output columns go to
+// ctx->obuf[7] (one uint64 per listed row).
 static int mttkrp_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_t K, int32_t R, const double* Bf,
-                      const double* Cf, int32_t* nrows_out) {
+                      const double* Cf, const uint64_t* mB, const uint64_t* mC, int32_t* nrows_out) {
   int rc = ctx_activate(ctx);
   if (rc) return rc;
   SPWS_REQUIRE(I >= 0 && J >= 0 && K >= 0 && R >= 1 && R <= 64, SPWS_ERR_INVALID,
                "MTTKRP needs non-negative dimensions and 1 <= R <= 64");
-  const int bi = bits_of(I > 1 ? I : 2), bj = bits_of(J > 1 ? J : 2), bk = bits_of(K > 1 ? K : 2);
+  SPWS_REQUIRE((mB == nullptr) == (mC == nullptr), SPWS_ERR_INVALID, "give both factor masks or neither");
+  const bool masked = mB != nullptr;
+  // with masks, row I is the key of the nonzeros that take no part
+  const int bi = bits_of(masked ? (int64_t)I + 1 : (I > 1 ? I : 2)), bj = bits_of(J > 1 ? J : 2),
+            bk = bits_of(K > 1 ? K : 2);
   SPWS_REQUIRE(bi + bj + bk <= 64, SPWS_ERR_UNSUPPORTED, "tensor dimensions exceed the 64-bit packed key");
   hipStream_t s = ctx->stream;
   memset(&ctx->stats, 0, sizeof(ctx->stats));
@@ -585,11 +652,16 @@ static int mttkrp_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_
       (rc = ctx->out_crd.ensure(ii * 4)) || (rc = ctx->keys[0].ensure(nz * 8)) || (rc = ctx->keys[1].ensure(nz * 8)) ||
       (rc = ctx->vals[0].ensure(nz * 8)) || (rc = ctx->vals[1].ensure(nz * 8)))
     return rc;
+  if (masked && ((rc = ctx->obuf[6].ensure(ii * 8 + 256)) || (rc = ctx->obuf[7].ensure(ii * 8)))) return rc;
   double* M = ctx->out_val.as<double>();
   int32_t* present = ctx->out_aux.as<int32_t>();
   int32_t* poff = present + ii;
+  unsigned long long* rmask = masked ? ctx->obuf[6].as<unsigned long long>() : nullptr;
+  unsigned long long* dropped = masked ? rmask + ii : nullptr;
   SPWS_HIP(hipMemsetAsync(M, 0, ii * R * 8, s));
   SPWS_HIP(hipMemsetAsync(present, 0, ii * 4, s));
+  if (masked) SPWS_HIP(hipMemsetAsync(rmask, 0, ii * 8 + 8, s));
+  int64_t kept = x.nnz;
   if (x.nnz > 0) {
     int32_t *leaf_fiber, *fiber_root;
     if ((rc = csf_owners(ctx, x, &leaf_fiber, &fiber_root))) return rc;
@@ -597,8 +669,14 @@ static int mttkrp_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_
     double* vals = ctx->vals[0].as<double>();
     SPWS_HIP(hipMemcpyAsync(vals, x.vals, (size_t)x.nnz * 8, hipMemcpyDeviceToDevice, s));
     const unsigned g = (unsigned)std::min<int64_t>(div_up(x.nnz, 256), 65536);
-    SPWS_LAUNCH(ctx, "mttkrp_keys", k_mttkrp_keys, dim3(g), dim3(256), 0, x.nnz, leaf_fiber, fiber_root, x.l1_crd,
-                x.l2_crd, x.l3_crd, bj, bk, keys);
+    if (masked) {
+      SPWS_LAUNCH(ctx, "mttkrp_keys", k_mttkrp_keys<true>, dim3(g), dim3(256), 0, x.nnz, leaf_fiber, fiber_root,
+                  x.l1_crd, x.l2_crd, x.l3_crd, bj, bk, keys, mB, mC, (uint64_t)I, dropped);
+    } else {
+      SPWS_LAUNCH(ctx, "mttkrp_keys", k_mttkrp_keys<false>, dim3(g), dim3(256), 0, x.nnz, leaf_fiber, fiber_root,
+                  x.l1_crd, x.l2_crd, x.l3_crd, bj, bk, keys, (const uint64_t*)nullptr, (const uint64_t*)nullptr,
+                  (uint64_t)0, (unsigned long long*)nullptr);
+    }
     bool in_alt = false;
     if ((rc = radix_sort_pairs<uint64_t>(ctx, keys, vals, ctx->keys[1].as<uint64_t>(), ctx->vals[1].as<double>(),
                                          x.nnz, bj + bk, bj + bk + bi, &in_alt)))
@@ -607,22 +685,39 @@ static int mttkrp_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_
       keys = ctx->keys[1].as<uint64_t>();
       vals = ctx->vals[1].as<double>();
     }
+    if (masked) {
+      // the nonzeros that take part come first after the sort
+      int64_t* hd = ctx->host_meta.as<int64_t>();
+      SPWS_HIP(hipMemcpyAsync(hd, dropped, 8, hipMemcpyDeviceToHost, s));
+      SPWS_HIP(hipStreamSynchronize(s));
+      kept = x.nnz - *hd;
+    }
     // segments of 128 nonzeros per half-wavefront
     const int64_t seg = 128;
-    const int64_t halves = div_up(x.nnz, seg);
-    if (R > 32) {
-      SPWS_LAUNCH(ctx, "mttkrp_accum", k_mttkrp_accum<true>, dim3((unsigned)div_up(halves, kAccBlock / 32)),
-                  dim3(kAccBlock), 0, x.nnz, seg, keys, vals, bj, bk, R, Bf, Cf, M, present);
-    } else {
-      SPWS_LAUNCH(ctx, "mttkrp_accum", k_mttkrp_accum<false>, dim3((unsigned)div_up(halves, kAccBlock / 32)),
-                  dim3(kAccBlock), 0, x.nnz, seg, keys, vals, bj, bk, R, Bf, Cf, M, present);
+    const int64_t halves = div_up(kept, seg);
+    const unsigned ga = (unsigned)std::max<int64_t>(1, div_up(halves, kAccBlock / 32));
+    if (kept > 0) {
+      if (R > 32 && masked) {
+        SPWS_LAUNCH(ctx, "mttkrp_accum", (k_mttkrp_accum<true, true>), dim3(ga), dim3(kAccBlock), 0, kept, seg, keys,
+                    vals, bj, bk, R, Bf, Cf, M, present, mB, mC, rmask);
+      } else if (R > 32) {
+        SPWS_LAUNCH(ctx, "mttkrp_accum", (k_mttkrp_accum<true, false>), dim3(ga), dim3(kAccBlock), 0, kept, seg, keys,
+                    vals, bj, bk, R, Bf, Cf, M, present, mB, mC, rmask);
+      } else if (masked) {
+        SPWS_LAUNCH(ctx, "mttkrp_accum", (k_mttkrp_accum<false, true>), dim3(ga), dim3(kAccBlock), 0, kept, seg, keys,
+                    vals, bj, bk, R, Bf, Cf, M, present, mB, mC, rmask);
+      } else {
+        SPWS_LAUNCH(ctx, "mttkrp_accum", (k_mttkrp_accum<false, false>), dim3(ga), dim3(kAccBlock), 0, kept, seg,
+                    keys, vals, bj, bk, R, Bf, Cf, M, present, mB, mC, rmask);
+      }
     }
-    ctx->stats.products = x.nnz * (int64_t)R;
+    ctx->stats.products = kept * (int64_t)R;
   }
   if ((rc = exclusive_scan<int32_t, int32_t>(ctx, present, poff, I))) return rc;
   if (I > 0)
     SPWS_LAUNCH(ctx, "rows_compact", k_rows_compact, dim3((unsigned)div_up(I, 256)), dim3(256), 0, I, present, poff,
-                ctx->out_crd.as<int32_t>());
+                ctx->out_crd.as<int32_t>(), (const unsigned long long*)rmask,
+                masked ? ctx->obuf[7].as<uint64_t>() : (uint64_t*)nullptr);
   int32_t* hm = ctx->host_meta.as<int32_t>();
   SPWS_HIP(hipMemcpyAsync(hm, poff + I, 4, hipMemcpyDeviceToHost, s));
   SPWS_HIP(hipStreamSynchronize(s));
@@ -672,13 +767,19 @@ static int ttm_accumulate(spws_ctx* ctx, int64_t nnz, int64_t F, int32_t R, int 
   return SPWS_OK;
 }
 
+// umask (device, K entries, or null): per U row, its stored columns as bits
+// (U given dense, zero where not stored); the fibers' stored output columns
+// then go to ctx->obuf[6] after its first 256 bytes (one uint64 per fiber).
 static int ttm_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_t K, int32_t R, const double* U,
-                   int64_t* nfib_out) {
+                   const uint64_t* umask, int64_t* nfib_out) {
   int rc = ctx_activate(ctx);
   if (rc) return rc;
   SPWS_REQUIRE(I >= 0 && J >= 0 && K >= 0 && R >= 1 && R <= 256, SPWS_ERR_INVALID,
                "TTM needs non-negative dimensions and 1 <= R <= 256");
-  const int bi = bits_of(I > 1 ? I : 2), bj = bits_of(J > 1 ? J : 2);
+  const bool masked = umask != nullptr;
+  SPWS_REQUIRE(!masked || R <= 64, SPWS_ERR_UNSUPPORTED, "a U row mask needs R <= 64");
+  // with a mask, root I is the key of the nonzeros that take no part
+  const int bi = bits_of(masked ? (int64_t)I + 1 : (I > 1 ? I : 2)), bj = bits_of(J > 1 ? J : 2);
   SPWS_REQUIRE(bi + bj <= 63, SPWS_ERR_UNSUPPORTED, "tensor dimensions exceed the 64-bit fiber key");
   SPWS_REQUIRE(x.nnz < ((int64_t)1 << 30), SPWS_ERR_UNSUPPORTED, "TTM supports fewer than 2^30 nonzeros");
   hipStream_t s = ctx->stream;
@@ -686,6 +787,8 @@ static int ttm_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_t K
   const size_t nz = (size_t)(x.nnz > 0 ? x.nnz : 1);
   const int bk_all = bits_of(K > 1 ? K : 2);
   const bool direct = bi + bj + bk_all <= 64;
+  SPWS_REQUIRE(!masked || direct, SPWS_ERR_UNSUPPORTED,
+               "a U row mask needs the root, leaf and contracted coordinates within 64 bits");
   // hash path: a table of 2 * nnz (power of two) slots
   uint64_t T = 1024;
   while (!direct && T < 2 * (uint64_t)nz) T <<= 1;
@@ -708,8 +811,18 @@ static int ttm_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_t K
     uint64_t* nkeys = ctx->keys[0].as<uint64_t>();
     double* nvals = ctx->vals[0].as<double>();
     const unsigned g = (unsigned)std::min<int64_t>(div_up(x.nnz, 256), 65536);
-    SPWS_LAUNCH(ctx, "ttm_keys", k_ttm_direct_keys, dim3(g), dim3(256), 0, x.nnz, leaf_fiber, fiber_root, x.l1_crd,
-                x.l2_crd, x.l3_crd, x.vals, bj, bk, nkeys, nvals);
+    unsigned long long* dropped = nullptr;
+    if (masked) {
+      if ((rc = ctx->obuf[6].ensure(256))) return rc;
+      dropped = ctx->obuf[6].as<unsigned long long>();
+      SPWS_HIP(hipMemsetAsync(dropped, 0, 8, s));
+      SPWS_LAUNCH(ctx, "ttm_keys", k_ttm_direct_keys<true>, dim3(g), dim3(256), 0, x.nnz, leaf_fiber, fiber_root,
+                  x.l1_crd, x.l2_crd, x.l3_crd, x.vals, bj, bk, nkeys, nvals, umask, (uint64_t)I, dropped);
+    } else {
+      SPWS_LAUNCH(ctx, "ttm_keys", k_ttm_direct_keys<false>, dim3(g), dim3(256), 0, x.nnz, leaf_fiber, fiber_root,
+                  x.l1_crd, x.l2_crd, x.l3_crd, x.vals, bj, bk, nkeys, nvals, (const uint64_t*)nullptr, (uint64_t)0,
+                  (unsigned long long*)nullptr);
+    }
     // the sort covers (i, j) only: within an i slice the CSF order already
     // has each (i, j)'s nonzeros in increasing k, and the sort is stable
     bool alt = false;
@@ -720,10 +833,18 @@ static int ttm_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_t K
       nkeys = ctx->keys[1].as<uint64_t>();
       nvals = ctx->vals[1].as<double>();
     }
+    // the nonzeros that take part come first after the sort
+    int64_t kept = x.nnz;
+    if (masked) {
+      int64_t* hd = ctx->host_meta.as<int64_t>() + 1;
+      SPWS_HIP(hipMemcpyAsync(hd, dropped, 8, hipMemcpyDeviceToHost, s));
+      SPWS_HIP(hipStreamSynchronize(s));
+      kept = x.nnz - *hd;
+    }
     // fiber flags and their prefix (flag / off hold nnz + 1 entries here)
-    SPWS_LAUNCH(ctx, "ttm_flags", k_ttm_fiber_flags, dim3(g), dim3(256), 0, x.nnz, nkeys, bk, flag);
-    if ((rc = exclusive_scan<int32_t, int32_t>(ctx, flag, off, x.nnz))) return rc;
-    SPWS_HIP(hipMemcpyAsync(hm, off + x.nnz, 4, hipMemcpyDeviceToHost, s));
+    if (kept > 0) SPWS_LAUNCH(ctx, "ttm_flags", k_ttm_fiber_flags, dim3(g), dim3(256), 0, kept, nkeys, bk, flag);
+    if ((rc = exclusive_scan<int32_t, int32_t>(ctx, flag, off, kept))) return rc;
+    SPWS_HIP(hipMemcpyAsync(hm, off + kept, 4, hipMemcpyDeviceToHost, s));
     SPWS_HIP(hipStreamSynchronize(s));
     F = *hm;
     if ((rc = ctx->out_crd.ensure((size_t)(F > 0 ? F : 1) * 4 * 2)) ||
@@ -733,10 +854,18 @@ static int ttm_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_t K
     int32_t* out_i = ctx->out_crd.as<int32_t>();
     int32_t* out_j = out_i + F;
     int32_t* foff = ctx->obuf[5].as<int32_t>();
-    SPWS_LAUNCH(ctx, "ttm_starts", k_ttm_fiber_starts, dim3(g), dim3(256), 0, x.nnz, nkeys, bj, bk, flag, off, foff,
-                out_i, out_j);
-    if ((rc = ttm_accumulate(ctx, x.nnz, F, R, bk, nkeys, nvals, foff, U, ctx->out_val.as<double>()))) return rc;
-    ctx->stats.products = x.nnz * (int64_t)R;
+    if (kept > 0) {
+      SPWS_LAUNCH(ctx, "ttm_starts", k_ttm_fiber_starts, dim3(g), dim3(256), 0, kept, nkeys, bj, bk, flag, off, foff,
+                  out_i, out_j);
+      if ((rc = ttm_accumulate(ctx, kept, F, R, bk, nkeys, nvals, foff, U, ctx->out_val.as<double>()))) return rc;
+    }
+    if (masked && F > 0) {
+      if ((rc = ctx->obuf[6].ensure(256 + (size_t)F * 8))) return rc;
+      const unsigned gfm = (unsigned)std::min<int64_t>(div_up(F, 256), 65536);
+      SPWS_LAUNCH(ctx, "ttm_fiber_mask", k_ttm_fiber_mask, dim3(gfm), dim3(256), 0, F, foff, nkeys, bk, umask,
+                  reinterpret_cast<uint64_t*>(ctx->obuf[6].as<char>() + 256));
+    }
+    ctx->stats.products = kept * (int64_t)R;
   } else if (x.nnz > 0) {
     int32_t *leaf_fiber, *fiber_root;
     if ((rc = csf_owners(ctx, x, &leaf_fiber, &fiber_root))) return rc;
@@ -812,16 +941,26 @@ int spws_mttkrp(spws_ctx* ctx, int32_t I, int32_t J, int32_t K, int32_t R, int32
                 const int32_t* l2_pos, const int32_t* l2_crd, int32_t nfibers, const int32_t* l3_pos,
                 const int32_t* l3_crd, const double* vals, int64_t nnz, const double* Bf, const double* Cf,
                 int32_t* nrows, int32_t** rows, double** M) {
+  return spws_mttkrp_masked(ctx, I, J, K, R, nroots, l1_crd, l2_pos, l2_crd, nfibers, l3_pos, l3_crd, vals, nnz, Bf,
+                            Cf, nullptr, nullptr, nrows, rows, nullptr, M);
+}
+
+int spws_mttkrp_masked(spws_ctx* ctx, int32_t I, int32_t J, int32_t K, int32_t R, int32_t nroots,
+                       const int32_t* l1_crd, const int32_t* l2_pos, const int32_t* l2_crd, int32_t nfibers,
+                       const int32_t* l3_pos, const int32_t* l3_crd, const double* vals, int64_t nnz, const double* Bf,
+                       const double* Cf, const uint64_t* Bmask, const uint64_t* Cmask, int32_t* nrows,
+                       int32_t** rows, uint64_t** row_mask, double** M) {
   SPWS_REQUIRE(nnz >= 0 && nroots >= 0 && nfibers >= 0, SPWS_ERR_INVALID, "negative size");
   SPWS_REQUIRE(nnz == 0 || (l1_crd && l2_pos && l2_crd && l3_pos && l3_crd && vals && Bf && Cf), SPWS_ERR_INVALID,
                "null pointer argument");
   SPWS_REQUIRE(nnz < ((int64_t)1 << 31), SPWS_ERR_UNSUPPORTED, "MTTKRP supports fewer than 2^31 nonzeros");
   Csf3 x{nroots, l1_crd, l2_pos, l2_crd, nfibers, l3_pos, l3_crd, vals, nnz};
   int32_t nr = 0;
-  int rc = mttkrp_run(ctx, x, I, J, K, R, Bf, Cf, &nr);
+  int rc = mttkrp_run(ctx, x, I, J, K, R, Bf, Cf, Bmask, Cmask, &nr);
   if (rc) return rc;
   if (nrows) *nrows = nr;
   if (rows) *rows = ctx->out_crd.as<int32_t>();
+  if (row_mask) *row_mask = Bmask ? ctx->obuf[7].as<uint64_t>() : nullptr;
   if (M) *M = ctx->out_val.as<double>();
   return SPWS_OK;
 }
@@ -830,16 +969,26 @@ int spws_ttm(spws_ctx* ctx, int32_t I, int32_t J, int32_t K, int32_t R, int32_t 
              const int32_t* l2_pos, const int32_t* l2_crd, int32_t nfibers, const int32_t* l3_pos,
              const int32_t* l3_crd, const double* vals, int64_t nnz, const double* U, int64_t* nfib,
              int32_t** fib_i, int32_t** fib_j, double** Y) {
+  return spws_ttm_masked(ctx, I, J, K, R, nroots, l1_crd, l2_pos, l2_crd, nfibers, l3_pos, l3_crd, vals, nnz, U,
+                         nullptr, nfib, fib_i, fib_j, nullptr, Y);
+}
+
+int spws_ttm_masked(spws_ctx* ctx, int32_t I, int32_t J, int32_t K, int32_t R, int32_t nroots,
+                    const int32_t* l1_crd, const int32_t* l2_pos, const int32_t* l2_crd, int32_t nfibers,
+                    const int32_t* l3_pos, const int32_t* l3_crd, const double* vals, int64_t nnz, const double* U,
+                    const uint64_t* Umask, int64_t* nfib, int32_t** fib_i, int32_t** fib_j, uint64_t** fib_mask,
+                    double** Y) {
   SPWS_REQUIRE(nnz >= 0 && nroots >= 0 && nfibers >= 0, SPWS_ERR_INVALID, "negative size");
   SPWS_REQUIRE(nnz == 0 || (l1_crd && l2_pos && l2_crd && l3_pos && l3_crd && vals && U), SPWS_ERR_INVALID,
                "null pointer argument");
   Csf3 x{nroots, l1_crd, l2_pos, l2_crd, nfibers, l3_pos, l3_crd, vals, nnz};
   int64_t F = 0;
-  int rc = ttm_run(ctx, x, I, J, K, R, U, &F);
+  int rc = ttm_run(ctx, x, I, J, K, R, U, Umask, &F);
   if (rc) return rc;
   if (nfib) *nfib = F;
   if (fib_i) *fib_i = ctx->out_crd.as<int32_t>();
   if (fib_j) *fib_j = ctx->out_crd.as<int32_t>() + F;
+  if (fib_mask) *fib_mask = Umask ? reinterpret_cast<uint64_t*>(ctx->obuf[6].as<char>() + 256) : nullptr;
   if (Y) *Y = ctx->out_val.as<double>();
   return SPWS_OK;
 }

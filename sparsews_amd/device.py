@@ -247,7 +247,8 @@ def set_outer_chunking(ctx: Context, max_products: int = 0, max_rows: int = 0):
 
 
 def set_outer_method(ctx: Context, method: int = 0, window_log2: int = 0):
-    """0 automatic, 1 LSD radix sort, 2 row/window buckets (see spws.h)."""
+    """0 automatic, 1 LSD radix sort, 2 row/window buckets, 3 fused row
+    windows (see spws.h)."""
     check(lib().spws_ctx_set_outer_method(ctx.h, int(method), int(window_log2)))
 
 

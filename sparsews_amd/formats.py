@@ -88,6 +88,35 @@ def csf3_from_coo(x: COO3, order) -> CSF3:
     return CSF3(tuple(x.dims), (a, b, c), l1_crd, l2_pos, l2_crd, l3_pos, l3_crd, vals)
 
 
+def partial_dcsr(m: np.ndarray, keep_rows, col_density: float, seed: int) -> DCSR:
+    """DCSR of the rows of m where keep_rows is true, each storing a random
+    subset of its columns (probability col_density each, at least one): a
+    hypersparse factor with missing rows and partial rows."""
+    rng = np.random.default_rng(seed)
+    rows = np.nonzero(np.asarray(keep_rows))[0].astype(np.int32)
+    pos, crd = [0], []
+    for r in rows:
+        cols = np.nonzero(rng.random(m.shape[1]) < col_density)[0]
+        if len(cols) == 0:
+            cols = np.array([int(rng.integers(m.shape[1]))])
+        crd.append(cols)
+        pos.append(pos[-1] + len(cols))
+    crd = np.concatenate(crd).astype(np.int32) if crd else np.zeros(0, np.int32)
+    vals = np.concatenate([m[r, c] for r, c in zip(rows, np.split(crd, pos[1:-1]))]) if len(rows) else np.zeros(0)
+    return DCSR(m.shape[0], m.shape[1], rows, np.asarray(pos, dtype=np.int32), crd, np.asarray(vals, np.float64))
+
+
+def dcsr_dense_and_mask(d: DCSR):
+    """(dense matrix, zero where not stored; boolean mask of stored entries)."""
+    D = np.zeros((d.nrows, d.ncols))
+    S = np.zeros((d.nrows, d.ncols), dtype=bool)
+    for t, r in enumerate(d.l1_crd):
+        c = d.l2_crd[d.l2_pos[t]:d.l2_pos[t + 1]]
+        D[r, c] = d.vals[d.l2_pos[t]:d.l2_pos[t + 1]]
+        S[r, c] = True
+    return D, S
+
+
 def dense_as_dcsr(m: np.ndarray) -> DCSR:
     """A dense matrix stored as a fully populated DCSR (the reference loads
     factor matrices this way, bench_hash_mttkrp.cpp:18-25)."""

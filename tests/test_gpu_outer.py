@@ -19,7 +19,7 @@ from sparsews_amd.gen import CSR
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
-METHODS = [1, 2]
+METHODS = [1, 2, 3]
 
 
 @contextlib.contextmanager
@@ -83,7 +83,8 @@ def test_outer_many_chunks(ctx, method):
             chunks = ctx.stats()["chunks"]
     finally:
         set_outer_chunking(ctx, 0, 0)
-    assert chunks >= 100
+    # the fused method (3) needs no product workspace, so it never chunks
+    assert chunks >= 100 if method != 3 else chunks == 1
     check(C, A, B)
 
 
@@ -118,48 +119,77 @@ def test_outer_edge_cases(ctx, method):
     np.testing.assert_array_equal(CI.vals, Bm.vals)
 
 
+@pytest.mark.parametrize("method", [2, 3])
 @pytest.mark.parametrize("wlog", [6, 8, 11])
-def test_outer_bucket_windows(ctx, wlog):
+def test_outer_bucket_windows(ctx, wlog, method):
     """Window widths from 64 to 2048 columns; the column count is not a
     multiple of the window, and some B rows run over many windows."""
-    print(f"case: outer bucket window 2^{wlog}", flush=True)
+    print(f"case: outer window 2^{wlog} method {method}", flush=True)
     A = powerlaw_rows(500, 400, 1.8, 250, 68, 0)
     B = powerlaw_rows(400, 3001, 1.8, 900, 68, 1)
-    with outer_method(ctx, 2, wlog):
+    with outer_method(ctx, method, wlog):
         C = run_outer(ctx, A, B)
         st = ctx.stats()
-    assert st["outer_method"] == 2 and st["window_bits"] == wlog
+    assert st["outer_method"] == method and st["window_bits"] == wlog
     check(C, A, B)
 
 
-def test_outer_bucket_wide_window_table(ctx):
+@pytest.mark.parametrize("method", [2, 3])
+def test_outer_bucket_wide_window_table(ctx, method):
     """More than 1024 windows per row (64-column windows over 80,000
-    columns): the expand reads each product's bucket slot from the global
-    window table instead of a per-entry copy in LDS."""
-    print("case: outer bucket 1250 windows per row", flush=True)
+    columns): the bucket expand reads each product's bucket slot from the
+    global window table instead of a per-entry copy in LDS; the fused walk
+    crosses 1,250 windows per row, most of them empty."""
+    print(f"case: outer 1250 windows per row method {method}", flush=True)
     A = powerlaw_rows(300, 400, 1.8, 200, 71, 0)
     B = powerlaw_rows(400, 80000, 1.8, 3000, 71, 1)
-    with outer_method(ctx, 2, 6):
+    with outer_method(ctx, method, 6):
         C = run_outer(ctx, A, B)
         st = ctx.stats()
-    assert st["outer_method"] == 2 and st["window_bits"] == 6
+    assert st["outer_method"] == method and st["window_bits"] == 6
     check(C, A, B)
 
 
-def test_outer_bucket_long_column_runs(ctx):
+@pytest.mark.parametrize("method", [2, 3])
+def test_outer_bucket_long_column_runs(ctx, method):
     """Outputs where one entry gathers many products (dense rows of A against
     a B whose rows all share a few columns), so one 64-lane stride holds
-    several products of the same column and of different j."""
-    print("case: outer bucket repeated columns", flush=True)
+    several products of the same column and of different j; rows of A have
+    150 to 300 entries, so the fused walk takes each window's entries in
+    chunks of 64."""
+    print(f"case: outer repeated columns method {method}", flush=True)
     rng = np.random.default_rng(69)
     A = uniform_rows(90, 300, 150, 300, 69, 0)
     lens = rng.integers(1, 6, 300)
     cols = np.concatenate([np.sort(rng.choice(8, size=l, replace=False)) for l in lens]).astype(np.int32)
     pos = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
     B = CSR(300, 8, pos, cols, rng.uniform(-1, 1, len(cols)))
-    with outer_method(ctx, 2, 6):
+    with outer_method(ctx, method, 6):
         C = run_outer(ctx, A, B)
     check(C, A, B)
+
+
+@pytest.mark.parametrize("wlog", [6, 11])
+def test_outer_fused_rows_over_many_chunks(ctx, wlog):
+    """Fused method: rows with 64 to 700 entries (one to eleven chunks of 64
+    per window) next to rows with none, against the bucket method bit for
+    bit and the oracle."""
+    print(f"case: outer fused long rows window 2^{wlog}", flush=True)
+    rng = np.random.default_rng(72)
+    lens = rng.integers(0, 700, 120)
+    lens[::5] = 0
+    lens[1] = 64
+    lens[2] = 65
+    A = random_csr(120, 900, lens, 72, 0)
+    B = powerlaw_rows(900, 5000, 1.8, 400, 72, 1)
+    with outer_method(ctx, 3, wlog):
+        C3 = run_outer(ctx, A, B)
+    with outer_method(ctx, 2, wlog):
+        C2 = run_outer(ctx, A, B)
+    np.testing.assert_array_equal(C3.pos, C2.pos)
+    np.testing.assert_array_equal(C3.crd, C2.crd)
+    np.testing.assert_array_equal(C3.vals, C2.vals)
+    check(C3, A, B)
 
 
 def test_outer_auto_method(ctx):

@@ -190,6 +190,58 @@ def test_ttm_taco_entry():
     assert oracle.max_value_error(gv, ov, absY[ok, oj, ol]) <= TOL
 
 
+@pytest.mark.parametrize("col_density", [1.0, 0.5])
+def test_mttkrp_taco_hypersparse_factors(col_density):
+    """DCSR factors with missing rows (and, at density 0.5, rows storing only
+    some columns) through the taco entry point: the co-iteration of
+    hash_free.h:277-300 on the device (factor row masks); entries, their
+    order and values against the oracle."""
+    print(f"case: mttkrp hypersparse factors density {col_density}", flush=True)
+    X = gen.zipf_coo3((120, 90, 70), 3000, (1.5, 1.5, 1.5), 16, 5, 0.0, 1.0)
+    R = 16
+    rng = np.random.default_rng(16)
+    Bf, Cf = rng.random((90, R)), rng.random((70, R))
+    Bd = formats.partial_dcsr(Bf, rng.random(90) < 0.6, col_density, 161)
+    Cd = formats.partial_dcsr(Cf, rng.random(70) < 0.7, col_density, 162)
+    csf = formats.csf3_from_coo(X, (1, 2, 0))
+    At = taco.init_taco_tensor_COO([120, R])
+    taco.mttkrp_compute(At, taco.csf_to_taco_tensor(csf), taco.CC_to_taco_tensor(Bd, 90, R),
+                        taco.CC_to_taco_tensor(Cd, 70, R), 1 << 10)
+    gi, gr, gv = taco.taco_coo(At)
+    oi, orr, ov = oracle.mttkrp_hash(csf, Bd, Cd, 120, R, 1 << 10)
+    assert len(oi) < 120 * R
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gr, orr)
+    Bz, _ = formats.dcsr_dense_and_mask(Bd)
+    Cz, _ = formats.dcsr_dense_and_mask(Cd)
+    absM = np.einsum("ijk,jr,kr->ir", np.abs(dense(X)), np.abs(Bz), np.abs(Cz))
+    assert oracle.max_value_error(gv, ov, absM[oi, orr]) <= TOL
+
+
+@pytest.mark.parametrize("col_density", [1.0, 0.5])
+def test_ttm_taco_hypersparse_factor(col_density):
+    """A DCSR U with missing rows (and partial rows at density 0.5) through
+    the taco entry point: hash_il's co-iteration of the contracted level with
+    U's rows on the device; fibers, entries and values against the oracle."""
+    print(f"case: ttm hypersparse factor density {col_density}", flush=True)
+    X = gen.zipf_coo3((80, 70, 30), 2500, (1.5, 1.5, None), 17, 5, -1.0, 1.0)
+    R = 16
+    rng = np.random.default_rng(17)
+    U = rng.uniform(-1, 1, (30, R))
+    Ud = formats.partial_dcsr(U, rng.random(30) < 0.5, col_density, 171)
+    csf = formats.csf3_from_coo(X, (0, 2, 1))
+    At = taco.init_taco_tensor_COO([80, 70, R])
+    taco.ttm_compute(At, taco.csf_to_taco_tensor(csf), taco.CC_to_taco_tensor(Ud, 30, R), 1)
+    gk, gj, gl, gv = taco.taco_coo(At)
+    ok, oj, ol, ov = oracle.ttm_hash_il(csf, Ud, 80, 70, R)
+    np.testing.assert_array_equal(gk, ok)
+    np.testing.assert_array_equal(gj, oj)
+    np.testing.assert_array_equal(gl, ol)
+    Uz, _ = formats.dcsr_dense_and_mask(Ud)
+    absY = np.einsum("ijk,kr->ijr", np.abs(dense(X)), np.abs(Uz))
+    assert oracle.max_value_error(gv, ov, absY[ok, oj, ol]) <= TOL
+
+
 def test_tensor_edge_cases(ctx):
     print("case: tensor edge cases", flush=True)
     X = gen.zipf_coo3((10, 9, 8), 1, (None, None, None), 15, 5, 0.5, 1.0)

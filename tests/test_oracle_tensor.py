@@ -58,6 +58,56 @@ def test_mttkrp_sparse_factor_rows():
     np.testing.assert_allclose(ov, M[rows].ravel(), rtol=1e-13, atol=1e-13)
 
 
+@pytest.mark.parametrize("col_density", [0.4, 0.8])
+def test_mttkrp_partial_factor_rows(col_density):
+    """Factors with missing rows and rows storing some columns: an entry
+    (i, r) exists when a nonzero X(i, j, k) meets stored Bf(j, r) and
+    Cf(k, r) (the rank co-iteration of hash_free.h:286-300), its value the
+    einsum over the stored entries (zero elsewhere)."""
+    X = small_tensor(4, lo=0.0, hi=1.0)
+    R = 6
+    rng = np.random.default_rng(4)
+    Bf, Cf = rng.random((X.dims[1], R)), rng.random((X.dims[2], R))
+    Bd = formats.partial_dcsr(Bf, rng.random(X.dims[1]) < 0.7, col_density, 41)
+    Cd = formats.partial_dcsr(Cf, rng.random(X.dims[2]) < 0.8, col_density, 42)
+    Bz, Bs = formats.dcsr_dense_and_mask(Bd)
+    Cz, Cs = formats.dcsr_dense_and_mask(Cd)
+    csf = formats.csf3_from_coo(X, (1, 2, 0))
+    oi, orr, ov = oracle.mttkrp_hash(csf, Bd, Cd, X.dims[0], R, 1 << 16)
+    present = np.zeros((X.dims[0], R), dtype=bool)
+    for i, j, k in X.idx:
+        present[i] |= Bs[j] & Cs[k]
+    ei, er = np.nonzero(present)
+    np.testing.assert_array_equal(oi, ei)
+    np.testing.assert_array_equal(orr, er)
+    M = np.einsum("ijk,jr,kr->ir", dense(X), Bz, Cz)
+    np.testing.assert_allclose(ov, M[ei, er], rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("col_density", [0.4, 1.0])
+def test_ttm_partial_factor_rows(col_density):
+    """U with missing rows and partial rows: fiber entry (i, j, r) exists
+    when a nonzero X(i, j, k) meets a stored U(k, r) (hash_il.h:294-303 and
+    its rank loop), its value the einsum over the stored entries."""
+    X = small_tensor(5, dims=(30, 25, 20), exps=(1.5, 1.5, None))
+    R = 5
+    rng = np.random.default_rng(5)
+    U = rng.uniform(-1, 1, (X.dims[2], R))
+    Ud = formats.partial_dcsr(U, rng.random(X.dims[2]) < 0.6, col_density, 51)
+    Uz, Us = formats.dcsr_dense_and_mask(Ud)
+    csf = formats.csf3_from_coo(X, (0, 2, 1))
+    ok, oj, ol, ov = oracle.ttm_hash_il(csf, Ud, X.dims[0], X.dims[1], R)
+    present = np.zeros((X.dims[0], X.dims[1], R), dtype=bool)
+    for i, j, k in X.idx:
+        present[i, j] |= Us[k]
+    ei, ej, er = np.nonzero(present)
+    np.testing.assert_array_equal(ok, ei)
+    np.testing.assert_array_equal(oj, ej)
+    np.testing.assert_array_equal(ol, er)
+    Y = np.einsum("ijk,kr->ijr", dense(X), Uz)
+    np.testing.assert_allclose(ov, Y[ei, ej, er], rtol=1e-13, atol=1e-13)
+
+
 def test_ttm_matches_einsum():
     X = small_tensor(3, dims=(30, 25, 20), exps=(1.5, 1.5, None))
     R = 6
