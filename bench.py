@@ -18,6 +18,8 @@ Workloads (sizes in the docstring of each runner):
   mttkrp    mode-1 MTTKRP, 50k x 40k x 30k, 20M nonzeros, rank 32
   ttm       mode-3 TTM, 100k x 80k x 2k, 10M nonzeros, U 2,000 x 16
   transpose CSR transpose on the device, 200k x 200k, 32 columns per row
+  fusion    fused vs non-fused transposed-output product over 15 cases,
+            output nnz 10^3 .. 10^8 (speedup per output-nnz decade)
   serve     4,096 SpGEMM requests, plain vs continuously batched
 """
 from __future__ import annotations
@@ -752,10 +754,180 @@ def run_transpose(args):
     return res
 
 
+# (rows m, inner k, columns n, entries per row r of both operands, seed);
+# nnz(C) is about m * r^2 while r^2 is far below n, so the cases span the
+# output-nnz decades 10^3 .. 10^8
+FUSION_CASES = [
+    (1_000, 1_000, 1_000, 2, 11), (2_000, 2_000, 2_000, 2, 12), (600, 600, 600, 3, 13),
+    (4_000, 4_000, 4_000, 3, 14), (10_000, 10_000, 10_000, 2, 15), (2_000, 2_000, 2_000, 6, 16),
+    (20_000, 20_000, 20_000, 4, 17), (50_000, 50_000, 50_000, 3, 18), (10_000, 10_000, 10_000, 8, 19),
+    (100_000, 100_000, 100_000, 4, 20), (100_000, 100_000, 100_000, 8, 21), (50_000, 50_000, 50_000, 12, 22),
+    (200_000, 200_000, 200_000, 8, 23), (500_000, 500_000, 500_000, 8, 24), (1_000_000, 1_000_000, 1_000_000, 8, 25),
+]
+FUSION_BINS = [0, 1e4, 1e5, 1e6, 1e7, 1e8]
+
+
+def _median_time(ctx, fn, budget_s=0.25, lo=3, hi=40):
+    """Median wall time of fn (each call followed by a stream sync): at
+    least lo calls, more while the budget lasts, at most hi."""
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < lo or (len(times) < hi and time.perf_counter() - t_start < budget_s):
+        t0 = time.perf_counter()
+        fn()
+        ctx.sync()
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times))
+
+
+def _bin_stats(xs, ys):
+    out = []
+    for lo, hi in zip(FUSION_BINS[:-1], FUSION_BINS[1:]):
+        sel = [y for x, y in zip(xs, ys) if lo <= x < hi]
+        if sel:
+            out.append({"output_nnz": [lo, hi], "cases": len(sel),
+                        "gmean": round(float(np.exp(np.mean(np.log(sel)))), 3),
+                        "min": round(min(sel), 3), "max": round(max(sel), 3)})
+    return out
+
+
+def run_fusion(args):
+    """Fused vs non-fused transposed-output product, C = (A B)^T with A CSC
+    (m x k) and B CSR (k x n), over FUSION_CASES (uniform random operands, r
+    distinct columns per row, FP64): the reference's one published figure,
+    the fusion speedup binned by output nnz (scripts/draw_bars.py:7-58, bins
+    [0, 1e4, 1e5, 1e6, 1e7, 1e8), geometric mean, min and max per bin).
+    Every array stays on the device.  Fused: one outer product B^T A^T, whose
+    row-major result is already (A B)^T.  Non-fused (nonfuse/CSC_CSR_T_hash.h:
+    compute_nofuse at :232, transpose at :352): the outer product A B, then
+    the transpose of its result into preallocated arrays.  A leg's time is
+    the median of repeated calls, each ended by a stream sync; fused and
+    non-fused outputs are compared bit for bit on every case up to 2*10^7
+    entries.  The CPU columns run the same two pipelines through the oracle
+    (the reference's coordinate workspace and transpose, one core) on the
+    cases up to 2*10^6 products."""
+    from sparsews_amd import formats, gen
+    from sparsews_amd._lib import check, lib
+    from sparsews_amd.device import Context, DeviceArray, DeviceDCSR, spgemm_outer
+    ctx = Context(0)
+    cases = []
+    big = None
+    for (m, k, n, r, seed) in FUSION_CASES:
+        A = gen.uniform_csr(m, k, r, seed, 0)
+        B = gen.uniform_csr(k, n, r, seed, 1)
+        dAc = DeviceDCSR.from_csr(ctx, A.transpose())   # A's columns: the stored outer level
+        dB = DeviceDCSR.from_csr(ctx, B)
+        R = spgemm_outer(ctx, dAc, dB, m, k, n)
+        nnz = R.nnz
+        products = ctx.stats()["products"]
+        t_pos = DeviceArray(ctx, n + 1, np.int32)
+        t_crd = DeviceArray(ctx, max(nnz, 1), np.int32)
+        t_val = DeviceArray(ctx, max(nnz, 1), np.float64)
+        held = {}
+
+        def transpose():
+            X = held["R"]
+            check(lib().spws_csr64_transpose_dev(ctx.h, m, n, X.nnz, X.pos.ptr, X.crd.ptr, X.vals.ptr, t_pos.ptr,
+                                                 t_crd.ptr, t_val.ptr))
+
+        def nonfused():
+            held["R"] = spgemm_outer(ctx, dAc, dB, m, k, n)
+            transpose()
+
+        def fused():
+            held["F"] = spgemm_outer(ctx, dB, dAc, n, k, m)
+
+        nonfused()
+        fused()
+        ctx.sync()
+        if nnz <= 20_000_000:
+            F = held["F"].download()
+            assert F.nnz == nnz
+            assert np.array_equal(np.asarray(F.pos, np.int64), t_pos.to_host().astype(np.int64))
+            assert np.array_equal(F.crd, t_crd.to_host()[:nnz]) and np.array_equal(F.vals, t_val.to_host()[:nnz])
+        t_f = t_n = float("inf")
+        for _ in range(2):   # alternate the legs so that drift affects both
+            t_n = min(t_n, _median_time(ctx, nonfused))
+            t_f = min(t_f, _median_time(ctx, fused))
+        nonfused()
+        t_t = _median_time(ctx, transpose)
+        case = {"m": m, "k": k, "n": n, "per_row": r, "seed": seed, "products": int(products), "nnz_C": int(nnz),
+                "fused_ms": round(t_f * 1e3, 4), "nonfused_ms": round(t_n * 1e3, 4),
+                "transpose_ms": round(t_t * 1e3, 4), "speedup": round(t_n / t_f, 3),
+                "bit_identical": nnz <= 20_000_000}
+        log(f"bench: fusion case {m}x{k}x{n} r={r}: nnz(C)={nnz} fused {t_f * 1e3:.3f} ms, "
+            f"non-fused {t_n * 1e3:.3f} ms (transpose {t_t * 1e3:.3f})")
+        if not args.no_cpu_baseline and products <= 2_000_000:
+            import oracle
+            Ad, Bd = formats.dcsr_from_csr(A.transpose()), formats.dcsr_from_csr(B)
+            w_cap = 1 << max(10, int(products).bit_length() - 1)
+
+            def cpu_fused():
+                oracle.outer_coord(Bd, Ad, n, m, w_cap)
+
+            def cpu_nonfused():
+                pos, crd, vals = oracle.outer_coord(Ad, Bd, m, n, w_cap)
+                oracle.csr_transpose(gen.CSR(m, n, pos.astype(np.int32), crd, vals))
+
+            cf = min(_median_time(ctx, cpu_fused, 0.5, 1, 10), _median_time(ctx, cpu_fused, 0.5, 1, 10))
+            cn = min(_median_time(ctx, cpu_nonfused, 0.5, 1, 10), _median_time(ctx, cpu_nonfused, 0.5, 1, 10))
+            case.update({"cpu_fused_ms": round(cf * 1e3, 4), "cpu_nonfused_ms": round(cn * 1e3, 4),
+                         "cpu_speedup": round(cn / cf, 3)})
+        cases.append(case)
+        if big is None or nnz > big[0]:
+            big = (nnz, m, k, n, dAc, dB, A.nnz + B.nnz)
+        else:
+            for a in (dAc.pos, dAc.crd, dAc.vals, dB.pos, dB.crd, dB.vals):
+                a.free()
+        for a in (t_pos, t_crd, t_val):
+            a.free()
+    # roofline of the largest case's fused product: compulsory bytes are both
+    # operands read once and the output's int64 row pointers and entries
+    # written once
+    nnz, m, k, n, dAc, dB, nin = big
+    out = {}
+
+    def step():
+        out["F"] = spgemm_outer(ctx, dB, dAc, n, k, m)
+
+    names = ("radix_scatter", "radix_hist", "outer_expand", "outer_reduce", "bk_expand", "bk_count", "bk_reduce",
+             "fz_count", "fz_reduce")
+    sec, dom, dom_times, bd = timed_steps(ctx, step, 3, 1,
+                                          lambda b: max((x for x in names if x in b), key=lambda x: b[x][0]))
+    compulsory = 4 * (2 * k + 2) + 12 * nin + 8 * (n + 1) + 12 * nnz
+    roof = roofline(dom, dom_times, 3, compulsory, compulsory, "fusion")
+    roof["kernel_io"] = None
+    ctx.close()
+    speed = [c["speedup"] for c in cases]
+    xs = [c["nnz_C"] for c in cases]
+    res = {
+        "metric": "fused vs non-fused transposed-output speedup (geometric mean)",
+        "value": round(float(np.exp(np.mean(np.log(speed)))), 3), "unit": "x",
+        "ms_per_step": round(sum(c["fused_ms"] for c in cases), 4), "dtype": "f64",
+        "config": {"workload": "csc_csr_transposed_output_sweep", "cases": len(cases),
+                   "operands": "uniform random, r distinct columns per row, m = k = n",
+                   "output_nnz_bins": FUSION_BINS},
+        "bins": _bin_stats(xs, speed),
+        "cases": cases,
+        "roofline": roof,
+        "roofline_case": {"m": m, "k": k, "n": n, "nnz_C": int(nnz), "ms_per_step": round(sec * 1e3, 4)},
+    }
+    cpu = [c for c in cases if "cpu_speedup" in c]
+    if cpu:
+        res["bins_cpu"] = _bin_stats([c["nnz_C"] for c in cpu], [c["cpu_speedup"] for c in cpu])
+        res["cpu_baseline"] = {
+            "value": round(float(np.exp(np.mean(np.log([c["cpu_speedup"] for c in cpu])))), 3), "unit": "x",
+            "cores": 1, "kind": "port",
+            "sample": f"{len(cpu)} cases up to 2e6 products: the oracle's coordinate workspace "
+                      "(DCSC_DCSR_coord_index_chase.h:194) fused (B^T A^T) vs A B then transpose() "
+                      "(nonfuse/CSC_CSR_T_hash.h:352)"}
+    return res
+
+
 WORKLOADS = {"rowwise": run_rowwise, "outer": run_outer, "mttkrp": run_mttkrp, "ttm": run_ttm, "serve": run_serve,
-             "transpose": run_transpose}
+             "transpose": run_transpose, "fusion": run_fusion}
 DEFAULT_STEPS = {"rowwise": (50, 5), "outer": (5, 1), "mttkrp": (20, 3), "ttm": (20, 3), "serve": (1, 1),
-                 "transpose": (20, 3)}
+                 "transpose": (20, 3), "fusion": (3, 1)}
 # the default run: the outer product (headline) and one record per other
 # workload of the measured set, each with its own roofline and CPU baseline
 ALL_HEAD = "outer"

@@ -172,6 +172,16 @@ int spws_csr_transpose_dev(spws_ctx* ctx, int32_t m, int32_t n, int64_t nnz, con
                            const int32_t* crd, const double* vals, int32_t* out_pos, int32_t* out_crd,
                            double* out_vals);
 
+/* spws_csr_transpose_dev for a CSR with int64 row pointers, the form
+ * spws_spgemm_outer returns its product in (nnz still below 2^31): the
+ * second half of the non-fused transposed product of
+ * benchmark/CSC_CSR_T/nonfuse/CSC_CSR_T_hash.h (compute_nofuse at :232,
+ * then transpose() at :352) with the product left on the device.  Same
+ * output arrays and order as spws_csr_transpose_dev. */
+int spws_csr64_transpose_dev(spws_ctx* ctx, int32_t m, int32_t n, int64_t nnz, const int64_t* pos,
+                             const int32_t* crd, const double* vals, int32_t* out_pos, int32_t* out_crd,
+                             double* out_vals);
+
 /* spws_csr_transpose_dev without a context and without allocating anything
  * (no hipMalloc/hipFree, no host allocation): the same kernels and the same
  * result, on `stream` (a hipStream_t; NULL = the null stream) of the current

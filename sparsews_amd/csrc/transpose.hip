@@ -31,14 +31,16 @@ __global__ __launch_bounds__(256) void k_tr_zero(int32_t n, int32_t* __restrict_
   for (int32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) counts[c] = 0;
 }
 
-// one wavefront per input row: keys and values
-__global__ __launch_bounds__(256) void k_tr_keys(int32_t m, const int32_t* __restrict__ pos,
+// one wavefront per input row: keys and values (row pointers int32, or
+// int64 as the outer product returns them; nnz stays below 2^31 either way)
+template <typename PosT>
+__global__ __launch_bounds__(256) void k_tr_keys(int32_t m, const PosT* __restrict__ pos,
                                                  const int32_t* __restrict__ crd, const double* __restrict__ vals,
                                                  uint64_t* __restrict__ keys, double* __restrict__ kv) {
   const int lane = lane_id();
   const int32_t waves = gridDim.x * (blockDim.x / kWave);
   for (int32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < m; i += waves) {
-    const int32_t b = pos[i], e = pos[i + 1];
+    const int32_t b = (int32_t)pos[i], e = (int32_t)pos[i + 1];
     for (int32_t p = b + lane; p < e; p += kWave) {
       keys[p] = ((uint64_t)(uint32_t)crd[p] << 32) | (uint32_t)i;
       kv[p] = vals[p];
@@ -65,7 +67,8 @@ __global__ __launch_bounds__(256) void k_tr_out(int64_t nnz, const uint64_t* __r
 
 // The transpose on a context: its workspaces are the context's buffers
 // (tr_keys, tr_vals, tr_counts, and the radix and scan scratch).
-static int transpose_run(spws_ctx* ctx, int32_t m, int32_t n, int64_t nnz, const int32_t* pos, const int32_t* crd,
+template <typename PosT>
+static int transpose_run(spws_ctx* ctx, int32_t m, int32_t n, int64_t nnz, const PosT* pos, const int32_t* crd,
                          const double* vals, int32_t* out_pos, int32_t* out_crd, double* out_vals) {
   int rc;
   const size_t nn = (size_t)(nnz > 0 ? nnz : 1);
@@ -79,7 +82,7 @@ static int transpose_run(spws_ctx* ctx, int32_t m, int32_t n, int64_t nnz, const
     SPWS_LAUNCH(ctx, "tr_zero", k_tr_zero, dim3((unsigned)std::min<int64_t>(div_up(n, 256), 4096)), dim3(256), 0, n,
                 counts);
   if (m > 0 && nnz > 0) {
-    SPWS_LAUNCH(ctx, "tr_keys", k_tr_keys, dim3((unsigned)std::min<int64_t>(div_up(m, 4), 65536)), dim3(256), 0, m,
+    SPWS_LAUNCH(ctx, "tr_keys", k_tr_keys<PosT>, dim3((unsigned)std::min<int64_t>(div_up(m, 4), 65536)), dim3(256), 0, m,
                 pos, crd, vals, keys, kv);
     bool in_alt = false;
     if ((rc = radix_sort_pairs<uint64_t>(ctx, keys, kv, keys + nn, kv + nn, nnz, 32, 32 + column_bits(n), &in_alt)))
@@ -90,7 +93,7 @@ static int transpose_run(spws_ctx* ctx, int32_t m, int32_t n, int64_t nnz, const
   return exclusive_scan<int32_t, int32_t>(ctx, counts, out_pos, n);
 }
 
-static int check_args(int32_t m, int32_t n, int64_t nnz, const int32_t* pos, const int32_t* crd, const double* vals,
+static int check_args(int32_t m, int32_t n, int64_t nnz, const void* pos, const int32_t* crd, const double* vals,
                       const int32_t* out_pos, const int32_t* out_crd, const double* out_vals) {
   SPWS_REQUIRE(m >= 0 && n >= 0 && nnz >= 0, SPWS_ERR_INVALID, "negative dimension or nnz");
   SPWS_REQUIRE(nnz <= INT32_MAX, SPWS_ERR_UNSUPPORTED, "nnz = %lld exceeds the int32 index range", (long long)nnz);
@@ -163,6 +166,15 @@ using namespace spws;
 extern "C" int spws_csr_transpose_dev(spws_ctx* ctx, int32_t m, int32_t n, int64_t nnz, const int32_t* pos,
                                       const int32_t* crd, const double* vals, int32_t* out_pos, int32_t* out_crd,
                                       double* out_vals) {
+  int rc = ctx_activate(ctx);
+  if (rc) return rc;
+  if ((rc = check_args(m, n, nnz, pos, crd, vals, out_pos, out_crd, out_vals))) return rc;
+  return transpose_run(ctx, m, n, nnz, pos, crd, vals, out_pos, out_crd, out_vals);
+}
+
+extern "C" int spws_csr64_transpose_dev(spws_ctx* ctx, int32_t m, int32_t n, int64_t nnz, const int64_t* pos,
+                                        const int32_t* crd, const double* vals, int32_t* out_pos, int32_t* out_crd,
+                                        double* out_vals) {
   int rc = ctx_activate(ctx);
   if (rc) return rc;
   if ((rc = check_args(m, n, nnz, pos, crd, vals, out_pos, out_crd, out_vals))) return rc;

@@ -281,6 +281,19 @@ def csr_transpose(ctx: Context, A: DeviceCSR) -> DeviceCSR:
     return DeviceCSR(A.ncols, A.nrows, A.nnz, pos, crd, vals)
 
 
+def csr64_transpose(ctx: Context, A: DeviceCSR64) -> DeviceCSR:
+    """The transpose of an outer-product result (int64 row pointers) as a
+    new device CSR with int32 row pointers: the second half of the non-fused
+    transposed product, its first half's output left on the device."""
+    pos = DeviceArray(ctx, A.ncols + 1, np.int32)
+    crd = DeviceArray(ctx, A.nnz, np.int32)
+    vals = DeviceArray(ctx, A.nnz, np.float64)
+    check(lib().spws_csr64_transpose_dev(ctx.h, int(A.nrows), int(A.ncols), int(A.nnz), A.pos.ptr, A.crd.ptr,
+                                         A.vals.ptr, pos.ptr, crd.ptr, vals.ptr))
+    ctx.sync()
+    return DeviceCSR(A.ncols, A.nrows, A.nnz, pos, crd, vals)
+
+
 @dataclass
 class DeviceCSF:
     """A 3-level CSF on the device (see formats.CSF3)."""

@@ -381,3 +381,31 @@ def test_device_transpose(ctx, m, n, lo, hi, seed):
     np.testing.assert_array_equal(T.pos, O.pos)
     np.testing.assert_array_equal(T.crd, O.crd)
     np.testing.assert_array_equal(T.vals, O.vals)
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("m,k,n,seed", [(300, 250, 400, 81), (2000, 1500, 1000, 82), (50, 3000, 70_000, 83)])
+def test_device_fused_vs_nonfused_transposed_output(ctx, method, m, k, n, seed):
+    """The transposed-output product with every array on the device, both
+    ways: fused (the outer product B^T A^T, which leaves (A B)^T in row
+    order) and non-fused (the outer product A B, then the transpose of its
+    int64-pointer result).  Both sum every entry over j in increasing order,
+    so they agree bit for bit, and with the oracle's Gustavson B^T A^T."""
+    print(f"case: device fused vs non-fused transposed output method={method} {m}x{k}x{n}", flush=True)
+    from sparsews_amd.device import csr64_transpose
+    A = powerlaw_rows(m, k, 1.8, 120, seed, 0)   # m x k (stored here as CSC: A^T's CSR arrays)
+    B = powerlaw_rows(k, n, 1.8, 150, seed, 1)   # k x n
+    set_outer_method(ctx, method)
+    try:
+        dAc = DeviceDCSR.from_csr(ctx, A.transpose())   # A's columns j = stored outer level
+        dB = DeviceDCSR.from_csr(ctx, B)
+        F = spgemm_outer(ctx, dB, dAc, n, k, m).download()        # (A B)^T = B^T A^T
+        N = csr64_transpose(ctx, spgemm_outer(ctx, dAc, dB, m, k, n)).download()
+    finally:
+        set_outer_method(ctx, 0)
+    G = oracle.spgemm_gustavson(B.transpose(), A.transpose())
+    for X in (F, N):
+        assert (X.nrows, X.ncols) == (n, m)
+        np.testing.assert_array_equal(np.asarray(X.pos, np.int64), np.asarray(G.pos, np.int64))
+        np.testing.assert_array_equal(X.crd, G.crd)
+        np.testing.assert_array_equal(X.vals, G.vals)
