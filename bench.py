@@ -169,20 +169,28 @@ def run_rowwise(args):
     def step():
         out["C"] = spgemm_rowwise(ctx, dA, dB)
 
-    # dominant kernel: the numeric kernel of the short-row bin
+    # dominant kernel: the longest of the row kernels (the short-row path's
+    # two kernels, or the binned path's per-row kernels)
     sec, dom, dom_times, bd = timed_steps(
         ctx, step, args.steps, args.warmup,
-        lambda bd: max((k for k in bd if k.startswith("row_")), key=lambda k: bd[k][0]))
+        lambda bd: max((k for k in bd if k.startswith(("row_", "short_"))), key=lambda k: bd[k][0]))
     calibration_copy(ctx, args.calibrate_copy)
     C = out["C"]
     P = ctx.stats()["products"]
     nnz = C.nnz
     m = A.nrows
-    # algorithmic bytes per launch of the dominant kernel: A rows (pos 8 B/row
-    # + 12 B/entry), B_pos gathers (8 B per A entry), B entries gathered (12 B
-    # per product), slot writes (12 B per output entry), per-row metadata (row
-    # id 4 + offset 8 + nnz 4 B).
-    dom_bytes = m * 8 + A.nnz * 12 + A.nnz * 8 + P * 12 + nnz * 12 + m * 16
+    # bytes each row kernel reads and writes itself, per launch:
+    #   short_rows     A row pointers (4 B/row) and entries (12 B), B_pos
+    #                  gathers (8 B per A entry), B entries (12 B per
+    #                  product), packed entries (12 B per output entry) and
+    #                  row counts (4 B/row) written
+    #   short_compact  row counts (4 B/row) and packed entries (12 B) read, C
+    #                  (4 B/row + 12 B per entry) written
+    #   row_*          as short_rows plus per-row metadata (row id 4 + offset
+    #                  8 + nnz 4 B)
+    io = {"short_rows": m * 8 + A.nnz * 20 + P * 12 + nnz * 12,
+          "short_compact": m * 8 + nnz * 24}
+    dom_bytes = io.get(dom, m * 8 + A.nnz * 12 + A.nnz * 8 + P * 12 + nnz * 12 + m * 16)
     # compulsory bytes: A and B read once, C written once (int32 pointers)
     compulsory = (4 * (m + 1) + 12 * A.nnz) + (4 * (B.nrows + 1) + 12 * B.nnz) + (4 * (m + 1) + 12 * nnz)
     res = {
@@ -263,7 +271,7 @@ def run_outer(args):
     def pick(bd):
         names = {1: ("radix_scatter", "radix_hist", "outer_expand", "outer_reduce"),
                  2: ("bk_expand", "bk_count", "bk_reduce"),
-                 3: ("fz_count", "fz_reduce")}[ctx.stats()["outer_method"]]
+                 3: ("fz_count", "fz_reduce"), 4: ("fz_count", "fz_reduce")}[ctx.stats()["outer_method"]]
         return max(names, key=lambda k: bd.get(k, (0, 1))[0])
 
     sec, dom, dom_times, bd = timed_steps(ctx, step, args.steps, args.warmup, pick)
@@ -285,7 +293,7 @@ def run_outer(args):
     # fused method
     #   fz_count       4 B of B columns gathered per product
     #   fz_reduce      12 B of B gathered per product + 12 B written per distinct entry
-    if st["outer_method"] == 3:
+    if st["outer_method"] in (3, 4):
         per_step = {"fz_count": 4 * P, "fz_reduce": 12 * P + 12 * nnz}
     elif st["outer_method"] == 2:
         per_step = {"bk_expand": 22 * P, "bk_count": 2 * P, "bk_reduce": 10 * P + 12 * nnz}
@@ -309,8 +317,8 @@ def run_outer(args):
         "dtype": "f64",
         "config": {"workload": "outer_product_spgemm_200k_zipf1.8", "n": N, "zipf_s": 1.8, "row_cap": 20000,
                    "nnz_B": int(B.nnz), "products": int(P), "nnz_C": int(nnz), "chunks": st["chunks"],
-                   "sort_method": {1: "lsd_radix", 2: "row_window_buckets",
-                                   3: "fused_row_windows"}[st["outer_method"]],
+                   "sort_method": {1: "lsd_radix", 2: "row_window_buckets", 3: "fused_row_windows",
+                                   4: "fused_row_windows_atomic"}[st["outer_method"]],
                    "window_bits": st["window_bits"], "sort_key_bits": st["sort_bits"],
                    "radix_passes": int(passes), "seed": 1},
         "roofline": roof,
@@ -446,12 +454,13 @@ def run_ttm(args):
     The CSF (levels i, k, j: contracted mode in the middle) is resident; the
     output is the sorted (i, j) fiber list with dense length-16 vectors."""
     from sparsews_amd import formats, gen
-    from sparsews_amd.device import Context, DeviceArray, DeviceCSF, ttm
+    from sparsews_amd.device import Context, DeviceArray, DeviceCSF, set_ttm_method, ttm
     X, U = gen.ttm_tensor()
     I, J, K = X.dims
     R = U.shape[1]
     csf = formats.csf3_from_coo(X, (0, 2, 1))
     ctx = Context(0)
+    set_ttm_method(ctx, args.ttm_method)
     dX = DeviceCSF.upload(ctx, csf)
     dU = DeviceArray.from_host(ctx, np.ascontiguousarray(U.ravel()))
     out = {}
@@ -459,24 +468,39 @@ def run_ttm(args):
     def step():
         out["r"] = ttm(ctx, dX, I, J, K, R, dU)
 
-    names = ("ttm_accum", "radix_scatter", "ttm_keys")
     sec, dom, dom_times, bd = timed_steps(ctx, step, args.steps, args.warmup,
-                                          lambda bd: max(names, key=lambda k: bd.get(k, (0, 1))[0]))
+                                          lambda bd: max(bd, key=lambda k: bd[k][0]))
     calibration_copy(ctx, args.calibrate_copy)
     nnz = X.nnz
     F = out["r"][0].count
-    # the nonzeros are grouped by one stable radix sort on the (i, j) bits of
-    # their packed (i, j, k) keys: 8-bit digits
+    roots_path = "tw_heavy" in bd or "tw_light" in bd
+    # sorting path: the nonzeros are grouped by one stable radix sort on the
+    # (i, j) bits of their packed (i, j, k) keys: 8-bit digits
     sort_bits = max(1, (I - 1).bit_length()) + max(1, (J - 1).bit_length())
     passes = -(-sort_bits // 8)
+    # roots path: nonzeros of heavy roots (more than 256) and light roots
+    rl = np.diff(csf.l3_pos[csf.l2_pos.astype(np.int64)])
+    heavy_nnz = int(rl[rl > 256].sum())
+    light_nnz = nnz - heavy_nnz
     per_step = {
-        # per nonzero: key + value read; per fiber: R values written; U read once
+        # sorting path
+        #   ttm_accum      per nonzero key + value read; per fiber R values written; U once
+        #   radix_scatter  per nonzero and pass key + value read and written
+        #   ttm_keys       per nonzero leaf fiber, fiber root, three coordinates and
+        #                  value read; key + value written
         "ttm_accum": nnz * (8 + 8) + 8 * R * F + 8 * R * K,
-        # per nonzero and pass: key + value read and written
         "radix_scatter": nnz * 32 * passes,
-        # per nonzero: leaf fiber, fiber root, root / middle / leaf coordinates
-        # and value read; key + value written
         "ttm_keys": nnz * (4 + 4 + 4 + 4 + 4 + 8 + 16),
+        # roots path
+        #   tw_scatter     per heavy nonzero column, value and k read, key + value written
+        #   tw_heavy       per heavy nonzero key + value read; its fibers' R values and
+        #                  (i, j) written
+        #   tw_light       per light nonzero column, value and k read; its fibers written
+        #   tw_hist        per heavy nonzero column read
+        "tw_scatter": heavy_nnz * (4 + 8 + 4 + 4 + 8),
+        "tw_heavy": heavy_nnz * (4 + 8),
+        "tw_light": light_nnz * (4 + 8 + 4),
+        "tw_hist": heavy_nnz * 4,
     }
     # compulsory bytes: X as COO once, U once, every output fiber's (i, j)
     # and R values written once
@@ -485,8 +509,11 @@ def run_ttm(args):
         "metric": "TTM GFLOP/s", "value": round(2 * R * nnz / sec / 1e9, 3), "unit": "GFLOP/s",
         "ms_per_step": round(sec * 1e3, 4), "dtype": "f64",
         "config": {"workload": "ttm_mode3_100k_80k_2k_10M_R16", "dims": [I, J, K], "nnz": nnz, "rank": R,
-                   "fibers_out": int(F), "sort_bits": sort_bits, "radix_passes": passes, "seed": 0},
-        "roofline": roofline(dom, dom_times, args.steps, compulsory, per_step[dom], "ttm"),
+                   "fibers_out": int(F), "seed": 0,
+                   "method": "roots_and_windows" if roots_path else "radix_sort",
+                   **({"heavy_root_nnz": heavy_nnz, "light_root_nnz": light_nnz} if roots_path else
+                      {"sort_bits": sort_bits, "radix_passes": passes})},
+        "roofline": roofline(dom, dom_times, args.steps, compulsory, per_step.get(dom, compulsory), "ttm"),
         "compulsory_bytes": int(compulsory),
         "compulsory_frac_of_hbm_roofline": round(compulsory / sec / 1e9 / HBM_PEAK_GBS, 5),
         "kernel_ms_per_step": {k: round(v[0], 4) for k, v in sorted(bd.items())},
@@ -579,9 +606,19 @@ def run_serve(args):
         if len(reqs) % 1024 == 0:
             log(f"serve: {len(reqs)} requests generated and uploaded ({time.perf_counter() - t0:.0f} s)")
     N = len(reqs)
-    n_engines, cap = args.serve_engines, 1 << args.serve_cap_log2
-    large = (1 << args.serve_large_log2) if args.serve_large_engines > 0 else 0
-    lanes = dict(max_batch_products=cap, large_products=large, large_engines=args.serve_large_engines)
+    # the autotuner's table (sparsews_amd/tuning.json) when present: row bins
+    # on the context (the engines copy them) and the serving shape; options
+    # given on the command line win
+    from sparsews_amd import autotune
+    table = None if args.no_tuning else autotune.load()
+    shape = autotune.apply(ctx, table)
+    n_engines = args.serve_engines if args.serve_engines is not None else shape["engines"]
+    cap = (1 << args.serve_cap_log2) if args.serve_cap_log2 is not None else shape["max_batch_products"]
+    large_engines = (args.serve_large_engines if args.serve_large_engines is not None
+                     else shape["large_engines"])
+    large = (((1 << args.serve_large_log2) if args.serve_large_log2 is not None else shape["large_products"])
+             if large_engines > 0 else 0)
+    lanes = dict(max_batch_products=cap, large_products=large, large_engines=large_engines)
     engines = [Engine(ctx) for _ in range(n_engines)]
     zero = np.zeros(N)
 
@@ -616,6 +653,7 @@ def run_serve(args):
     bound["speedup"] = round(bound["batched"]["req_per_s"] / max(bound["plain"]["req_per_s"], 1e-9), 3)
     for e in engines:
         e.close()
+    ctx_bins = {k: v for k, v in ctx.rowwise_tuning().items() if k in ("wave_ub", "wave2_ub", "block_ub", "block2_ub")}
     sizes = np.array([r.products for r in reqs])
     del reqs[:], runs
     ctx.close()
@@ -635,7 +673,9 @@ def run_serve(args):
                    "products_total": int(products), "nnz_total": int(nnz_b.sum()),
                    "products_median": int(np.median(sizes)), "products_q99": int(np.percentile(sizes, 99)),
                    "engines": n_engines, "max_batch_products": cap, "large_lane_products": large,
-                   "large_lane_engines": args.serve_large_engines, "seed": 0},
+                   "large_lane_engines": large_engines, "seed": 0,
+                   "tuning": "sparsews_amd/tuning.json" if table is not None else "defaults",
+                   "row_bins": ctx_bins},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "batch pipeline (all graphs, wall time of the saturated batched run)",
@@ -965,19 +1005,27 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=None,
                     help="CPU baseline seconds per workload (default 10; 5 in the combined run)")
     ap.add_argument("--requests", type=int, default=4096, help="serve workload: number of requests")
-    ap.add_argument("--serve-engines", type=int, default=4, help="serve workload: batch engines")
-    ap.add_argument("--serve-cap-log2", type=int, default=26,
-                    help="serve workload: log2 of the products (output upper bound) per batch")
-    ap.add_argument("--serve-large-log2", type=int, default=24,
-                    help="serve workload: requests of at least 2^this many products form the large lane")
-    ap.add_argument("--serve-large-engines", type=int, default=1,
-                    help="serve workload: engines reserved for the large lane (0 = one lane)")
-    ap.add_argument("--outer-method", type=int, default=0, choices=[0, 1, 2, 3],
+    ap.add_argument("--serve-engines", type=int, default=None,
+                    help="serve workload: batch engines (default: the tuning table's, else 4)")
+    ap.add_argument("--serve-cap-log2", type=int, default=None,
+                    help="serve workload: log2 of the products (output upper bound) per batch "
+                         "(default: the tuning table's, else 26)")
+    ap.add_argument("--serve-large-log2", type=int, default=None,
+                    help="serve workload: requests of at least 2^this many products form the large lane "
+                         "(default: the tuning table's, else 24)")
+    ap.add_argument("--serve-large-engines", type=int, default=None,
+                    help="serve workload: engines reserved for the large lane, 0 = one lane "
+                         "(default: the tuning table's, else 1)")
+    ap.add_argument("--no-tuning", action="store_true",
+                    help="serve workload: ignore sparsews_amd/tuning.json (the engine defaults)")
+    ap.add_argument("--outer-method", type=int, default=0, choices=[0, 1, 2, 3, 4],
                     help="outer workload: 0 automatic, 1 LSD radix sort, 2 row/window buckets, "
-                         "3 fused row windows")
+                         "3 fused row windows, 4 fused row windows with atomic adds")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="after the timed steps, one plain device copy of this many bytes (the reference "
                          "point for rocprofv3 FETCH_SIZE / WRITE_SIZE runs)")
+    ap.add_argument("--ttm-method", type=int, default=0, choices=[0, 1, 2],
+                    help="ttm workload: 0 automatic, 1 one radix sort of the tensor, 2 roots and column windows")
     ap.add_argument("--outer-window", type=int, default=0,
                     help="outer workload, bucket method: log2 of the column window (0 = automatic)")
     args = ap.parse_args(argv)

@@ -156,9 +156,13 @@ int spws_ctx_set_outer_chunking(spws_ctx* ctx, int64_t max_products, int32_t max
  *      walks its column windows, gathering each window's products straight
  *      from B (symbolic pass for the exact row pointers, then the numeric
  *      pass writing every entry in place);
- *   0  automatic (2 unless the windows would be too sparse or too many).
- * window_log2 sets the column window of methods 2 and 3 (6..11; 0 = chosen
- * from the average products per row).  All methods sum each entry in
+ *   4  as 3, the products added into the LDS window by atomic adds (no
+ *      ordering of equal columns): sums reassociated within the 1e-12 bar,
+ *      not reproducible bit for bit;
+ *   0  automatic (3 unless the windows would be too sparse or too many,
+ *      then 1).
+ * window_log2 sets the column window of methods 2-4 (6..11; 0 = chosen
+ * from the average products per row).  Methods 1-3 sum each entry in
  * increasing j and give identical results. */
 int spws_ctx_set_outer_method(spws_ctx* ctx, int32_t method, int32_t window_log2);
 
@@ -214,6 +218,17 @@ int spws_mttkrp(spws_ctx* ctx, int32_t I, int32_t J, int32_t K, int32_t R,
                 int32_t nroots, const int32_t* l1_crd, const int32_t* l2_pos, const int32_t* l2_crd,
                 int32_t nfibers, const int32_t* l3_pos, const int32_t* l3_crd, const double* vals, int64_t nnz,
                 const double* Bf, const double* Cf, int32_t* nrows, int32_t** rows, double** M);
+
+/* How spws_ttm groups the nonzeros of each output fiber (no counterpart in
+ * the reference, whose hash_il workspace does it on the host): 0 automatic
+ * (roots and column windows when the shapes allow it and the heavy roots'
+ * column windows hold at least 32 nonzeros on average, else 1), 1 one stable
+ * radix sort of the whole tensor on the fiber key, 2 roots and column
+ * windows (SPWS_ERR_UNSUPPORTED outside its shapes: R <= 64, at most 4,096
+ * windows of 1024 / R' columns with R' = R rounded up to 16, 32 or 64).
+ * Both sum every fiber over k in increasing order and give identical
+ * results; a U row mask always takes method 1. */
+int spws_ctx_set_ttm_method(spws_ctx* ctx, int32_t method);
 
 /* TTM, Y(i,j,r) = sum_k X(i,k,j) U(k,r), for a CSF with levels (i, k, j) --
  * the contracted mode is the middle level, as in the reference's hash_il

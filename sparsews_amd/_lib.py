@@ -111,6 +111,7 @@ SIGNATURES = {
     "spws_csr_transpose_dev": (C.c_int, [VP, I32, I32, I64, VP, VP, VP, VP, VP, VP]),
     "spws_csr_transpose_dev_noalloc": (C.c_int, [VP, I32, I32, I64, VP, VP, VP, VP, VP, VP]),
     "spws_csr64_transpose_dev": (C.c_int, [VP, I32, I32, I64, VP, VP, VP, VP, VP, VP]),
+    "spws_ctx_set_ttm_method": (C.c_int, [VP, I32]),
     "spws_mttkrp": (C.c_int, [VP, I32, I32, I32, I32, I32, VP, VP, VP, I32, VP, VP, VP, I64, VP, VP,
                               C.POINTER(I32), PVP, PVP]),
     "spws_ttm": (C.c_int, [VP, I32, I32, I32, I32, I32, VP, VP, VP, I32, VP, VP, VP, I64, VP,

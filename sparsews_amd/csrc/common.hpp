@@ -90,12 +90,15 @@ struct spws_ctx {
   spws::DevBuf row_ub, row_off, row_nnz, bin_rows, bin_meta;
   bool rw_meta_clean = false;  // bin_meta is zero (left so by the last completed product)
   bool rw_fixed_last = false;  // the last row-wise product took the fixed-slot path
+  // short-row product: result counters and per-wavefront totals
+  spws::DevBuf short_meta;
   spws::DevBuf slot_crd, slot_val, wide_tab, dense_val, dense_flag;
   spws::DevBuf out_pos, out_crd, out_val, out_aux;
   spws::DevBuf keys[2], vals[2], chunk_a, chunk_b;
   spws::DevBuf tr_keys, tr_vals, tr_counts, tr_out[3];  // CSR transpose: sort pairs, column counts, outputs
   spws::DevBuf stage[12];  // device copies of host inputs (TACO entry points)
   spws::DevBuf radix_hist;
+  spws::DevBuf tw[10];     // TTM by roots and windows: counts, prefixes, bucketed nonzeros
   spws::DevBuf obuf[8];    // outer-product per-column metadata
   spws::DevBuf bk[11];     // outer-product bucket path: pairs, window bounds, bucket tables (two sets)
   spws::PinnedBuf host_meta;
@@ -108,6 +111,8 @@ struct spws_ctx {
   // bucket window width log2 (0 = chosen from the average row product count)
   int32_t outer_method = 0;
   int32_t outer_window_log2 = 0;
+  // TTM method (0 auto, 1 sorting, 2 roots and column windows)
+  int32_t ttm_method = 0;
   // second stream (default priority) and its events, created on first use:
   // the outer product's per-chunk reduce overlaps the next chunk's expand
   hipStream_t aux_stream = nullptr;
@@ -169,10 +174,48 @@ __device__ __forceinline__ T wave_incl_scan(T v) {
 }
 
 template <class T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d, kWave));
+  return v;
+}
+
+template <class T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
   for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
   return v;
+}
+
+// Wavefront bitonic sort, ascending ("flip" network: stage k first compares
+// e with e ^ (k - 1), then e with e ^ j for j = k/4 .. 1, so padding with
+// INT_MAX sorts any n).
+// R keys per lane, element e = r * 64 + lane.
+template <int R>
+__device__ __forceinline__ void wave_bitonic_step(int32_t (&v)[R], int d) {
+  const int lane = lane_id();
+  const int dl = d & (kWave - 1);
+  const int dr = d >> 6;
+  int32_t pv[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int32_t src = v[r ^ dr];
+    pv[r] = dl ? __shfl_xor(src, dl, kWave) : src;
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int e = r * kWave + lane;
+    const int p = e ^ d;
+    v[r] = (e < p) ? min(v[r], pv[r]) : max(v[r], pv[r]);
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void wave_bitonic(int32_t (&v)[R], int n_pow2) {
+  for (int k = 2; k <= n_pow2; k <<= 1) {
+    wave_bitonic_step<R>(v, k - 1);
+    for (int j = k >> 2; j >= 1; j >>= 1) wave_bitonic_step<R>(v, j);
+  }
 }
 
 // Exclusive scan over a block of BLOCK threads; returns this thread's prefix

@@ -408,34 +408,6 @@ __device__ __forceinline__ double lds_lookup(const int32_t* keys, const double* 
 // compares e with e ^ (k - 1), then e with e ^ j for j = k/4 .. 1.  Padding
 // with INT_MAX (or skipping partners beyond n) therefore sorts any n.
 
-// R keys per lane, element e = r * 64 + lane.
-template <int R>
-__device__ __forceinline__ void wave_bitonic_step(int32_t (&v)[R], int d) {
-  const int lane = lane_id();
-  const int dl = d & (kWave - 1);
-  const int dr = d >> 6;
-  int32_t pv[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int32_t src = v[r ^ dr];
-    pv[r] = dl ? __shfl_xor(src, dl, kWave) : src;
-  }
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int e = r * kWave + lane;
-    const int p = e ^ d;
-    v[r] = (e < p) ? min(v[r], pv[r]) : max(v[r], pv[r]);
-  }
-}
-
-template <int R>
-__device__ __forceinline__ void wave_bitonic(int32_t (&v)[R], int n_pow2) {
-  for (int k = 2; k <= n_pow2; k <<= 1) {
-    wave_bitonic_step<R>(v, k - 1);
-    for (int j = k >> 2; j >= 1; j >>= 1) wave_bitonic_step<R>(v, j);
-  }
-}
-
 // LDS bitonic of P (power of two) keys, all threads of the block participate.
 template <int BLOCK>
 __device__ __forceinline__ void block_bitonic_lds(int32_t* list, int P) {
@@ -585,6 +557,283 @@ constexpr int kCfRows = 64;  // a multiple of 4: the earlier rows' counts are re
 constexpr int kCfBlock = 256;
 constexpr int kCfPart = 8;  // partial sums (16-byte loads in flight per thread) of the row-count prefix
 constexpr int32_t kCfMaxRows = 1 << 16;
+
+// ------------------------------------------------------------------ short-row product in two launches
+//
+// When every row has at most 64 products (and at most 64 A entries):
+//   k_short_rows     wavefront w owns the q consecutive rows from w q
+//                    (q <= kShortRows).  It gathers the products of all its
+//                    rows together -- row pointers, A entries, B row
+//                    pointers, B entries: four dependent loads in all, not
+//                    four per row -- and stages them in LDS (column, A value
+//                    * B value); then per row: LDS open-addressing table,
+//                    compaction, register bitonic sort of the columns, the
+//                    sorted entries written back over the row's own staged
+//                    products.  The wavefront's rows go out packed into its
+//                    slot region (q * 64 entries from w q 64), with the row
+//                    counts and the wavefront's totals (entries, products).
+//   k_short_compact  a wavefront per wavefront of the first kernel: the
+//                    totals of all earlier wavefronts (each workgroup sums
+//                    them itself: at most kShortMaxWaves values), their
+//                    prefix, then C_pos and a coalesced copy of the
+//                    wavefront's packed entries into C, every load issued
+//                    before the first store.
+// No upper-bound pass, no bins, no host round trip in between.  A single
+// launch with a look-back across wavefronts was measured slower than both
+// kernels together: status words polled across the XCDs cost a coherent
+// round trip each, and a same-address ticket counter serialises.
+// A row that does not fit sets flags bit 1; the caller then runs the
+// general pipeline (which recomputes C).
+//   out[0] products, out[1] flags (set only on failure; cleared by the
+//   caller's memset), out[2] nnz(C).
+constexpr int kShortBlock = 256;
+constexpr int kShortWaves = kShortBlock / kWave;
+constexpr int kShortRows = 8;           // rows per wavefront at most
+constexpr int kShortAEntries = 128;     // A entries per wavefront at most
+constexpr int kShortProducts = 512;     // staged products per wavefront at most
+constexpr int kShortMaxWaves = 1 << 14; // wavefronts of the first kernel at most
+constexpr int kShortCompactWaves = kShortWaves;  // first-kernel wavefronts per compaction workgroup (one per wavefront)
+
+[[maybe_unused]] __global__ __launch_bounds__(kShortBlock) void k_short_rows(
+    int32_t m, int32_t q, int32_t nwaves, const int32_t* __restrict__ Ap, const int32_t* __restrict__ Ai,
+    const double* __restrict__ Ax, const int32_t* __restrict__ Bp, const int32_t* __restrict__ Bi,
+    const double* __restrict__ Bx, int32_t* __restrict__ slot_crd, double* __restrict__ slot_val,
+    int32_t* __restrict__ row_nnz, int32_t* __restrict__ wave_tot, int32_t* __restrict__ wave_prod,
+    unsigned long long* __restrict__ out) {
+  constexpr int TS = 128;
+  constexpr uint32_t kMask = TS - 1;
+  __shared__ int32_t s_keys[kShortWaves][TS];
+  __shared__ double s_vals[kShortWaves][TS];
+  __shared__ int32_t s_list[kShortWaves][kWave];
+  __shared__ int32_t s_col[kShortWaves][kShortProducts];
+  __shared__ double s_pv[kShortWaves][kShortProducts];
+  const int wv = threadIdx.x / kWave;
+  int32_t* keys = s_keys[wv];
+  double* vals = s_vals[wv];
+  int32_t* list = s_list[wv];
+  int32_t* scol = s_col[wv];
+  double* spv = s_pv[wv];
+  const uint32_t shift = 32 - log2_pow2(TS);
+  const int lane = lane_id();
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int32_t w = blockIdx.x * kShortWaves + wv;
+  if (w >= nwaves) return;  // the last workgroup's spare wavefronts own no rows and no totals
+  const int32_t r0 = (int32_t)min((int64_t)w * q, (int64_t)m);
+  const int32_t nr = min(q, m - r0);
+
+  // row pointers, A entries and B row pointers of all the rows
+  const int32_t rp = (lane <= nr && nr > 0) ? Ap[r0 + lane] : 0;
+  const int32_t a0 = __shfl(rp, 0, kWave), a1 = __shfl(rp, nr, kWave);
+  const int32_t nA = a1 - a0;
+  bool over = nA > kShortAEntries;
+  int32_t bs0 = 0, bl0 = 0, bs1 = 0, bl1 = 0;
+  double av0 = 0.0, av1 = 0.0;
+  if (!over) {
+    if (lane < nA) {
+      const int32_t j = Ai[a0 + lane];
+      av0 = Ax[a0 + lane];
+      bs0 = Bp[j];
+      bl0 = Bp[j + 1] - bs0;
+    }
+    if (kWave + lane < nA) {
+      const int32_t j = Ai[a0 + kWave + lane];
+      av1 = Ax[a0 + kWave + lane];
+      bs1 = Bp[j];
+      bl1 = Bp[j + 1] - bs1;
+    }
+  }
+  // exclusive product offsets over the wavefront's A entries (two halves)
+  const int32_t inc0 = wave_incl_scan(bl0);
+  const int32_t tot0 = __shfl(inc0, kWave - 1, kWave);
+  const int32_t inc1 = wave_incl_scan(bl1) + tot0;
+  const int32_t P = __shfl(inc1, kWave - 1, kWave);
+  const int32_t ex0 = inc0 - bl0, ex1 = inc1 - bl1;
+  // lane r < nr: its row's A entries [lo, hi) and products [pb, pe)
+  auto E = [&](int32_t x) -> int32_t {  // exclusive offset of A entry x (P at x = nA)
+    const int32_t e0 = __shfl(ex0, x & (kWave - 1), kWave);
+    const int32_t e1 = __shfl(ex1, x & (kWave - 1), kWave);
+    return x >= nA ? P : (x < kWave ? e0 : e1);
+  };
+  const int32_t lo = rp - a0, hi = __shfl(rp, min(lane + 1, kWave - 1), kWave) - a0;
+  const int32_t pb = E(lo), pe = E(hi);
+  if (lane < nr && (hi - lo > kWave || pe - pb > kWave)) over = true;
+  over = __any(over || P > kShortProducts);
+  if (over) {  // the caller discards the result; the totals stay consistent (zero)
+    if (lane == 0) {
+      atomicOr(&out[1], 1ull);
+      wave_tot[w] = 0;
+      wave_prod[w] = 0;
+    }
+    if (lane < nr) row_nnz[r0 + lane] = 0;
+    return;
+  }
+
+  // every product staged in LDS at its A entry's offset: each lane walks
+  // its A entry's B row (the wavefront's longest row sets the trip count,
+  // eight loads in flight per lane)
+  {
+    const int32_t maxlen = wave_max(max(bl0, bl1));
+    for (int32_t t0 = 0; t0 < maxlen; t0 += 8) {
+      int32_t c0[8], c1[8];
+      double x0[8], x1[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int32_t t = t0 + u;
+        c0[u] = t < bl0 ? Bi[bs0 + t] : 0;
+        x0[u] = t < bl0 ? Bx[bs0 + t] : 0.0;
+        c1[u] = t < bl1 ? Bi[bs1 + t] : 0;
+        x1[u] = t < bl1 ? Bx[bs1 + t] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int32_t t = t0 + u;
+        if (t < bl0) {
+          scol[ex0 + t] = c0[u];
+          spv[ex0 + t] = av0 * x0[u];
+        }
+        if (t < bl1) {
+          scol[ex1 + t] = c1[u];
+          spv[ex1 + t] = av1 * x1[u];
+        }
+      }
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  // per row: table, compaction, sort, entries back over the row's products
+  int32_t my_nnz = 0;
+  for (int r = 0; r < nr; ++r) {
+    const int32_t rb = __builtin_amdgcn_readlane(pb, r);
+    const int32_t len = __builtin_amdgcn_readlane(pe, r) - rb;
+    keys[lane] = keys[lane + kWave] = -1;
+    vals[lane] = vals[lane + kWave] = 0.0;
+    int32_t col = -1;
+    double pv = 0.0;
+    if (lane < len) {
+      col = scol[rb + lane];
+      pv = spv[rb + lane];
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (col >= 0) lds_insert<true>(keys, vals, shift, kMask, col, pv);
+    __builtin_amdgcn_wave_barrier();
+    int32_t nnz = 0;
+#pragma unroll
+    for (int s0 = 0; s0 < TS; s0 += kWave) {
+      const int32_t key = keys[s0 + lane];
+      const uint64_t occ = __ballot(key != -1);
+      if (key != -1) list[nnz + __popcll(occ & lt)] = key;
+      nnz += __popcll(occ);
+    }
+    __builtin_amdgcn_wave_barrier();
+    int32_t vv[1] = {lane < nnz ? list[lane] : INT_MAX};
+    int n_pow2 = 2;
+    while (n_pow2 < nnz) n_pow2 <<= 1;
+    wave_bitonic<1>(vv, n_pow2);
+    if (lane < nnz) {
+      scol[rb + lane] = vv[0];
+      spv[rb + lane] = lds_lookup(keys, vals, shift, kMask, vv[0]);
+    }
+    if (lane == r) my_nnz = nnz;
+    __builtin_amdgcn_wave_barrier();
+  }
+  // the rows packed into the wavefront's slot region
+  const int32_t loc_incl = wave_incl_scan(lane < nr ? my_nnz : 0);
+  const int32_t loc = loc_incl - (lane < nr ? my_nnz : 0);
+  const int64_t base = (int64_t)w * q * kWave;
+  for (int r = 0; r < nr; ++r) {
+    const int32_t rb = __builtin_amdgcn_readlane(pb, r);
+    const int32_t cnt = __builtin_amdgcn_readlane(my_nnz, r);
+    const int64_t dst = base + __builtin_amdgcn_readlane(loc, r);
+    if (lane < cnt) {
+      slot_crd[dst + lane] = scol[rb + lane];
+      slot_val[dst + lane] = spv[rb + lane];
+    }
+  }
+  if (lane < nr) row_nnz[r0 + lane] = my_nnz;
+  if (lane == 0) {
+    wave_tot[w] = __builtin_amdgcn_readlane(loc_incl, kWave - 1);
+    wave_prod[w] = P;
+  }
+}
+
+[[maybe_unused]] __global__ __launch_bounds__(kShortBlock) void k_short_compact(
+    int32_t m, int32_t q, int32_t nwaves, const int32_t* __restrict__ row_nnz, const int32_t* __restrict__ wave_tot,
+    const int32_t* __restrict__ wave_prod, const int32_t* __restrict__ slot_crd, const double* __restrict__ slot_val,
+    int32_t* __restrict__ C_pos, int32_t* __restrict__ C_crd, double* __restrict__ C_val,
+    unsigned long long* __restrict__ out) {
+  if (out[1] != 0) return;  // a row did not fit: the caller recomputes C
+  __shared__ int64_t s_part[2][kShortWaves];
+  __shared__ int32_t s_off[kShortCompactWaves + 1];
+  const int lane = lane_id();
+  const int wv = threadIdx.x / kWave;
+  const int32_t w0 = blockIdx.x * kShortCompactWaves;
+  const int32_t w1 = min(w0 + kShortCompactWaves, nwaves);
+  // totals of every earlier wavefront (entries, and products for the last
+  // workgroup's statistics)
+  int64_t sn = 0, sp = 0;
+  for (int32_t t = threadIdx.x; t < w0; t += kShortBlock) {
+    sn += wave_tot[t];
+    sp += wave_prod[t];
+  }
+  sn = wave_sum(sn);
+  sp = wave_sum(sp);
+  if (lane == 0) {
+    s_part[0][wv] = sn;
+    s_part[1][wv] = sp;
+  }
+  __syncthreads();
+  if (wv == 0) {
+    int64_t n = 0, pr = 0;
+#pragma unroll
+    for (int i = 0; i < kShortWaves; ++i) {
+      n += s_part[0][i];
+      pr += s_part[1][i];
+    }
+    // this workgroup's wavefronts (one per lane): exclusive offsets, and the
+    // end at index w1 - w0 (lanes past w1 add nothing)
+    const int32_t t = w0 + lane;
+    const bool mine = lane < kShortCompactWaves && t < w1;
+    const int32_t tot = mine ? wave_tot[t] : 0;
+    const int64_t prod = mine ? wave_prod[t] : 0;
+    const int32_t incl = wave_incl_scan(tot);
+    if (lane <= kShortCompactWaves) s_off[lane] = (int32_t)n + incl - tot;
+    const int64_t psum = wave_sum(prod);
+    if (w1 == nwaves && lane == kWave - 1) {
+      C_pos[m] = (int32_t)n + incl;
+      out[0] = (unsigned long long)(pr + psum);
+      out[2] = (unsigned long long)((int32_t)n + incl);
+    }
+  }
+  __syncthreads();
+  // one wavefront per first-kernel wavefront: row pointers, then the copy
+  // (all of its packed entries loaded before any is stored)
+  const int32_t t = w0 + wv;
+  if (t >= w1) return;
+  const int32_t r0 = (int32_t)min((int64_t)t * q, (int64_t)m);
+  const int32_t nr = min(q, m - r0);
+  const int32_t off = s_off[wv];
+  const int32_t cnt = lane < nr ? row_nnz[r0 + lane] : 0;
+  const int32_t incl = wave_incl_scan(cnt);
+  if (lane < nr) C_pos[r0 + lane] = off + incl - cnt;
+  const int32_t tot = s_off[wv + 1] - off;
+  const int64_t base = (int64_t)t * q * kWave;
+  constexpr int kU = kShortRows;  // at most kShortRows * 64 entries per wavefront
+  int32_t c[kU];
+  double v[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const int32_t e = u * kWave + lane;
+    c[u] = e < tot ? slot_crd[base + e] : 0;
+    v[u] = e < tot ? slot_val[base + e] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const int32_t e = u * kWave + lane;
+    if (e < tot) {
+      C_crd[off + e] = c[u];
+      C_val[off + e] = v[u];
+    }
+  }
+}
 
 template <bool kValues>
 __global__ __launch_bounds__(kCfBlock) void k_compact_fixed(int32_t m, const int32_t* __restrict__ row_nnz,

@@ -312,12 +312,13 @@ int spws_ctx_destroy(spws_ctx* ctx) {
                     &ctx->dense_flag, &ctx->out_pos, &ctx->out_crd,   &ctx->out_val,   &ctx->out_aux,
                     &ctx->keys[0],  &ctx->keys[1],  &ctx->vals[0],    &ctx->vals[1],   &ctx->chunk_a,
                     &ctx->chunk_b,  &ctx->tr_keys,  &ctx->tr_vals,    &ctx->tr_counts, &ctx->tr_out[0],
-                    &ctx->tr_out[1], &ctx->tr_out[2]};
+                    &ctx->tr_out[1], &ctx->tr_out[2], &ctx->short_meta};
   if (ctx->aux_stream) (void)hipStreamSynchronize(ctx->aux_stream);
   for (DevBuf* b : bufs) b->release();
   for (DevBuf& b : ctx->stage) b.release();
   for (DevBuf& b : ctx->obuf) b.release();
   for (DevBuf& b : ctx->bk) b.release();
+  for (DevBuf& b : ctx->tw) b.release();
   if (ctx->aux_ready) (void)hipEventDestroy(ctx->aux_ready);
   for (hipEvent_t e : ctx->aux_done)
     if (e) (void)hipEventDestroy(e);

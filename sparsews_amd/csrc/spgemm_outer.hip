@@ -937,7 +937,11 @@ struct FzBatch {
   bool win_end;   // last batch of the window
 };
 
-template <bool kNumeric>
+// kOrdered (numeric pass): entries summed in position order, bit-identical
+// to the other methods; otherwise every product goes into the window by an
+// LDS atomic add (ds_add_f64), with no conflict resolution -- the sums are
+// then reassociated within the 1e-12 bar and not reproducible bit for bit.
+template <bool kNumeric, bool kOrdered = true>
 __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, const int32_t* __restrict__ rp_off,
                                                    const int32_t* __restrict__ rec_b,
                                                    const int32_t* __restrict__ rec_ja,
@@ -1096,6 +1100,13 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
           continue;
         }
         const double v = av[slot][u] * bv[slot][u];
+        if (!kOrdered) {
+          if (ok) {
+            atomicAdd(&acc[k], v);
+            atomicOr(&seen[wi], bit);
+          }
+          continue;
+        }
         uint32_t old = 0;
         double a = 0.0;
         if (ok) {
@@ -1569,7 +1580,7 @@ constexpr int kFzNumWavesPerCu = 8;   // k_fz_rows<true>: bounded by the LDS win
 // (exact row pointers, and an output allocated to the exact size), then the
 // numeric pass writing every entry in place.
 static int outer_fused(spws_ctx* ctx, const OuterArgs& a, const std::vector<int64_t>& pair_prefix,
-                       const int32_t* row_pairs, int s, int64_t* nnz_out) {
+                       const int32_t* row_pairs, int s, bool ordered, int64_t* nnz_out) {
   hipStream_t st = ctx->stream;
   int rc;
   const int32_t m = a.m;
@@ -1612,7 +1623,7 @@ static int outer_fused(spws_ctx* ctx, const OuterArgs& a, const std::vector<int6
   int32_t* rec_ja = rec_b + ecap;
   double* rec_a = reinterpret_cast<double*>(rec_ja + ecap);
   int64_t* out_pos = ctx->out_pos.as<int64_t>();
-  ctx->stats.outer_method = 3;
+  ctx->stats.outer_method = ordered ? 3 : 4;
   ctx->stats.window_bits = s;
   ctx->stats.chunks = 1;
   const unsigned gcol = (unsigned)div_up(na > 0 ? na : 1, 256);
@@ -1666,9 +1677,14 @@ static int outer_fused(spws_ctx* ctx, const OuterArgs& a, const std::vector<int6
   // numeric pass: every entry written at its final position
   SPWS_HIP(hipMemsetAsync(work, 0, 4, st));
   const unsigned gnum = (unsigned)std::max<int64_t>(1, std::min<int64_t>(m, (int64_t)dev_cus * kFzNumWavesPerCu));
-  SPWS_LAUNCH(ctx, "fz_reduce", k_fz_rows<true>, dim3(gnum), dim3(kWave), bk_reduce_lds_per_wave(s), m, W, s, rp_off,
-              rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, (int32_t*)nullptr, (const int64_t*)out_pos,
-              ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>(), work);
+  if (ordered)
+    SPWS_LAUNCH(ctx, "fz_reduce", (k_fz_rows<true, true>), dim3(gnum), dim3(kWave), bk_reduce_lds_per_wave(s), m, W,
+                s, rp_off, rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, (int32_t*)nullptr, (const int64_t*)out_pos,
+                ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>(), work);
+  else
+    SPWS_LAUNCH(ctx, "fz_reduce", (k_fz_rows<true, false>), dim3(gnum), dim3(kWave), bk_reduce_lds_per_wave(s), m, W,
+                s, rp_off, rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, (int32_t*)nullptr, (const int64_t*)out_pos,
+                ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>(), work);
   SPWS_HIP(hipStreamSynchronize(st));
   *nnz_out = total;
   return SPWS_OK;
@@ -1720,8 +1736,10 @@ static int outer_run(spws_ctx* ctx, const OuterArgs& a, int64_t* nnz_out) {
   const int64_t max_products = ctx->outer_max_products > 0 ? ctx->outer_max_products : ((int64_t)1 << 30);
   const int64_t user_rows = ctx->outer_max_rows > 0 ? ctx->outer_max_rows : INT32_MAX;
 
-  // method: buckets unless the windows would hold too few products each
-  // (very sparse, very wide outputs) -- then the LSD sort over whole chunks
+  // method: the fused row windows unless the windows would hold too few
+  // products each (very sparse, very wide outputs) -- then the LSD sort over
+  // whole chunks.  The fused method needs no product workspace; on the 200k
+  // workload its step is about 7% shorter than the bucket method's.
   int method = ctx->outer_method;
   int wlog = ctx->outer_window_log2;
   if (wlog == 0) {
@@ -1736,10 +1754,10 @@ static int outer_run(spws_ctx* ctx, const OuterArgs& a, int64_t* nnz_out) {
     const double per_bucket = busy_rows > 0 ? (double)prefix[m] / ((double)busy_rows * (double)W) : 0.0;
     int32_t max_pairs = 0;
     for (int32_t i = 0; i < m; ++i) max_pairs = std::max(max_pairs, pairs[i]);
-    method = (per_bucket >= 64.0 && W <= 4096 && (int64_t)max_pairs * W <= ((int64_t)1 << 28)) ? 2 : 1;
+    method = (per_bucket >= 64.0 && W <= 4096 && (int64_t)max_pairs * W <= ((int64_t)1 << 28)) ? 3 : 1;
   }
-  if (method == 3) {
-    rc = outer_fused(ctx, a, pair_prefix, row_pairs, wlog, nnz_out);
+  if (method == 3 || method == 4) {
+    rc = outer_fused(ctx, a, pair_prefix, row_pairs, wlog, method == 3, nnz_out);
   } else if (method == 2) {
     rc = outer_chunks_bucket(ctx, a, prefix, pair_prefix, row_pairs, wlog, max_products,
                              (int32_t)std::min<int64_t>(user_rows, INT32_MAX), nnz_out);
@@ -1776,7 +1794,7 @@ int spws_ctx_set_outer_chunking(spws_ctx* ctx, int64_t max_products, int32_t max
 
 int spws_ctx_set_outer_method(spws_ctx* ctx, int32_t method, int32_t window_log2) {
   SPWS_REQUIRE(ctx != nullptr, SPWS_ERR_INVALID, "null context");
-  SPWS_REQUIRE(method >= 0 && method <= 3, SPWS_ERR_INVALID, "outer method must be 0, 1, 2 or 3");
+  SPWS_REQUIRE(method >= 0 && method <= 4, SPWS_ERR_INVALID, "outer method must be 0, 1, 2, 3 or 4");
   SPWS_REQUIRE(window_log2 == 0 || (window_log2 >= kBkMinLog2 && window_log2 <= kBkMaxLog2), SPWS_ERR_INVALID,
                "window_log2 must be 0 or in [%d, %d]", kBkMinLog2, kBkMaxLog2);
   ctx->outer_method = method;

@@ -66,6 +66,53 @@ static int validate_tuning(const spws_rowwise_tuning& t) {
   return SPWS_OK;
 }
 
+// The short-row product in two launches (k_short_rows, k_short_compact)
+// into the context's C arrays.  *ok is false when it does not apply (more
+// wavefronts than the compaction sums) or a row does not fit: the caller
+// then runs the general pipeline, which recomputes C.
+static int run_short(spws_ctx* ctx, int32_t m, const int32_t* Ap, const int32_t* Ai, const double* Ax,
+                     const int32_t* Bp, const int32_t* Bi, const double* Bx, bool* ok, int64_t* nnz,
+                     int64_t* products) {
+  hipStream_t s = ctx->stream;
+  int rc;
+  *ok = false;
+  if (m <= 0) return SPWS_OK;
+  int cus = 0;
+  SPWS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  // about 16 wavefronts per CU, at most kShortRows rows each
+  const int32_t q = (int32_t)std::min<int64_t>(kShortRows, std::max<int64_t>(1, div_up(m, (int64_t)cus * 16)));
+  const int64_t nwaves = div_up(m, q);
+  if (nwaves > kShortMaxWaves) return SPWS_OK;
+  const size_t slots = (size_t)nwaves * q * kWave;
+  const size_t nw = (size_t)nwaves;
+  if ((rc = ctx->slot_crd.ensure(slots * 4)) || (rc = ctx->slot_val.ensure(slots * 8)) ||
+      (rc = ctx->row_nnz.ensure((size_t)m * 4)) || (rc = ctx->short_meta.ensure(nw * 8 + 64)) ||
+      (rc = ctx->out_pos.ensure(((size_t)m + 1) * 4)) || (rc = ctx->out_crd.ensure(slots * 4)) ||
+      (rc = ctx->out_val.ensure(slots * 8)))
+    return rc;
+  unsigned long long* out = ctx->short_meta.as<unsigned long long>();
+  int32_t* wave_tot = reinterpret_cast<int32_t*>(out + 8);
+  int32_t* wave_prod = wave_tot + nw;
+  SPWS_HIP(hipMemsetAsync(out, 0, 24, s));
+  const unsigned g1 = (unsigned)div_up(nwaves, kShortWaves);
+  SPWS_LAUNCH(ctx, "short_rows", k_short_rows, dim3(g1), dim3(kShortBlock), 0, m, q, (int32_t)nwaves, Ap, Ai, Ax,
+              Bp, Bi, Bx,
+              ctx->slot_crd.as<int32_t>(), ctx->slot_val.as<double>(), ctx->row_nnz.as<int32_t>(), wave_tot,
+              wave_prod, out);
+  const unsigned g2 = (unsigned)div_up(nwaves, kShortCompactWaves);
+  SPWS_LAUNCH(ctx, "short_compact", k_short_compact, dim3(g2), dim3(kShortBlock), 0, m, q, (int32_t)nwaves,
+              (const int32_t*)ctx->row_nnz.as<int32_t>(), (const int32_t*)wave_tot, (const int32_t*)wave_prod,
+              (const int32_t*)ctx->slot_crd.as<int32_t>(), (const double*)ctx->slot_val.as<double>(),
+              ctx->out_pos.as<int32_t>(), ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>(), out);
+  unsigned long long* h = ctx->host_meta.as<unsigned long long>() + 64;
+  SPWS_HIP(hipMemcpyAsync(h, out, 3 * 8, hipMemcpyDeviceToHost, s));
+  SPWS_HIP(hipStreamSynchronize(s));
+  *ok = h[1] == 0;
+  *nnz = (int64_t)h[2];
+  *products = (int64_t)h[0];
+  return SPWS_OK;
+}
+
 template <bool kValues>
 static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const int32_t* Ap, const int32_t* Ai,
                        const double* Ax, const int32_t* Bp, const int32_t* Bi, const double* Bx, int64_t* nnz_out) {
@@ -78,6 +125,22 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
   // the forced paths must still fit the compiled LDS tables
   PathParams pp{t.wave_ub, t.wave2_ub, t.block_ub, t.block2_ub, t.dense_ratio, t.force_path};
   memset(&ctx->stats, 0, sizeof(ctx->stats));
+
+  // the previous product had only short rows: try the short-row path first,
+  // with no upper-bound pass (its kernels check every row themselves)
+  if (kValues && t.force_path < 0 && m > 0 && ctx->rw_fixed_last) {
+    bool ok = false;
+    int64_t nnz = 0, prods = 0;
+    if ((rc = run_short(ctx, m, Ap, Ai, Ax, Bp, Bi, Bx, &ok, &nnz, &prods))) return rc;
+    if (ok) {
+      ctx->stats.products = prods;
+      ctx->stats.nnz = nnz;
+      ctx->stats.rows_per_path[P_WAVE] = m;
+      *nnz_out = nnz;
+      return SPWS_OK;
+    }
+    ctx->rw_fixed_last = false;
+  }
 
   const size_t mm = (size_t)(m > 0 ? m : 1);
   if ((rc = ctx->row_ub.ensure(mm * 8)) || (rc = ctx->row_off.ensure((mm + 1) * 8)) ||
@@ -130,7 +193,8 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
     SPWS_HIP(hipMemcpyAsync(h32, ctx->out_pos.as<int32_t>() + m, 4, hipMemcpyDeviceToHost, s));
     return SPWS_OK;
   };
-  const bool spec = ctx->rw_fixed_last && t.force_path < 0 && m > 0 && m <= kCfMaxRows;
+  // (the numeric product took the one-launch path above when this was set)
+  const bool spec = !kValues && ctx->rw_fixed_last && t.force_path < 0 && m > 0 && m <= kCfMaxRows;
   // at most 64 entries per row whichever rows the wave kernel skips
   if (spec && (rc = launch_fixed((size_t)m * kWave, bmeta))) return rc;
   SPWS_HIP(hipStreamSynchronize(s));
@@ -152,9 +216,28 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
                    (long long)maxub[t.force_path]);
   }
 
-  const bool fixed_ok = t.force_path < 0 && m > 0 && m <= kCfMaxRows && counts[P_WAVE] + counts[P_EMPTY] == m &&
-                        maxub[P_WAVE] <= kWaveUbMax;
-  ctx->rw_fixed_last = fixed_ok;
+  // every row has at most 64 products: the two-launch short-row path
+  // (numeric), or the fixed-slot path (symbolic, or when the short-row path
+  // declines)
+  const bool short_rows = t.force_path < 0 && m > 0 && counts[P_WAVE] + counts[P_EMPTY] == m &&
+                          maxub[P_WAVE] <= kWaveUbMax;
+  const bool fixed_ok = short_rows && m <= kCfMaxRows;
+  ctx->rw_fixed_last = short_rows;
+  if (kValues && short_rows) {
+    bool ok = false;
+    int64_t nnz = 0, prods = 0;
+    // the metadata block is zero again for the next product
+    SPWS_LAUNCH(ctx, "meta_clear", k_meta_clear, dim3(1), dim3(64), 0, bmeta, maxub_d);
+    ctx->rw_meta_clean = true;
+    if ((rc = run_short(ctx, m, Ap, Ai, Ax, Bp, Bi, Bx, &ok, &nnz, &prods))) return rc;
+    if (ok) {
+      *nnz_out = nnz;
+      ctx->stats.nnz = nnz;
+      return SPWS_OK;
+    }
+    ctx->rw_meta_clean = false;
+    SPWS_HIP(hipMemcpyAsync(bmeta, hc, 32 * 4 + 9 * 8, hipMemcpyHostToDevice, s));
+  }
   if (fixed_ok) {
     if (!spec) {
       if ((rc = launch_fixed((size_t)(total_ub > 0 ? total_ub : 1), bmeta))) return rc;

@@ -51,6 +51,7 @@
 #include <climits>
 
 #include "radix.hpp"
+#include "tensor.hpp"
 
 namespace spws {
 
@@ -529,18 +530,6 @@ __global__ __launch_bounds__(256) void k_ttm_combine(int64_t F, int R, const int
 
 // ------------------------------------------------------------------ drivers
 
-struct Csf3 {
-  int32_t nroots;
-  const int32_t* l1_crd;
-  const int32_t* l2_pos;
-  const int32_t* l2_crd;
-  int32_t nfibers;
-  const int32_t* l3_pos;
-  const int32_t* l3_crd;
-  const double* vals;
-  int64_t nnz;
-};
-
 // Direct TTM grouping: the nonzeros carry packed (i, j, k) keys, and one
 // stable radix sort on the (i, j) bits puts each output fiber together, in
 // (i, j) order, with its nonzeros in increasing k (the CSF order (i, k, j)
@@ -778,6 +767,16 @@ static int ttm_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_t K
                "TTM needs non-negative dimensions and 1 <= R <= 256");
   const bool masked = umask != nullptr;
   SPWS_REQUIRE(!masked || R <= 64, SPWS_ERR_UNSUPPORTED, "a U row mask needs R <= 64");
+  // the roots-and-windows path (no sort of the whole tensor) unless the
+  // sorting path is asked for or the shapes are outside what it supports
+  if (!masked && ctx->ttm_method != 1) {
+    bool handled = false;
+    if ((rc = ttm_roots_run(ctx, x, I, J, K, R, U, ctx->ttm_method == 2, nfib_out, &handled))) return rc;
+    if (handled) return SPWS_OK;
+    SPWS_REQUIRE(ctx->ttm_method != 2, SPWS_ERR_UNSUPPORTED,
+                 "the roots-and-windows TTM needs 1 <= R <= 64, at most 4,096 column windows of 1024 / R' "
+                 "columns (R' = R rounded up to 16, 32 or 64), J < 2^23 and K times the window below 2^32");
+  }
   // with a mask, root I is the key of the nonzeros that take no part
   const int bi = bits_of(masked ? (int64_t)I + 1 : (I > 1 ? I : 2)), bj = bits_of(J > 1 ? J : 2);
   SPWS_REQUIRE(bi + bj <= 63, SPWS_ERR_UNSUPPORTED, "tensor dimensions exceed the 64-bit fiber key");
@@ -936,6 +935,13 @@ static int ttm_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_t K
 using namespace spws;
 
 extern "C" {
+
+int spws_ctx_set_ttm_method(spws_ctx* ctx, int32_t method) {
+  SPWS_REQUIRE(ctx != nullptr, SPWS_ERR_INVALID, "null context");
+  SPWS_REQUIRE(method >= 0 && method <= 2, SPWS_ERR_INVALID, "TTM method %d is not 0, 1 or 2", method);
+  ctx->ttm_method = method;
+  return SPWS_OK;
+}
 
 int spws_mttkrp(spws_ctx* ctx, int32_t I, int32_t J, int32_t K, int32_t R, int32_t nroots, const int32_t* l1_crd,
                 const int32_t* l2_pos, const int32_t* l2_crd, int32_t nfibers, const int32_t* l3_pos,

@@ -248,7 +248,7 @@ def set_outer_chunking(ctx: Context, max_products: int = 0, max_rows: int = 0):
 
 def set_outer_method(ctx: Context, method: int = 0, window_log2: int = 0):
     """0 automatic, 1 LSD radix sort, 2 row/window buckets, 3 fused row
-    windows (see spws.h)."""
+    windows, 4 fused row windows with atomic (unordered) adds (see spws.h)."""
     check(lib().spws_ctx_set_outer_method(ctx.h, int(method), int(window_log2)))
 
 
@@ -324,6 +324,12 @@ def mttkrp(ctx: Context, X: DeviceCSF, I: int, J: int, K: int, R: int, Bf: Devic
                             C.byref(nrows), C.byref(pr), C.byref(pm)))
     return (DeviceArray.view(ctx, pr.value, nrows.value, np.int32),
             DeviceArray.view(ctx, pm.value, int(I) * int(R), np.float64))
+
+
+def set_ttm_method(ctx: Context, method: int = 0):
+    """0 automatic, 1 one radix sort of the whole tensor, 2 roots and
+    column windows (see spws.h); identical results."""
+    check(lib().spws_ctx_set_ttm_method(ctx.h, int(method)))
 
 
 def ttm(ctx: Context, X: DeviceCSF, I: int, J: int, K: int, R: int, U: DeviceArray):

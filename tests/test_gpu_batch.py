@@ -12,7 +12,7 @@ import pytest
 import oracle
 from helpers import powerlaw_rows, random_csr, uniform_rows
 from sparsews_amd import SpwsError, gen
-from sparsews_amd.device import DeviceCSR, spgemm_rowwise
+from sparsews_amd.device import Context, DeviceCSR, spgemm_rowwise
 from sparsews_amd.serve import DeviceRequest, Engine, default_caps, latency_summary, serve
 
 pytestmark = pytest.mark.gpu
@@ -315,3 +315,35 @@ def test_config4_stream_served_continuously(ctx):
         assert oracle.max_value_error(C.vals, alone.vals, absum) <= TOL
         for arr in (r.C_pos, r.C_crd, r.C_vals):
             arr.free()
+
+
+def test_tuned_settings_serve_correctly():
+    """The autotuner's table (sparsews_amd/tuning.json, the defaults when it
+    is absent) applied as the serve bench applies it -- row bins on the
+    context, engines, products per batch and the large lane -- on a prefix
+    of the served request mix: every served product against the oracle."""
+    from sparsews_amd import autotune
+    print("case: tuned settings serve correctly", flush=True)
+    with Context(0) as tctx:
+        shape = autotune.apply(tctx, autotune.load())
+        stream = gen.request_stream(count=4096, seed=0)
+        idx = config4_selection(stream, 60, 2)
+        pairs = [gen.request_operands(stream[i]) for i in idx]
+        reqs = [DeviceRequest.upload(tctx, A, B, output=True) for A, B in pairs]
+        gaps = np.random.default_rng(45).exponential(1.0 / 20_000.0, len(reqs))
+        arrivals = np.cumsum(gaps) - gaps[0]
+        engines = [Engine(tctx) for _ in range(shape["engines"])]
+        try:
+            fin, bid, st, nnz = serve(tctx, engines, reqs, arrivals, "batched",
+                                      max_batch_products=shape["max_batch_products"],
+                                      large_products=shape["large_products"],
+                                      large_engines=min(shape["large_engines"], shape["engines"] - 1),
+                                      with_nnz=True)
+        finally:
+            for e in engines:
+                e.close()
+        assert st["fallback_requests"] == 0
+        for t, ((A, B), r) in enumerate(zip(pairs, reqs)):
+            check_result(r.result(int(nnz[t])), A, B)
+            for arr in (r.C_pos, r.C_crd, r.C_vals):
+                arr.free()

@@ -19,16 +19,23 @@ from sparsews_amd.gen import CSR
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
-METHODS = [1, 2, 3]
+METHODS = [1, 2, 3, 4]
+# methods 1-3 sum every entry in increasing j (bit-identical to the oracle's
+# Gustavson); method 4 adds the products into the LDS window atomically, so
+# its sums are reassociated (within the tolerance only)
+UNORDERED = {4}
+_current = {"method": 0}
 
 
 @contextlib.contextmanager
 def outer_method(ctx, method, window_log2=0):
     set_outer_method(ctx, method, window_log2)
+    _current["method"] = method
     try:
         yield
     finally:
         set_outer_method(ctx, 0, 0)
+        _current["method"] = 0
 
 
 def run_outer(ctx, A, B, dense_outer=False):
@@ -41,7 +48,9 @@ def run_outer(ctx, A, B, dense_outer=False):
     return spgemm_outer(ctx, dA, dB, A.nrows, A.ncols, B.ncols).download()
 
 
-def check(C, A, B, exact_order=True, nthreads=0):
+def check(C, A, B, exact_order=None, nthreads=0):
+    if exact_order is None:
+        exact_order = _current["method"] not in UNORDERED
     G = oracle.spgemm_gustavson(A, B, nthreads)
     assert C.nnz == G.nnz
     np.testing.assert_array_equal(np.asarray(C.pos, np.int64), np.asarray(G.pos, np.int64))
@@ -193,13 +202,13 @@ def test_outer_fused_rows_over_many_chunks(ctx, wlog):
 
 
 def test_outer_auto_method(ctx):
-    """Automatic choice: buckets for the power-law workload shape, the LSD
-    sort for very wide, very sparse outputs."""
+    """Automatic choice: the fused row windows for the power-law workload
+    shape, the LSD sort for very wide, very sparse outputs."""
     print("case: outer automatic method", flush=True)
     B = gen.zipf_csr(4000, 1.8, 400, seed=3, stream=2)
     dB = DeviceDCSR.upload(ctx, formats.dcsr_from_csr(B))
     spgemm_outer(ctx, dB, dB, 4000, 4000, 4000)
-    assert ctx.stats()["outer_method"] == 2
+    assert ctx.stats()["outer_method"] == 3
     A = uniform_rows(100, 80, 0, 3, 70, 0)
     Bw = random_csr(80, 1 << 22, np.full(80, 3), 70, 1)
     C = run_outer(ctx, A, Bw)
@@ -404,8 +413,12 @@ def test_device_fused_vs_nonfused_transposed_output(ctx, method, m, k, n, seed):
     finally:
         set_outer_method(ctx, 0)
     G = oracle.spgemm_gustavson(B.transpose(), A.transpose())
+    _, absum = oracle.spgemm_refsum(B.transpose(), A.transpose(), G.pos, G.crd)
     for X in (F, N):
         assert (X.nrows, X.ncols) == (n, m)
         np.testing.assert_array_equal(np.asarray(X.pos, np.int64), np.asarray(G.pos, np.int64))
         np.testing.assert_array_equal(X.crd, G.crd)
-        np.testing.assert_array_equal(X.vals, G.vals)
+        if method in UNORDERED:  # atomically added: within the tolerance
+            assert oracle.max_value_error(X.vals, G.vals, absum) <= TOL
+        else:
+            np.testing.assert_array_equal(X.vals, G.vals)

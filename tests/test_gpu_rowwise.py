@@ -86,6 +86,67 @@ def test_fixed_slot_path_launched_ahead(ctx):
         assert (st["rows_per_path"][0] + st["rows_per_path"][6] == A.nrows) == fits, t
 
 
+@pytest.mark.parametrize("m", [1, 63, 10_000, 40_000, 130_000])
+def test_short_row_path(ctx, m):
+    """Every row has at most 64 products: two launches, each wavefront of
+    the first owning up to 8 consecutive rows (130,000 rows give 8 per
+    wavefront on a 256-CU device) and packing their sorted entries, the
+    second summing the earlier wavefronts' totals and copying into C.  Empty
+    rows among them; three calls in a row (the second and third start on the
+    short-row path with no upper-bound pass), each against the oracle, with
+    the product count."""
+    print(f"case: short-row path, {m} rows", flush=True)
+    A = uniform_rows(m, 5000, 0, 8, 31, 0)
+    B = uniform_rows(5000, 6000, 0, 8, 31, 1)
+    ub = oracle.spgemm_row_ub(A, B)
+    assert ub.max() <= 64
+    try:
+        ctx.set_profiling(True)
+        for rep in range(3):
+            ctx.reset_kernel_times()
+            C = run(ctx, A, B)
+            check(C, A, B)
+            assert ctx.stats()["products"] == int(ub.sum()), rep
+            times = ctx.kernel_times()
+            assert "short_rows" in times and "short_compact" in times and "row_wave" not in times, (rep, times)
+            if rep > 0:
+                assert "rowstat" not in times, times
+    finally:
+        ctx.set_profiling(False)
+
+
+def test_short_rows_beyond_short_path(ctx):
+    """300,000 short rows need more first-kernel wavefronts than the
+    compaction sums: the short-row path declines before launching and the
+    general pipeline computes the product, twice in a row, each against the
+    oracle."""
+    print("case: short rows beyond the short-row path, 300k rows", flush=True)
+    A = uniform_rows(300_000, 5000, 0, 8, 31, 0)
+    B = uniform_rows(5000, 6000, 0, 8, 31, 1)
+    ub = oracle.spgemm_row_ub(A, B)
+    for rep in range(2):
+        C = run(ctx, A, B)
+        check(C, A, B)
+        assert ctx.stats()["products"] == int(ub.sum()), rep
+
+
+def test_short_path_declines_rows_of_many_a_entries(ctx):
+    """Rows with more than 64 A entries but at most 64 products (most B rows
+    empty): the short-row path declines them and the general pipeline
+    computes the product; then a short product takes the short-row path
+    again."""
+    print("case: short-row path declines long A rows", flush=True)
+    lensB = np.where(np.arange(900) % 10 == 0, 3, 0)
+    B = random_csr(900, 700, lensB, 32, 1)
+    A = random_csr(500, 900, np.where(np.arange(500) % 7 == 0, 100, 5), 32, 0)
+    assert oracle.spgemm_row_ub(A, B).max() <= 64 and A.row_lengths().max() > 64
+    check(run(ctx, A, B), A, B)
+    A2, B2 = uniform_rows(4000, 700, 0, 8, 33, 0), uniform_rows(700, 900, 0, 8, 33, 1)
+    for _ in range(2):
+        check(run(ctx, A2, B2), A2, B2)
+    check(run(ctx, A, B), A, B)
+
+
 def test_matrix_market_file(ctx, tmp_path):
     """A symmetric Matrix Market file (lower triangle stored) read with its
     values, squared on the device."""
@@ -309,14 +370,21 @@ def test_kernel_timing_filter(ctx):
         ctx.reset_kernel_times()
         check(run(ctx, *short), *short)
         every = ctx.kernel_times()
-        assert {"rowstat", "row_wave", "compact_fixed"} <= set(every), every
-        ctx.set_profiling_only("row_wave")
+        assert "short_rows" in every and "compact_fixed" not in every, every
+        ctx.set_profiling_only("short_rows")
         ctx.reset_kernel_times()
         for A, B in (short, binned, short):
             check(run(ctx, A, B), A, B)
         only = ctx.kernel_times()
-        assert set(only) == {"row_wave"}, only
-        assert only["row_wave"][1] >= 3 and only["row_wave"][0] > 0.0
+        # each short product takes the short-row kernels; the binned one,
+        # following a short one, tries it first (it declines, then the bins run)
+        assert set(only) == {"short_rows"}, only
+        assert only["short_rows"][1] == 3 and only["short_rows"][0] > 0.0
+        # a binned product after a binned one does not try it
+        check(run(ctx, *binned), *binned)
+        ctx.reset_kernel_times()
+        check(run(ctx, *binned), *binned)
+        assert ctx.kernel_times() == {}, ctx.kernel_times()
     finally:
         ctx.set_profiling_only(None)
         ctx.set_profiling(False)

@@ -8,7 +8,7 @@ import pytest
 
 import oracle
 from sparsews_amd import formats, gen, taco
-from sparsews_amd.device import Context, DeviceArray, DeviceCSF, mttkrp, ttm
+from sparsews_amd.device import Context, DeviceArray, DeviceCSF, mttkrp, set_ttm_method, ttm
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
@@ -29,12 +29,16 @@ def run_mttkrp(ctx, X, Bf, Cf):
     return rows.to_host(), M.to_host().reshape(X.dims[0], R)
 
 
-def run_ttm(ctx, X, U):
+def run_ttm(ctx, X, U, method=0):
     R = U.shape[1]
     csf = formats.csf3_from_coo(X, (0, 2, 1))
-    fi, fj, Y = ttm(ctx, DeviceCSF.upload(ctx, csf), X.dims[0], X.dims[1], X.dims[2], R,
-                    DeviceArray.from_host(ctx, np.ascontiguousarray(U.ravel())))
-    return fi.to_host(), fj.to_host(), Y.to_host().reshape(-1, R)
+    set_ttm_method(ctx, method)
+    try:
+        fi, fj, Y = ttm(ctx, DeviceCSF.upload(ctx, csf), X.dims[0], X.dims[1], X.dims[2], R,
+                        DeviceArray.from_host(ctx, np.ascontiguousarray(U.ravel())))
+        return fi.to_host(), fj.to_host(), Y.to_host().reshape(-1, R)
+    finally:
+        set_ttm_method(ctx, 0)
 
 
 @pytest.mark.parametrize("R", [32, 8, 48])
@@ -86,6 +90,60 @@ def test_ttm_small(ctx, R):
     np.testing.assert_array_equal(np.repeat(fj, R), oj)
     absY = np.einsum("ijk,kr->ijr", np.abs(dense(X)), np.abs(U))
     assert oracle.max_value_error(Y.ravel(), ov, absY[fi, fj].ravel()) <= TOL
+
+
+TTM_ROOT_CASES = [
+    # (dims, nnz, exponents, seed): Zipf roots -- a few heavy roots of many
+    # 2,048-nonzero tiles and light ones; uniform roots -- light roots only;
+    # Zipf roots over many roots -- both kinds, heavy ones of one tile
+    ((60, 5000, 300), 200_000, (1.2, 1.5, None), 23),
+    ((5000, 3000, 50), 100_000, (None, None, None), 24),
+    ((20_000, 4000, 200), 300_000, (1.1, 1.3, None), 25),
+]
+
+
+@pytest.mark.parametrize("R", [16, 5, 32, 40, 64])
+@pytest.mark.parametrize("case", range(len(TTM_ROOT_CASES)))
+def test_ttm_roots_and_windows(ctx, R, case):
+    """The roots-and-windows path (light roots sorted in registers, heavy
+    roots bucketed by column window and added in dense LDS windows of
+    1024 / R' columns): every fiber summed from 0.0 over k in increasing
+    order, so structure and values equal the sequential hash_il oracle bit
+    for bit; the sorting path gives the same structure and values within
+    the tolerance (it splits fibers of more than 128 nonzeros into partial
+    sums)."""
+    dims, nnz, exps, seed = TTM_ROOT_CASES[case]
+    print(f"case: ttm roots and windows R={R} dims={dims} nnz={nnz}", flush=True)
+    X = gen.zipf_coo3(dims, nnz, exps, seed, 5, -1.0, 1.0)
+    U = np.random.default_rng(50 + R).uniform(-1, 1, (dims[2], R))
+    fi, fj, Y = run_ttm(ctx, X, U, method=2)
+    csf = formats.csf3_from_coo(X, (0, 2, 1))
+    ok, oj, ol, ov = oracle.ttm_hash_il(csf, formats.dense_as_dcsr(U), dims[0], dims[1], R)
+    np.testing.assert_array_equal(np.repeat(fi, R), ok)
+    np.testing.assert_array_equal(np.repeat(fj, R), oj)
+    np.testing.assert_array_equal(Y.ravel(), ov)
+    si, sj, SY = run_ttm(ctx, X, U, method=1)
+    np.testing.assert_array_equal(si, fi)
+    np.testing.assert_array_equal(sj, fj)
+    scale = np.abs(Y).max() + 1.0
+    assert np.max(np.abs(SY - Y)) <= 1e-12 * scale * 16
+
+
+def test_ttm_roots_declines_wide_output_mode(ctx):
+    """More column windows than the per-root counters hold (J = 300,000 at
+    R = 16): method 2 refuses with SPWS_ERR_UNSUPPORTED, the automatic method
+    takes the sorting path."""
+    print("case: ttm roots path declines wide J", flush=True)
+    from sparsews_amd._lib import SpwsError
+    X = gen.zipf_coo3((40, 300_000, 30), 5000, (None, None, None), 26, 5, -1.0, 1.0)
+    U = np.random.default_rng(7).uniform(-1, 1, (30, 16))
+    with pytest.raises(SpwsError):
+        run_ttm(ctx, X, U, method=2)
+    fi, fj, Y = run_ttm(ctx, X, U)
+    csf = formats.csf3_from_coo(X, (0, 2, 1))
+    ok, oj, ol, ov = oracle.ttm_hash_il(csf, formats.dense_as_dcsr(U), 40, 300_000, 16)
+    np.testing.assert_array_equal(np.repeat(fi, 16), ok)
+    np.testing.assert_array_equal(np.repeat(fj, 16), oj)
 
 
 @pytest.mark.parametrize("R", [16, 40])
