@@ -842,8 +842,9 @@ def run_fusion(args):
     compute_nofuse at :232, transpose at :352): the outer product A B, then
     the transpose of its result into preallocated arrays.  A leg's time is
     the median of repeated calls, each ended by a stream sync; fused and
-    non-fused outputs are compared bit for bit on every case up to 2*10^7
-    entries.  The CPU columns run the same two pipelines through the oracle
+    non-fused outputs are compared on every case up to 2*10^7 entries (bit
+    for bit when the automatic method sums in order, within 1e-12 when it is
+    the atomic-add variant).  The CPU columns run the same two pipelines through the oracle
     (the reference's coordinate workspace and transpose, one core) on the
     cases up to 2*10^6 products."""
     from sparsews_amd import formats, gen
@@ -884,7 +885,12 @@ def run_fusion(args):
             F = held["F"].download()
             assert F.nnz == nnz
             assert np.array_equal(np.asarray(F.pos, np.int64), t_pos.to_host().astype(np.int64))
-            assert np.array_equal(F.crd, t_crd.to_host()[:nnz]) and np.array_equal(F.vals, t_val.to_host()[:nnz])
+            assert np.array_equal(F.crd, t_crd.to_host()[:nnz])
+            tv = t_val.to_host()[:nnz]
+            if ctx.stats()["outer_method"] == 4:  # atomic adds: equal within the reassociation bound
+                assert np.max(np.abs(F.vals - tv), initial=0.0) <= 1e-12 * max(1.0, float(np.abs(tv).max(initial=0)))
+            else:
+                assert np.array_equal(F.vals, tv)
         t_f = t_n = float("inf")
         for _ in range(2):   # alternate the legs so that drift affects both
             t_n = min(t_n, _median_time(ctx, nonfused))
@@ -894,7 +900,9 @@ def run_fusion(args):
         case = {"m": m, "k": k, "n": n, "per_row": r, "seed": seed, "products": int(products), "nnz_C": int(nnz),
                 "fused_ms": round(t_f * 1e3, 4), "nonfused_ms": round(t_n * 1e3, 4),
                 "transpose_ms": round(t_t * 1e3, 4), "speedup": round(t_n / t_f, 3),
-                "bit_identical": nnz <= 20_000_000}
+                "outer_method": ctx.stats()["outer_method"],
+                "compared": "bit for bit" if ctx.stats()["outer_method"] != 4 else "within 1e-12",
+                "checked": nnz <= 20_000_000}
         log(f"bench: fusion case {m}x{k}x{n} r={r}: nnz(C)={nnz} fused {t_f * 1e3:.3f} ms, "
             f"non-fused {t_n * 1e3:.3f} ms (transpose {t_t * 1e3:.3f})")
         if not args.no_cpu_baseline and products <= 2_000_000:

@@ -159,7 +159,7 @@ int spws_ctx_set_outer_chunking(spws_ctx* ctx, int64_t max_products, int32_t max
  *   4  as 3, the products added into the LDS window by atomic adds (no
  *      ordering of equal columns): sums reassociated within the 1e-12 bar,
  *      not reproducible bit for bit;
- *   0  automatic (3 unless the windows would be too sparse or too many,
+ *   0  automatic (4 unless the windows would be too sparse or too many,
  *      then 1).
  * window_log2 sets the column window of methods 2-4 (6..11; 0 = chosen
  * from the average products per row).  Methods 1-3 sum each entry in

@@ -111,6 +111,9 @@ struct spws_ctx {
   // bucket window width log2 (0 = chosen from the average row product count)
   int32_t outer_method = 0;
   int32_t outer_window_log2 = 0;
+  // the automatic method keeps the sums in order (3 rather than 4): set by
+  // the taco entry points, whose results stand in for the reference's
+  bool outer_auto_ordered = false;
   // TTM method (0 auto, 1 sorting, 2 roots and column windows)
   int32_t ttm_method = 0;
   // second stream (default priority) and its events, created on first use:

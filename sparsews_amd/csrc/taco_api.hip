@@ -292,6 +292,16 @@ int spws_csr_csr_assemble(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B) 
   return SPWS_OK;
 }
 
+// The taco entry points stand in for the reference's compute(): their sums
+// stay reproducible run to run, so an automatic outer-product method is the
+// ordered fused one there (not the atomic-add variant).
+struct OrderedOuterAuto {
+  spws_ctx* ctx;
+  bool saved;
+  explicit OrderedOuterAuto(spws_ctx* c) : ctx(c), saved(c->outer_auto_ordered) { ctx->outer_auto_ordered = true; }
+  ~OrderedOuterAuto() { ctx->outer_auto_ordered = saved; }
+};
+
 int spws_csr_csr_compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B) {
   int rc;
   if ((rc = check_csr(A, "A")) || (rc = check_csr(B, "B"))) return rc;
@@ -332,8 +342,11 @@ int spws_dcsc_dcsr_coord_compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor
   int64_t* dpos;
   int32_t* dcrd;
   double* dval;
-  rc = spws_spgemm_outer(ctx, m, A->dimensions[1], n, a.nouter, a.l1, a.pos, a.crd, a.vals, b.nouter, b.l1, b.pos,
-                         b.crd, b.vals, &nnz, &dpos, &dcrd, &dval);
+  {
+    OrderedOuterAuto ordered(ctx);
+    rc = spws_spgemm_outer(ctx, m, A->dimensions[1], n, a.nouter, a.l1, a.pos, a.crd, a.vals, b.nouter, b.l1, b.pos,
+                           b.crd, b.vals, &nnz, &dpos, &dcrd, &dval);
+  }
   if (rc) return rc;
   SPWS_REQUIRE(nnz <= INT32_MAX, SPWS_ERR_UNSUPPORTED, "nnz(C) = %lld does not fit the int32 CSR of taco_tensor_t",
                (long long)nnz);
@@ -421,8 +434,11 @@ int spws_csc_csr_t_hash_compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_
   int64_t* dpos;
   int32_t* dcrd;
   double* dval;
-  rc = spws_spgemm_outer(ctx, n, k, m, k, nullptr, bpos, bcrd, bval, k, nullptr, apos, acrd, aval, &nnz, &dpos, &dcrd,
-                         &dval);
+  {
+    OrderedOuterAuto ordered(ctx);
+    rc = spws_spgemm_outer(ctx, n, k, m, k, nullptr, bpos, bcrd, bval, k, nullptr, apos, acrd, aval, &nnz, &dpos, &dcrd,
+                           &dval);
+  }
   if (rc) return rc;
   return outer_result_to_taco(ctx, C, n, nnz, dpos, dcrd, dval);
 }
@@ -456,8 +472,11 @@ int spws_csc_csr_nofuse_compute(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_
   int64_t* dpos;
   int32_t* dcrd;
   double* dval;
-  rc = spws_spgemm_outer(ctx, m, k, n, k, nullptr, apos, acrd, aval, k, nullptr, bpos, bcrd, bval, &nnz, &dpos, &dcrd,
-                         &dval);
+  {
+    OrderedOuterAuto ordered(ctx);
+    rc = spws_spgemm_outer(ctx, m, k, n, k, nullptr, apos, acrd, aval, k, nullptr, bpos, bcrd, bval, &nnz, &dpos, &dcrd,
+                           &dval);
+  }
   if (rc) return rc;
   return outer_result_to_taco(ctx, C, m, nnz, dpos, dcrd, dval);
 }
