@@ -36,6 +36,7 @@
 // every later kernel has nothing to do, and the batch reports an error.
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <thread>
 #include <vector>
 
@@ -466,6 +467,10 @@ static int batch_build(spws_batch* b) {
   memset(b->h_desc, 0, sizeof(spws_batch_desc));
   SPWS_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   SPWS_HIP(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+  // SPWS_BATCH_EAGER=1: no graph; every launch enqueues the same kernels
+  // on the engine's stream (for kernel tracers that do not follow graphs)
+  const char* eager = getenv("SPWS_BATCH_EAGER");
+  if (eager && eager[0] == '1') return SPWS_OK;
   SPWS_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   enqueue_batch(b);
   hipError_t e = hipStreamEndCapture(b->stream, &b->graph);
@@ -606,7 +611,12 @@ int spws_batch_launch(spws_batch* b) {
   if (rc) return rc;
   *b->h_desc = b->cur;
   *b->h_total = -1;
-  SPWS_HIP(hipGraphLaunch(b->exec, b->stream));
+  if (b->exec) {
+    SPWS_HIP(hipGraphLaunch(b->exec, b->stream));
+  } else {
+    enqueue_batch(b);
+    SPWS_CHECK_LAUNCH();
+  }
   SPWS_HIP(hipEventRecord(b->done, b->stream));
   b->in_flight = true;
   return SPWS_OK;

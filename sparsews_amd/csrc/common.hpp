@@ -221,6 +221,96 @@ __device__ __forceinline__ void wave_bitonic(int32_t (&v)[R], int n_pow2) {
   }
 }
 
+// v of lane (lane ^ D) without LDS: DPP quad permutes, half-row and row
+// mirrors and the row rotate by 8, and gfx950's 16- and 32-lane swaps
+// (each swap returns both halves exchanged; the lane keeps the half that
+// came from its partner).
+template <int D>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v) {
+  if constexpr (D == 1) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true);  // quad_perm [1,0,3,2]
+  } else if constexpr (D == 2) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
+  } else if constexpr (D == 3) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x1B, 0xF, 0xF, true);  // quad_perm [3,2,1,0]
+  } else if constexpr (D == 4) {
+    return lane_xor<3>(lane_xor<7>(v));
+  } else if constexpr (D == 7) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, true);  // row_half_mirror
+  } else if constexpr (D == 8) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, true);  // row_ror:8
+  } else if constexpr (D == 15) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xF, 0xF, true);  // row_mirror
+  } else if constexpr (D == 16) {
+    const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (lane_id() & 16) ? p[0] : p[1];
+  } else if constexpr (D == 31) {
+    return lane_xor<16>(lane_xor<15>(v));
+  } else if constexpr (D == 32) {
+    const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (lane_id() & 32) ? p[0] : p[1];
+  } else {
+    static_assert(D == 63, "lane_xor: unsupported distance");
+    return lane_xor<32>(lane_xor<31>(v));
+  }
+}
+
+// One comparator column of the flip bitonic network on R independent
+// 64-key sequences (register r of every lane): the lower lane of each pair
+// keeps the minimum.
+template <int D, int R>
+__device__ __forceinline__ void wave_sort64_step(uint32_t (&k)[R], bool low) {
+  uint32_t p[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) p[r] = lane_xor<D>(k[r]);
+  // the median of (key, partner's key, 0 or ~0): the minimum in the lower
+  // lane, the maximum in the upper one
+  const uint32_t bound = low ? 0u : ~0u;
+#pragma unroll
+  for (int r = 0; r < R; ++r) k[r] = max(min(k[r], p[r]), min(max(k[r], p[r]), bound));
+}
+
+// Sorts R independent sequences of 64 unsigned keys (register r of every
+// lane is sequence r) ascending; only the first n_pow2 (a power of two,
+// 2..64) entries of each take part, the rest must already hold 0xFFFFFFFF.
+// Every exchange is a DPP move or a lane swap, none goes through LDS.
+template <int R>
+__device__ __forceinline__ void wave_sort64(uint32_t (&k)[R], int n_pow2) {
+  const int lane = lane_id();
+  const bool b0 = !(lane & 1), b1 = !(lane & 2), b2 = !(lane & 4), b3 = !(lane & 8), b4 = !(lane & 16);
+  wave_sort64_step<1>(k, b0);
+  if (n_pow2 >= 4) {
+    wave_sort64_step<3>(k, b1);
+    wave_sort64_step<1>(k, b0);
+  }
+  if (n_pow2 >= 8) {
+    wave_sort64_step<7>(k, b2);
+    wave_sort64_step<2>(k, b1);
+    wave_sort64_step<1>(k, b0);
+  }
+  if (n_pow2 >= 16) {
+    wave_sort64_step<15>(k, b3);
+    wave_sort64_step<4>(k, b2);
+    wave_sort64_step<2>(k, b1);
+    wave_sort64_step<1>(k, b0);
+  }
+  if (n_pow2 >= 32) {
+    wave_sort64_step<31>(k, b4);
+    wave_sort64_step<8>(k, b3);
+    wave_sort64_step<4>(k, b2);
+    wave_sort64_step<2>(k, b1);
+    wave_sort64_step<1>(k, b0);
+  }
+  if (n_pow2 >= 64) {
+    wave_sort64_step<63>(k, !(lane & 32));
+    wave_sort64_step<16>(k, b4);
+    wave_sort64_step<8>(k, b3);
+    wave_sort64_step<4>(k, b2);
+    wave_sort64_step<2>(k, b1);
+    wave_sort64_step<1>(k, b0);
+  }
+}
+
 // Exclusive scan over a block of BLOCK threads; returns this thread's prefix
 // and writes the block total to *total.  `sh` holds BLOCK/64 + 1 entries.
 template <class T, int BLOCK>

@@ -566,26 +566,29 @@ constexpr int32_t kCfMaxRows = 1 << 16;
 //                    rows together -- row pointers, A entries, B row
 //                    pointers, B entries: four dependent loads in all, not
 //                    four per row -- and stages them in LDS (column, A value
-//                    * B value); then per row: LDS open-addressing table,
-//                    compaction, register bitonic sort of the columns, the
-//                    sorted entries written back over the row's own staged
-//                    products.  The wavefront's rows go out packed into its
+//                    * B value); then all its rows at once, row r in
+//                    register r across the lanes: a bitonic sort by column
+//                    carrying the values, and a scan that sums runs of equal
+//                    columns.  The wavefront's rows go out packed into its
 //                    slot region (q * 64 entries from w q 64), with the row
 //                    counts and the wavefront's totals (entries, products).
+//                    Each workgroup also writes its own totals.
 //   k_short_compact  a wavefront per wavefront of the first kernel: the
-//                    totals of all earlier wavefronts (each workgroup sums
-//                    them itself: at most kShortMaxWaves values), their
-//                    prefix, then C_pos and a coalesced copy of the
-//                    wavefront's packed entries into C, every load issued
-//                    before the first store.
+//                    totals of all earlier workgroups (each workgroup sums
+//                    them itself: at most kShortMaxWaves / 4 values), the
+//                    prefix over its own wavefronts, then C_pos and a
+//                    coalesced copy of the wavefront's packed entries into
+//                    C, every load issued before the first store.
 // No upper-bound pass, no bins, no host round trip in between.  A single
 // launch with a look-back across wavefronts was measured slower than both
 // kernels together: status words polled across the XCDs cost a coherent
 // round trip each, and a same-address ticket counter serialises.
-// A row that does not fit sets flags bit 1; the caller then runs the
-// general pipeline (which recomputes C).
-//   out[0] products, out[1] flags (set only on failure; cleared by the
-//   caller's memset), out[2] nnz(C).
+// A wavefront with a row that does not fit reports a total of -1; the
+// compaction then writes nothing past the totals, out[1] reads 1, and the
+// caller runs the general pipeline (which recomputes C).  The last
+// compaction workgroup writes all of out with plain stores (nothing is
+// cleared beforehand):
+//   out[0] products, out[1] 1 if a row did not fit (else 0), out[2] nnz(C).
 constexpr int kShortBlock = 256;
 constexpr int kShortWaves = kShortBlock / kWave;
 constexpr int kShortRows = 8;           // rows per wavefront at most
@@ -594,32 +597,67 @@ constexpr int kShortProducts = 512;     // staged products per wavefront at most
 constexpr int kShortMaxWaves = 1 << 14; // wavefronts of the first kernel at most
 constexpr int kShortCompactWaves = kShortWaves;  // first-kernel wavefronts per compaction workgroup (one per wavefront)
 
+// Sorts R independent 64-entry sequences (register r of every lane is
+// sequence r, entry = lane) by key, carrying the values along; the first
+// n_pow2 entries of each take part (the rest must be INT_MAX already).
+// Equal keys keep their own values: a lane takes its partner's entry only
+// on a strict comparison.
+template <int R>
+__device__ __forceinline__ void wave_sort_pairs(int32_t (&k)[R], double (&v)[R], int n_pow2) {
+  const int lane = lane_id();
+  auto step = [&](int d, bool low) {
+    int32_t pk[R];
+    double pv[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      pk[r] = __shfl_xor(k[r], d, kWave);
+      pv[r] = __shfl_xor(v[r], d, kWave);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const bool take = low ? pk[r] < k[r] : pk[r] > k[r];
+      if (take) {
+        k[r] = pk[r];
+        v[r] = pv[r];
+      }
+    }
+  };
+  for (int kk = 2; kk <= n_pow2; kk <<= 1) {
+    step(kk - 1, (lane & (kk >> 1)) == 0);
+    for (int j = kk >> 2; j >= 1; j >>= 1) step(j, (lane & j) == 0);
+  }
+}
+
+// kPacked: columns below 2^26 - 1, so that (column << 6 | position in the
+// row) fits 32 bits below the padding key; the sort then moves one dword
+// per exchange and the values are read from LDS at the sorted positions.
+template <int R, bool kPacked>
 [[maybe_unused]] __global__ __launch_bounds__(kShortBlock) void k_short_rows(
     int32_t m, int32_t q, int32_t nwaves, const int32_t* __restrict__ Ap, const int32_t* __restrict__ Ai,
     const double* __restrict__ Ax, const int32_t* __restrict__ Bp, const int32_t* __restrict__ Bi,
     const double* __restrict__ Bx, int32_t* __restrict__ slot_crd, double* __restrict__ slot_val,
-    int32_t* __restrict__ row_nnz, int32_t* __restrict__ wave_tot, int32_t* __restrict__ wave_prod,
-    unsigned long long* __restrict__ out) {
-  constexpr int TS = 128;
-  constexpr uint32_t kMask = TS - 1;
-  __shared__ int32_t s_keys[kShortWaves][TS];
-  __shared__ double s_vals[kShortWaves][TS];
-  __shared__ int32_t s_list[kShortWaves][kWave];
+    int32_t* __restrict__ row_nnz, int32_t* __restrict__ wave_tot, int32_t* __restrict__ wg_tot,
+    int32_t* __restrict__ wg_prod) {
   __shared__ int32_t s_col[kShortWaves][kShortProducts];
   __shared__ double s_pv[kShortWaves][kShortProducts];
+  __shared__ double s_av[kShortWaves][kShortAEntries];
+  __shared__ uint8_t s_ai[kShortWaves][kShortProducts];
+  static_assert(kShortAEntries <= 256, "A entry indices are stored in a byte");
+  static_assert(kShortAEntries == 2 * kWave, "two A entries per lane");
   const int wv = threadIdx.x / kWave;
-  int32_t* keys = s_keys[wv];
-  double* vals = s_vals[wv];
-  int32_t* list = s_list[wv];
   int32_t* scol = s_col[wv];
   double* spv = s_pv[wv];
-  const uint32_t shift = 32 - log2_pow2(TS);
+  double* sav = s_av[wv];
+  uint8_t* sai = s_ai[wv];
   const int lane = lane_id();
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int32_t w = blockIdx.x * kShortWaves + wv;
-  if (w >= nwaves) return;  // the last workgroup's spare wavefronts own no rows and no totals
+  __shared__ int32_t s_wt[kShortWaves], s_wp[kShortWaves];
+  // the wavefront's totals: entries (-1 when a row does not fit), products
+  int32_t my_tot = 0, my_prod = 0;
+  auto wave_rows = [&]() {
   const int32_t r0 = (int32_t)min((int64_t)w * q, (int64_t)m);
-  const int32_t nr = min(q, m - r0);
+  const int32_t nr = min(q, m - r0);  // <= q <= R
 
   // row pointers, A entries and B row pointers of all the rows
   const int32_t rp = (lane <= nr && nr > 0) ? Ap[r0 + lane] : 0;
@@ -658,152 +696,212 @@ constexpr int kShortCompactWaves = kShortWaves;  // first-kernel wavefronts per 
   const int32_t pb = E(lo), pe = E(hi);
   if (lane < nr && (hi - lo > kWave || pe - pb > kWave)) over = true;
   over = __any(over || P > kShortProducts);
-  if (over) {  // the caller discards the result; the totals stay consistent (zero)
-    if (lane == 0) {
-      atomicOr(&out[1], 1ull);
-      wave_tot[w] = 0;
-      wave_prod[w] = 0;
-    }
+  if (over) {  // the caller discards the result
+    my_tot = -1;
     if (lane < nr) row_nnz[r0 + lane] = 0;
     return;
   }
 
-  // every product staged in LDS at its A entry's offset: each lane walks
-  // its A entry's B row (the wavefront's longest row sets the trip count,
-  // eight loads in flight per lane)
+  // every product staged in LDS at its A entry's offset, in two steps so
+  // that the B loads are coalesced: each A entry first writes, for each of
+  // its products, the product's B index (over scol) and its own index;
+  // then lane l loads products l, l + 64, ... (consecutive products of a B
+  // row sit in consecutive lanes), all loads of the wavefront in flight
   {
+    sav[lane] = av0;
+    sav[kWave + lane] = av1;
     const int32_t maxlen = wave_max(max(bl0, bl1));
-    for (int32_t t0 = 0; t0 < maxlen; t0 += 8) {
-      int32_t c0[8], c1[8];
-      double x0[8], x1[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int32_t t = t0 + u;
-        c0[u] = t < bl0 ? Bi[bs0 + t] : 0;
-        x0[u] = t < bl0 ? Bx[bs0 + t] : 0.0;
-        c1[u] = t < bl1 ? Bi[bs1 + t] : 0;
-        x1[u] = t < bl1 ? Bx[bs1 + t] : 0.0;
+    for (int32_t t = 0; t < maxlen; ++t) {
+      if (t < bl0) {
+        scol[ex0 + t] = bs0 + t;
+        sai[ex0 + t] = (uint8_t)lane;
       }
+      if (t < bl1) {
+        scol[ex1 + t] = bs1 + t;
+        sai[ex1 + t] = (uint8_t)(kWave + lane);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    constexpr int kU = kShortProducts / kWave;
+    int32_t c[kU];
+    double x[kU];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int32_t t = t0 + u;
-        if (t < bl0) {
-          scol[ex0 + t] = c0[u];
-          spv[ex0 + t] = av0 * x0[u];
-        }
-        if (t < bl1) {
-          scol[ex1 + t] = c1[u];
-          spv[ex1 + t] = av1 * x1[u];
-        }
+    for (int u = 0; u < kU; ++u) {
+      const int32_t p = u * kWave + lane;
+      if (u * kWave < P) {  // uniform
+        const int32_t idx = p < P ? scol[p] : 0;
+        c[u] = p < P ? Bi[idx] : 0;
+        x[u] = p < P ? Bx[idx] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int32_t p = u * kWave + lane;
+      if (u * kWave < P && p < P) {
+        scol[p] = c[u];
+        spv[p] = sav[sai[p]] * x[u];
       }
     }
   }
   __builtin_amdgcn_wave_barrier();
-  // per row: table, compaction, sort, entries back over the row's products
-  int32_t my_nnz = 0;
-  for (int r = 0; r < nr; ++r) {
-    const int32_t rb = __builtin_amdgcn_readlane(pb, r);
-    const int32_t len = __builtin_amdgcn_readlane(pe, r) - rb;
-    keys[lane] = keys[lane + kWave] = -1;
-    vals[lane] = vals[lane + kWave] = 0.0;
-    int32_t col = -1;
-    double pv = 0.0;
-    if (lane < len) {
-      col = scol[rb + lane];
-      pv = spv[rb + lane];
-    }
-    __builtin_amdgcn_wave_barrier();
-    if (col >= 0) lds_insert<true>(keys, vals, shift, kMask, col, pv);
-    __builtin_amdgcn_wave_barrier();
-    int32_t nnz = 0;
+  // all rows at once, row r in register r (its products across the lanes,
+  // INT_MAX past the end): sort by column, then sum runs of equal columns
+  const int32_t rlen = lane < nr ? pe - pb : 0;
+  const int32_t maxrow = wave_max(rlen);
+  int n_pow2 = 2;
+  while (n_pow2 < maxrow) n_pow2 <<= 1;
+  int32_t kc[R];
+  double kv[R];
+  if constexpr (kPacked) {
+    uint32_t key[R];
 #pragma unroll
-    for (int s0 = 0; s0 < TS; s0 += kWave) {
-      const int32_t key = keys[s0 + lane];
-      const uint64_t occ = __ballot(key != -1);
-      if (key != -1) list[nnz + __popcll(occ & lt)] = key;
-      nnz += __popcll(occ);
+    for (int r = 0; r < R; ++r) {
+      const int32_t rb = __builtin_amdgcn_readlane(pb, r);
+      const int32_t len = r < nr ? __builtin_amdgcn_readlane(rlen, r) : 0;
+      key[r] = lane < len ? ((uint32_t)scol[rb + lane] << 6) | (uint32_t)lane : 0xFFFFFFFFu;
     }
-    __builtin_amdgcn_wave_barrier();
-    int32_t vv[1] = {lane < nnz ? list[lane] : INT_MAX};
-    int n_pow2 = 2;
-    while (n_pow2 < nnz) n_pow2 <<= 1;
-    wave_bitonic<1>(vv, n_pow2);
-    if (lane < nnz) {
-      scol[rb + lane] = vv[0];
-      spv[rb + lane] = lds_lookup(keys, vals, shift, kMask, vv[0]);
+    wave_sort64<R>(key, n_pow2);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int32_t rb = __builtin_amdgcn_readlane(pb, r);
+      const bool valid = key[r] != 0xFFFFFFFFu;
+      kc[r] = valid ? (int32_t)(key[r] >> 6) : INT_MAX;
+      kv[r] = valid ? spv[rb + (int32_t)(key[r] & 63u)] : 0.0;
     }
-    if (lane == r) my_nnz = nnz;
-    __builtin_amdgcn_wave_barrier();
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int32_t rb = __builtin_amdgcn_readlane(pb, r);
+      const int32_t len = r < nr ? __builtin_amdgcn_readlane(rlen, r) : 0;
+      kc[r] = lane < len ? scol[rb + lane] : INT_MAX;
+      kv[r] = lane < len ? spv[rb + lane] : 0.0;
+    }
+    wave_sort_pairs<R>(kc, kv, n_pow2);
   }
-  // the rows packed into the wavefront's slot region
-  const int32_t loc_incl = wave_incl_scan(lane < nr ? my_nnz : 0);
-  const int32_t loc = loc_incl - (lane < nr ? my_nnz : 0);
-  const int64_t base = (int64_t)w * q * kWave;
-  for (int r = 0; r < nr; ++r) {
-    const int32_t rb = __builtin_amdgcn_readlane(pb, r);
-    const int32_t cnt = __builtin_amdgcn_readlane(my_nnz, r);
-    const int64_t dst = base + __builtin_amdgcn_readlane(loc, r);
-    if (lane < cnt) {
-      slot_crd[dst + lane] = scol[rb + lane];
-      slot_val[dst + lane] = spv[rb + lane];
+  // runs of equal columns are contiguous: an inclusive scan restricted to
+  // equal keys leaves each run's sum at its last lane (only rows that have
+  // a repeated column take it)
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int32_t prev = __shfl_up(kc[r], 1, kWave);
+    if (__ballot(lane > 0 && kc[r] != INT_MAX && prev == kc[r])) {
+#pragma unroll
+      for (int d = 1; d < kWave; d <<= 1) {
+        const int32_t pk = __shfl_up(kc[r], d, kWave);
+        const double ps = __shfl_up(kv[r], d, kWave);
+        if (lane >= d && pk == kc[r]) kv[r] += ps;
+      }
     }
+  }
+  // the rows packed into the wavefront's slot region: each run's last lane
+  // writes (column, sum) at its rank
+  const int64_t base = (int64_t)w * q * kWave;
+  int32_t loc = 0, my_nnz = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int32_t nxt = __shfl_down(kc[r], 1, kWave);
+    const bool last = kc[r] != INT_MAX && (lane == kWave - 1 || nxt != kc[r]);
+    const uint64_t lm = __ballot(last);
+    if (last) {
+      const int64_t dst = base + loc + __popcll(lm & lt);
+      slot_crd[dst] = kc[r];
+      slot_val[dst] = kv[r];
+    }
+    const int32_t cnt = __popcll(lm);
+    if (lane == r) my_nnz = cnt;
+    loc += cnt;
   }
   if (lane < nr) row_nnz[r0 + lane] = my_nnz;
+  my_tot = loc;
+  my_prod = P;
+  };
+  // the last workgroup's spare wavefronts own no rows and add nothing
+  if (w < nwaves) {
+    wave_rows();
+    if (lane == 0) wave_tot[w] = my_tot;
+  }
   if (lane == 0) {
-    wave_tot[w] = __builtin_amdgcn_readlane(loc_incl, kWave - 1);
-    wave_prod[w] = P;
+    s_wt[wv] = my_tot;
+    s_wp[wv] = my_prod;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t t = 0, pr = 0;
+    bool bad = false;
+#pragma unroll
+    for (int i = 0; i < kShortWaves; ++i) {
+      bad |= s_wt[i] < 0;
+      t += s_wt[i];
+      pr += s_wp[i];
+    }
+    wg_tot[blockIdx.x] = bad ? -1 : t;
+    wg_prod[blockIdx.x] = pr;
   }
 }
 
 [[maybe_unused]] __global__ __launch_bounds__(kShortBlock) void k_short_compact(
     int32_t m, int32_t q, int32_t nwaves, const int32_t* __restrict__ row_nnz, const int32_t* __restrict__ wave_tot,
-    const int32_t* __restrict__ wave_prod, const int32_t* __restrict__ slot_crd, const double* __restrict__ slot_val,
+    const int32_t* __restrict__ wg_tot, const int32_t* __restrict__ wg_prod, const int32_t* __restrict__ slot_crd,
+    const double* __restrict__ slot_val,
     int32_t* __restrict__ C_pos, int32_t* __restrict__ C_crd, double* __restrict__ C_val,
     unsigned long long* __restrict__ out) {
-  if (out[1] != 0) return;  // a row did not fit: the caller recomputes C
   __shared__ int64_t s_part[2][kShortWaves];
+  __shared__ int32_t s_bad[kShortWaves + 1];  // per wavefront, then the workgroup's verdict
   __shared__ int32_t s_off[kShortCompactWaves + 1];
   const int lane = lane_id();
   const int wv = threadIdx.x / kWave;
   const int32_t w0 = blockIdx.x * kShortCompactWaves;
   const int32_t w1 = min(w0 + kShortCompactWaves, nwaves);
-  // totals of every earlier wavefront (entries, and products for the last
-  // workgroup's statistics)
+  // entries of every earlier workgroup of the first kernel (one per
+  // workgroup here too), and the products of all of them for the last
+  // workgroup's statistics; a total of -1 marks a workgroup with a row that
+  // did not fit
+  static_assert(kShortCompactWaves == kShortWaves, "one compaction workgroup per first-kernel workgroup");
   int64_t sn = 0, sp = 0;
-  for (int32_t t = threadIdx.x; t < w0; t += kShortBlock) {
-    sn += wave_tot[t];
-    sp += wave_prod[t];
+  bool bad = false;
+  for (int32_t t = threadIdx.x; t < (int32_t)blockIdx.x; t += kShortBlock) {
+    const int32_t x = wg_tot[t];
+    sn += x;
+    bad |= x < 0;
   }
+  if (w1 == nwaves)
+    for (int32_t t = threadIdx.x; t < (int32_t)gridDim.x; t += kShortBlock) sp += wg_prod[t];
   sn = wave_sum(sn);
   sp = wave_sum(sp);
+  const bool wbad = __any(bad);
   if (lane == 0) {
     s_part[0][wv] = sn;
     s_part[1][wv] = sp;
+    s_bad[wv] = wbad;
   }
   __syncthreads();
   if (wv == 0) {
     int64_t n = 0, pr = 0;
+    bool any_bad = false;
 #pragma unroll
     for (int i = 0; i < kShortWaves; ++i) {
       n += s_part[0][i];
       pr += s_part[1][i];
+      any_bad |= s_bad[i] != 0;
     }
     // this workgroup's wavefronts (one per lane): exclusive offsets, and the
     // end at index w1 - w0 (lanes past w1 add nothing)
     const int32_t t = w0 + lane;
     const bool mine = lane < kShortCompactWaves && t < w1;
     const int32_t tot = mine ? wave_tot[t] : 0;
-    const int64_t prod = mine ? wave_prod[t] : 0;
+    any_bad |= __any(mine && tot < 0);
     const int32_t incl = wave_incl_scan(tot);
     if (lane <= kShortCompactWaves) s_off[lane] = (int32_t)n + incl - tot;
-    const int64_t psum = wave_sum(prod);
+    if (lane == 0) s_bad[kShortWaves] = any_bad;
     if (w1 == nwaves && lane == kWave - 1) {
-      C_pos[m] = (int32_t)n + incl;
-      out[0] = (unsigned long long)(pr + psum);
+      if (!any_bad) C_pos[m] = (int32_t)n + incl;
+      out[0] = (unsigned long long)pr;
+      out[1] = any_bad ? 1ull : 0ull;
       out[2] = (unsigned long long)((int32_t)n + incl);
     }
   }
   __syncthreads();
+  if (s_bad[kShortWaves]) return;  // a row did not fit: the caller recomputes C
   // one wavefront per first-kernel wavefront: row pointers, then the copy
   // (all of its packed entries loaded before any is stored)
   const int32_t t = w0 + wv;

@@ -70,7 +70,7 @@ static int validate_tuning(const spws_rowwise_tuning& t) {
 // into the context's C arrays.  *ok is false when it does not apply (more
 // wavefronts than the compaction sums) or a row does not fit: the caller
 // then runs the general pipeline, which recomputes C.
-static int run_short(spws_ctx* ctx, int32_t m, const int32_t* Ap, const int32_t* Ai, const double* Ax,
+static int run_short(spws_ctx* ctx, int32_t m, int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax,
                      const int32_t* Bp, const int32_t* Bi, const double* Bx, bool* ok, int64_t* nnz,
                      int64_t* products) {
   hipStream_t s = ctx->stream;
@@ -85,27 +85,49 @@ static int run_short(spws_ctx* ctx, int32_t m, const int32_t* Ap, const int32_t*
   if (nwaves > kShortMaxWaves) return SPWS_OK;
   const size_t slots = (size_t)nwaves * q * kWave;
   const size_t nw = (size_t)nwaves;
+  const size_t nwg = (size_t)div_up(nwaves, kShortWaves);  // workgroups of either kernel
   if ((rc = ctx->slot_crd.ensure(slots * 4)) || (rc = ctx->slot_val.ensure(slots * 8)) ||
-      (rc = ctx->row_nnz.ensure((size_t)m * 4)) || (rc = ctx->short_meta.ensure(nw * 8 + 64)) ||
+      (rc = ctx->row_nnz.ensure((size_t)m * 4)) || (rc = ctx->short_meta.ensure((nw + 2 * nwg) * 4)) ||
       (rc = ctx->out_pos.ensure(((size_t)m + 1) * 4)) || (rc = ctx->out_crd.ensure(slots * 4)) ||
       (rc = ctx->out_val.ensure(slots * 8)))
     return rc;
-  unsigned long long* out = ctx->short_meta.as<unsigned long long>();
-  int32_t* wave_tot = reinterpret_cast<int32_t*>(out + 8);
-  int32_t* wave_prod = wave_tot + nw;
-  SPWS_HIP(hipMemsetAsync(out, 0, 24, s));
-  const unsigned g1 = (unsigned)div_up(nwaves, kShortWaves);
-  SPWS_LAUNCH(ctx, "short_rows", k_short_rows, dim3(g1), dim3(kShortBlock), 0, m, q, (int32_t)nwaves, Ap, Ai, Ax,
-              Bp, Bi, Bx,
-              ctx->slot_crd.as<int32_t>(), ctx->slot_val.as<double>(), ctx->row_nnz.as<int32_t>(), wave_tot,
-              wave_prod, out);
-  const unsigned g2 = (unsigned)div_up(nwaves, kShortCompactWaves);
-  SPWS_LAUNCH(ctx, "short_compact", k_short_compact, dim3(g2), dim3(kShortBlock), 0, m, q, (int32_t)nwaves,
-              (const int32_t*)ctx->row_nnz.as<int32_t>(), (const int32_t*)wave_tot, (const int32_t*)wave_prod,
-              (const int32_t*)ctx->slot_crd.as<int32_t>(), (const double*)ctx->slot_val.as<double>(),
-              ctx->out_pos.as<int32_t>(), ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>(), out);
+  // the compaction's last workgroup writes products, the fit flag and
+  // nnz(C) straight into pinned host memory (no copy after the kernels)
   unsigned long long* h = ctx->host_meta.as<unsigned long long>() + 64;
-  SPWS_HIP(hipMemcpyAsync(h, out, 3 * 8, hipMemcpyDeviceToHost, s));
+  int32_t* wave_tot = ctx->short_meta.as<int32_t>();
+  int32_t* wg_tot = wave_tot + nw;
+  int32_t* wg_prod = wg_tot + nwg;
+  const unsigned g1 = (unsigned)nwg;
+  // one register per row of a wavefront (R = q), packed 32-bit sort keys
+  // when the columns allow them
+  const bool packed = n <= (1 << 26) - 1;
+#define SPWS_SHORT_ROWS(R)                                                                                  \
+  do {                                                                                                      \
+    if (packed)                                                                                             \
+      SPWS_LAUNCH(ctx, "short_rows", (k_short_rows<R, true>), dim3(g1), dim3(kShortBlock), 0, m, q,        \
+                  (int32_t)nwaves, Ap, Ai, Ax, Bp, Bi, Bx, ctx->slot_crd.as<int32_t>(),                    \
+                  ctx->slot_val.as<double>(), ctx->row_nnz.as<int32_t>(), wave_tot, wg_tot, wg_prod);     \
+    else                                                                                                    \
+      SPWS_LAUNCH(ctx, "short_rows", (k_short_rows<R, false>), dim3(g1), dim3(kShortBlock), 0, m, q,       \
+                  (int32_t)nwaves, Ap, Ai, Ax, Bp, Bi, Bx, ctx->slot_crd.as<int32_t>(),                    \
+                  ctx->slot_val.as<double>(), ctx->row_nnz.as<int32_t>(), wave_tot, wg_tot, wg_prod);     \
+  } while (0)
+  switch (q) {
+    case 1: SPWS_SHORT_ROWS(1); break;
+    case 2: SPWS_SHORT_ROWS(2); break;
+    case 3: SPWS_SHORT_ROWS(3); break;
+    case 4: SPWS_SHORT_ROWS(4); break;
+    case 5: SPWS_SHORT_ROWS(5); break;
+    case 6: SPWS_SHORT_ROWS(6); break;
+    case 7: SPWS_SHORT_ROWS(7); break;
+    default: SPWS_SHORT_ROWS(8); break;
+  }
+#undef SPWS_SHORT_ROWS
+  SPWS_LAUNCH(ctx, "short_compact", k_short_compact, dim3(g1), dim3(kShortBlock), 0, m, q, (int32_t)nwaves,
+              (const int32_t*)ctx->row_nnz.as<int32_t>(), (const int32_t*)wave_tot, (const int32_t*)wg_tot,
+              (const int32_t*)wg_prod,
+              (const int32_t*)ctx->slot_crd.as<int32_t>(), (const double*)ctx->slot_val.as<double>(),
+              ctx->out_pos.as<int32_t>(), ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>(), h);
   SPWS_HIP(hipStreamSynchronize(s));
   *ok = h[1] == 0;
   *nnz = (int64_t)h[2];
@@ -131,7 +153,7 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
   if (kValues && t.force_path < 0 && m > 0 && ctx->rw_fixed_last) {
     bool ok = false;
     int64_t nnz = 0, prods = 0;
-    if ((rc = run_short(ctx, m, Ap, Ai, Ax, Bp, Bi, Bx, &ok, &nnz, &prods))) return rc;
+    if ((rc = run_short(ctx, m, n, Ap, Ai, Ax, Bp, Bi, Bx, &ok, &nnz, &prods))) return rc;
     if (ok) {
       ctx->stats.products = prods;
       ctx->stats.nnz = nnz;
@@ -229,7 +251,7 @@ static int rowwise_run(spws_ctx* ctx, int32_t m, int32_t kdim, int32_t n, const 
     // the metadata block is zero again for the next product
     SPWS_LAUNCH(ctx, "meta_clear", k_meta_clear, dim3(1), dim3(64), 0, bmeta, maxub_d);
     ctx->rw_meta_clean = true;
-    if ((rc = run_short(ctx, m, Ap, Ai, Ax, Bp, Bi, Bx, &ok, &nnz, &prods))) return rc;
+    if ((rc = run_short(ctx, m, n, Ap, Ai, Ax, Bp, Bi, Bx, &ok, &nnz, &prods))) return rc;
     if (ok) {
       *nnz_out = nnz;
       ctx->stats.nnz = nnz;
