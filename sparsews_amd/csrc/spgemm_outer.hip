@@ -927,6 +927,34 @@ __global__ __launch_bounds__(256) void k_fz_records(int32_t En, const double* __
 }
 
 constexpr int kFzU = 16;  // strides of 64 products per batch
+constexpr int kFzOwn = kFzU * kWave;  // owner marks of a batch (u16 each)
+constexpr uint32_t kFzTagFirst = 32;  // (32 << 6) exceeds every window column (< 2^11)
+constexpr uint32_t kFzTagEnd = 1024;  // tags stay below 2^16 with the lane in the low 6 bits
+
+// Entries of the u16 area shared by the emitted column list (2^s) and the
+// owner marks of a batch.
+__host__ __device__ constexpr int fz_list_len(int s) { return (1 << s) > kFzOwn ? (1 << s) : kFzOwn; }
+
+// LDS of one k_fz_rows wavefront: numeric: the window's values, the seen
+// and stride bitmaps, the list / marks, the per-lane B bases and A values;
+// symbolic: the seen bitmap, the marks, the B bases.
+__host__ __device__ constexpr size_t fz_lds_bytes(int s, bool numeric) {
+  return (numeric ? ((size_t)8 << s) + 2 * (((size_t)1 << s) >> 3) : (((size_t)1 << s) >> 3)) +
+         2 * (size_t)fz_list_len(s) + 4 * kWave + (numeric ? 8 * kWave : 0);
+}
+
+// Inclusive max-scan over the wavefront: row shifts by 1, 2, 4, 8 within
+// each row of 16, then the ends of rows 0 and 0-1 broadcast into the rows
+// after them (lanes without a source add 0).
+__device__ __forceinline__ uint32_t wave_incl_max(uint32_t v) {
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false));  // row_shr:1
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false));  // row_shr:2
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false));  // row_shr:4
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false));  // row_shr:8
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false));  // row_bcast:15
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false));  // row_bcast:31
+  return v;
+}
 
 // A batch: up to kFzU strides of one (window, entry chunk) of the row.
 struct FzBatch {
@@ -957,14 +985,24 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
   double* acc = fz_lds;                                          // numeric: 2^s values
   uint32_t* seen = kNumeric ? reinterpret_cast<uint32_t*>(acc + S) : reinterpret_cast<uint32_t*>(fz_lds);
   uint32_t* sb = seen + words;                                   // numeric: this stride's columns
-  uint16_t* list = reinterpret_cast<uint16_t*>(sb + words);      // numeric: emitted columns
+  // numeric: the emitted columns of a window; both passes: a batch's owner
+  // marks (the two uses never overlap in time)
+  uint16_t* list = reinterpret_cast<uint16_t*>(kNumeric ? sb + words : seen + words);
+  uint16_t* own = list;
+  int32_t* s_base = reinterpret_cast<int32_t*>(list + fz_list_len(s));  // per lane: B index - product index
+  double* s_av = reinterpret_cast<double*>(s_base + kWave);              // numeric: per lane: A value
   if (lane < words) {
     seen[lane] = 0u;
     if (kNumeric) sb[lane] = 0u;
   }
   if (kNumeric)
     for (int q = lane; q < S; q += kWave) acc[q] = -0.0;
+  for (int q = lane; q < kFzOwn; q += kWave) own[q] = 0;
   __builtin_amdgcn_wave_barrier();
+  // owner marks carry a batch tag (tagc << 6 | lane) above every column
+  // value the list holds (< 2^11 = 32 << 6), so marks of earlier batches
+  // and list entries never win the max-scan against this batch's
+  uint32_t tagc = kFzTagFirst;
 
   for (;;) {
     int32_t r = 0;
@@ -1013,6 +1051,8 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
       excl = incl - len;
       base = b_start + seg_lo - excl;
       b.total = __builtin_amdgcn_readlane(incl, kWave - 1);
+      s_base[lane] = base;
+      if (kNumeric) s_av[lane] = a_val;
     };
     // the batch after b (advancing chunks and windows; empty ones skipped)
     auto next = [&](const FzBatch& b) -> FzBatch {
@@ -1054,14 +1094,29 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
     double bv[2][kFzU], av[2][kFzU];
     auto load = [&](const FzBatch& b, int slot) {
       const int32_t len = seg_hi - seg_lo;
+      // this batch's owner marks: each segment that starts inside the
+      // batch marks its first product with its lane
+      if (tagc >= kFzTagEnd) {
+        for (int q = lane; q < kFzOwn; q += kWave) own[q] = 0;
+        tagc = kFzTagFirst;
+      }
+      const uint32_t tag = tagc << 6;
+      ++tagc;
+      {
+        const int32_t rel = excl - b.q0;
+        if (len > 0 && rel >= 0 && rel < kFzOwn) own[rel] = (uint16_t)(tag | (uint32_t)lane);
+      }
+      __builtin_amdgcn_wave_barrier();
 #pragma unroll
       for (int u = 0; u < kFzU; ++u) {
         const int32_t s0 = b.q0 + u * kWave;  // the stride's first product (uniform)
         if (s0 >= b.total) break;             // apply stops at the same stride
         const int32_t q = min(s0 + lane, b.total - 1);
         // a lane whose segment covers the whole stride owns every product of
-        // it (the common case: most products lie in long segments); else
-        // each lane finds its owner, the last lane t with excl_t <= q
+        // it (the common case for long segments); else the owner of product
+        // q is the last lane t with excl_t <= q: the latest mark at or
+        // before q (a wavefront max-scan of the marks), or, before the
+        // stride's first mark, the segment that holds the stride's start
         const int32_t s_end = min(s0 + kWave, b.total);
         const uint64_t cov = __ballot(len > 0 && excl <= s0 && excl + len >= s_end);
         int32_t idx;
@@ -1071,14 +1126,12 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
           idx = __builtin_amdgcn_readlane(base, o) + q;
           if (kNumeric) a = readlane_f64(a_val, o);
         } else {
-          int lo = 0, hi = kWave - 1;
-#pragma unroll
-          for (int it = 0; it < 6; ++it) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (__shfl(excl, mid, kWave) <= q) lo = mid; else hi = mid - 1;
-          }
-          idx = __shfl(base, lo, kWave) + q;
-          if (kNumeric) a = __shfl(a_val, lo, kWave);
+          const uint64_t before = __ballot(len > 0 && excl <= s0);
+          const uint32_t carry = tag | (uint32_t)(63 - __builtin_clzll(before));
+          const uint32_t mark = wave_incl_max(own[u * kWave + lane]);
+          const int o = (int)(max(mark, carry) & 63u);
+          idx = s_base[o] + q;
+          if (kNumeric) a = s_av[o];
         }
         kc[slot][u] = (uint32_t)B2_crd[idx];
         if (kNumeric) {
@@ -1660,7 +1713,7 @@ static int outer_fused(spws_ctx* ctx, const OuterArgs& a, const std::vector<int6
   // symbolic pass: entries per row
   SPWS_HIP(hipMemsetAsync(work, 0, 4, st));
   const unsigned gsym = (unsigned)std::max<int64_t>(1, std::min<int64_t>(m, (int64_t)dev_cus * kFzSymWavesPerCu));
-  SPWS_LAUNCH(ctx, "fz_count", k_fz_rows<false>, dim3(gsym), dim3(kWave), (((size_t)1 << s) >> 3), m, W, s, rp_off,
+  SPWS_LAUNCH(ctx, "fz_count", k_fz_rows<false>, dim3(gsym), dim3(kWave), fz_lds_bytes(s, false), m, W, s, rp_off,
               rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, row_cnt, (const int64_t*)nullptr, (int32_t*)nullptr,
               (double*)nullptr, work);
   if ((rc = exclusive_scan<int32_t, int64_t>(ctx, row_cnt, out_pos, m))) return rc;
@@ -1678,11 +1731,11 @@ static int outer_fused(spws_ctx* ctx, const OuterArgs& a, const std::vector<int6
   SPWS_HIP(hipMemsetAsync(work, 0, 4, st));
   const unsigned gnum = (unsigned)std::max<int64_t>(1, std::min<int64_t>(m, (int64_t)dev_cus * kFzNumWavesPerCu));
   if (ordered)
-    SPWS_LAUNCH(ctx, "fz_reduce", (k_fz_rows<true, true>), dim3(gnum), dim3(kWave), bk_reduce_lds_per_wave(s), m, W,
+    SPWS_LAUNCH(ctx, "fz_reduce", (k_fz_rows<true, true>), dim3(gnum), dim3(kWave), fz_lds_bytes(s, true), m, W,
                 s, rp_off, rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, (int32_t*)nullptr, (const int64_t*)out_pos,
                 ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>(), work);
   else
-    SPWS_LAUNCH(ctx, "fz_reduce", (k_fz_rows<true, false>), dim3(gnum), dim3(kWave), bk_reduce_lds_per_wave(s), m, W,
+    SPWS_LAUNCH(ctx, "fz_reduce", (k_fz_rows<true, false>), dim3(gnum), dim3(kWave), fz_lds_bytes(s, true), m, W,
                 s, rp_off, rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, (int32_t*)nullptr, (const int64_t*)out_pos,
                 ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>(), work);
   SPWS_HIP(hipStreamSynchronize(st));
