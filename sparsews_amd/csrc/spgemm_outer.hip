@@ -487,6 +487,8 @@ __global__ __launch_bounds__(256) void k_bk_expand(int32_t En, const double* __r
 // everything; a kernel that issues this just before a batch's loads makes
 // every such wait one on operations issued a batch earlier.
 __device__ __forceinline__ void vm_drain() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+// s_waitcnt lgkmcnt(0), vector memory and export counts left alone
+__device__ __forceinline__ void lgkm_drain() { __builtin_amdgcn_s_waitcnt(0xC07F); }
 
 constexpr int kBkExpandLdsW = 1024;  // widest window table k_bk_expand_lds keeps in LDS
 
@@ -1139,6 +1141,11 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
           av[slot][u] = a;
         }
       }
+      // the A values read from LDS above are used only in apply: wait for
+      // them here, where they have long arrived, so that apply's first use
+      // does not wait for its own LDS atomics as well (LDS operations
+      // retire in order)
+      lgkm_drain();
     };
     auto apply = [&](const FzBatch& b, int slot) {
       const uint32_t wmask = (uint32_t)S - 1;
@@ -1221,6 +1228,10 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
           }
         }
         __builtin_amdgcn_wave_barrier();
+        // nothing of the emit stays in flight: without this, a list or
+        // value read on a path with no lane left to store it counts as
+        // pending at every stride of the next batch
+        lgkm_drain();
       }
       run += total;
     };
