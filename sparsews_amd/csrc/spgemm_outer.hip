@@ -940,8 +940,8 @@ __host__ __device__ constexpr int fz_list_len(int s) { return (1 << s) > kFzOwn 
 // LDS of one k_fz_rows wavefront: numeric: the window's values, the seen
 // and stride bitmaps, the list / marks, the per-lane B bases and A values;
 // symbolic: the seen bitmap, the marks, the B bases.
-__host__ __device__ constexpr size_t fz_lds_bytes(int s, bool numeric) {
-  return (numeric ? ((size_t)8 << s) + 2 * (((size_t)1 << s) >> 3) : (((size_t)1 << s) >> 3)) +
+__host__ __device__ constexpr size_t fz_lds_bytes(int s, int d, bool numeric) {
+  return (numeric ? ((size_t)8 << s) + 2 * (((size_t)1 << s) >> 3) : (((size_t)1 << (s + d)) >> 3)) +
          2 * (size_t)fz_list_len(s) + 4 * kWave + (numeric ? 8 * kWave : 0);
 }
 
@@ -972,7 +972,8 @@ struct FzBatch {
 // LDS atomic add (ds_add_f64), with no conflict resolution -- the sums are
 // then reassociated within the 1e-12 bar and not reproducible bit for bit.
 template <bool kNumeric, bool kOrdered = true>
-__global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, const int32_t* __restrict__ rp_off,
+__global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, int d,
+                                                   const int32_t* __restrict__ rp_off,
                                                    const int32_t* __restrict__ rec_b,
                                                    const int32_t* __restrict__ rec_ja,
                                                    const double* __restrict__ rec_a,
@@ -982,7 +983,11 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
                                                    const int64_t* __restrict__ C_pos, int32_t* __restrict__ C_crd,
                                                    double* __restrict__ C_val, int32_t* __restrict__ work) {
   extern __shared__ double fz_lds[];
-  const int S = 1 << s, words = S >> 5;  // words <= 64: one per lane
+  // the walk's windows: 2^d segment-table windows each (the symbolic pass
+  // only counts, so it takes wide windows: fewer per-window steps); the
+  // numeric pass keeps d = 0 (its emit holds one bitmap word per lane)
+  const int S = 1 << (s + d), words = S >> 5;
+  const int32_t Wd = (W + (1 << d) - 1) >> d;
   const int lane = lane_id();
   double* acc = fz_lds;                                          // numeric: 2^s values
   uint32_t* seen = kNumeric ? reinterpret_cast<uint32_t*>(acc + S) : reinterpret_cast<uint32_t*>(fz_lds);
@@ -993,9 +998,9 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
   uint16_t* own = list;
   int32_t* s_base = reinterpret_cast<int32_t*>(list + fz_list_len(s));  // per lane: B index - product index
   double* s_av = reinterpret_cast<double*>(s_base + kWave);              // numeric: per lane: A value
-  if (lane < words) {
-    seen[lane] = 0u;
-    if (kNumeric) sb[lane] = 0u;
+  for (int q = lane; q < words; q += kWave) {
+    seen[q] = 0u;
+    if (kNumeric) sb[q] = 0u;
   }
   if (kNumeric)
     for (int q = lane; q < S; q += kWave) acc[q] = -0.0;
@@ -1034,9 +1039,9 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
         b_start = rec_b[q_beg + q];
         a_val = kNumeric ? rec_a[q_beg + q] : 0.0;
         bs_row = bseg + (int64_t)rec_ja[q_beg + q] * (W + 1);
-        seg_lo = bs_row[w];
-        seg_hi = bs_row[w + 1];
-        nxt_hi = (one_chunk && w + 2 <= W) ? bs_row[w + 2] : seg_hi;
+        seg_lo = bs_row[min(w << d, W)];
+        seg_hi = bs_row[min((w + 1) << d, W)];
+        nxt_hi = (one_chunk && w + 2 <= Wd) ? bs_row[min((w + 2) << d, W)] : seg_hi;
       } else {
         b_start = 0;
         a_val = 0.0;
@@ -1071,7 +1076,7 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
         } else {
           n.w += 1;
           n.c = 0;
-          if (n.w >= W) {
+          if (n.w >= Wd) {
             n.w = -1;
             return n;
           }
@@ -1079,7 +1084,7 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
             seg_lo = seg_hi;
             seg_hi = nxt_hi;
             // the next window's end, a window ahead
-            if (lane < E && n.w + 2 <= W) nxt_hi = bs_row[n.w + 2];
+            if (lane < E && n.w + 2 <= Wd) nxt_hi = bs_row[min((n.w + 2) << d, W)];
           } else {
             load_chunk(0, n.w);
           }
@@ -1198,9 +1203,14 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
       if (!b.win_end) return;
       __builtin_amdgcn_wave_barrier();
       const uint32_t myw = lane < words ? seen[lane] : 0u;
+      int cnt = __popc(myw);
+      if (!kNumeric)  // wide windows: more than one bitmap word per lane
+        for (int q = lane + kWave; q < words; q += kWave) {
+          cnt += __popc(seen[q]);
+          seen[q] = 0u;
+        }
       __builtin_amdgcn_wave_barrier();
       if (lane < words) seen[lane] = 0u;
-      const int cnt = __popc(myw);
       const int incl = wave_incl_scan(cnt);
       const int total = __builtin_amdgcn_readlane(incl, kWave - 1);
       if (kNumeric && total > 0 && out0 + run + total <= out_end) {
@@ -1635,7 +1645,8 @@ static int outer_chunks_bucket(spws_ctx* ctx, const OuterArgs& a, const std::vec
   return SPWS_OK;
 }
 
-constexpr int kFzSymWavesPerCu = 16;  // k_fz_rows<false>: one wave per workgroup, tiny LDS
+constexpr int kFzSymWavesPerCu = 16;  // k_fz_rows<false>: one wave per workgroup, small LDS
+constexpr int kFzSymWindowLog2 = 15;  // widest window of the symbolic walk (a 4 KiB bitmap)
 constexpr int kFzNumWavesPerCu = 8;   // k_fz_rows<true>: bounded by the LDS window
 
 // Fused method over all rows at once (see the fused-method comment): A's
@@ -1724,7 +1735,11 @@ static int outer_fused(spws_ctx* ctx, const OuterArgs& a, const std::vector<int6
   // symbolic pass: entries per row
   SPWS_HIP(hipMemsetAsync(work, 0, 4, st));
   const unsigned gsym = (unsigned)std::max<int64_t>(1, std::min<int64_t>(m, (int64_t)dev_cus * kFzSymWavesPerCu));
-  SPWS_LAUNCH(ctx, "fz_count", k_fz_rows<false>, dim3(gsym), dim3(kWave), fz_lds_bytes(s, false), m, W, s, rp_off,
+  // the symbolic walk's windows: 2^dsym segment-table windows each
+  int dsym = 0;
+  while (s + dsym < kFzSymWindowLog2 && ((int64_t)1 << dsym) < W) ++dsym;
+  SPWS_LAUNCH(ctx, "fz_count", k_fz_rows<false>, dim3(gsym), dim3(kWave), fz_lds_bytes(s, dsym, false), m, W, s,
+              dsym, rp_off,
               rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, row_cnt, (const int64_t*)nullptr, (int32_t*)nullptr,
               (double*)nullptr, work);
   if ((rc = exclusive_scan<int32_t, int64_t>(ctx, row_cnt, out_pos, m))) return rc;
@@ -1742,12 +1757,12 @@ static int outer_fused(spws_ctx* ctx, const OuterArgs& a, const std::vector<int6
   SPWS_HIP(hipMemsetAsync(work, 0, 4, st));
   const unsigned gnum = (unsigned)std::max<int64_t>(1, std::min<int64_t>(m, (int64_t)dev_cus * kFzNumWavesPerCu));
   if (ordered)
-    SPWS_LAUNCH(ctx, "fz_reduce", (k_fz_rows<true, true>), dim3(gnum), dim3(kWave), fz_lds_bytes(s, true), m, W,
-                s, rp_off, rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, (int32_t*)nullptr, (const int64_t*)out_pos,
+    SPWS_LAUNCH(ctx, "fz_reduce", (k_fz_rows<true, true>), dim3(gnum), dim3(kWave), fz_lds_bytes(s, 0, true), m, W,
+                s, 0, rp_off, rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, (int32_t*)nullptr, (const int64_t*)out_pos,
                 ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>(), work);
   else
-    SPWS_LAUNCH(ctx, "fz_reduce", (k_fz_rows<true, false>), dim3(gnum), dim3(kWave), fz_lds_bytes(s, true), m, W,
-                s, rp_off, rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, (int32_t*)nullptr, (const int64_t*)out_pos,
+    SPWS_LAUNCH(ctx, "fz_reduce", (k_fz_rows<true, false>), dim3(gnum), dim3(kWave), fz_lds_bytes(s, 0, true), m, W,
+                s, 0, rp_off, rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, (int32_t*)nullptr, (const int64_t*)out_pos,
                 ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>(), work);
   SPWS_HIP(hipStreamSynchronize(st));
   *nnz_out = total;
