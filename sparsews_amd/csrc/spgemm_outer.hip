@@ -930,6 +930,8 @@ __global__ __launch_bounds__(256) void k_fz_records(int32_t En, const double* __
 
 constexpr int kFzU = 16;  // strides of 64 products per batch
 constexpr int kFzOwn = kFzU * kWave;  // owner marks of a batch (u16 each)
+constexpr int kFzG = 4;  // numeric pass: strides whose owner searches overlap
+static_assert(kFzU % kFzG == 0, "whole groups of strides per batch");
 constexpr uint32_t kFzTagFirst = 32;  // (32 << 6) exceeds every window column (< 2^11)
 constexpr uint32_t kFzTagEnd = 1024;  // tags stay below 2^16 with the lane in the low 6 bits
 
@@ -1114,36 +1116,66 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
         if (len > 0 && rel >= 0 && rel < kFzOwn) own[rel] = (uint16_t)(tag | (uint32_t)lane);
       }
       __builtin_amdgcn_wave_barrier();
+      // The owner of product q is the last lane t with excl_t <= q: the
+      // latest mark at or before q (a wavefront max-scan of the stride's
+      // marks), or, before the stride's first mark, the segment that holds
+      // the stride's start (the carry; a segment covering the whole stride
+      // is that case).
+      if constexpr (kNumeric) {
+        // strides in groups of kFzG: all the group's mark reads, then its
+        // scans and base / A-value reads, then its B loads, so that the LDS
+        // round trips of a group overlap
 #pragma unroll
-      for (int u = 0; u < kFzU; ++u) {
-        const int32_t s0 = b.q0 + u * kWave;  // the stride's first product (uniform)
-        if (s0 >= b.total) break;             // apply stops at the same stride
-        const int32_t q = min(s0 + lane, b.total - 1);
-        // a lane whose segment covers the whole stride owns every product of
-        // it (the common case for long segments); else the owner of product
-        // q is the last lane t with excl_t <= q: the latest mark at or
-        // before q (a wavefront max-scan of the marks), or, before the
-        // stride's first mark, the segment that holds the stride's start
-        const int32_t s_end = min(s0 + kWave, b.total);
-        const uint64_t cov = __ballot(len > 0 && excl <= s0 && excl + len >= s_end);
-        int32_t idx;
-        double a = 0.0;
-        if (cov) {
-          const int o = __builtin_ctzll(cov);
-          idx = __builtin_amdgcn_readlane(base, o) + q;
-          if (kNumeric) a = readlane_f64(a_val, o);
-        } else {
-          const uint64_t before = __ballot(len > 0 && excl <= s0);
-          const uint32_t carry = tag | (uint32_t)(63 - __builtin_clzll(before));
-          const uint32_t mark = wave_incl_max(own[u * kWave + lane]);
-          const int o = (int)(max(mark, carry) & 63u);
-          idx = s_base[o] + q;
-          if (kNumeric) a = s_av[o];
+        for (int g = 0; g < kFzU; g += kFzG) {
+          if (b.q0 + g * kWave >= b.total) break;  // uniform; apply stops at the same stride
+          uint32_t mk[kFzG], cr[kFzG];
+#pragma unroll
+          for (int i = 0; i < kFzG; ++i) {
+            const int32_t s0 = b.q0 + (g + i) * kWave;
+            const uint64_t before = __ballot(len > 0 && excl <= s0);  // nonzero: total > 0
+            cr[i] = tag | (uint32_t)(63 - __builtin_clzll(before));
+            mk[i] = own[(g + i) * kWave + lane];
+          }
+          int32_t bi[kFzG];
+          double ai[kFzG];
+#pragma unroll
+          for (int i = 0; i < kFzG; ++i) {
+            const int o = (int)(max(wave_incl_max(mk[i]), cr[i]) & 63u);
+            bi[i] = s_base[o];
+            ai[i] = s_av[o];
+          }
+#pragma unroll
+          for (int i = 0; i < kFzG; ++i) {
+            const int u = g + i;
+            const int32_t s0 = b.q0 + u * kWave;  // the stride's first product (uniform)
+            if (s0 < b.total) {
+              const int32_t idx = bi[i] + min(s0 + lane, b.total - 1);
+              kc[slot][u] = (uint32_t)B2_crd[idx];
+              bv[slot][u] = Bx[idx];
+              av[slot][u] = ai[i];
+            }
+          }
         }
-        kc[slot][u] = (uint32_t)B2_crd[idx];
-        if (kNumeric) {
-          bv[slot][u] = Bx[idx];
-          av[slot][u] = a;
+      } else {
+        // the symbolic walk's wide windows make covering segments common:
+        // they take the owner from registers, the rest search the marks
+#pragma unroll
+        for (int u = 0; u < kFzU; ++u) {
+          const int32_t s0 = b.q0 + u * kWave;  // the stride's first product (uniform)
+          if (s0 >= b.total) break;             // apply stops at the same stride
+          const int32_t q = min(s0 + lane, b.total - 1);
+          const int32_t s_end = min(s0 + kWave, b.total);
+          const uint64_t cov = __ballot(len > 0 && excl <= s0 && excl + len >= s_end);
+          int32_t idx;
+          if (cov) {
+            idx = __builtin_amdgcn_readlane(base, __builtin_ctzll(cov)) + q;
+          } else {
+            const uint64_t before = __ballot(len > 0 && excl <= s0);
+            const uint32_t carry = tag | (uint32_t)(63 - __builtin_clzll(before));
+            const int o = (int)(max(wave_incl_max(own[u * kWave + lane]), carry) & 63u);
+            idx = s_base[o] + q;
+          }
+          kc[slot][u] = (uint32_t)B2_crd[idx];
         }
       }
       // the A values read from LDS above are used only in apply: wait for
