@@ -4,11 +4,24 @@
 
 namespace spws {
 
+// Tiles of the LSD method's own run and reduce kernels (spgemm_outer.hip).
 constexpr int kRadixBlock = 256;
 constexpr int kRadixItems = 8;
 constexpr int kRadixTile = kRadixBlock * kRadixItems;  // keys per workgroup tile
-constexpr int kRadixDigit = 8;                          // bits per pass
-constexpr int kRadixBuckets = 1 << kRadixDigit;
+
+// The sort: tiles of kSortTile keys, passes of at most kSortDigitMax bits
+// (the key bits are split evenly over ceil(bits / kSortDigitMax) passes).
+constexpr int kSortBlock = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortBlock * kSortItems;
+constexpr int kSortDigitMax = 9;
+constexpr int kSortBucketsMax = 1 << kSortDigitMax;
+
+// int32 histogram entries one pass of radix_sort_pairs over n keys uses at
+// most (the scan of a pass covers as many).
+inline size_t radix_hist_entries(int64_t n) {
+  return (size_t)div_up(n > 0 ? n : 1, kSortTile) * kSortBucketsMax;
+}
 
 // Sorts keys[0, n) by bits [begin_bit, end_bit) carrying vals along, using
 // keys_alt / vals_alt as the ping-pong buffers.  On return *in_alt says

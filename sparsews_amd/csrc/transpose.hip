@@ -145,7 +145,7 @@ struct TransposeScratch {
     keys = align256(2 * nn * 8);
     vals = align256(2 * nn * 8);
     counts = align256((size_t)(n > 0 ? n : 1) * 4);
-    const size_t hn = (size_t)div_up((int64_t)nn, kRadixTile) * kRadixBuckets;
+    const size_t hn = radix_hist_entries((int64_t)nn);
     hist = align256((2 * hn + 1) * 4);
     scan = align256(std::max(scan_scratch_bytes((int64_t)hn, 4), scan_scratch_bytes((int64_t)n, 4)) + 4);
     total = keys + vals + counts + hist + scan;
