@@ -1,5 +1,6 @@
 // Shared device helpers and the per-context state of libspws.
 #pragma once
+#include <limits>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -165,29 +166,67 @@ int noalloc_scratch_init(int device);
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
+// v moved by a DPP pattern (CTRL) within the rows of ROW_MASK; lanes with no
+// source lane take `none` (32- or 64-bit T, moved as one or two dwords).
+template <int CTRL, int ROW_MASK, class T>
+__device__ __forceinline__ T dpp_move(T v, T none) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, none),
+                                                             __builtin_bit_cast(int, v), CTRL, ROW_MASK, 0xF, false));
+  } else {
+    const uint64_t x = __builtin_bit_cast(uint64_t, v), o = __builtin_bit_cast(uint64_t, none);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)o, (int)(uint32_t)x, CTRL, ROW_MASK, 0xF,
+                                                              false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)(o >> 32), (int)(uint32_t)(x >> 32),
+                                                              CTRL, ROW_MASK, 0xF, false);
+    return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+  }
+}
+
+// Inclusive wavefront scan with an associative op and its identity, all in
+// DPP moves (no LDS): row shifts by 1, 2, 4 and 8 give each row of 16 its
+// own prefixes, then the ends of rows 0 and 2 are broadcast into rows 1
+// and 3, and the end of rows 0-1 into rows 2-3.  Every lane of the
+// wavefront must be active.
+template <class T, class Op>
+__device__ __forceinline__ T wave_incl_scan_op(T v, T id, Op op) {
+  v = op(v, dpp_move<0x111, 0xF>(v, id));  // row_shr:1
+  v = op(v, dpp_move<0x112, 0xF>(v, id));  // row_shr:2
+  v = op(v, dpp_move<0x114, 0xF>(v, id));  // row_shr:4
+  v = op(v, dpp_move<0x118, 0xF>(v, id));  // row_shr:8
+  v = op(v, dpp_move<0x142, 0xA>(v, id));  // row_bcast:15
+  v = op(v, dpp_move<0x143, 0xC>(v, id));  // row_bcast:31
+  return v;
+}
+
+// The value of lane 63, in every lane.
+template <class T>
+__device__ __forceinline__ T wave_last(T v) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), kWave - 1));
+  } else {
+    const uint64_t x = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, kWave - 1);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), kWave - 1);
+    return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+  }
+}
+
+// Integer wavefront scans and reductions (every lane active).
 template <class T>
 __device__ __forceinline__ T wave_incl_scan(T v) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    T u = __shfl_up(v, d, kWave);
-    if (lane >= d) v += u;
-  }
-  return v;
+  return wave_incl_scan_op(v, (T)0, [](T a, T b) { return a + b; });
 }
 
 template <class T>
 __device__ __forceinline__ T wave_max(T v) {
-#pragma unroll
-  for (int d = kWave / 2; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d, kWave));
-  return v;
+  return wave_last(wave_incl_scan_op(v, std::numeric_limits<T>::lowest(), [](T a, T b) { return max(a, b); }));
 }
 
 template <class T>
 __device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
-  return v;
+  return wave_last(wave_incl_scan(v));
 }
 
 // Wavefront bitonic sort, ascending ("flip" network: stage k first compares
