@@ -140,13 +140,18 @@ void prof_collect(spws_ctx* ctx) {
 
 // ---------------------------------------------------------------- scan
 //
-// Reduce-then-scan: kernel 1 reduces tiles of kScanTile values, kernel 2
-// scans the tile sums in one workgroup, kernel 3 rescans each tile with its
-// offset.  Inputs below one tile take a single-workgroup path.
+// Reduce-then-scan: kernel 1 reduces tiles of kScanTile values; then, up to
+// kScanFuseTiles tiles, one kernel in which each workgroup sums the earlier
+// tiles' sums itself and scans its tile (coalesced: each wavefront takes a
+// quarter of the tile 64 values at a time, DPP wavefront scans with a
+// running carry); beyond that, a single-workgroup scan of the tile sums
+// and a rescan of each tile with its offset.  Inputs of at most four tiles
+// take a single-workgroup path.
 
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanBlock * kScanItems;
+constexpr int64_t kScanFuseTiles = 8192;  // earlier-tile sums each workgroup reads at most
 
 template <class Tin, class Tout>
 __global__ __launch_bounds__(kScanBlock) void k_scan_tile_reduce(const Tin* __restrict__ in, int64_t n,
@@ -227,6 +232,59 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_tile_apply(const Tin* __res
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = tile_off[gridDim.x];
 }
 
+// Tile scan with the tile's offset summed in place from the tile sums:
+// out[i] = sum of in[0, i); the last workgroup also writes out[n].
+template <class Tin, class Tout>
+__global__ __launch_bounds__(kScanBlock) void k_scan_tile_fused(const Tin* __restrict__ in, int64_t n,
+                                                                const Tout* __restrict__ tile_sums,
+                                                                Tout* __restrict__ out) {
+  constexpr int kWaves = kScanBlock / kWave;
+  constexpr int kRounds = kScanTile / kScanBlock;  // rounds of 64 values per wavefront
+  __shared__ Tout sh[2 * kWaves];
+  const int lane = lane_id();
+  const int w = threadIdx.x / kWave;
+  // this tile's offset: the sums of the tiles before it
+  Tout off = 0;
+  for (int64_t t = threadIdx.x; t < (int64_t)blockIdx.x; t += kScanBlock) off += tile_sums[t];
+  // this wavefront's quarter of the tile
+  const int64_t wbase = (int64_t)blockIdx.x * kScanTile + (int64_t)w * (kRounds * kWave);
+  Tout v[kRounds];
+  Tout mine = 0;
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    const int64_t idx = wbase + r * kWave + lane;
+    v[r] = idx < n ? (Tout)in[idx] : (Tout)0;
+    mine += v[r];
+  }
+  off = wave_sum(off);
+  mine = wave_sum(mine);
+  if (lane == 0) {
+    sh[w] = off;
+    sh[kWaves + w] = mine;
+  }
+  __syncthreads();
+  Tout carry = 0, total = 0;
+#pragma unroll
+  for (int q = 0; q < kWaves; ++q) {
+    carry += sh[q];  // the tile offset, summed over the threads
+    if (q < w) carry += sh[kWaves + q];
+    total += sh[kWaves + q];
+  }
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    const int64_t idx = wbase + r * kWave + lane;
+    const Tout incl = wave_incl_scan(v[r]) + carry;
+    if (idx < n) out[idx] = incl - v[r];
+    carry = wave_last(incl);
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    Tout base = 0;
+#pragma unroll
+    for (int q = 0; q < kWaves; ++q) base += sh[q];
+    out[n] = base + total;
+  }
+}
+
 size_t scan_scratch_bytes(int64_t n, size_t elem_bytes) {
   if (n <= 4 * (int64_t)kScanTile) return 0;
   return (size_t)(div_up(n, kScanTile) + 1) * 2 * elem_bytes;
@@ -245,6 +303,11 @@ int exclusive_scan(spws_ctx* ctx, const Tin* in, Tout* out, int64_t n) {
   Tout* offs = sums + (tiles + 1);
   SPWS_LAUNCH(ctx, "scan_tile_reduce", (k_scan_tile_reduce<Tin, Tout>), dim3((unsigned)tiles), dim3(kScanBlock),
               0, in, n, sums);
+  if (tiles <= kScanFuseTiles) {
+    SPWS_LAUNCH(ctx, "scan_tile_apply", (k_scan_tile_fused<Tin, Tout>), dim3((unsigned)tiles), dim3(kScanBlock), 0,
+                in, n, (const Tout*)sums, out);
+    return SPWS_OK;
+  }
   SPWS_LAUNCH(ctx, "scan_single", (k_scan_single<Tout, Tout>), dim3(1), dim3(kScanBlock), 0, sums, tiles, offs);
   SPWS_LAUNCH(ctx, "scan_tile_apply", (k_scan_tile_apply<Tin, Tout>), dim3((unsigned)tiles), dim3(kScanBlock), 0,
               in, n, offs, out);
