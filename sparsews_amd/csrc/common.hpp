@@ -200,6 +200,31 @@ __device__ __forceinline__ T wave_incl_scan_op(T v, T id, Op op) {
   return v;
 }
 
+// v of lane n of this lane's row of 16 (DPP row_newbcast: no LDS); n must
+// fold to a constant where it is used (unrolled loops do).
+template <class T>
+__device__ __forceinline__ T row_lane(T v, int n) {
+  switch (n & 15) {
+    case 0: return dpp_move<0x150, 0xF>(v, v);
+    case 1: return dpp_move<0x151, 0xF>(v, v);
+    case 2: return dpp_move<0x152, 0xF>(v, v);
+    case 3: return dpp_move<0x153, 0xF>(v, v);
+    case 4: return dpp_move<0x154, 0xF>(v, v);
+    case 5: return dpp_move<0x155, 0xF>(v, v);
+    case 6: return dpp_move<0x156, 0xF>(v, v);
+    case 7: return dpp_move<0x157, 0xF>(v, v);
+    case 8: return dpp_move<0x158, 0xF>(v, v);
+    case 9: return dpp_move<0x159, 0xF>(v, v);
+    case 10: return dpp_move<0x15A, 0xF>(v, v);
+    case 11: return dpp_move<0x15B, 0xF>(v, v);
+    case 12: return dpp_move<0x15C, 0xF>(v, v);
+    case 13: return dpp_move<0x15D, 0xF>(v, v);
+    case 14: return dpp_move<0x15E, 0xF>(v, v);
+    case 15: return dpp_move<0x15F, 0xF>(v, v);
+  }
+  return v;
+}
+
 // The value of lane 63, in every lane.
 template <class T>
 __device__ __forceinline__ T wave_last(T v) {

@@ -372,7 +372,7 @@ __global__ __launch_bounds__(256) void k_ttm_heads(int64_t nnz, const uint64_t* 
 // Work items of at most kTtmChunk sorted nonzeros: a fiber of n nonzeros
 // has ceil(n / kTtmChunk) items (fibers reach 2,000 nonzeros, one per k).
 constexpr int kTtmChunk = 128;
-constexpr int kTtmLanes = 16;
+constexpr int kTtmLanes = 16;  // one DPP row per group (broadcasts by row_newbcast)
 constexpr int kTtmU = 16;
 constexpr int kTtmBlocksPerCu = 4;  // workgroups per CU in the accumulation grid (4 waves per SIMD)
 
@@ -420,12 +420,12 @@ __device__ __forceinline__ double ttm_item_tail(double acc, int32_t base, int32_
     double uv[kTtmU];
 #pragma unroll
     for (int u = 0; u < kTtmU; ++u) {
-      const uint32_t k = __shfl(k2, u < cnt2 ? u : cnt2 - 1, kTtmLanes);
+      const uint32_t k = row_lane(k2, u);  // lanes past cnt2 hold the last entry
       uv[u] = ttm_load_u<kOff32>(U, k, R, rr);
     }
 #pragma unroll
     for (int u = 0; u < kTtmU; ++u) {
-      const double x = __shfl(v2, u, kTtmLanes);
+      const double x = row_lane(v2, u);
       if (u < cnt2) acc += x * uv[u];
     }
   }
@@ -486,12 +486,12 @@ __global__ __launch_bounds__(256) void k_ttm_accum(int64_t nitems, int R, const 
         double uv[kTtmU];
 #pragma unroll
         for (int u = 0; u < kTtmU; ++u) {
-          const uint32_t k = __shfl(myk, u < cnt ? u : cnt - 1, kTtmLanes);
+          const uint32_t k = row_lane(myk, u);  // lanes past cnt hold the last entry
           uv[u] = ttm_load_u<kOff32>(U, k, R, rr);
         }
 #pragma unroll
         for (int u = 0; u < kTtmU; ++u) {
-          const double x = __shfl(myval, u, kTtmLanes);
+          const double x = row_lane(myval, u);
           if (u < cnt) acc += x * uv[u];
         }
       }
