@@ -166,9 +166,13 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const K* __restric
 
 template <class K>
 int radix_sort_pairs(spws_ctx* ctx, K* keys, double* vals, K* keys_alt, double* vals_alt, int64_t n, int begin_bit,
-                     int end_bit, bool* in_alt) {
+                     int end_bit, bool* in_alt, const double* vals_first) {
   *in_alt = false;
-  if (n <= 1 || end_bit <= begin_bit) return SPWS_OK;
+  if (n <= 0) return SPWS_OK;
+  if (n <= 1 || end_bit <= begin_bit) {
+    if (vals_first) SPWS_HIP(hipMemcpyAsync(vals, vals_first, (size_t)n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    return SPWS_OK;
+  }
   SPWS_REQUIRE(n < ((int64_t)1 << 31), SPWS_ERR_UNSUPPORTED, "radix sort of %lld keys exceeds int32 offsets",
                (long long)n);
   const int64_t ntiles = div_up(n, kSortTile);
@@ -177,7 +181,7 @@ int radix_sort_pairs(spws_ctx* ctx, K* keys, double* vals, K* keys_alt, double* 
   int32_t* hist = ctx->radix_hist.as<int32_t>();
   int32_t* offs = hist + radix_hist_entries(n);
   K* kin = keys;
-  double* vin = vals;
+  const double* vin = vals_first ? vals_first : vals;
   K* kout = keys_alt;
   double* vout = vals_alt;
   const int passes = (int)div_up(end_bit - begin_bit, kSortDigitMax);
@@ -193,19 +197,23 @@ int radix_sort_pairs(spws_ctx* ctx, K* keys, double* vals, K* keys_alt, double* 
     if (rc) return rc;
     SPWS_LAUNCH(ctx, "radix_scatter", (k_radix_scatter<K>), dim3((unsigned)ntiles), dim3(kSortBlock), 0, kin, vin,
                 kout, vout, n, shift, mask, nbits, offs, ntiles);
+    // the next pass reads what this one wrote and writes the other buffer
+    // (never vals_first)
     K* tk = kin;
     kin = kout;
     kout = tk;
-    double* tv = vin;
-    vin = vout;
-    vout = tv;
+    double* written = vout;
+    vout = (written == vals_alt) ? vals : vals_alt;
+    vin = written;
     *in_alt = !*in_alt;
     shift += nbits;
   }
   return SPWS_OK;
 }
 
-template int radix_sort_pairs<uint32_t>(spws_ctx*, uint32_t*, double*, uint32_t*, double*, int64_t, int, int, bool*);
-template int radix_sort_pairs<uint64_t>(spws_ctx*, uint64_t*, double*, uint64_t*, double*, int64_t, int, int, bool*);
+template int radix_sort_pairs<uint32_t>(spws_ctx*, uint32_t*, double*, uint32_t*, double*, int64_t, int, int, bool*,
+                                        const double*);
+template int radix_sort_pairs<uint64_t>(spws_ctx*, uint64_t*, double*, uint64_t*, double*, int64_t, int, int, bool*,
+                                        const double*);
 
 }  // namespace spws

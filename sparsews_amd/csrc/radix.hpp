@@ -26,9 +26,11 @@ inline size_t radix_hist_entries(int64_t n) {
 // Sorts keys[0, n) by bits [begin_bit, end_bit) carrying vals along, using
 // keys_alt / vals_alt as the ping-pong buffers.  On return *in_alt says
 // whether the sorted data ended up in the _alt buffers.  K is uint32_t or
-// uint64_t.  Uses ctx->radix_hist and the scan scratch.
+// uint64_t.  Uses ctx->radix_hist and the scan scratch.  With vals_first,
+// the values are read from there instead of vals (which then needs no
+// copy of them; it is still filled when no pass runs).
 template <class K>
 int radix_sort_pairs(spws_ctx* ctx, K* keys, double* vals, K* keys_alt, double* vals_alt, int64_t n, int begin_bit,
-                     int end_bit, bool* in_alt);
+                     int end_bit, bool* in_alt, const double* vals_first = nullptr);
 
 }  // namespace spws

@@ -29,7 +29,8 @@
 //   benchmark/TTM/COO_CSF_DCSR_hash_il.h:247), whose hash workspace is keyed
 //   on the output fiber (i,j).  Here the fibers are grouped by sorting:
 //     1. k_seg_owner (x2)   leaf -> fiber and fiber -> root owners.
-//     2. k_ttm_direct_keys  packed (i, j, k) key and value per nonzero.
+//     2. k_ttm_direct_keys  packed (i, j, k) key per nonzero (the sort reads
+//                           the values straight from the CSF).
 //     3. radix sort         stable, on the (i, j) bits only: each fiber's
 //                           nonzeros end up together and keep the CSF order,
 //                           increasing k -- the order hash_il sums them.
@@ -544,9 +545,8 @@ __global__ __launch_bounds__(256) void k_ttm_direct_keys(int64_t nnz, const int3
                                                          const int32_t* __restrict__ fiber_root,
                                                          const int32_t* __restrict__ l1_crd,
                                                          const int32_t* __restrict__ l2_crd,
-                                                         const int32_t* __restrict__ l3_crd,
-                                                         const double* __restrict__ xvals, int bj, int bk,
-                                                         uint64_t* __restrict__ keys, double* __restrict__ vals,
+                                                         const int32_t* __restrict__ l3_crd, int bj, int bk,
+                                                         uint64_t* __restrict__ keys,
                                                          const uint64_t* __restrict__ umask, uint64_t drop_i,
                                                          unsigned long long* __restrict__ dropped) {
   unsigned long long nd = 0;
@@ -560,7 +560,6 @@ __global__ __launch_bounds__(256) void k_ttm_direct_keys(int64_t nnz, const int3
       ++nd;
     }
     keys[e] = (i << (bj + bk)) | (j << bk) | k;
-    vals[e] = xvals[e];
   }
   if (kMask) {
     nd = wave_sum(nd);
@@ -656,7 +655,6 @@ static int mttkrp_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_
     if ((rc = csf_owners(ctx, x, &leaf_fiber, &fiber_root))) return rc;
     uint64_t* keys = ctx->keys[0].as<uint64_t>();
     double* vals = ctx->vals[0].as<double>();
-    SPWS_HIP(hipMemcpyAsync(vals, x.vals, (size_t)x.nnz * 8, hipMemcpyDeviceToDevice, s));
     const unsigned g = (unsigned)std::min<int64_t>(div_up(x.nnz, 256), 65536);
     if (masked) {
       SPWS_LAUNCH(ctx, "mttkrp_keys", k_mttkrp_keys<true>, dim3(g), dim3(256), 0, x.nnz, leaf_fiber, fiber_root,
@@ -668,7 +666,7 @@ static int mttkrp_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_
     }
     bool in_alt = false;
     if ((rc = radix_sort_pairs<uint64_t>(ctx, keys, vals, ctx->keys[1].as<uint64_t>(), ctx->vals[1].as<double>(),
-                                         x.nnz, bj + bk, bj + bk + bi, &in_alt)))
+                                         x.nnz, bj + bk, bj + bk + bi, &in_alt, x.vals)))
       return rc;
     if (in_alt) {
       keys = ctx->keys[1].as<uint64_t>();
@@ -816,17 +814,17 @@ static int ttm_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_t K
       dropped = ctx->obuf[6].as<unsigned long long>();
       SPWS_HIP(hipMemsetAsync(dropped, 0, 8, s));
       SPWS_LAUNCH(ctx, "ttm_keys", k_ttm_direct_keys<true>, dim3(g), dim3(256), 0, x.nnz, leaf_fiber, fiber_root,
-                  x.l1_crd, x.l2_crd, x.l3_crd, x.vals, bj, bk, nkeys, nvals, umask, (uint64_t)I, dropped);
+                  x.l1_crd, x.l2_crd, x.l3_crd, bj, bk, nkeys, umask, (uint64_t)I, dropped);
     } else {
       SPWS_LAUNCH(ctx, "ttm_keys", k_ttm_direct_keys<false>, dim3(g), dim3(256), 0, x.nnz, leaf_fiber, fiber_root,
-                  x.l1_crd, x.l2_crd, x.l3_crd, x.vals, bj, bk, nkeys, nvals, (const uint64_t*)nullptr, (uint64_t)0,
+                  x.l1_crd, x.l2_crd, x.l3_crd, bj, bk, nkeys, (const uint64_t*)nullptr, (uint64_t)0,
                   (unsigned long long*)nullptr);
     }
     // the sort covers (i, j) only: within an i slice the CSF order already
     // has each (i, j)'s nonzeros in increasing k, and the sort is stable
     bool alt = false;
     if ((rc = radix_sort_pairs<uint64_t>(ctx, nkeys, nvals, ctx->keys[1].as<uint64_t>(), ctx->vals[1].as<double>(),
-                                         x.nnz, bk, bi + bj + bk, &alt)))
+                                         x.nnz, bk, bi + bj + bk, &alt, x.vals)))
       return rc;
     if (alt) {
       nkeys = ctx->keys[1].as<uint64_t>();
