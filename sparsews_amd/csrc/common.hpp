@@ -238,6 +238,11 @@ __device__ __forceinline__ T wave_last(T v) {
   }
 }
 
+// Inclusive wavefront max-scan of unsigned values (every lane active).
+__device__ __forceinline__ uint32_t wave_incl_max(uint32_t v) {
+  return wave_incl_scan_op(v, 0u, [](uint32_t x, uint32_t y) { return max(x, y); });
+}
+
 // Integer wavefront scans and reductions (every lane active).
 template <class T>
 __device__ __forceinline__ T wave_incl_scan(T v) {

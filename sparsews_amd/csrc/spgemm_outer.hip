@@ -947,18 +947,6 @@ __host__ __device__ constexpr size_t fz_lds_bytes(int s, int d, bool numeric) {
          2 * (size_t)fz_list_len(s) + 4 * kWave + (numeric ? 8 * kWave : 0);
 }
 
-// Inclusive max-scan over the wavefront: row shifts by 1, 2, 4, 8 within
-// each row of 16, then the ends of rows 0 and 0-1 broadcast into the rows
-// after them (lanes without a source add 0).
-__device__ __forceinline__ uint32_t wave_incl_max(uint32_t v) {
-  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false));  // row_shr:1
-  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false));  // row_shr:2
-  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false));  // row_shr:4
-  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false));  // row_shr:8
-  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false));  // row_bcast:15
-  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false));  // row_bcast:31
-  return v;
-}
 
 // A batch: up to kFzU strides of one (window, entry chunk) of the row.
 struct FzBatch {
