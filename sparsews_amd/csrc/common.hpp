@@ -1,5 +1,6 @@
 // Shared device helpers and the per-context state of libspws.
 #pragma once
+#include <climits>
 #include <limits>
 #include <hip/hip_runtime.h>
 
@@ -259,37 +260,6 @@ __device__ __forceinline__ T wave_sum(T v) {
   return wave_last(wave_incl_scan(v));
 }
 
-// Wavefront bitonic sort, ascending ("flip" network: stage k first compares
-// e with e ^ (k - 1), then e with e ^ j for j = k/4 .. 1, so padding with
-// INT_MAX sorts any n).
-// R keys per lane, element e = r * 64 + lane.
-template <int R>
-__device__ __forceinline__ void wave_bitonic_step(int32_t (&v)[R], int d) {
-  const int lane = lane_id();
-  const int dl = d & (kWave - 1);
-  const int dr = d >> 6;
-  int32_t pv[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int32_t src = v[r ^ dr];
-    pv[r] = dl ? __shfl_xor(src, dl, kWave) : src;
-  }
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int e = r * kWave + lane;
-    const int p = e ^ d;
-    v[r] = (e < p) ? min(v[r], pv[r]) : max(v[r], pv[r]);
-  }
-}
-
-template <int R>
-__device__ __forceinline__ void wave_bitonic(int32_t (&v)[R], int n_pow2) {
-  for (int k = 2; k <= n_pow2; k <<= 1) {
-    wave_bitonic_step<R>(v, k - 1);
-    for (int j = k >> 2; j >= 1; j >>= 1) wave_bitonic_step<R>(v, j);
-  }
-}
-
 // v of lane (lane ^ D) without LDS: DPP quad permutes, half-row and row
 // mirrors and the row rotate by 8, and gfx950's 16- and 32-lane swaps
 // (each swap returns both halves exchanged; the lane keeps the half that
@@ -378,6 +348,59 @@ __device__ __forceinline__ void wave_sort64(uint32_t (&k)[R], int n_pow2) {
     wave_sort64_step<2>(k, b1);
     wave_sort64_step<1>(k, b0);
   }
+}
+
+// Wavefront bitonic sort, ascending ("flip" network: stage k first compares
+// e with e ^ (k - 1), then e with e ^ j for j = k/4 .. 1, so padding with
+// INT_MAX sorts any n).  R keys per lane, element e = r * 64 + lane; the
+// first n_pow2 elements (a power of two, at least 2) take part.  Every
+// distance is a compile-time constant: its lane part is a DPP move or lane
+// swap (lane_xor), its register part a renaming, and each comparator one
+// median of (key, partner, INT_MIN or INT_MAX).
+template <int D, int R>
+__device__ __forceinline__ void wave_bitonic_step(int32_t (&v)[R]) {
+  constexpr int DL = D & (kWave - 1), DR = D >> 6;
+  // the lower element of each pair (bit M of e clear) keeps the minimum
+  constexpr int M = (D & (D + 1)) == 0 ? (D + 1) >> 1 : D;  // flip distance 2^k - 1, or a power of two
+  const int lane = lane_id();
+  int32_t p[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int32_t src = v[r ^ DR];
+    if constexpr (DL != 0) p[r] = (int32_t)lane_xor<DL>((uint32_t)src);
+    else p[r] = src;
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const bool low = M < kWave ? (lane & M) == 0 : ((r << 6) & M) == 0;
+    const int32_t bound = low ? INT_MIN : INT_MAX;
+    v[r] = max(min(v[r], p[r]), min(max(v[r], p[r]), bound));
+  }
+}
+
+template <int J, int R>
+__device__ __forceinline__ void wave_bitonic_merge(int32_t (&v)[R]) {
+  if constexpr (J >= 1) {
+    wave_bitonic_step<J, R>(v);
+    wave_bitonic_merge<J / 2, R>(v);
+  }
+}
+
+template <int KK, int R>
+__device__ __forceinline__ void wave_bitonic_stages(int32_t (&v)[R], int n_pow2) {
+  if constexpr (KK <= kWave * R) {
+    if (n_pow2 >= KK) {  // uniform
+      wave_bitonic_step<KK - 1, R>(v);
+      wave_bitonic_merge<KK / 4, R>(v);
+      wave_bitonic_stages<KK * 2, R>(v, n_pow2);
+    }
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void wave_bitonic(int32_t (&v)[R], int n_pow2) {
+  static_assert(R == 1 || R == 2 || R == 4 || R == 8, "R keys per lane: a power of two up to 8");
+  wave_bitonic_stages<2, R>(v, n_pow2);
 }
 
 // Exclusive scan over a block of BLOCK threads; returns this thread's prefix
