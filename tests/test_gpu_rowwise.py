@@ -115,6 +115,47 @@ def test_short_row_path(ctx, m):
         ctx.set_profiling(False)
 
 
+def test_short_row_path_wide_columns(ctx):
+    """B with 2^26 + 1000 columns: a packed sort key (column and A-entry
+    index in 32 bits) no longer fits, so the short-row kernel sorts
+    (column, value) pairs instead.  B's columns come from a pool of about
+    400 values that includes 0, both sides of 2^26 and the last column, so
+    rows repeat columns and the run sums are exercised.  Two calls against
+    the single-threaded oracle (its dense workspace is one entry per
+    column), with the product count."""
+    print("case: short-row path, 3000 rows, 2^26 + 1000 columns", flush=True)
+    n = (1 << 26) + 1000
+    rng = np.random.default_rng(77)
+    pool = np.unique(np.concatenate([[0, 1, (1 << 26) - 2, (1 << 26) - 1, 1 << 26, n - 1],
+                                     rng.integers(0, n, 400)])).astype(np.int32)
+    kdim = 200
+    lengths = rng.integers(0, 9, kdim)
+    cols = [np.sort(rng.choice(pool, int(L), replace=False)) for L in lengths]
+    pos = np.zeros(kdim + 1, dtype=np.int32)
+    np.cumsum(lengths, out=pos[1:])
+    B = CSR(kdim, n, pos, np.concatenate(cols).astype(np.int32), rng.uniform(-1.0, 1.0, int(pos[-1])))
+    A = uniform_rows(3000, kdim, 0, 8, 57, 0)
+    ub = oracle.spgemm_row_ub(A, B)
+    assert ub.max() <= 64
+    G = oracle.spgemm_gustavson(A, B, nthreads=1)
+    assert int(G.crd.max()) >= 1 << 26
+    assert G.nnz < int(ub.sum())  # repeated columns within rows
+    exact, absum = oracle.spgemm_refsum(A, B, G.pos, G.crd)
+    try:
+        ctx.set_profiling(True)
+        for rep in range(2):
+            ctx.reset_kernel_times()
+            C = run(ctx, A, B)
+            np.testing.assert_array_equal(C.pos, G.pos)
+            np.testing.assert_array_equal(C.crd, G.crd)
+            assert oracle.max_value_error(C.vals, G.vals, absum) <= TOL
+            assert oracle.max_value_error(C.vals, exact, absum) <= TOL
+            assert ctx.stats()["products"] == int(ub.sum()), rep
+            assert "short_rows" in ctx.kernel_times(), rep
+    finally:
+        ctx.set_profiling(False)
+
+
 def test_short_rows_beyond_short_path(ctx):
     """300,000 short rows need more first-kernel wavefronts than the
     compaction sums: the short-row path declines before launching and the
