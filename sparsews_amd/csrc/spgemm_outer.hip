@@ -1023,21 +1023,27 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
     double a_val = 0.0;
     const int32_t* bs_row = bseg;
     int32_t seg_lo = 0, seg_hi = 0, nxt_hi = 0;
+    // Straight-line loads and selects (lanes past the chunk read entry 0 of
+    // the row and keep empty segments): conditional writes of this state
+    // compile to stores through a selected address in private memory.
+    // Past the row's last window the clamped reads give the end of the
+    // segment table's row, so nxt_hi then equals seg_hi.
     auto load_chunk = [&](int32_t c, int32_t w) {
       const int32_t q = c + lane;
-      if (q < E) {
-        b_start = rec_b[q_beg + q];
-        a_val = kNumeric ? rec_a[q_beg + q] : 0.0;
-        bs_row = bseg + (int64_t)rec_ja[q_beg + q] * (W + 1);
-        seg_lo = bs_row[min(w << d, W)];
-        seg_hi = bs_row[min((w + 1) << d, W)];
-        nxt_hi = (one_chunk && w + 2 <= Wd) ? bs_row[min((w + 2) << d, W)] : seg_hi;
-      } else {
-        b_start = 0;
-        a_val = 0.0;
-        bs_row = bseg;
-        seg_lo = seg_hi = nxt_hi = 0;
-      }
+      const bool in = q < E;
+      const int32_t e = q_beg + (in ? q : 0);
+      const int32_t rb = rec_b[e];
+      const double ra = kNumeric ? rec_a[e] : 0.0;
+      const int32_t* row = bseg + (int64_t)rec_ja[e] * (W + 1);
+      const int32_t lo = row[min(w << d, W)];
+      const int32_t hi = row[min((w + 1) << d, W)];
+      const int32_t nh = row[min((w + (one_chunk ? 2 : 1)) << d, W)];
+      b_start = in ? rb : 0;
+      a_val = in ? ra : 0.0;
+      bs_row = in ? row : bseg;
+      seg_lo = in ? lo : 0;
+      seg_hi = in ? hi : 0;
+      nxt_hi = in ? nh : 0;
     };
     // the segment scan of the current (window, chunk): base + q = B index of
     // product q for the lane owning it
@@ -1073,8 +1079,10 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
           if (one_chunk) {
             seg_lo = seg_hi;
             seg_hi = nxt_hi;
-            // the next window's end, a window ahead
-            if (lane < E && n.w + 2 <= Wd) nxt_hi = bs_row[min((n.w + 2) << d, W)];
+            // the next window's end, a window ahead (lanes past the row's
+            // entries read the table's first row and stay empty)
+            const int32_t nh = bs_row[min((n.w + 2) << d, W)];
+            nxt_hi = lane < E ? nh : 0;
           } else {
             load_chunk(0, n.w);
           }
