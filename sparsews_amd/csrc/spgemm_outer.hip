@@ -934,6 +934,8 @@ constexpr int kFzG = 4;  // numeric pass: strides whose owner searches overlap
 static_assert(kFzU % kFzG == 0, "whole groups of strides per batch");
 constexpr uint32_t kFzTagFirst = 32;  // (32 << 6) exceeds every window column (< 2^11)
 constexpr uint32_t kFzTagEnd = 1024;  // tags stay below 2^16 with the lane in the low 6 bits
+// integer compare predicates of __builtin_amdgcn_sicmp (LLVM's ICMP_SGT, SGE, SLE)
+constexpr int kIcmpSgt = 38, kIcmpSge = 39, kIcmpSle = 41;
 
 // Entries of the u16 area shared by the emitted column list (2^s) and the
 // owner marks of a batch.
@@ -944,7 +946,7 @@ __host__ __device__ constexpr int fz_list_len(int s) { return (1 << s) > kFzOwn 
 // symbolic: the seen bitmap, the marks, the B bases.
 __host__ __device__ constexpr size_t fz_lds_bytes(int s, int d, bool numeric) {
   return (numeric ? ((size_t)8 << s) + 2 * (((size_t)1 << s) >> 3) : (((size_t)1 << (s + d)) >> 3)) +
-         2 * (size_t)fz_list_len(s) + 4 * kWave + (numeric ? 8 * kWave : 0);
+         2 * (size_t)fz_list_len(s) + (numeric ? 16 : 4) * kWave;
 }
 
 
@@ -961,7 +963,26 @@ struct FzBatch {
 // to the other methods; otherwise every product goes into the window by an
 // LDS atomic add (ds_add_f64), with no conflict resolution -- the sums are
 // then reassociated within the 1e-12 bar and not reproducible bit for bit.
-template <bool kNumeric, bool kOrdered = true>
+// B's column and value at entry idx.  kOff32 (B has fewer than 2^29
+// entries): the byte offsets fit in 32 unsigned bits, and the loads take
+// the array base from scalar registers with a 32-bit offset per lane
+// instead of a sign-extended 64-bit address per lane.
+template <bool kOff32>
+__device__ __forceinline__ uint32_t fz_col(const int32_t* __restrict__ B2_crd, int32_t idx) {
+  if constexpr (kOff32)
+    return *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(B2_crd) + ((uint32_t)idx << 2));
+  else
+    return (uint32_t)B2_crd[idx];
+}
+template <bool kOff32>
+__device__ __forceinline__ double fz_val(const double* __restrict__ Bx, int32_t idx) {
+  if constexpr (kOff32)
+    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(Bx) + ((uint32_t)idx << 3));
+  else
+    return Bx[idx];
+}
+
+template <bool kNumeric, bool kOrdered = true, bool kOff32 = false>
 __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, int d,
                                                    const int32_t* __restrict__ rp_off,
                                                    const int32_t* __restrict__ rec_b,
@@ -986,8 +1007,12 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
   // marks (the two uses never overlap in time)
   uint16_t* list = reinterpret_cast<uint16_t*>(kNumeric ? sb + words : seen + words);
   uint16_t* own = list;
-  int32_t* s_base = reinterpret_cast<int32_t*>(list + fz_list_len(s));  // per lane: B index - product index
-  double* s_av = reinterpret_cast<double*>(s_base + kWave);              // numeric: per lane: A value
+  // per lane: B index - product index, and (numeric) the A value: one
+  // 16-byte record per lane {base, unused, A value}, so that one address
+  // serves both reads of an owner's record
+  constexpr int kRec = kNumeric ? 4 : 1;  // record length in int32
+  int32_t* s_base = reinterpret_cast<int32_t*>(list + fz_list_len(s));
+  double* s_av = reinterpret_cast<double*>(s_base) + 1;  // numeric: A value of lane t at s_av[2 t]
   for (int q = lane; q < words; q += kWave) {
     seen[q] = 0u;
     if (kNumeric) sb[q] = 0u;
@@ -1054,8 +1079,8 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
       excl = incl - len;
       base = b_start + seg_lo - excl;
       b.total = __builtin_amdgcn_readlane(incl, kWave - 1);
-      s_base[lane] = base;
-      if (kNumeric) s_av[lane] = a_val;
+      s_base[kRec * lane] = base;
+      if (kNumeric) s_av[2 * lane] = a_val;
     };
     // the batch after b (advancing chunks and windows; empty ones skipped)
     auto next = [&](const FzBatch& b) -> FzBatch {
@@ -1117,6 +1142,9 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
       // marks), or, before the stride's first mark, the segment that holds
       // the stride's start (the carry; a segment covering the whole stride
       // is that case).
+      // lanes holding a segment; with the compares below written as lane
+      // masks (sicmp) rather than ballots of a bool, which widen it first
+      const uint64_t live = __builtin_amdgcn_sicmp(len, 0, kIcmpSgt);
       if constexpr (kNumeric) {
         // strides in groups of kFzG: all the group's mark reads, then its
         // scans and base / A-value reads, then its B loads, so that the LDS
@@ -1128,7 +1156,7 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
 #pragma unroll
           for (int i = 0; i < kFzG; ++i) {
             const int32_t s0 = b.q0 + (g + i) * kWave;
-            const uint64_t before = __ballot(len > 0 && excl <= s0);  // nonzero: total > 0
+            const uint64_t before = live & __builtin_amdgcn_sicmp(excl, s0, kIcmpSle);  // nonzero: total > 0
             cr[i] = tag | (uint32_t)(63 - __builtin_clzll(before));
             mk[i] = own[(g + i) * kWave + lane];
           }
@@ -1137,8 +1165,8 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
 #pragma unroll
           for (int i = 0; i < kFzG; ++i) {
             const int o = (int)(max(wave_incl_max(mk[i]), cr[i]) & 63u);
-            bi[i] = s_base[o];
-            ai[i] = s_av[o];
+            bi[i] = s_base[kRec * o];
+            ai[i] = s_av[2 * o];
           }
 #pragma unroll
           for (int i = 0; i < kFzG; ++i) {
@@ -1146,8 +1174,8 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
             const int32_t s0 = b.q0 + u * kWave;  // the stride's first product (uniform)
             if (s0 < b.total) {
               const int32_t idx = bi[i] + min(s0 + lane, b.total - 1);
-              kc[slot][u] = (uint32_t)B2_crd[idx];
-              bv[slot][u] = Bx[idx];
+              kc[slot][u] = fz_col<kOff32>(B2_crd, idx);
+              bv[slot][u] = fz_val<kOff32>(Bx, idx);
               av[slot][u] = ai[i];
             }
           }
@@ -1161,17 +1189,18 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
           if (s0 >= b.total) break;             // apply stops at the same stride
           const int32_t q = min(s0 + lane, b.total - 1);
           const int32_t s_end = min(s0 + kWave, b.total);
-          const uint64_t cov = __ballot(len > 0 && excl <= s0 && excl + len >= s_end);
+          const uint64_t at_or_before = live & __builtin_amdgcn_sicmp(excl, s0, kIcmpSle);
+          const uint64_t cov = at_or_before & __builtin_amdgcn_sicmp(excl + len, s_end, kIcmpSge);
           int32_t idx;
           if (cov) {
             idx = __builtin_amdgcn_readlane(base, __builtin_ctzll(cov)) + q;
           } else {
-            const uint64_t before = __ballot(len > 0 && excl <= s0);
+            const uint64_t before = at_or_before;
             const uint32_t carry = tag | (uint32_t)(63 - __builtin_clzll(before));
             const int o = (int)(max(wave_incl_max(own[u * kWave + lane]), carry) & 63u);
             idx = s_base[o] + q;
           }
-          kc[slot][u] = (uint32_t)B2_crd[idx];
+          kc[slot][u] = fz_col<kOff32>(B2_crd, idx);
         }
       }
       // the A values read from LDS above are used only in apply: wait for
@@ -1683,7 +1712,7 @@ constexpr int kFzNumWavesPerCu = 8;   // k_fz_rows<true>: bounded by the LDS win
 // (exact row pointers, and an output allocated to the exact size), then the
 // numeric pass writing every entry in place.
 static int outer_fused(spws_ctx* ctx, const OuterArgs& a, const std::vector<int64_t>& pair_prefix,
-                       const int32_t* row_pairs, int s, bool ordered, int64_t* nnz_out) {
+                       const int32_t* row_pairs, int s, bool ordered, bool off32, int64_t* nnz_out) {
   hipStream_t st = ctx->stream;
   int rc;
   const int32_t m = a.m;
@@ -1766,10 +1795,14 @@ static int outer_fused(spws_ctx* ctx, const OuterArgs& a, const std::vector<int6
   // the symbolic walk's windows: 2^dsym segment-table windows each
   int dsym = 0;
   while (s + dsym < kFzSymWindowLog2 && ((int64_t)1 << dsym) < W) ++dsym;
-  SPWS_LAUNCH(ctx, "fz_count", k_fz_rows<false>, dim3(gsym), dim3(kWave), fz_lds_bytes(s, dsym, false), m, W, s,
-              dsym, rp_off,
-              rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, row_cnt, (const int64_t*)nullptr, (int32_t*)nullptr,
-              (double*)nullptr, work);
+  if (off32)
+    SPWS_LAUNCH(ctx, "fz_count", (k_fz_rows<false, true, true>), dim3(gsym), dim3(kWave), fz_lds_bytes(s, dsym, false),
+                m, W, s, dsym, rp_off, rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, row_cnt, (const int64_t*)nullptr,
+                (int32_t*)nullptr, (double*)nullptr, work);
+  else
+    SPWS_LAUNCH(ctx, "fz_count", (k_fz_rows<false, true, false>), dim3(gsym), dim3(kWave), fz_lds_bytes(s, dsym, false),
+                m, W, s, dsym, rp_off, rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, row_cnt, (const int64_t*)nullptr,
+                (int32_t*)nullptr, (double*)nullptr, work);
   if ((rc = exclusive_scan<int32_t, int64_t>(ctx, row_cnt, out_pos, m))) return rc;
   int64_t* hm = ctx->host_meta.as<int64_t>();
   SPWS_HIP(hipMemcpyAsync(hm, out_pos + m, 8, hipMemcpyDeviceToHost, st));
@@ -1784,14 +1817,19 @@ static int outer_fused(spws_ctx* ctx, const OuterArgs& a, const std::vector<int6
   // numeric pass: every entry written at its final position
   SPWS_HIP(hipMemsetAsync(work, 0, 4, st));
   const unsigned gnum = (unsigned)std::max<int64_t>(1, std::min<int64_t>(m, (int64_t)dev_cus * kFzNumWavesPerCu));
-  if (ordered)
-    SPWS_LAUNCH(ctx, "fz_reduce", (k_fz_rows<true, true>), dim3(gnum), dim3(kWave), fz_lds_bytes(s, 0, true), m, W,
-                s, 0, rp_off, rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, (int32_t*)nullptr, (const int64_t*)out_pos,
-                ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>(), work);
+#define SPWS_FZ_REDUCE(ORD, OFF)                                                                              \
+  SPWS_LAUNCH(ctx, "fz_reduce", (k_fz_rows<true, ORD, OFF>), dim3(gnum), dim3(kWave), fz_lds_bytes(s, 0, true), m, \
+              W, s, 0, rp_off, rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, (int32_t*)nullptr,                     \
+              (const int64_t*)out_pos, ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>(), work)
+  if (ordered && off32)
+    SPWS_FZ_REDUCE(true, true);
+  else if (ordered)
+    SPWS_FZ_REDUCE(true, false);
+  else if (off32)
+    SPWS_FZ_REDUCE(false, true);
   else
-    SPWS_LAUNCH(ctx, "fz_reduce", (k_fz_rows<true, false>), dim3(gnum), dim3(kWave), fz_lds_bytes(s, 0, true), m, W,
-                s, 0, rp_off, rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, (int32_t*)nullptr, (const int64_t*)out_pos,
-                ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>(), work);
+    SPWS_FZ_REDUCE(false, false);
+#undef SPWS_FZ_REDUCE
   SPWS_HIP(hipStreamSynchronize(st));
   *nnz_out = total;
   return SPWS_OK;
@@ -1828,9 +1866,11 @@ static int outer_run(spws_ctx* ctx, const OuterArgs& a, int64_t* nnz_out) {
   }
   std::vector<int64_t> ub(mm), prefix((size_t)m + 1, 0), pair_prefix((size_t)m + 1, 0);
   std::vector<int32_t> pairs(mm);
+  int32_t b_nnz = 0;  // B's entries (the fused method's offset width)
   if (m > 0) {
     SPWS_HIP(hipMemcpyAsync(ub.data(), ctx->row_ub.ptr, (size_t)m * 8, hipMemcpyDeviceToHost, s));
     SPWS_HIP(hipMemcpyAsync(pairs.data(), row_pairs, (size_t)m * 4, hipMemcpyDeviceToHost, s));
+    if (a.b_nrows > 0) SPWS_HIP(hipMemcpyAsync(&b_nnz, a.B2_pos + a.b_nrows, 4, hipMemcpyDeviceToHost, s));
     SPWS_HIP(hipStreamSynchronize(s));
   }
   int64_t busy_rows = 0;
@@ -1867,7 +1907,11 @@ static int outer_run(spws_ctx* ctx, const OuterArgs& a, int64_t* nnz_out) {
     method = (per_bucket >= 64.0 && W <= 4096 && (int64_t)max_pairs * W <= ((int64_t)1 << 28)) ? fused : 1;
   }
   if (method == 3 || method == 4) {
-    rc = outer_fused(ctx, a, pair_prefix, row_pairs, wlog, method == 3, nnz_out);
+    // 32-bit byte offsets into B's values while B has fewer than 2^29
+    // entries; SPWS_OUTER_WIDE_OFFSETS=1 takes the 64-bit addresses anyway
+    const char* wide = getenv("SPWS_OUTER_WIDE_OFFSETS");
+    const bool off32 = b_nnz < (1 << 29) && !(wide && wide[0] == '1');
+    rc = outer_fused(ctx, a, pair_prefix, row_pairs, wlog, method == 3, off32, nnz_out);
   } else if (method == 2) {
     rc = outer_chunks_bucket(ctx, a, prefix, pair_prefix, row_pairs, wlog, max_products,
                              (int32_t)std::min<int64_t>(user_rows, INT32_MAX), nnz_out);

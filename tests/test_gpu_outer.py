@@ -72,6 +72,26 @@ def test_outer_powerlaw(ctx, seed, method):
     check(C, A, B)
 
 
+@pytest.mark.parametrize("method", [3, 4])
+def test_outer_fused_wide_offsets(ctx, method, monkeypatch):
+    """The fused kernels address B with 32-bit byte offsets while B has
+    fewer than 2^29 entries; SPWS_OUTER_WIDE_OFFSETS=1 makes them take the
+    64-bit addresses that larger B needs.  Both variants against the
+    oracle, and against each other bit for bit (method 3, in order)."""
+    print(f"case: outer fused 64-bit offsets method {method}", flush=True)
+    A, B = powerlaw_rows(600, 500, 1.8, 300, 65, 0), powerlaw_rows(500, 700, 1.8, 300, 65, 1)
+    with outer_method(ctx, method):
+        C32 = run_outer(ctx, A, B)
+        monkeypatch.setenv("SPWS_OUTER_WIDE_OFFSETS", "1")
+        C64 = run_outer(ctx, A, B)
+        monkeypatch.delenv("SPWS_OUTER_WIDE_OFFSETS")
+        assert ctx.stats()["outer_method"] == method
+        check(C64, A, B)
+        check(C32, A, B)
+    if method == 3:
+        np.testing.assert_array_equal(C32.vals, C64.vals)
+
+
 @pytest.mark.parametrize("method", METHODS)
 def test_outer_dense_outer_levels(ctx, method):
     print(f"case: outer CSC x CSR method {method}", flush=True)
