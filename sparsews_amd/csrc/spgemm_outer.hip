@@ -937,16 +937,21 @@ constexpr uint32_t kFzTagEnd = 1024;  // tags stay below 2^16 with the lane in t
 // integer compare predicates of __builtin_amdgcn_sicmp (LLVM's ICMP_SGT, SGE, SLE)
 constexpr int kIcmpSgt = 38, kIcmpSge = 39, kIcmpSle = 41;
 
-// Entries of the u16 area shared by the emitted column list (2^s) and the
-// owner marks of a batch.
-__host__ __device__ constexpr int fz_list_len(int s) { return (1 << s) > kFzOwn ? (1 << s) : kFzOwn; }
+// Entries of the u16 area shared by the emitted column list and the owner
+// marks of a batch.  A window of 2^11 columns can hold more entries: its
+// emit then lists and writes lanes 0-31's columns, then lanes 32-63's (at
+// most 32 x 32 each).
+constexpr int kFzList = kFzOwn;
+static_assert(kFzList >= 32 * 32, "half a wavefront's bitmap words fit the list");
 
 // LDS of one k_fz_rows wavefront: numeric: the window's values, the seen
-// and stride bitmaps, the list / marks, the per-lane B bases and A values;
-// symbolic: the seen bitmap, the marks, the B bases.
-__host__ __device__ constexpr size_t fz_lds_bytes(int s, int d, bool numeric) {
-  return (numeric ? ((size_t)8 << s) + 2 * (((size_t)1 << s) >> 3) : (((size_t)1 << (s + d)) >> 3)) +
-         2 * (size_t)fz_list_len(s) + (numeric ? 16 : 4) * kWave;
+// bitmap, (ordered) the stride bitmap, the list / marks, one 16-byte record
+// per lane (B base, A value); symbolic: the seen bitmap, the marks, the B
+// bases.  The window of 2^11 columns takes 19,712 bytes (unordered), so
+// that eight wavefronts share a CU's 160 KiB.
+__host__ __device__ constexpr size_t fz_lds_bytes(int s, int d, bool numeric, bool ordered) {
+  return (numeric ? ((size_t)8 << s) + (ordered ? 2 : 1) * (((size_t)1 << s) >> 3) : (((size_t)1 << (s + d)) >> 3)) +
+         2 * (size_t)kFzList + (numeric ? 16 : 4) * kWave;
 }
 
 
@@ -959,10 +964,6 @@ struct FzBatch {
   bool win_end;   // last batch of the window
 };
 
-// kOrdered (numeric pass): entries summed in position order, bit-identical
-// to the other methods; otherwise every product goes into the window by an
-// LDS atomic add (ds_add_f64), with no conflict resolution -- the sums are
-// then reassociated within the 1e-12 bar and not reproducible bit for bit.
 // B's column and value at entry idx.  kOff32 (B has fewer than 2^29
 // entries): the byte offsets fit in 32 unsigned bits, and the loads take
 // the array base from scalar registers with a 32-bit offset per lane
@@ -982,6 +983,10 @@ __device__ __forceinline__ double fz_val(const double* __restrict__ Bx, int32_t 
     return Bx[idx];
 }
 
+// kOrdered (numeric pass): entries summed in position order, bit-identical
+// to the other methods; otherwise every product goes into the window by an
+// LDS atomic add (ds_add_f64), with no conflict resolution -- the sums are
+// then reassociated within the 1e-12 bar and not reproducible bit for bit.
 template <bool kNumeric, bool kOrdered = true, bool kOff32 = false>
 __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, int d,
                                                    const int32_t* __restrict__ rp_off,
@@ -1002,20 +1007,21 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
   const int lane = lane_id();
   double* acc = fz_lds;                                          // numeric: 2^s values
   uint32_t* seen = kNumeric ? reinterpret_cast<uint32_t*>(acc + S) : reinterpret_cast<uint32_t*>(fz_lds);
-  uint32_t* sb = seen + words;                                   // numeric: this stride's columns
+  constexpr bool kStrideMap = kNumeric && kOrdered;
+  uint32_t* sb = seen + words;                                   // ordered numeric: this stride's columns
   // numeric: the emitted columns of a window; both passes: a batch's owner
   // marks (the two uses never overlap in time)
-  uint16_t* list = reinterpret_cast<uint16_t*>(kNumeric ? sb + words : seen + words);
+  uint16_t* list = reinterpret_cast<uint16_t*>(kStrideMap ? sb + words : seen + words);
   uint16_t* own = list;
   // per lane: B index - product index, and (numeric) the A value: one
   // 16-byte record per lane {base, unused, A value}, so that one address
   // serves both reads of an owner's record
   constexpr int kRec = kNumeric ? 4 : 1;  // record length in int32
-  int32_t* s_base = reinterpret_cast<int32_t*>(list + fz_list_len(s));
+  int32_t* s_base = reinterpret_cast<int32_t*>(list + kFzList);
   double* s_av = reinterpret_cast<double*>(s_base) + 1;  // numeric: A value of lane t at s_av[2 t]
   for (int q = lane; q < words; q += kWave) {
     seen[q] = 0u;
-    if (kNumeric) sb[q] = 0u;
+    if (kStrideMap) sb[q] = 0u;
   }
   if (kNumeric)
     for (int q = lane; q < S; q += kWave) acc[q] = -0.0;
@@ -1271,30 +1277,43 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
       const int incl = wave_incl_scan(cnt);
       const int total = __builtin_amdgcn_readlane(incl, kWave - 1);
       if (kNumeric && total > 0 && out0 + run + total <= out_end) {
-        int rr = incl - cnt;
-        for (uint32_t bits = myw; bits; bits &= bits - 1) list[rr++] = (uint16_t)(lane * 32 + __builtin_ctz(bits));
-        __builtin_amdgcn_wave_barrier();
-        int32_t* oc = C_crd + out0 + run;
-        double* ov = C_val + out0 + run;
+        // at most kFzList entries are listed at a time: a denser window is
+        // listed and written in two phases (uniform), lanes 0-31's columns
+        // (the entries before split) and then lanes 32-63's
+        const int split = __builtin_amdgcn_readlane(incl, 31);
+        const int phases = total > kFzList ? 2 : 1;
         const int32_t wb = b.w << s;
-        for (int r0 = 0; r0 < total; r0 += 4 * kWave) {
-          uint32_t kq[4];
-          double aq[4];
+        for (int h = 0; h < phases; ++h) {
+          const int lo = h == 0 ? 0 : split;  // the phase's first entry of the window
+          const int n = phases == 1 ? total : (h == 0 ? split : total - split);
+          if (phases == 1 || (lane >> 5) == h) {
+            int rr = incl - cnt - lo;
+            for (uint32_t bits = myw; bits; bits &= bits - 1)
+              list[rr++] = (uint16_t)(lane * 32 + __builtin_ctz(bits));
+          }
+          __builtin_amdgcn_wave_barrier();
+          int32_t* oc = C_crd + out0 + run + lo;
+          double* ov = C_val + out0 + run + lo;
+          for (int r0 = 0; r0 < n; r0 += 4 * kWave) {
+            uint32_t kq[4];
+            double aq[4];
 #pragma unroll
-          for (int i = 0; i < 4; ++i) kq[i] = list[min(r0 + i * kWave + lane, total - 1)];
+            for (int i = 0; i < 4; ++i) kq[i] = list[min(r0 + i * kWave + lane, n - 1)];
 #pragma unroll
-          for (int i = 0; i < 4; ++i) aq[i] = acc[kq[i]];
+            for (int i = 0; i < 4; ++i) aq[i] = acc[kq[i]];
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int q = r0 + i * kWave + lane;
-            if (q < total) {
-              oc[q] = wb + (int32_t)kq[i];
-              ov[q] = aq[i];
-              acc[kq[i]] = -0.0;
+            for (int i = 0; i < 4; ++i) {
+              const int q = r0 + i * kWave + lane;
+              if (q < n) {
+                oc[q] = wb + (int32_t)kq[i];
+                ov[q] = aq[i];
+                acc[kq[i]] = -0.0;
+              }
             }
           }
+          // the next phase's list writes follow this phase's reads
+          __builtin_amdgcn_wave_barrier();
         }
-        __builtin_amdgcn_wave_barrier();
         // nothing of the emit stays in flight: without this, a list or
         // value read on a path with no lane left to store it counts as
         // pending at every stride of the next batch
@@ -1796,12 +1815,12 @@ static int outer_fused(spws_ctx* ctx, const OuterArgs& a, const std::vector<int6
   int dsym = 0;
   while (s + dsym < kFzSymWindowLog2 && ((int64_t)1 << dsym) < W) ++dsym;
   if (off32)
-    SPWS_LAUNCH(ctx, "fz_count", (k_fz_rows<false, true, true>), dim3(gsym), dim3(kWave), fz_lds_bytes(s, dsym, false),
-                m, W, s, dsym, rp_off, rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, row_cnt, (const int64_t*)nullptr,
+    SPWS_LAUNCH(ctx, "fz_count", (k_fz_rows<false, true, true>), dim3(gsym), dim3(kWave),
+                fz_lds_bytes(s, dsym, false, false), m, W, s, dsym, rp_off, rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, row_cnt, (const int64_t*)nullptr,
                 (int32_t*)nullptr, (double*)nullptr, work);
   else
-    SPWS_LAUNCH(ctx, "fz_count", (k_fz_rows<false, true, false>), dim3(gsym), dim3(kWave), fz_lds_bytes(s, dsym, false),
-                m, W, s, dsym, rp_off, rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, row_cnt, (const int64_t*)nullptr,
+    SPWS_LAUNCH(ctx, "fz_count", (k_fz_rows<false, true, false>), dim3(gsym), dim3(kWave),
+                fz_lds_bytes(s, dsym, false, false), m, W, s, dsym, rp_off, rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, row_cnt, (const int64_t*)nullptr,
                 (int32_t*)nullptr, (double*)nullptr, work);
   if ((rc = exclusive_scan<int32_t, int64_t>(ctx, row_cnt, out_pos, m))) return rc;
   int64_t* hm = ctx->host_meta.as<int64_t>();
@@ -1818,7 +1837,7 @@ static int outer_fused(spws_ctx* ctx, const OuterArgs& a, const std::vector<int6
   SPWS_HIP(hipMemsetAsync(work, 0, 4, st));
   const unsigned gnum = (unsigned)std::max<int64_t>(1, std::min<int64_t>(m, (int64_t)dev_cus * kFzNumWavesPerCu));
 #define SPWS_FZ_REDUCE(ORD, OFF)                                                                              \
-  SPWS_LAUNCH(ctx, "fz_reduce", (k_fz_rows<true, ORD, OFF>), dim3(gnum), dim3(kWave), fz_lds_bytes(s, 0, true), m, \
+  SPWS_LAUNCH(ctx, "fz_reduce", (k_fz_rows<true, ORD, OFF>), dim3(gnum), dim3(kWave), fz_lds_bytes(s, 0, true, ORD), m, \
               W, s, 0, rp_off, rec_b, rec_ja, rec_a, bseg, a.B2_crd, a.Bx, (int32_t*)nullptr,                     \
               (const int64_t*)out_pos, ctx->out_crd.as<int32_t>(), ctx->out_val.as<double>(), work)
   if (ordered && off32)

@@ -92,6 +92,26 @@ def test_outer_fused_wide_offsets(ctx, method, monkeypatch):
         np.testing.assert_array_equal(C32.vals, C64.vals)
 
 
+@pytest.mark.parametrize("method", [3, 4])
+def test_outer_fused_dense_windows(ctx, method):
+    """Windows of 2^11 columns holding more entries than the emit lists at
+    once (1,024): those windows are listed and written in two phases, lanes
+    0-31's columns then lanes 32-63's.  Rows of every density, so that
+    single-phase and two-phase windows (and phases with no entries) mix."""
+    print(f"case: outer fused dense windows method {method}", flush=True)
+    A = uniform_rows(300, 400, 0, 30, 66, 0)
+    B = uniform_rows(400, 4096, 0, 400, 66, 1)
+    G = oracle.spgemm_gustavson(A, B)
+    per_window = np.bincount(np.repeat(np.arange(A.nrows), np.diff(G.pos)) * 2 + G.crd // 2048,
+                             minlength=2 * A.nrows)
+    assert (per_window > 1024).sum() > 50 and ((per_window > 0) & (per_window <= 1024)).sum() > 50
+    with outer_method(ctx, method, 11):
+        C = run_outer(ctx, A, B)
+        assert ctx.stats()["outer_method"] == method
+        assert ctx.stats()["window_bits"] == 11
+        check(C, A, B)
+
+
 @pytest.mark.parametrize("method", METHODS)
 def test_outer_dense_outer_levels(ctx, method):
     print(f"case: outer CSC x CSR method {method}", flush=True)
