@@ -928,7 +928,8 @@ __global__ __launch_bounds__(256) void k_fz_records(int32_t En, const double* __
   }
 }
 
-constexpr int kFzU = 16;  // strides of 64 products per batch
+constexpr int kFzU = 16;  // strides of 64 products per batch (numeric pass)
+constexpr int kFzUSym = 8;  // the symbolic pass's (fewer registers, more wavefronts)
 constexpr int kFzOwn = kFzU * kWave;  // owner marks of a batch (u16 each)
 constexpr int kFzG = 4;  // numeric pass: strides whose owner searches overlap
 static_assert(kFzU % kFzG == 0, "whole groups of strides per batch");
@@ -999,6 +1000,10 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
                                                    const int64_t* __restrict__ C_pos, int32_t* __restrict__ C_crd,
                                                    double* __restrict__ C_val, int32_t* __restrict__ work) {
   extern __shared__ double fz_lds[];
+  // strides per batch, and the owner marks of a batch
+  constexpr int kU = kNumeric ? kFzU : kFzUSym;
+  constexpr int kOwn = kU * kWave;
+  static_assert(kOwn <= kFzList && kU % kFzG == 0, "a batch's marks fit the list area");
   // the walk's windows: 2^d segment-table windows each (the symbolic pass
   // only counts, so it takes wide windows: fewer per-window steps); the
   // numeric pass keeps d = 0 (its emit holds one bitmap word per lane)
@@ -1025,7 +1030,7 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
   }
   if (kNumeric)
     for (int q = lane; q < S; q += kWave) acc[q] = -0.0;
-  for (int q = lane; q < kFzOwn; q += kWave) own[q] = 0;
+  for (int q = lane; q < kOwn; q += kWave) own[q] = 0;
   __builtin_amdgcn_wave_barrier();
   // owner marks carry a batch tag (tagc << 6 | lane) above every column
   // value the list holds (< 2^11 = 32 << 6), so marks of earlier batches
@@ -1091,9 +1096,9 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
     // the batch after b (advancing chunks and windows; empty ones skipped)
     auto next = [&](const FzBatch& b) -> FzBatch {
       FzBatch n = b;
-      if (b.q0 + kWave * kFzU < b.total) {
-        n.q0 = b.q0 + kWave * kFzU;
-        n.win_end = (n.q0 + kWave * kFzU >= n.total) && (n.c + kWave >= E);
+      if (b.q0 + kWave * kU < b.total) {
+        n.q0 = b.q0 + kWave * kU;
+        n.win_end = (n.q0 + kWave * kU >= n.total) && (n.c + kWave >= E);
         return n;
       }
       for (;;) {
@@ -1120,27 +1125,27 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
         }
         n.q0 = 0;
         scan(n);
-        n.win_end = (kWave * kFzU >= n.total) && (n.c + kWave >= E);
+        n.win_end = (kWave * kU >= n.total) && (n.c + kWave >= E);
         // an empty (window, chunk) still closes its window when it is the last chunk
         if (n.total > 0 || n.c + kWave >= E) return n;
       }
     };
     // loads of one batch: column and value (numeric) of each lane's product
-    uint32_t kc[2][kFzU];
-    double bv[2][kFzU], av[2][kFzU];
+    uint32_t kc[2][kU];
+    double bv[2][kU], av[2][kU];
     auto load = [&](const FzBatch& b, int slot) {
       const int32_t len = seg_hi - seg_lo;
       // this batch's owner marks: each segment that starts inside the
       // batch marks its first product with its lane
       if (tagc >= kFzTagEnd) {
-        for (int q = lane; q < kFzOwn; q += kWave) own[q] = 0;
+        for (int q = lane; q < kOwn; q += kWave) own[q] = 0;
         tagc = kFzTagFirst;
       }
       const uint32_t tag = tagc << 6;
       ++tagc;
       {
         const int32_t rel = excl - b.q0;
-        if (len > 0 && rel >= 0 && rel < kFzOwn) own[rel] = (uint16_t)(tag | (uint32_t)lane);
+        if (len > 0 && rel >= 0 && rel < kOwn) own[rel] = (uint16_t)(tag | (uint32_t)lane);
       }
       __builtin_amdgcn_wave_barrier();
       // The owner of product q is the last lane t with excl_t <= q: the
@@ -1156,7 +1161,7 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
         // scans and base / A-value reads, then its B loads, so that the LDS
         // round trips of a group overlap
 #pragma unroll
-        for (int g = 0; g < kFzU; g += kFzG) {
+        for (int g = 0; g < kU; g += kFzG) {
           if (b.q0 + g * kWave >= b.total) break;  // uniform; apply stops at the same stride
           uint32_t mk[kFzG], cr[kFzG];
 #pragma unroll
@@ -1190,7 +1195,7 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
         // the symbolic walk's wide windows make covering segments common:
         // they take the owner from registers, the rest search the marks
 #pragma unroll
-        for (int u = 0; u < kFzU; ++u) {
+        for (int u = 0; u < kU; ++u) {
           const int32_t s0 = b.q0 + u * kWave;  // the stride's first product (uniform)
           if (s0 >= b.total) break;             // apply stops at the same stride
           const int32_t q = min(s0 + lane, b.total - 1);
@@ -1218,7 +1223,7 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
     auto apply = [&](const FzBatch& b, int slot) {
       const uint32_t wmask = (uint32_t)S - 1;
 #pragma unroll
-      for (int u = 0; u < kFzU; ++u) {
+      for (int u = 0; u < kU; ++u) {
         if (b.q0 + u * kWave >= b.total) break;
         const bool ok = b.q0 + u * kWave + lane < b.total;
         const uint32_t k = kc[slot][u] & wmask;
@@ -1328,7 +1333,7 @@ __global__ __launch_bounds__(kWave) void k_fz_rows(int32_t m, int32_t W, int s, 
     c0.q0 = 0;
     load_chunk(0, 0);
     scan(c0);
-    c0.win_end = (kWave * kFzU >= c0.total) && (kWave >= E);
+    c0.win_end = (kWave * kU >= c0.total) && (kWave >= E);
     load(c0, 0);
     while (c0.w >= 0) {
       vm_drain();
@@ -1721,7 +1726,7 @@ static int outer_chunks_bucket(spws_ctx* ctx, const OuterArgs& a, const std::vec
   return SPWS_OK;
 }
 
-constexpr int kFzSymWavesPerCu = 16;  // k_fz_rows<false>: one wave per workgroup, small LDS
+constexpr int kFzSymWavesPerCu = 24;  // k_fz_rows<false>: one wave per workgroup, 76 VGPRs, 6.4 KB of LDS
 constexpr int kFzSymWindowLog2 = 15;  // widest window of the symbolic walk (a 4 KiB bitmap)
 constexpr int kFzNumWavesPerCu = 8;   // k_fz_rows<true>: bounded by the LDS window
 
