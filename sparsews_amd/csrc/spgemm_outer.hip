@@ -942,8 +942,8 @@ constexpr int kIcmpSgt = 38, kIcmpSge = 39, kIcmpSle = 41;
 // marks of a batch.  A window of 2^11 columns can hold more entries: its
 // emit then lists and writes lanes 0-31's columns, then lanes 32-63's (at
 // most 32 x 32 each).
-constexpr int kFzList = kFzOwn;
-static_assert(kFzList >= 32 * 32, "half a wavefront's bitmap words fit the list");
+constexpr int kFzList = 32 * 32;
+static_assert(kFzList >= kFzOwn, "a batch's owner marks fit the list area");
 
 // LDS of one k_fz_rows wavefront: numeric: the window's values, the seen
 // bitmap, (ordered) the stride bitmap, the list / marks, one 16-byte record
