@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256) void k_mttkrp_keys(int64_t nnz, const int32_t*
 // before any is used, so their latencies overlap; the sums still run in
 // bucket order.
 constexpr int kAccBlock = 256;
-constexpr int kAccU = 8;
+constexpr int kAccU = 4;
 
 // kMask: per row, the OR of its nonzeros' shared stored factor columns
 // (mB[j] & mC[k]) is or-ed into rmask[i], the row's stored output columns.
