@@ -375,7 +375,7 @@ __global__ __launch_bounds__(256) void k_ttm_heads(int64_t nnz, const uint64_t* 
 constexpr int kTtmChunk = 128;
 constexpr int kTtmLanes = 16;  // one DPP row per group (broadcasts by row_newbcast)
 constexpr int kTtmU = 16;
-constexpr int kTtmBlocksPerCu = 4;  // workgroups per CU in the accumulation grid (4 waves per SIMD)
+constexpr int kTtmBlocksPerCu = 5;  // workgroups per CU in the accumulation grid (5 waves per SIMD: 96 VGPRs)
 
 __global__ __launch_bounds__(256) void k_ttm_chunks(int64_t F, const int32_t* __restrict__ foff,
                                                     int32_t* __restrict__ nchunk) {
