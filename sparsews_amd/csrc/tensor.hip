@@ -579,30 +579,83 @@ __global__ __launch_bounds__(256) void k_ttm_fiber_mask(int64_t F, const int32_t
   }
 }
 
-// flag[p] = 1 where sorted nonzero p starts a fiber ((i, j) differs from p - 1)
-__global__ __launch_bounds__(256) void k_ttm_fiber_flags(int64_t nnz, const uint64_t* __restrict__ keys, int bk,
-                                                         int32_t* __restrict__ flag) {
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x)
-    flag[p] = (p == 0 || (keys[p - 1] >> bk) != (keys[p] >> bk)) ? 1 : 0;
+// Fiber starts of the sorted nonzeros (p starts a fiber when its (i, j)
+// differs from p - 1's), in tiles of kFibTile: a count per tile, then (with
+// the counts' prefix) each tile's starts ranked in order, with no flag or
+// rank array per nonzero.  A tile is walked in kFibItems coalesced rounds of
+// kFibBlock nonzeros; ballots give the ranks within a round's wavefronts.
+constexpr int kFibBlock = 256;
+constexpr int kFibItems = 16;
+constexpr int kFibTile = kFibBlock * kFibItems;
+constexpr int kFibWaves = kFibBlock / kWave;
+
+// Whether nonzero p starts a fiber (false past the end).
+__device__ __forceinline__ bool ttm_fiber_start(int64_t nnz, const uint64_t* __restrict__ keys, int bk, int64_t p) {
+  if (p >= nnz) return false;
+  return p == 0 || (keys[p - 1] >> bk) != (keys[p] >> bk);
 }
 
-// With rank = exclusive prefix of the flags (rank[nnz] = fiber count F):
-// fiber rank[p] starts at p; its coordinates come from the key.
-__global__ __launch_bounds__(256) void k_ttm_fiber_starts(int64_t nnz, const uint64_t* __restrict__ keys, int bj,
-                                                          int bk, const int32_t* __restrict__ flag,
-                                                          const int32_t* __restrict__ rank, int32_t* __restrict__ foff,
-                                                          int32_t* __restrict__ out_i, int32_t* __restrict__ out_j) {
+__global__ __launch_bounds__(kFibBlock) void k_ttm_fiber_count(int64_t nnz, const uint64_t* __restrict__ keys, int bk,
+                                                                int32_t* __restrict__ tile_cnt) {
+  __shared__ int32_t wsum[kFibWaves];
+  const int64_t base = (int64_t)blockIdx.x * kFibTile + threadIdx.x;
+  int32_t c = 0;
+#pragma unroll
+  for (int it = 0; it < kFibItems; ++it) c += __popcll(__ballot(ttm_fiber_start(nnz, keys, bk, base + it * kFibBlock)));
+  if (lane_id() == 0) wsum[threadIdx.x / kWave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t t = 0;
+#pragma unroll
+    for (int w = 0; w < kFibWaves; ++w) t += wsum[w];
+    tile_cnt[blockIdx.x] = t;
+  }
+}
+
+// tile_off = exclusive prefix of the tile counts (tile_off[ntiles] = F):
+// fiber q starts at foff[q]; its coordinates come from the key.
+__global__ __launch_bounds__(kFibBlock) void k_ttm_fiber_starts(int64_t nnz, const uint64_t* __restrict__ keys, int bj,
+                                                                int bk, const int32_t* __restrict__ tile_off,
+                                                                int64_t ntiles, int32_t* __restrict__ foff,
+                                                                int32_t* __restrict__ out_i,
+                                                                int32_t* __restrict__ out_j) {
+  // starts per (round, wavefront), then their exclusive prefix in that order
+  __shared__ int32_t cnt[kFibItems * kFibWaves];
+  static_assert(kFibItems * kFibWaves == kWave, "one wavefront scans the per-round counts");
   const uint64_t jm = (1ull << bj) - 1;
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
-    if (flag[p]) {
-      const int32_t q = rank[p];
+  const int lane = lane_id();
+  const int w = threadIdx.x / kWave;
+  const int64_t base = (int64_t)blockIdx.x * kFibTile + threadIdx.x;
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+  uint32_t mine = 0;  // bit it: this thread's nonzero of round it starts a fiber
+  int32_t before[kFibItems];  // starts of this wavefront's lanes below this one, per round
+#pragma unroll
+  for (int it = 0; it < kFibItems; ++it) {
+    const bool st = ttm_fiber_start(nnz, keys, bk, base + it * kFibBlock);
+    const uint64_t m = __ballot(st);
+    mine |= (st ? 1u : 0u) << it;
+    before[it] = __popcll(m & below);
+    if (lane == 0) cnt[it * kFibWaves + w] = __popcll(m);
+  }
+  __syncthreads();
+  if (w == 0) {
+    const int32_t v = cnt[lane];
+    cnt[lane] = wave_incl_scan(v) - v;
+  }
+  __syncthreads();
+  const int32_t t0 = tile_off[blockIdx.x];
+#pragma unroll
+  for (int it = 0; it < kFibItems; ++it) {
+    if ((mine >> it) & 1u) {
+      const int64_t p = base + it * kFibBlock;
+      const int32_t q = t0 + cnt[it * kFibWaves + w] + before[it];
       const uint64_t key = keys[p];
       foff[q] = (int32_t)p;
       out_i[q] = (int32_t)(key >> (bj + bk));
       out_j[q] = (int32_t)((key >> bk) & jm);
     }
-    if (p == 0) foff[rank[nnz]] = (int32_t)nnz;
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0) foff[tile_off[ntiles]] = (int32_t)nnz;
 }
 
 static int csf_owners(spws_ctx* ctx, const Csf3& x, int32_t** leaf_fiber, int32_t** fiber_root) {
@@ -838,10 +891,14 @@ static int ttm_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_t K
       SPWS_HIP(hipStreamSynchronize(s));
       kept = x.nnz - *hd;
     }
-    // fiber flags and their prefix (flag / off hold nnz + 1 entries here)
-    if (kept > 0) SPWS_LAUNCH(ctx, "ttm_flags", k_ttm_fiber_flags, dim3(g), dim3(256), 0, kept, nkeys, bk, flag);
-    if ((rc = exclusive_scan<int32_t, int32_t>(ctx, flag, off, kept))) return rc;
-    SPWS_HIP(hipMemcpyAsync(hm, off + kept, 4, hipMemcpyDeviceToHost, s));
+    // fiber starts counted per tile and the counts' prefix (flag / off hold
+    // nnz + 1 entries here, more than the tiles)
+    const int64_t ftiles = div_up(kept, kFibTile);
+    if (kept > 0)
+      SPWS_LAUNCH(ctx, "ttm_fibcount", k_ttm_fiber_count, dim3((unsigned)ftiles), dim3(kFibBlock), 0, kept, nkeys, bk,
+                  flag);
+    if ((rc = exclusive_scan<int32_t, int32_t>(ctx, flag, off, ftiles))) return rc;
+    SPWS_HIP(hipMemcpyAsync(hm, off + ftiles, 4, hipMemcpyDeviceToHost, s));
     SPWS_HIP(hipStreamSynchronize(s));
     F = *hm;
     if ((rc = ctx->out_crd.ensure((size_t)(F > 0 ? F : 1) * 4 * 2)) ||
@@ -852,8 +909,8 @@ static int ttm_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_t K
     int32_t* out_j = out_i + F;
     int32_t* foff = ctx->obuf[5].as<int32_t>();
     if (kept > 0) {
-      SPWS_LAUNCH(ctx, "ttm_starts", k_ttm_fiber_starts, dim3(g), dim3(256), 0, kept, nkeys, bj, bk, flag, off, foff,
-                  out_i, out_j);
+      SPWS_LAUNCH(ctx, "ttm_starts", k_ttm_fiber_starts, dim3((unsigned)ftiles), dim3(kFibBlock), 0, kept, nkeys, bj,
+                  bk, off, ftiles, foff, out_i, out_j);
       if ((rc = ttm_accumulate(ctx, kept, F, R, bk, nkeys, nvals, foff, U, ctx->out_val.as<double>()))) return rc;
     }
     if (masked && F > 0) {
