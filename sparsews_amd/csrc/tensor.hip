@@ -61,36 +61,40 @@ constexpr int kOwnItems = 8;
 constexpr int kOwnTile = kOwnBlock * kOwnItems;
 
 // owner[e] = s for e in [pos[s], pos[s+1]), s in [0, nseg); pos[nseg] = nitems.
-// One tile of items per workgroup; the owners of the tile's first and last
-// item by binary search, then per item a binary search in the LDS slice.
-__global__ __launch_bounds__(kOwnBlock) void k_seg_owner(const int32_t* __restrict__ pos, int32_t nseg, int64_t nitems,
+// First the owner of every tile's first item (one thread per tile, a binary
+// search over pos: the last s with pos[s] <= e, which is the non-empty
+// segment holding e); then one tile of items per workgroup, each item's
+// owner by a binary search in the LDS slice of pos between its tile's first
+// owner and the next tile's (the owner of the tile's last item is at most
+// the latter).
+__global__ __launch_bounds__(256) void k_seg_tile_first(const int32_t* __restrict__ pos, int32_t nseg, int64_t nitems,
+                                                        int64_t ntiles, int32_t* __restrict__ first) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > ntiles) return;
+  if (t == ntiles) {
+    first[t] = nseg - 1;
+    return;
+  }
+  const int64_t x = t * kOwnTile;
+  int32_t lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (pos[mid] <= x) lo = mid; else hi = mid - 1;
+  }
+  first[t] = lo;
+}
+
+__global__ __launch_bounds__(kOwnBlock) void k_seg_owner(const int32_t* __restrict__ pos, int64_t nitems,
+                                                         const int32_t* __restrict__ first,
                                                          int32_t* __restrict__ owner) {
   __shared__ int32_t spos[kOwnTile + 1];
-  __shared__ int32_t s_lo, s_span;
   const int64_t e0 = (int64_t)blockIdx.x * kOwnTile;
   const int64_t e_end = e0 + kOwnTile < nitems ? e0 + kOwnTile : nitems;
-  if (threadIdx.x == 0) {
-    // last s with pos[s] <= x and pos[s+1] > x: search the last s with pos[s] <= x
-    // among non-empty segments (pos is non-decreasing)
-    auto last_le = [&](int64_t x) {
-      int32_t lo = 0, hi = nseg - 1;
-      while (lo < hi) {
-        const int32_t mid = (lo + hi + 1) >> 1;
-        if (pos[mid] <= x) lo = mid; else hi = mid - 1;
-      }
-      return lo;
-    };
-    const int32_t a = last_le(e0);
-    const int32_t b = last_le(e_end - 1);
-    s_lo = a;
-    s_span = b - a + 1;
-  }
-  __syncthreads();
-  const int32_t lo0 = s_lo;
-  const int32_t span = s_span;
+  const int32_t lo0 = first[blockIdx.x];
+  const int32_t span = first[blockIdx.x + 1] - lo0 + 1;  // candidate owners lo0 .. lo0 + span - 1
   const bool fits = span <= kOwnTile;
   if (fits)
-    for (int i = threadIdx.x; i <= span; i += kOwnBlock) spos[i] = pos[lo0 + i];
+    for (int i = threadIdx.x; i < span; i += kOwnBlock) spos[i] = pos[lo0 + i];
   __syncthreads();
   for (int it = 0; it < kOwnItems; ++it) {
     const int64_t e = e0 + (int64_t)it * kOwnBlock + threadIdx.x;
@@ -111,10 +115,14 @@ __global__ __launch_bounds__(kOwnBlock) void k_seg_owner(const int32_t* __restri
   }
 }
 
-static int seg_owner(spws_ctx* ctx, const int32_t* pos, int32_t nseg, int64_t nitems, int32_t* owner) {
+// first: ntiles + 1 entries of scratch (ntiles = tiles of kOwnTile items).
+static int seg_owner(spws_ctx* ctx, const int32_t* pos, int32_t nseg, int64_t nitems, int32_t* owner,
+                     int32_t* first) {
   if (nitems <= 0) return SPWS_OK;
-  SPWS_LAUNCH(ctx, "seg_owner", k_seg_owner, dim3((unsigned)div_up(nitems, kOwnTile)), dim3(kOwnBlock), 0, pos,
-              nseg, nitems, owner);
+  const int64_t ntiles = div_up(nitems, kOwnTile);
+  SPWS_LAUNCH(ctx, "seg_owner", k_seg_tile_first, dim3((unsigned)div_up(ntiles + 1, 256)), dim3(256), 0, pos, nseg,
+              nitems, ntiles, first);
+  SPWS_LAUNCH(ctx, "seg_owner", k_seg_owner, dim3((unsigned)ntiles), dim3(kOwnBlock), 0, pos, nitems, first, owner);
   return SPWS_OK;
 }
 
@@ -660,13 +668,18 @@ __global__ __launch_bounds__(kFibBlock) void k_ttm_fiber_starts(int64_t nnz, con
 
 static int csf_owners(spws_ctx* ctx, const Csf3& x, int32_t** leaf_fiber, int32_t** fiber_root) {
   int rc;
-  if ((rc = ctx->obuf[0].ensure((size_t)(x.nnz > 0 ? x.nnz : 1) * 4)) ||
+  // obuf[0]: the nonzeros' fibers, then the tile owners both passes use in
+  // turn (nfibers <= nnz)
+  const size_t nz = (size_t)(x.nnz > 0 ? x.nnz : 1);
+  const size_t tiles = (size_t)div_up((int64_t)nz, kOwnTile) + 1;
+  if ((rc = ctx->obuf[0].ensure((nz + tiles) * 4)) ||
       (rc = ctx->obuf[1].ensure((size_t)(x.nfibers > 0 ? x.nfibers : 1) * 4)))
     return rc;
   *leaf_fiber = ctx->obuf[0].as<int32_t>();
   *fiber_root = ctx->obuf[1].as<int32_t>();
-  if ((rc = seg_owner(ctx, x.l3_pos, x.nfibers, x.nnz, *leaf_fiber))) return rc;
-  return seg_owner(ctx, x.l2_pos, x.nroots, x.nfibers, *fiber_root);
+  int32_t* first = *leaf_fiber + nz;
+  if ((rc = seg_owner(ctx, x.l3_pos, x.nfibers, x.nnz, *leaf_fiber, first))) return rc;
+  return seg_owner(ctx, x.l2_pos, x.nroots, x.nfibers, *fiber_root, first);
 }
 
 // mB / mC (device, J and K entries; both null or both set): per factor row,
