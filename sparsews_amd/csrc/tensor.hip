@@ -5,7 +5,8 @@
 //   reference (B(k,l,i) in benchmark/MTTKRP/COO_CSF_DCSR_DCSR_hash_free.h:233;
 //   its C and D factors are Bf and Cf here, given dense, row-major R wide).
 //   Bucket workspace over output rows:
-//     1. k_seg_owner (x2)   leaf -> fiber and fiber -> root owners.
+//     1. k_seg_tile_first + k_seg_owner (x2)   leaf -> fiber and fiber ->
+//                           root owners.
 //     2. k_mttkrp_keys      packed key (i << (bj+bk) | j << bk | k), value X.
 //     3. radix sort         stable on the i bits only: the buckets are the
 //                           output rows, CSF (j,k) order kept inside each.
@@ -28,13 +29,15 @@
 //   level, as in the reference's hash_il (B(k,i,j) in
 //   benchmark/TTM/COO_CSF_DCSR_hash_il.h:247), whose hash workspace is keyed
 //   on the output fiber (i,j).  Here the fibers are grouped by sorting:
-//     1. k_seg_owner (x2)   leaf -> fiber and fiber -> root owners.
+//     1. k_seg_tile_first + k_seg_owner (x2)   leaf -> fiber and fiber ->
+//                           root owners.
 //     2. k_ttm_direct_keys  packed (i, j, k) key per nonzero (the sort reads
 //                           the values straight from the CSF).
 //     3. radix sort         stable, on the (i, j) bits only: each fiber's
 //                           nonzeros end up together and keep the CSF order,
 //                           increasing k -- the order hash_il sums them.
-//     4. k_ttm_fiber_flags + scan + k_ttm_fiber_starts: fiber heads, fiber
+//     4. k_ttm_fiber_count + scan of the tile counts + k_ttm_fiber_starts:
+//                           fiber heads ranked within their tile, fiber
 //                           offsets and the output coordinates (i, j).
 //     5. k_ttm_accum        one 16-lane group per work item (at most 128
 //                           nonzeros of one fiber) sums X * U(k,:) in order;
@@ -452,7 +455,8 @@ __device__ __forceinline__ double ttm_item_tail(double acc, int32_t base, int32_
 // a longer fiber write partial sums, which k_ttm_combine adds in item order
 // (so the result does not depend on timing).
 template <bool kOff32>
-__global__ __launch_bounds__(256) void k_ttm_accum(int64_t nitems, int R, const int4* __restrict__ items,
+__global__ __launch_bounds__(256) void k_ttm_accum(const int32_t* __restrict__ nitems_p, int R,
+                                                   const int4* __restrict__ items,
                                                    const uint64_t* __restrict__ keys, const double* __restrict__ vals,
                                                    int bk, const double* __restrict__ U, double* __restrict__ Y,
                                                    double* __restrict__ part) {
@@ -461,6 +465,7 @@ __global__ __launch_bounds__(256) void k_ttm_accum(int64_t nitems, int R, const 
   const uint64_t km = (1ull << bk) - 1;
   const uint32_t km32 = (uint32_t)km;  // k < 2^31: the key's low word holds it
   const uint32_t* __restrict__ keys_lo = reinterpret_cast<const uint32_t*>(keys);  // little-endian low words
+  const int64_t nitems = *nitems_p;  // the items' count, on the device (the grid is sized for at most that many)
   const int64_t stride = (int64_t)gridDim.x * groups;
   int64_t c = (int64_t)blockIdx.x * groups + threadIdx.x / kTtmLanes;
   if (c >= nitems) return;  // uniform per group
@@ -784,8 +789,6 @@ static int mttkrp_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_
 // the partial sums of long fibers combined in item order.
 static int ttm_accumulate(spws_ctx* ctx, int64_t nnz, int64_t F, int32_t R, int bk, const uint64_t* nkeys,
                           const double* nvals, const int32_t* foff, const double* U, double* Y) {
-  hipStream_t s = ctx->stream;
-  int32_t* hm = ctx->host_meta.as<int32_t>();
   int rc;
   // coff = prefix of items per fiber
   if ((rc = ctx->obuf[2].ensure(((size_t)F + 1) * 4 * 2))) return rc;
@@ -798,23 +801,22 @@ static int ttm_accumulate(spws_ctx* ctx, int64_t nnz, int64_t F, int32_t R, int 
   const int64_t max_items = F + nnz / kTtmChunk + 1;
   if ((rc = ctx->obuf[3].ensure((size_t)max_items * R * 8)) || (rc = ctx->obuf[4].ensure((size_t)max_items * 16)))
     return rc;
-  SPWS_HIP(hipMemcpyAsync(hm, coff + F, 4, hipMemcpyDeviceToHost, s));
-  SPWS_HIP(hipStreamSynchronize(s));
-  const int64_t nitems = *hm;
+  // the item count stays on the device (coff[F]): no wait for it here
+  const int32_t* nitems_dev = coff + F;
   int4* items = ctx->obuf[4].as<int4>();
   SPWS_LAUNCH(ctx, "ttm_itemmap", k_ttm_itemmap, dim3(gf2), dim3(256), 0, F, foff, coff, items);
   // one workgroup per slot of a few per CU: each group walks many items
   int cus = 0;
   SPWS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-  const unsigned ga = (unsigned)std::max<int64_t>(1, std::min<int64_t>(div_up(nitems, 256 / kTtmLanes),
+  const unsigned ga = (unsigned)std::max<int64_t>(1, std::min<int64_t>(div_up(max_items, 256 / kTtmLanes),
                                                                         (int64_t)std::max(cus, 1) * kTtmBlocksPerCu));
   const bool off32 = ((int64_t)1 << bk) * R * 8 <= (int64_t)UINT32_MAX;  // every U offset fits in 32 bits
-  if (nitems > 0 && off32)
-    SPWS_LAUNCH(ctx, "ttm_accum", k_ttm_accum<true>, dim3(ga), dim3(256), 0, nitems, R, items, nkeys, nvals, bk, U,
-                Y, ctx->obuf[3].as<double>());
-  else if (nitems > 0)
-    SPWS_LAUNCH(ctx, "ttm_accum", k_ttm_accum<false>, dim3(ga), dim3(256), 0, nitems, R, items, nkeys, nvals, bk, U,
-                Y, ctx->obuf[3].as<double>());
+  if (off32)
+    SPWS_LAUNCH(ctx, "ttm_accum", k_ttm_accum<true>, dim3(ga), dim3(256), 0, nitems_dev, R, items, nkeys, nvals, bk,
+                U, Y, ctx->obuf[3].as<double>());
+  else
+    SPWS_LAUNCH(ctx, "ttm_accum", k_ttm_accum<false>, dim3(ga), dim3(256), 0, nitems_dev, R, items, nkeys, nvals, bk,
+                U, Y, ctx->obuf[3].as<double>());
   const unsigned gc = (unsigned)std::min<int64_t>(div_up(F, 256 / kTtmLanes), 65536);
   SPWS_LAUNCH(ctx, "ttm_combine", k_ttm_combine, dim3(gc), dim3(256), 0, F, R, coff, ctx->obuf[3].as<double>(), Y);
   return SPWS_OK;
