@@ -87,9 +87,36 @@ __global__ __launch_bounds__(256) void k_seg_tile_first(const int32_t* __restric
   first[t] = lo;
 }
 
+// What k_seg_owner does with item e of segment s: store s.
+struct OwnerOut {
+  int32_t* __restrict__ owner;
+  __device__ void operator()(int64_t e, int32_t s) const { owner[e] = s; }
+};
+
+// Or, over a CSF's leaves (s = the nonzero's fiber): its packed sort key
+// (hi << (bj + bk)) | (mid << bk) | k with k = the fiber's coordinate and,
+// for TTM, hi = the root's and mid = the leaf's coordinate; for MTTKRP the
+// other way round.
+template <bool kMttkrp>
+struct CsfKeyOut {
+  const int32_t* __restrict__ fiber_root;
+  const int32_t* __restrict__ l1_crd;
+  const int32_t* __restrict__ l2_crd;
+  const int32_t* __restrict__ l3_crd;
+  int bj, bk;
+  uint64_t* __restrict__ keys;
+  __device__ void operator()(int64_t e, int32_t f) const {
+    const uint64_t root = (uint32_t)l1_crd[fiber_root[f]];
+    const uint64_t k = (uint32_t)l2_crd[f];
+    const uint64_t leaf = (uint32_t)l3_crd[e];
+    const uint64_t hi = kMttkrp ? leaf : root, mid = kMttkrp ? root : leaf;
+    keys[e] = (hi << (bj + bk)) | (mid << bk) | k;
+  }
+};
+
+template <class Out>
 __global__ __launch_bounds__(kOwnBlock) void k_seg_owner(const int32_t* __restrict__ pos, int64_t nitems,
-                                                         const int32_t* __restrict__ first,
-                                                         int32_t* __restrict__ owner) {
+                                                         const int32_t* __restrict__ first, Out out) {
   __shared__ int32_t spos[kOwnTile + 1];
   const int64_t e0 = (int64_t)blockIdx.x * kOwnTile;
   const int64_t e_end = e0 + kOwnTile < nitems ? e0 + kOwnTile : nitems;
@@ -114,18 +141,19 @@ __global__ __launch_bounds__(kOwnBlock) void k_seg_owner(const int32_t* __restri
         if (pos[lo0 + mid] <= e) lo = mid; else hi = mid - 1;
       }
     }
-    owner[e] = lo0 + lo;
+    out(e, lo0 + lo);
   }
 }
 
 // first: ntiles + 1 entries of scratch (ntiles = tiles of kOwnTile items).
-static int seg_owner(spws_ctx* ctx, const int32_t* pos, int32_t nseg, int64_t nitems, int32_t* owner,
-                     int32_t* first) {
+template <class Out>
+static int seg_owner(spws_ctx* ctx, const int32_t* pos, int32_t nseg, int64_t nitems, Out out, int32_t* first,
+                     const char* name = "seg_owner") {
   if (nitems <= 0) return SPWS_OK;
   const int64_t ntiles = div_up(nitems, kOwnTile);
   SPWS_LAUNCH(ctx, "seg_owner", k_seg_tile_first, dim3((unsigned)div_up(ntiles + 1, 256)), dim3(256), 0, pos, nseg,
               nitems, ntiles, first);
-  SPWS_LAUNCH(ctx, "seg_owner", k_seg_owner, dim3((unsigned)ntiles), dim3(kOwnBlock), 0, pos, nitems, first, owner);
+  SPWS_LAUNCH(ctx, name, k_seg_owner<Out>, dim3((unsigned)ntiles), dim3(kOwnBlock), 0, pos, nitems, first, out);
   return SPWS_OK;
 }
 
@@ -683,8 +711,27 @@ static int csf_owners(spws_ctx* ctx, const Csf3& x, int32_t** leaf_fiber, int32_
   *leaf_fiber = ctx->obuf[0].as<int32_t>();
   *fiber_root = ctx->obuf[1].as<int32_t>();
   int32_t* first = *leaf_fiber + nz;
-  if ((rc = seg_owner(ctx, x.l3_pos, x.nfibers, x.nnz, *leaf_fiber, first))) return rc;
-  return seg_owner(ctx, x.l2_pos, x.nroots, x.nfibers, *fiber_root, first);
+  if ((rc = seg_owner(ctx, x.l3_pos, x.nfibers, x.nnz, OwnerOut{*leaf_fiber}, first))) return rc;
+  return seg_owner(ctx, x.l2_pos, x.nroots, x.nfibers, OwnerOut{*fiber_root}, first);
+}
+
+// The nonzeros' packed sort keys straight from the CSF (CsfKeyOut): the
+// fibers' roots first, then the keys from the leaves' owner walk, with no
+// per-nonzero fiber array.
+template <bool kMttkrp>
+static int csf_keys(spws_ctx* ctx, const Csf3& x, int bj, int bk, uint64_t* keys) {
+  int rc;
+  const size_t nz = (size_t)(x.nnz > 0 ? x.nnz : 1);
+  const size_t tiles = (size_t)div_up((int64_t)nz, kOwnTile) + 1;
+  if ((rc = ctx->obuf[0].ensure(tiles * 4)) ||
+      (rc = ctx->obuf[1].ensure((size_t)(x.nfibers > 0 ? x.nfibers : 1) * 4)))
+    return rc;
+  int32_t* first = ctx->obuf[0].as<int32_t>();
+  int32_t* fiber_root = ctx->obuf[1].as<int32_t>();
+  if ((rc = seg_owner(ctx, x.l2_pos, x.nroots, x.nfibers, OwnerOut{fiber_root}, first))) return rc;
+  return seg_owner(ctx, x.l3_pos, x.nfibers, x.nnz,
+                   CsfKeyOut<kMttkrp>{fiber_root, x.l1_crd, x.l2_crd, x.l3_crd, bj, bk, keys}, first,
+                   kMttkrp ? "mttkrp_keys" : "ttm_keys");
 }
 
 // mB / mC (device, J and K entries; both null or both set): per factor row,
@@ -722,18 +769,16 @@ static int mttkrp_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_
   if (masked) SPWS_HIP(hipMemsetAsync(rmask, 0, ii * 8 + 8, s));
   int64_t kept = x.nnz;
   if (x.nnz > 0) {
-    int32_t *leaf_fiber, *fiber_root;
-    if ((rc = csf_owners(ctx, x, &leaf_fiber, &fiber_root))) return rc;
     uint64_t* keys = ctx->keys[0].as<uint64_t>();
     double* vals = ctx->vals[0].as<double>();
-    const unsigned g = (unsigned)std::min<int64_t>(div_up(x.nnz, 256), 65536);
     if (masked) {
+      int32_t *leaf_fiber, *fiber_root;
+      if ((rc = csf_owners(ctx, x, &leaf_fiber, &fiber_root))) return rc;
+      const unsigned g = (unsigned)std::min<int64_t>(div_up(x.nnz, 256), 65536);
       SPWS_LAUNCH(ctx, "mttkrp_keys", k_mttkrp_keys<true>, dim3(g), dim3(256), 0, x.nnz, leaf_fiber, fiber_root,
                   x.l1_crd, x.l2_crd, x.l3_crd, bj, bk, keys, mB, mC, (uint64_t)I, dropped);
-    } else {
-      SPWS_LAUNCH(ctx, "mttkrp_keys", k_mttkrp_keys<false>, dim3(g), dim3(256), 0, x.nnz, leaf_fiber, fiber_root,
-                  x.l1_crd, x.l2_crd, x.l3_crd, bj, bk, keys, (const uint64_t*)nullptr, (const uint64_t*)nullptr,
-                  (uint64_t)0, (unsigned long long*)nullptr);
+    } else if ((rc = csf_keys<true>(ctx, x, bj, bk, keys))) {
+      return rc;
     }
     bool in_alt = false;
     if ((rc = radix_sort_pairs<uint64_t>(ctx, keys, vals, ctx->keys[1].as<uint64_t>(), ctx->vals[1].as<double>(),
@@ -870,23 +915,21 @@ static int ttm_run(spws_ctx* ctx, const Csf3& x, int32_t I, int32_t J, int32_t K
   int64_t F = 0;
   if (x.nnz > 0 && direct) {
     // one sort of the nonzeros on (i, j, k)
-    int32_t *leaf_fiber, *fiber_root;
-    if ((rc = csf_owners(ctx, x, &leaf_fiber, &fiber_root))) return rc;
     const int bk = bk_all;
     uint64_t* nkeys = ctx->keys[0].as<uint64_t>();
     double* nvals = ctx->vals[0].as<double>();
     const unsigned g = (unsigned)std::min<int64_t>(div_up(x.nnz, 256), 65536);
     unsigned long long* dropped = nullptr;
     if (masked) {
+      int32_t *leaf_fiber, *fiber_root;
+      if ((rc = csf_owners(ctx, x, &leaf_fiber, &fiber_root))) return rc;
       if ((rc = ctx->obuf[6].ensure(256))) return rc;
       dropped = ctx->obuf[6].as<unsigned long long>();
       SPWS_HIP(hipMemsetAsync(dropped, 0, 8, s));
       SPWS_LAUNCH(ctx, "ttm_keys", k_ttm_direct_keys<true>, dim3(g), dim3(256), 0, x.nnz, leaf_fiber, fiber_root,
                   x.l1_crd, x.l2_crd, x.l3_crd, bj, bk, nkeys, umask, (uint64_t)I, dropped);
-    } else {
-      SPWS_LAUNCH(ctx, "ttm_keys", k_ttm_direct_keys<false>, dim3(g), dim3(256), 0, x.nnz, leaf_fiber, fiber_root,
-                  x.l1_crd, x.l2_crd, x.l3_crd, bj, bk, nkeys, (const uint64_t*)nullptr, (uint64_t)0,
-                  (unsigned long long*)nullptr);
+    } else if ((rc = csf_keys<false>(ctx, x, bj, bk, nkeys))) {
+      return rc;
     }
     // the sort covers (i, j) only: within an i slice the CSF order already
     // has each (i, j)'s nonzeros in increasing k, and the sort is stable
