@@ -5,12 +5,14 @@
 //   reference (B(k,l,i) in benchmark/MTTKRP/COO_CSF_DCSR_DCSR_hash_free.h:233;
 //   its C and D factors are Bf and Cf here, given dense, row-major R wide).
 //   Bucket workspace over output rows:
-//     1. k_seg_tile_first + k_seg_owner (x2)   leaf -> fiber and fiber ->
-//                           root owners.
-//     2. k_mttkrp_keys      packed key (i << (bj+bk) | j << bk | k), value X.
-//     3. radix sort         stable on the i bits only: the buckets are the
+//     1. k_seg_tile_first + k_seg_owner   fiber -> root owners, then the
+//                           leaves' owner walk writes each nonzero's packed
+//                           key (i << (bj+bk) | j << bk | k) directly
+//                           (CsfKeyOut); with factor masks, leaf -> fiber
+//                           owners and k_mttkrp_keys instead.
+//     2. radix sort         stable on the i bits only: the buckets are the
 //                           output rows, CSF (j,k) order kept inside each.
-//     4. k_mttkrp_accum     each half-wavefront (32 lanes = 32 ranks) walks a
+//     3. k_mttkrp_accum     each half-wavefront (32 lanes = 32 ranks) walks a
 //                           segment of the bucketed nonzeros, loading factor
 //                           rows Bf(j,:), Cf(k,:) as coalesced 256-byte rows
 //                           (Bf(j,:) only when j changes: within a row the
@@ -19,7 +21,7 @@
 //                           a fiber is added into M when its row changes (only
 //                           segment-boundary rows are shared, so the atomics
 //                           are few and uncontended).
-//     5. k_rows_present + scan + k_rows_compact: the sorted list of rows that
+//     4. k_rows_present + scan + k_rows_compact: the sorted list of rows that
 //                           hold a nonzero (the reference's COO output rows).
 //   The product is (X * Bf) * Cf, the reference's B_vals * C_vals * D_vals
 //   (:296), and each segment sums in CSF order.
@@ -29,17 +31,19 @@
 //   level, as in the reference's hash_il (B(k,i,j) in
 //   benchmark/TTM/COO_CSF_DCSR_hash_il.h:247), whose hash workspace is keyed
 //   on the output fiber (i,j).  Here the fibers are grouped by sorting:
-//     1. k_seg_tile_first + k_seg_owner (x2)   leaf -> fiber and fiber ->
-//                           root owners.
-//     2. k_ttm_direct_keys  packed (i, j, k) key per nonzero (the sort reads
-//                           the values straight from the CSF).
-//     3. radix sort         stable, on the (i, j) bits only: each fiber's
+//     1. k_seg_tile_first + k_seg_owner   fiber -> root owners, then the
+//                           leaves' owner walk writes each nonzero's packed
+//                           (i, j, k) key directly (CsfKeyOut); with a U row
+//                           mask, leaf -> fiber owners and k_ttm_direct_keys
+//                           instead.  The sort reads the values straight
+//                           from the CSF.
+//     2. radix sort         stable, on the (i, j) bits only: each fiber's
 //                           nonzeros end up together and keep the CSF order,
 //                           increasing k -- the order hash_il sums them.
-//     4. k_ttm_fiber_count + scan of the tile counts + k_ttm_fiber_starts:
+//     3. k_ttm_fiber_count + scan of the tile counts + k_ttm_fiber_starts:
 //                           fiber heads ranked within their tile, fiber
 //                           offsets and the output coordinates (i, j).
-//     5. k_ttm_accum        one 16-lane group per work item (at most 128
+//     4. k_ttm_accum        one 16-lane group per work item (at most 128
 //                           nonzeros of one fiber) sums X * U(k,:) in order;
 //                           a fiber of one item writes its vector, longer
 //                           fibers write partial sums that k_ttm_combine adds
