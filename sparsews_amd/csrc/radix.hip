@@ -42,7 +42,7 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_hist(const K* __restrict__
 }
 
 template <class K>
-__global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const K* __restrict__ kin, const double* __restrict__ vin,
+__global__ __launch_bounds__(kSortBlock, 4) void k_radix_scatter(const K* __restrict__ kin, const double* __restrict__ vin,
                                                               K* __restrict__ kout, double* __restrict__ vout,
                                                               int64_t n, int shift, uint32_t mask, int nbits,
                                                               const int32_t* __restrict__ offs, int64_t ntiles) {
@@ -51,7 +51,11 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const K* __restric
   static_assert(kSortBucketsMax == 2 * kSortBlock, "two digits per thread in the prefix step");
   static_assert(sizeof(K) <= sizeof(double), "one staging buffer for keys and values");
   __shared__ double stage[kSortTile];  // keys, then values, in digit order
-  __shared__ int32_t wcnt[kWaves][kSortBucketsMax];
+  // per (wavefront, digit): counts, then their prefixes over the wavefronts
+  // (at most kSortTile = 4096, so 16 bits: 38.9 KB of LDS in all, four
+  // workgroups per CU)
+  __shared__ uint16_t wcnt[kWaves][kSortBucketsMax];
+  static_assert(kSortTile < 65536, "per-wavefront digit prefixes fit 16 bits");
   __shared__ int32_t dstart[kSortBucketsMax];
   __shared__ int32_t scan_sh[kSortBlock / kWave + 1];
   K* sk = reinterpret_cast<K*>(stage);
@@ -94,7 +98,7 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const K* __restric
       const int before = __popcll(match & lt);
       const int32_t old = wcnt[w][d];
       rank[r] = old + before;
-      if (before == 0) wcnt[w][d] = old + __popcll(match);
+      if (before == 0) wcnt[w][d] = (uint16_t)(old + __popcll(match));
     }
   }
   __syncthreads();
@@ -110,7 +114,7 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const K* __restric
 #pragma unroll
         for (int q = 0; q < kWaves; ++q) {
           const int32_t c = wcnt[q][d];
-          wcnt[q][d] = run;
+          wcnt[q][d] = (uint16_t)run;
           run += c;
         }
       }
